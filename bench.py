@@ -1,0 +1,161 @@
+"""Benchmark: PageRank GTEPS on the scale-24 rMat graph (n=2^24, m=2^28) on one
+MI355X, 20 iterations per step (numa-PageRank.C semantics, fp32).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line with the metric, the roofline of the dominant kernel
+(the pull iteration, measured with HIP events on the library stream) and a
+CPU baseline (the oracle's fp64 OpenMP PageRank on a bounded sample of the
+same workload: 3 of the 20 iterations on the same graph).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pagerank_bytes(n, edges, iters):
+    """Algorithmic bytes of the pull iterations: 8 per active out-edge (4 B
+    neighbour id + 4 B gathered contribution) and 20 per vertex (offsets,
+    rank write, contribution write, frontier read/write, out-degree)."""
+    return 8.0 * edges + 20.0 * n * iters
+
+
+def cpu_baseline(pa, g, iters_sample=3):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as orc
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    off, adj, _ = g.download()
+    hg = orc.Graph(g.n, off, adj)
+    hg.transpose()
+    _, st = g.pagerank(iters_sample, 0.85, out=False)  # exact sum of E_t for the sample
+    t0 = time.perf_counter()
+    orc.pagerank_pull_omp(hg, iters_sample, 0.85, threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(st.edges / dt / 1e9, 4),
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d of the 20 PageRank iterations (fp64 pull, OpenMP) on the same "
+                  "scale-24 graph, %.1f s" % (iters_sample, dt),
+    }
+
+
+def bfs_extra(pa, scale=22, m=1 << 26, nsrc=8):
+    gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True)
+    rng = np.random.default_rng(7)
+    srcs = [0] + [int(x) for x in rng.integers(0, gs.n, nsrc)]
+    gs.bfs(0, out=False)  # warm-up / workspace
+    tot_ms, tot_edges = 0.0, 0
+    for s in srcs:
+        _, _, st = gs.bfs(s, out=False)
+        tot_ms += st.ms
+        tot_edges += st.edges  # directed entries of the reached vertices
+    return {"bfs_scale22_gteps_graph500": round(tot_edges / 2 / (tot_ms / 1e3) / 1e9, 3),
+            "bfs_scale22_gteps_directed": round(tot_edges / (tot_ms / 1e3) / 1e9, 3),
+            "bfs_sources": len(srcs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edges", type=int, default=1 << 28)
+    ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import polymer_amd as pa
+
+    pa.init(0)
+    log("device:", pa.device_name())
+    t0 = time.perf_counter()
+    g = pa.Graph.rmat(args.scale, args.edges, seed=1)
+    log("graph scale %d m %d built in %.1f s" % (args.scale, g.m, time.perf_counter() - t0))
+
+    for _ in range(args.warmup):
+        g.pagerank(args.iters, 0.85, out=False)
+    # timed region: K steps, each a full 20-iteration PageRank with inputs resident in HBM
+    edges = 0
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    for _ in range(args.steps):
+        _, st = g.pagerank(args.iters, 0.85, out=False)
+        edges += st.edges
+        dev_ms += st.ms
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    gteps = edges / wall / 1e9
+    log("pagerank: %.3f ms/step wall, %.3f ms/step device, %.2f GTEPS" %
+        (ms_per_step, dev_ms / args.steps, gteps))
+
+    # dominant kernel: the pull iteration, timed launch by launch with HIP events
+    _, stp = g.pagerank(args.iters, 0.85, out=False, profile=True)
+    avg_kernel_s = stp.kernel_ms / max(stp.kernel_launches, 1) / 1e3
+    bytes_per_launch = pagerank_bytes(g.n, stp.edges, args.iters) / args.iters
+    achieved = bytes_per_launch / avg_kernel_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_pull<PrOp>", "avg_launch_us": round(avg_kernel_s * 1e6, 2),
+                "bytes_per_launch": int(bytes_per_launch)}
+    log("kernel avg %.1f us, %.1f GB/s algorithmic" % (avg_kernel_s * 1e6, achieved))
+
+    extra = {}
+    if not args.no_extra:
+        try:
+            extra.update(bfs_extra(pa))
+        except Exception as e:  # report, do not hide
+            extra["bfs_error"] = str(e)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pa, g)
+        log("cpu baseline:", cpu)
+
+    out = {
+        "metric": "PageRank GTEPS (scale-24 rMat), % of HBM peak in roofline.frac",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: device-generated rMat a/b/c=0.57/0.19/0.19 seed 1, no self-loops "
+                "or duplicates",
+        "config": {"workload": "pagerank rmat scale-%d m=%d, %d iterations per step, "
+                               "damping 0.85" % (args.scale, g.m, args.iters),
+                   "n": g.n, "m": g.m, "iters": args.iters},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

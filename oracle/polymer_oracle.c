@@ -216,6 +216,19 @@ float or_range_f32(uint64_t seed, uint64_t i, float lo, float hi) {
     return lo + t;
 }
 
+/* BP edge potential W_e[k/2][k%2] of the edge s->d in [lo,hi) */
+float or_bp_pot(uint64_t seed, int32_t s, int32_t d, int k, float lo, float hi) {
+    uint64_t i = or_mix64(((uint64_t)(uint32_t)s << 32) | (uint32_t)d) + (uint64_t)k;
+    return or_range_f32(seed ^ 0xB5297A4D3F84D5B5ull, i, lo, hi);
+}
+
+void or_fill_bp_pot(uint64_t seed, int64_t n, const int64_t *off, const int32_t *adj,
+                    float lo, float hi, float *pot) {
+    for (int64_t s = 0; s < n; s++)
+        for (int64_t e = off[s]; e < off[s + 1]; e++)
+            for (int k = 0; k < 4; k++) pot[4 * e + k] = or_bp_pot(seed, (int32_t)s, adj[e], k, lo, hi);
+}
+
 void or_fill_weights_i32(uint64_t seed, int64_t n, const int64_t *off,
                          const int32_t *adj, int32_t *w) {
     for (int64_t s = 0; s < n; s++)
@@ -713,9 +726,29 @@ int or_spmv(int64_t n, const int64_t *off, const int32_t *adj, const double *w,
 /* The second product buffer starts at zero (mmap'd, numa-BP.C:557).    */
 /* ------------------------------------------------------------------ */
 
+static int bp_impl(int64_t n, const int64_t *off, const int32_t *adj,
+                   const float *vpot, const float *epot, const double *prod0, int iters,
+                   double *prod_out, int state_f32);
+
 int or_bp(int64_t n, const int64_t *off, const int32_t *adj,
           const float *vpot, const float *epot, const double *prod0, int iters,
           double *prod_out) {
+    return bp_impl(n, off, adj, vpot, epot, prod0, iters, prod_out, 0);
+}
+
+/* fp64 arithmetic on a state rounded to fp32 after every iteration: the error
+ * left between this and an fp32 implementation is one iteration's arithmetic
+ * (the state's own rounding is raised to ~in-degree power by the next
+ * iteration, which no fp32 state can avoid). */
+int or_bp_state_f32(int64_t n, const int64_t *off, const int32_t *adj,
+                    const float *vpot, const float *epot, const double *prod0, int iters,
+                    double *prod_out) {
+    return bp_impl(n, off, adj, vpot, epot, prod0, iters, prod_out, 1);
+}
+
+static int bp_impl(int64_t n, const int64_t *off, const int32_t *adj,
+                   const float *vpot, const float *epot, const double *prod0, int iters,
+                   double *prod_out, int state_f32) {
     double *cur = (double *)malloc(sizeof(double) * 2 * (size_t)n);
     double *nxt = (double *)calloc(2 * (size_t)n, sizeof(double));
     uint8_t *F = (uint8_t *)malloc((size_t)n);
@@ -738,6 +771,8 @@ int or_bp(int64_t n, const int64_t *off, const int32_t *adj,
                 N[d] = 1;
             }
         }
+        if (state_f32)
+            for (int64_t i = 0; i < 2 * n; i++) nxt[i] = (double)(float)nxt[i];
         double *tp = cur; cur = nxt; nxt = tp;
         uint8_t *tf = F; F = N; N = tf;
     }

@@ -56,6 +56,9 @@ int32_t or_weight_i32(uint64_t seed, int32_t u, int32_t v);   /* [1,255], symmet
 float or_value_f32(uint64_t seed, int32_t u, int32_t v);      /* (0,1], directed */
 float or_unit_f32(uint64_t seed, uint64_t i);                 /* [0,1) */
 float or_range_f32(uint64_t seed, uint64_t i, float lo, float hi); /* [lo,hi) */
+float or_bp_pot(uint64_t seed, int32_t s, int32_t d, int k, float lo, float hi);
+void or_fill_bp_pot(uint64_t seed, int64_t n, const int64_t *off, const int32_t *adj,
+                    float lo, float hi, float *pot);
 void or_fill_weights_i32(uint64_t seed, int64_t n, const int64_t *off,
                          const int32_t *adj, int32_t *w);
 void or_fill_values_f32(uint64_t seed, int64_t n, const int64_t *off,
@@ -99,6 +102,9 @@ int or_spmv(int64_t n, const int64_t *off, const int32_t *adj, const double *w,
 int or_bp(int64_t n, const int64_t *off, const int32_t *adj,
           const float *vpot, const float *epot, const double *prod0, int iters,
           double *prod_out);
+int or_bp_state_f32(int64_t n, const int64_t *off, const int32_t *adj,
+                    const float *vpot, const float *epot, const double *prod0, int iters,
+                    double *prod_out);
 int or_bp_f32(int64_t n, const int64_t *off, const int32_t *adj,
               const float *vpot, const float *epot, const float *prod0,
               int iters, float *prod_out);

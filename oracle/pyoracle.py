@@ -34,6 +34,7 @@ def lib():
             "or_symmetrize": (i64, [i64, I64P, I32P, ctypes.c_void_p, ctypes.c_void_p]),
             "or_fill_weights_i32": (None, [u64, i64, I64P, I32P, I32P]),
             "or_fill_values_f32": (None, [u64, i64, I64P, I32P, F32P]),
+            "or_fill_bp_pot": (None, [u64, i64, I64P, I32P, ctypes.c_float, ctypes.c_float, F32P]),
             "or_fill_vertex_f32": (None, [u64, i64, ctypes.c_float, ctypes.c_float, F32P]),
             "or_write_adj": (ctypes.c_int, [ctypes.c_char_p, i64, i64, I64P, I32P]),
             "or_write_wadj": (ctypes.c_int, [ctypes.c_char_p, i64, i64, I64P, I32P, I32P]),
@@ -53,6 +54,8 @@ def lib():
             "or_spmv": (ctypes.c_int, [i64, I64P, I32P, F64P, F64P, ctypes.c_int, F64P]),
             "or_bp": (ctypes.c_int, [i64, I64P, I32P, F32P, F32P, F64P, ctypes.c_int, F64P]),
             "or_bp_f32": (ctypes.c_int, [i64, I64P, I32P, F32P, F32P, F32P, ctypes.c_int, F32P]),
+            "or_bp_state_f32": (ctypes.c_int, [i64, I64P, I32P, F32P, F32P, F64P, ctypes.c_int,
+                                               F64P]),
             "or_ppr": (ctypes.c_int, [i64, I64P, I32P, ctypes.c_int, f64, F64P, F64P]),
             "or_wpr_forward": (ctypes.c_int, [i64, I64P, I32P, F64P, F64P, f64, ctypes.c_int, f64,
                                               F64P, ctypes.POINTER(ctypes.c_int)]),
@@ -135,6 +138,12 @@ def values_f32(g, seed):
     w = np.zeros(max(g.m, 1), np.float32)
     lib().or_fill_values_f32(seed, g.n, g.off, g.adj, w)
     return w[: g.m]
+
+
+def bp_potentials(g, seed, lo=0.1, hi=1.0):
+    pot = np.zeros(max(4 * g.m, 1), np.float32)
+    lib().or_fill_bp_pot(seed, g.n, g.off, g.adj, lo, hi, pot)
+    return pot[: 4 * g.m]
 
 
 def vertex_f32(n, seed, lo=0.0, hi=1.0):
@@ -242,6 +251,15 @@ def bp(g, vpot, epot, prod0, iters):
     lib().or_bp(g.n, g.off, g.adj, np.ascontiguousarray(vpot, np.float32),
                 np.ascontiguousarray(epot, np.float32),
                 np.ascontiguousarray(prod0, np.float64).ravel(), iters, out)
+    return out.reshape(g.n, 2)
+
+
+def bp_state_f32(g, vpot, epot, prod0, iters):
+    """fp64 arithmetic, state rounded to fp32 after every iteration."""
+    out = np.zeros(2 * g.n, np.float64)
+    lib().or_bp_state_f32(g.n, g.off, g.adj, np.ascontiguousarray(vpot, np.float32),
+                          np.ascontiguousarray(epot, np.float32),
+                          np.ascontiguousarray(prod0, np.float64).ravel(), iters, out)
     return out.reshape(g.n, 2)
 
 

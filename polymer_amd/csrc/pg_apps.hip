@@ -1,0 +1,954 @@
+// pg_apps.hip -- the six Polymer applications on the MI355X engine.
+//
+// Each app is an edge operator (the reference's functor) plugged into the
+// pull engine (k_pull) or the push engine (k_push) plus a host loop that
+// mirrors the app's iteration protocol (the pthread SPMD loop of numa-*.C),
+// with kernel boundaries in place of the reference's barriers.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pg_engine.h"
+#include "pg_internal.h"
+
+namespace pg {
+
+#define PG_LAUNCH_K(kern, grid, ...)                                                  \
+    do {                                                                              \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream(), __VA_ARGS__); \
+        PG_CHECK_HIP(hipGetLastError());                                              \
+    } while (0)
+
+static inline int64_t resolve_threshold(int64_t thr, int64_t m, int64_t div) {
+    return thr < 0 ? m / div : thr;
+}
+
+// ======================================================================================
+// PageRank  (numa-PageRank.C:64-148 PR_F / PR_Vertex_F / PR_Vertex_Reset, loop :273-335)
+//
+// The reference pushes p_curr[s]/outdeg(s) from every active s into p_next
+// (edgeMapDenseForwardOTHER :177-230) and marks every out-neighbour active;
+// then, on the current frontier only, p_next = d*p_next + (1-d)/n and
+// p_curr = 0; swap.  Pulled over the CSC this is, per active destination v:
+//     sum    = sum_{s in in(v)} contrib[s],  contrib[s] = F[s] ? cur[s]/outdeg(s) : 0
+//     nxt[v] = d*sum + (1-d)/n,   F'[v] = (sum > 0),   cur[v] = 0 if v leaves F
+// and contrib' = F'[v] ? nxt[v]/outdeg(v) : 0 is produced in the same pass.
+// sum > 0 is exactly "some in-neighbour is active": an active s has
+// cur[s] >= (1-d)/n > 0.  Frontiers only shrink (N_out(F_t) is a subset of F_t), so an
+// inactive vertex is never written (its value freezes, trap P1).
+// ======================================================================================
+struct PrOp {
+    using Acc = float;
+    using Ctx = int;
+    using Stat = unsigned long long;
+    static constexpr bool kEarlyExit = false;
+    static constexpr bool kStats = true;
+    const uint8_t *F;
+    uint8_t *Fn;
+    const float *contrib;
+    float *contrib_n;
+    float *cur;
+    float *nxt;
+    const int32_t *out_off;
+    const float *addc_v;  // per-vertex (1-d)/n_g for many graphs, else null
+    float damp, addc;
+    __device__ bool active(int v) const { return F[v] != 0; }
+    __device__ int ctx(int v) const { return v; }
+    __device__ float identity() const { return 0.f; }
+    __device__ float combine(float a, float b) const { return a + b; }
+    __device__ float edge(int s, int, const int &) const { return contrib[s]; }
+    __device__ bool found(float, const int &) const { return false; }
+    __device__ Stat finish(int v, float sum, const int &) const {
+        const float c = addc_v ? addc_v[v] : addc;
+        const float r = damp * sum + c;
+        nxt[v] = r;
+        const bool any = sum > 0.f;
+        const int od = out_off[v + 1] - out_off[v];
+        Fn[v] = any ? 1 : 0;
+        contrib_n[v] = (any && od > 0) ? r / (float)od : 0.f;
+        if (!any) cur[v] = 0.f;  // PR_Vertex_Reset on a vertex leaving the frontier
+        return any ? (Stat)od : 0ull;
+    }
+    __device__ void skip(int v) const {
+        Fn[v] = 0;
+        contrib_n[v] = 0.f;
+    }
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
+    __device__ static void stat_commit(Stat *p, Stat v) { atomicAdd(p, v); }
+};
+
+__global__ void k_pr_init(int64_t n, const int32_t *out_off, const float *inv_n, float one_over_n,
+                          float *cur, float *nxt, uint8_t *F, float *contrib) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const float x = inv_n ? inv_n[v] : one_over_n;
+        const int od = out_off[v + 1] - out_off[v];
+        cur[v] = x;
+        nxt[v] = 0.f;
+        F[v] = 1;
+        contrib[v] = od > 0 ? x / (float)od : 0.f;
+    }
+}
+
+__global__ void k_scale(const float *inv, int64_t n, float s, float *out) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        out[v] = s * inv[v];
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+struct PrBuffers {
+    float *a, *b, *c0, *c1;
+    uint8_t *f0, *f1;
+    unsigned long long *stats;  // per iteration sum of out-degrees of F_{t+1}
+};
+
+static int pr_buffers(pg_graph *g, int iters, PrBuffers *B) {
+    const size_t n = (size_t)g->n;
+    PG_TRY(workspace_t(g, "pr_a", n, &B->a));
+    PG_TRY(workspace_t(g, "pr_b", n, &B->b));
+    PG_TRY(workspace_t(g, "pr_c0", n, &B->c0));
+    PG_TRY(workspace_t(g, "pr_c1", n, &B->c1));
+    PG_TRY(workspace_t(g, "pr_f0", n, &B->f0));
+    PG_TRY(workspace_t(g, "pr_f1", n, &B->f1));
+    PG_TRY(workspace_t(g, "pr_stats", (size_t)iters + 1, &B->stats));
+    return PG_OK;
+}
+
+// Runs the iterations; on return *result points at the buffer holding p_curr.
+int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stats *st,
+                 float **result) {
+    PrBuffers B;
+    PG_TRY(pr_buffers(g, iters, &B));
+    const int64_t n = g->n;
+    PG_LAUNCH_K(k_pr_init, grid_for(n), n, g->out_off, g->inv_n, (float)(1.0 / (double)n), B.a,
+                B.b, B.f0, B.c0);
+    PG_LAUNCH_K(k_zero_u64, (unsigned)((iters + 1 + kBlock - 1) / kBlock), B.stats, iters + 1);
+    PrOp op{};
+    op.out_off = g->out_off;
+    op.addc_v = nullptr;
+    op.damp = (float)damping;
+    op.addc = (float)((1.0 - damping) * (1.0 / (double)n));
+    float *inv_n_addc = nullptr;
+    if (g->inv_n) {
+        PG_TRY(workspace_t(g, "pr_addc", (size_t)n, &inv_n_addc));
+        op.addc_v = inv_n_addc;
+    }
+    if (inv_n_addc)  // (1-d)/n_g per vertex
+        PG_LAUNCH_K(k_scale, grid_for(n), g->inv_n, n, (float)(1.0 - damping), inv_n_addc);
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    float *cur = B.a, *nxt = B.b, *c = B.c0, *cn = B.c1;
+    uint8_t *F = B.f0, *Fn = B.f1;
+    for (int t = 0; t < iters; t++) {
+        op.F = F;
+        op.Fn = Fn;
+        op.contrib = c;
+        op.contrib_n = cn;
+        op.cur = cur;
+        op.nxt = nxt;
+        PG_TRY(kt.begin());
+        PG_TRY(launch_pull(op, g->in_sched, &B.stats[t]));
+        PG_TRY(kt.end());
+        std::swap(cur, nxt);
+        std::swap(F, Fn);
+        std::swap(c, cn);
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    *result = cur;
+    if (st) {
+        std::vector<unsigned long long> hs((size_t)iters + 1);
+        PG_CHECK_HIP(hipMemcpy(hs.data(), B.stats, 8 * (iters + 1), hipMemcpyDeviceToHost));
+        st->rounds = iters;
+        st->dense_rounds = iters;
+        int64_t e = iters > 0 ? g->m : 0;  // F_1 = V
+        for (int t = 0; t + 1 < iters; t++) e += (int64_t)hs[t];
+        st->edges = e;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    return PG_OK;
+}
+
+// ======================================================================================
+// SpMV (numa-SPMV.C:56-108 SPMV_F / SPMV_Vertex_Reset, loop :168-193): frontier All is
+// never switched, so every iteration is y[d] = sum_{s->d} x[s] w(s,d) over all vertices.
+//
+// The reference iterates in fp64, and its values leave fp32's range (integer weights
+// reach 1e42 in 10 iterations on a scale-16 rMat).  The fp32 iteration therefore
+// carries one power-of-two exponent per iteration, shared by all vertices: iteration k
+// stores y_k * 2^-E_k with E_k chosen from the previous maximum (|y| ~ 2^64).  Scaling
+// by a power of two is exact, so every product and sum rounds exactly as unscaled
+// fp32 would; only the range grows.  The result is returned as fp64 y * 2^E.
+// ======================================================================================
+struct SpmvOp {
+    using Acc = float;
+    using Ctx = int;
+    using Stat = unsigned;  // bits of max |y| (non-negative floats order as integers)
+    static constexpr bool kEarlyExit = false;
+    static constexpr bool kStats = true;
+    const float *x;
+    const float *w;  // CSC order
+    float *y;
+    const int32_t *shift;  // this iteration's exponent shift
+    __device__ bool active(int) const { return true; }
+    __device__ int ctx(int v) const { return v; }
+    __device__ float identity() const { return 0.f; }
+    __device__ float combine(float a, float b) const { return a + b; }
+    __device__ float edge(int s, int e, const int &) const { return x[s] * w[e]; }
+    __device__ bool found(float, const int &) const { return false; }
+    __device__ Stat finish(int v, float sum, const int &) const {
+        const float r = ldexpf(sum, -*shift);
+        y[v] = r;
+        return __float_as_uint(fabsf(r));
+    }
+    __device__ void skip(int) const {}
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a > b ? a : b; }
+    __device__ static void stat_commit(Stat *p, Stat v) { atomicMax(p, v); }
+};
+
+// shift[k+1] from the maximum of iteration k; total[0] accumulates the exponent
+__global__ void k_spmv_shift(const unsigned *maxbits, int32_t *shift_next, long long *total,
+                             int32_t this_shift) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const float mx = __uint_as_float(*maxbits);
+        int s = 0;
+        if (mx > 0.f && isfinite(mx)) s = ilogbf(mx) - 64;
+        *shift_next = s;
+        total[0] += this_shift;
+    }
+}
+
+__global__ void k_absmax_bits(const float *x, int64_t n, unsigned *out) {
+    unsigned m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, __float_as_uint(fabsf(x[i])));
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned)__shfl_xor(m, o, 64));
+    if (lane_id() == 0) atomicMax(out, m);
+}
+
+__global__ void k_ldexp_inplace(float *x, int64_t n, const int32_t *shift) {
+    const int s = *shift;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = ldexpf(x[i], -s);
+}
+
+__global__ void k_to_f64_scaled(const float *a, const int32_t *perm, int64_t n,
+                                const long long *total, double *out) {
+    const int e = (int)total[0];
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (int64_t)gridDim.x * blockDim.x)
+        out[u] = ldexp((double)a[perm[u]], e);
+}
+
+__global__ void k_i32_to_f32(const int32_t *a, int64_t m, float *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)a[i];
+}
+
+static int spmv_weights(pg_graph *g, const float **w) {
+    if (g->w_dtype == PG_W_F32) {
+        *w = g->in_w_f32;
+        return PG_OK;
+    }
+    PG_REQUIRE(g->w_dtype == PG_W_I32, PG_ERR_STATE, "SpMV needs a weighted graph");
+    float *wf = nullptr;
+    auto it = g->ws.find("spmv_wf");
+    const bool have = it != g->ws.end();
+    PG_TRY(workspace_t(g, "spmv_wf", (size_t)g->m, &wf));
+    if (!have) PG_LAUNCH_K(k_i32_to_f32, grid_for(g->m), g->in_w_i32, g->m, wf);
+    *w = wf;
+    return PG_OK;
+}
+
+// ======================================================================================
+// Belief propagation (numa-BP.C:89-140 BP_F / BP_Vertex_Reset, push :179-230, loop
+// :280-316).  For an edge e = s->d of an active s the reference multiplies
+//     P_next[d][k] *= b_e[k],  b_e[k] = sum_l phi_d[l] W_e[k][l] P_cur[d][l],
+// after P_next = (1,1) on the current frontier, and marks d active.  Pulled:
+//     P_next[d] = prod over active in-neighbours of b_e   (F'[d] = any active).
+// b_e does not depend on s (the reference's message uses the destination's
+// potential and product), so q_l = phi_d[l] P_cur[d][l] is formed once per d.
+// ======================================================================================
+template <class T>
+struct BpAcc {
+    T p0, p1;
+    int any;
+    int pad;
+};
+template <class T>
+__device__ __forceinline__ BpAcc<T> shfl_xor_t(BpAcc<T> v, int m) {
+    BpAcc<T> r;
+    r.p0 = __shfl_xor(v.p0, m, 64);
+    r.p1 = __shfl_xor(v.p1, m, 64);
+    r.any = __shfl_xor(v.any, m, 64);
+    r.pad = 0;
+    return r;
+}
+
+template <class T>
+struct Vec2 {
+    T x, y;
+};
+
+// T = double: the product state and the messages are fp64 (default); T = float:
+// fp32 throughout (PG_BP_FP32).  Potentials are fp32 inputs either way
+// (numa-BP.C:59-73 stores them as float).
+template <class T>
+struct BpOp {
+    using Acc = BpAcc<T>;
+    using Ctx = Vec2<T>;  // q_l = phi_d[l] * P_cur[d][l]
+    using Stat = unsigned long long;
+    static constexpr bool kEarlyExit = false;
+    static constexpr bool kStats = false;
+    const uint8_t *F;
+    uint8_t *Fn;
+    const float4 *pot;   // CSC order, W_e[0][0], W_e[0][1], W_e[1][0], W_e[1][1]
+    const float2 *vpot;  // phi
+    const Vec2<T> *cur;
+    Vec2<T> *nxt;
+    __device__ bool active(int v) const { return F[v] != 0; }
+    __device__ Ctx ctx(int v) const {
+        const float2 phi = vpot[v];
+        const Vec2<T> p = cur[v];
+        return Ctx{(T)phi.x * p.x, (T)phi.y * p.y};
+    }
+    __device__ Acc identity() const { return Acc{T(1), T(1), 0, 0}; }
+    __device__ Acc combine(Acc a, Acc b) const {
+        return Acc{a.p0 * b.p0, a.p1 * b.p1, a.any | b.any, 0};
+    }
+    __device__ Acc edge(int s, int e, const Ctx &q) const {
+        if (!F[s]) return identity();
+        const float4 w = pot[e];
+        return Acc{(T)w.x * q.x + (T)w.y * q.y, (T)w.z * q.x + (T)w.w * q.y, 1, 0};
+    }
+    __device__ bool found(Acc, const Ctx &) const { return false; }
+    __device__ Stat finish(int v, Acc a, const Ctx &) const {
+        nxt[v] = Vec2<T>{a.p0, a.p1};
+        Fn[v] = a.any ? 1 : 0;
+        return 0ull;
+    }
+    __device__ void skip(int v) const { Fn[v] = 0; }
+};
+
+__global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ======================================================================================
+// BFS (numa-BFS.C:59-79 BFS_F: cond = unvisited, updateAtomic = CAS(parent,-1,s);
+// loop :258-296).  Dense rounds pull bottom-up (edgeMapDenseNoRep :113-159: the first
+// in-neighbour found in the frontier becomes the parent, then break); sparse rounds push
+// top-down from the frontier list (edgeMapSparseV3).  Levels are the round numbers.
+// ======================================================================================
+struct BfsPullOp {
+    using Acc = int;
+    using Ctx = int;
+    using Stat = unsigned long long;
+    static constexpr bool kEarlyExit = true;
+    static constexpr bool kStats = false;
+    const uint32_t *F;
+    uint32_t *Fn;
+    int32_t *list_n;
+    unsigned long long *cnt_n;
+    int32_t *level, *parent;
+    const int32_t *in_adj;
+    int32_t round;
+    __device__ bool active(int v) const { return level[v] < 0; }
+    __device__ int ctx(int v) const { return v; }
+    __device__ int identity() const { return 0x7FFFFFFF; }
+    __device__ int combine(int a, int b) const { return min(a, b); }
+    __device__ int edge(int s, int e, const int &) const {
+        return bit_test(F, s) ? e : 0x7FFFFFFF;
+    }
+    __device__ bool found(int a, const int &) const { return a != 0x7FFFFFFF; }
+    __device__ Stat finish(int v, int a, const int &) const {
+        const bool hit = a != 0x7FFFFFFF;
+        if (hit) {
+            parent[v] = in_adj[a];
+            level[v] = round;
+            atomicOr(&Fn[v >> 5], 1u << (v & 31));
+        }
+        agg_append(list_n, cnt_n, hit, v);
+        return 0ull;
+    }
+    __device__ void skip(int) const {}
+};
+
+struct BfsPushOp {
+    uint32_t *Fn;
+    int32_t *list_n;
+    unsigned long long *cnt_n;
+    int32_t *level, *parent;
+    int32_t round;
+    __device__ bool src_active(int) const { return true; }
+    __device__ void push(int s, int d, int) const {
+        bool won = false;
+        if (level[d] < 0 && atomicCAS(&parent[d], -1, s) == -1) {
+            level[d] = round;
+            atomicOr(&Fn[d >> 5], 1u << (d & 31));
+            won = true;
+        }
+        agg_append(list_n, cnt_n, won, d);
+    }
+};
+
+// ======================================================================================
+// Connected components (numa-Components.C:55-119 CC_F / CC_Vertex_F, edgeMapCustom
+// :121-175 with the switch at m/8, loop :233-281).  prev = ID on the frontier; dense
+// rounds pull the minimum label of active in-neighbours (edgeMapDenseReduce); sparse
+// rounds push writeMin (edgeMapSparseV3).  A vertex joins the next frontier on its
+// first decrease in the round, i.e. when the value it replaced equals prev.
+// ======================================================================================
+struct CcPullOp {
+    using Acc = int;
+    using Ctx = int;
+    using Stat = unsigned long long;
+    static constexpr bool kEarlyExit = false;
+    static constexpr bool kStats = false;
+    const uint32_t *F;
+    uint32_t *Fn;
+    int32_t *list_n;
+    unsigned long long *cnt_n;
+    int32_t *label;
+    __device__ bool active(int) const { return true; }
+    __device__ int ctx(int v) const { return v; }
+    __device__ int identity() const { return 0x7FFFFFFF; }
+    __device__ int combine(int a, int b) const { return min(a, b); }
+    __device__ int edge(int s, int, const int &) const {
+        return bit_test(F, s) ? label[s] : 0x7FFFFFFF;
+    }
+    __device__ bool found(int, const int &) const { return false; }
+    __device__ Stat finish(int v, int mn, const int &) const {
+        const bool dec = mn < label[v];
+        if (dec) {
+            label[v] = mn;
+            atomicOr(&Fn[v >> 5], 1u << (v & 31));
+        }
+        agg_append(list_n, cnt_n, dec, v);
+        return 0ull;
+    }
+    __device__ void skip(int) const {}
+};
+
+struct CcPushOp {
+    uint32_t *Fn;
+    int32_t *list_n;
+    unsigned long long *cnt_n;
+    int32_t *label;
+    const int32_t *prev;
+    __device__ bool src_active(int) const { return true; }
+    __device__ void push(int s, int d, int) const {
+        const int ls = label[s];
+        bool act = false;
+        if (ls < label[d]) {
+            const int old = atomicMin(&label[d], ls);
+            act = ls < old && old == prev[d];
+            if (act) atomicOr(&Fn[d >> 5], 1u << (d & 31));
+        }
+        agg_append(list_n, cnt_n, act, d);
+    }
+};
+
+__global__ void k_cc_prev(const int32_t *list, const unsigned long long *cnt, const int32_t *label,
+                          int32_t *prev) {
+    const int64_t c = (int64_t)*cnt;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = list[i];
+        prev[v] = label[v];
+    }
+}
+
+__global__ void k_iota_labels(const int32_t *iperm, int64_t n, int32_t *label, int32_t *list) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        label[v] = iperm[v];  // IDs[i] = i in input ids (numa-Components.C:387-391)
+        list[v] = (int32_t)v;
+    }
+}
+
+// ======================================================================================
+// SSSP (numa-BellmanFord.C:61-95 BF_F: writeMin(dist[d], dist[s]+w) and activate d,
+// loop :166-211, switch at m/20).  Both directions push (edgeMapDenseForward over the
+// bitmap, edgeMapSparseV3 over the list).  Activation is "first improvement this round"
+// (a new bit in the next frontier), the Ligra protocol, so distances are exactly the
+// shortest ones; the reference's reset of Visited on the current frontier (P6) can stop
+// short of them and is not reproduced.
+// ======================================================================================
+template <bool kDense>
+struct SsspPushOp {
+    const uint32_t *F;
+    uint32_t *Fn;
+    int32_t *list_n;
+    unsigned long long *cnt_n;
+    int32_t *dist;
+    const int32_t *w;  // CSR order
+    __device__ bool src_active(int s) const { return kDense ? bit_test(F, s) : true; }
+    __device__ void push(int s, int d, int e) const {
+        const int nd = dist[s] + w[e];
+        bool act = false;
+        if (nd < dist[d]) {
+            const int old = atomicMin(&dist[d], nd);
+            if (nd < old) act = bit_set_new(Fn, d);
+        }
+        agg_append(list_n, cnt_n, act, d);
+    }
+};
+
+// ---- traversal state ---------------------------------------------------------------
+struct Frontier {
+    uint32_t *bits;
+    int32_t *list;
+    unsigned long long *ctr;  // [0] = count, [1] = out-edges
+};
+
+static int frontiers(pg_graph *g, Frontier *a, Frontier *b) {
+    const size_t nw = (size_t)((g->n + 31) / 32) + 1;
+    PG_TRY(workspace_t(g, "fr_bits0", nw, &a->bits));
+    PG_TRY(workspace_t(g, "fr_bits1", nw, &b->bits));
+    PG_TRY(workspace_t(g, "fr_list0", (size_t)g->n + 1, &a->list));
+    PG_TRY(workspace_t(g, "fr_list1", (size_t)g->n + 1, &b->list));
+    PG_TRY(workspace_t(g, "fr_ctr0", 4, &a->ctr));
+    PG_TRY(workspace_t(g, "fr_ctr1", 4, &b->ctr));
+    return PG_OK;
+}
+
+static int clear_frontier(pg_graph *g, Frontier &f) {
+    PG_CHECK_HIP(hipMemsetAsync(f.bits, 0, 4 * ((g->n + 31) / 32 + 1), stream()));
+    PG_CHECK_HIP(hipMemsetAsync(f.ctr, 0, 32, stream()));
+    return PG_OK;
+}
+
+// |F_next| and the sum of its out-degrees, read back for the direction switch
+static int frontier_stats(pg_graph *g, Frontier &f, int64_t *count, int64_t *edges) {
+    PG_TRY(list_degree_sum(g, f.list, &f.ctr[0], g->out_off, &f.ctr[1]));
+    unsigned long long h[2];
+    PG_CHECK_HIP(hipMemcpyAsync(h, f.ctr, 16, hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    *count = (int64_t)h[0];
+    *edges = (int64_t)h[1];
+    return PG_OK;
+}
+
+__global__ void k_seed_source(int32_t src, int32_t *level, int32_t *parent, int32_t *dist,
+                              uint32_t *bits, int32_t *list, unsigned long long *ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (level) level[src] = 0;
+        if (parent) parent[src] = src;
+        if (dist) dist[src] = 0;
+        bits[src >> 5] |= 1u << (src & 31);
+        list[0] = src;
+        ctr[0] = 1ull;
+    }
+}
+
+static int lookup_internal(pg_graph *g, int64_t src, int32_t *out) {
+    PG_REQUIRE(src >= 0 && src < g->n, PG_ERR_ARG, "source %lld out of range", (long long)src);
+    PG_CHECK_HIP(hipMemcpy(out, g->perm + src, 4, hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+int bfs_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_stats *st,
+            int32_t **level_out, int32_t **parent_out) {
+    int32_t s_int;
+    PG_TRY(lookup_internal(g, src, &s_int));
+    const int64_t thr = resolve_threshold(threshold, g->m, 20);
+    int32_t *level = nullptr, *parent = nullptr;
+    PG_TRY(workspace_t(g, "bfs_level", (size_t)g->n, &level));
+    PG_TRY(workspace_t(g, "bfs_parent", (size_t)g->n, &parent));
+    Frontier A, B;
+    PG_TRY(frontiers(g, &A, &B));
+    Timer tm;
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    PG_TRY(tm.start());
+    PG_TRY(fill(level, -1, g->n));
+    PG_TRY(fill(parent, -1, g->n));
+    PG_TRY(clear_frontier(g, A));
+    PG_LAUNCH_K(k_seed_source, 1, s_int, level, parent, (int32_t *)nullptr, A.bits, A.list, A.ctr);
+    int64_t cnt = 1, edges = 0;
+    PG_TRY(frontier_stats(g, A, &cnt, &edges));
+    int rounds = 0, dense_rounds = 0;
+    int64_t tot_active = 0, tot_edges = 0;
+    Frontier *F = &A, *N = &B;
+    while (cnt > 0) {
+        rounds++;
+        tot_active += cnt;
+        tot_edges += edges;
+        PG_TRY(clear_frontier(g, *N));
+        const bool dense = cnt + edges >= thr;  // P9: >=
+        PG_TRY(kt.begin());
+        if (dense) {
+            dense_rounds++;
+            BfsPullOp op{F->bits, N->bits, N->list, &N->ctr[0], level, parent, g->in_adj, rounds};
+            PG_TRY(launch_pull(op, g->in_sched));
+        } else {
+            BfsPushOp op{N->bits, N->list, &N->ctr[0], level, parent, rounds};
+            PG_TRY((launch_push<BfsPushOp, false>(op, g->out_off, g->out_adj, F->list, cnt)));
+        }
+        PG_TRY(kt.end());
+        PG_TRY(frontier_stats(g, *N, &cnt, &edges));
+        std::swap(F, N);
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = rounds;
+        st->dense_rounds = dense_rounds;
+        st->edges = tot_edges;
+        st->active = tot_active;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    *level_out = level;
+    *parent_out = parent;
+    return PG_OK;
+}
+
+int cc_run(pg_graph *g, int64_t threshold, uint32_t flags, pg_stats *st, int32_t **label_out) {
+    const int64_t thr = resolve_threshold(threshold, g->m, 8);
+    int32_t *label = nullptr, *prev = nullptr;
+    PG_TRY(workspace_t(g, "cc_label", (size_t)g->n, &label));
+    PG_TRY(workspace_t(g, "cc_prev", (size_t)g->n, &prev));
+    Frontier A, B;
+    PG_TRY(frontiers(g, &A, &B));
+    Timer tm;
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    PG_TRY(tm.start());
+    PG_TRY(clear_frontier(g, A));
+    PG_TRY(fill_bits_all(g, A.bits));
+    PG_LAUNCH_K(k_iota_labels, grid_for(g->n), g->iperm, g->n, label, A.list);
+    unsigned long long hc[2] = {(unsigned long long)g->n, (unsigned long long)g->m};
+    PG_CHECK_HIP(hipMemcpyAsync(A.ctr, hc, 16, hipMemcpyHostToDevice, stream()));
+    int64_t cnt = g->n, edges = g->m;
+    int rounds = 0, dense_rounds = 0;
+    int64_t tot_active = 0, tot_edges = 0;
+    Frontier *F = &A, *N = &B;
+    while (cnt > 0 || rounds == 0) {
+        rounds++;
+        tot_active += cnt;
+        tot_edges += edges;
+        // vertexMap(CC_Vertex_F): prev = ID on the frontier
+        PG_LAUNCH_K(k_cc_prev, grid_for(cnt), F->list, &F->ctr[0], label, prev);
+        PG_TRY(clear_frontier(g, *N));
+        const bool dense = cnt + edges >= thr;
+        PG_TRY(kt.begin());
+        if (dense) {
+            dense_rounds++;
+            CcPullOp op{F->bits, N->bits, N->list, &N->ctr[0], label};
+            PG_TRY(launch_pull(op, g->in_sched));
+        } else {
+            CcPushOp op{N->bits, N->list, &N->ctr[0], label, prev};
+            PG_TRY((launch_push<CcPushOp, false>(op, g->out_off, g->out_adj, F->list, cnt)));
+        }
+        PG_TRY(kt.end());
+        PG_TRY(frontier_stats(g, *N, &cnt, &edges));
+        std::swap(F, N);
+        if (rounds > 4 * g->n + 8) break;
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = rounds;
+        st->dense_rounds = dense_rounds;
+        st->edges = tot_edges;
+        st->active = tot_active;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    *label_out = label;
+    return PG_OK;
+}
+
+int sssp_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_stats *st,
+             int32_t **dist_out) {
+    PG_REQUIRE(g->w_dtype == PG_W_I32, PG_ERR_STATE, "SSSP needs int32 weights");
+    int32_t s_int;
+    PG_TRY(lookup_internal(g, src, &s_int));
+    const int64_t thr = resolve_threshold(threshold, g->m, 20);
+    int32_t *dist = nullptr;
+    PG_TRY(workspace_t(g, "sssp_dist", (size_t)g->n, &dist));
+    Frontier A, B;
+    PG_TRY(frontiers(g, &A, &B));
+    Timer tm;
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    PG_TRY(tm.start());
+    PG_TRY(fill(dist, (int32_t)1073741823, g->n));  // INT_MAX/2, numa-BellmanFord.C:283
+    PG_TRY(clear_frontier(g, A));
+    PG_LAUNCH_K(k_seed_source, 1, s_int, (int32_t *)nullptr, (int32_t *)nullptr, dist, A.bits,
+                A.list, A.ctr);
+    int64_t cnt = 1, edges = 0;
+    PG_TRY(frontier_stats(g, A, &cnt, &edges));
+    int rounds = 0, dense_rounds = 0;
+    int64_t tot_active = 0, tot_edges = 0;
+    Frontier *F = &A, *N = &B;
+    while (cnt > 0) {
+        rounds++;
+        tot_active += cnt;
+        tot_edges += edges;
+        PG_TRY(clear_frontier(g, *N));
+        const bool dense = cnt + edges >= thr;
+        PG_TRY(kt.begin());
+        if (dense) {
+            dense_rounds++;
+            SsspPushOp<true> op{F->bits, N->bits, N->list, &N->ctr[0], dist, g->out_w_i32};
+            PG_TRY((launch_push<SsspPushOp<true>, true>(op, g->out_off, g->out_adj, nullptr, g->n)));
+        } else {
+            SsspPushOp<false> op{F->bits, N->bits, N->list, &N->ctr[0], dist, g->out_w_i32};
+            PG_TRY((launch_push<SsspPushOp<false>, false>(op, g->out_off, g->out_adj, F->list, cnt)));
+        }
+        PG_TRY(kt.end());
+        PG_TRY(frontier_stats(g, *N, &cnt, &edges));
+        std::swap(F, N);
+        if (rounds > 4 * g->n + 8) break;
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = rounds;
+        st->dense_rounds = dense_rounds;
+        st->edges = tot_edges;
+        st->active = tot_active;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    *dist_out = dist;
+    return PG_OK;
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+// ====================================================================================
+// C ABI
+// ====================================================================================
+static int check_graph(pg_graph *g) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    return PG_OK;
+}
+
+extern "C" int pg_pagerank(pg_graph *g, int iters, double damping, float *rank, uint32_t flags,
+                           pg_stats *st) {
+    PG_TRY(check_graph(g));
+    PG_REQUIRE(iters >= 0, PG_ERR_ARG, "iters < 0");
+    PG_REQUIRE(damping >= 0.0 && damping <= 1.0, PG_ERR_ARG, "damping out of [0,1]");
+    float *res = nullptr;
+    PG_TRY(pagerank_run(g, iters, damping, flags, st, &res));
+    return to_input_order_f32(g, res, rank, 1);
+}
+
+__global__ void k_spmv_copy(const float *a, int64_t n, float *b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint32_t flags,
+                       pg_stats *st) {
+    PG_TRY(check_graph(g));
+    PG_REQUIRE(iters >= 0 && x0, PG_ERR_ARG, "bad argument");
+    const float *w = nullptr;
+    PG_TRY(spmv_weights(g, &w));
+    const int64_t n = g->n;
+    float *xa = nullptr, *xb = nullptr;
+    unsigned *maxbits = nullptr;
+    int32_t *shift = nullptr;
+    long long *total = nullptr;
+    PG_TRY(workspace_t(g, "spmv_xa", (size_t)n, &xa));
+    PG_TRY(workspace_t(g, "spmv_xb", (size_t)n, &xb));
+    PG_TRY(workspace_t(g, "spmv_max", (size_t)iters + 2, &maxbits));
+    PG_TRY(workspace_t(g, "spmv_shift", (size_t)iters + 2, &shift));
+    PG_TRY(workspace_t(g, "spmv_total", 2, &total));
+    PG_TRY(from_input_order_f32(g, x0, xa, 1));
+    PG_TRY(fill(maxbits, 0u, iters + 2));
+    PG_TRY(fill(shift, 0, iters + 2));
+    PG_TRY(fill(total, 0ll, 2));
+    // exponent of x0 (slot 0), applied to x0 itself
+    PG_LAUNCH_K(k_absmax_bits, grid_for(n, 2048), xa, n, maxbits);
+    hipLaunchKernelGGL(k_spmv_shift, dim3(1), dim3(64), 0, stream(), maxbits, shift, total, 0);
+    PG_CHECK_HIP(hipGetLastError());
+    PG_LAUNCH_K(k_ldexp_inplace, grid_for(n), xa, n, shift);
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    float *cur = xa, *nxt = xb;
+    for (int t = 0; t < iters; t++) {
+        // iteration t scales its output by 2^-shift[t+1]; shift[t+1] comes from max(x_t)
+        SpmvOp op{cur, w, nxt, shift + t + 1};
+        PG_TRY(kt.begin());
+        PG_TRY(launch_pull(op, g->in_sched, maxbits + t + 1));
+        PG_TRY(kt.end());
+        hipLaunchKernelGGL(k_spmv_shift, dim3(1), dim3(64), 0, stream(), maxbits + t + 1,
+                           shift + t + 2, total, 0);
+        PG_CHECK_HIP(hipGetLastError());
+        std::swap(cur, nxt);  // SPMV_Vertex_Reset + swap
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = iters;
+        st->dense_rounds = iters;
+        st->edges = (int64_t)iters * g->m;
+        st->active = (int64_t)iters * g->n;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    if (!y) return PG_OK;
+    // total exponent = sum of the applied shifts shift[0..iters]
+    std::vector<int32_t> hs((size_t)iters + 2);
+    PG_CHECK_HIP(hipMemcpy(hs.data(), shift, 4 * (iters + 2), hipMemcpyDeviceToHost));
+    long long E = 0;
+    for (int t = 0; t <= iters; t++) E += hs[t];
+    PG_CHECK_HIP(hipMemcpy(total, &E, 8, hipMemcpyHostToDevice));
+    double *tmp = nullptr;
+    PG_TRY(workspace_t(g, "spmv_out", (size_t)n, &tmp));
+    PG_LAUNCH_K(k_to_f64_scaled, grid_for(n), cur, g->perm, n, total, tmp);
+    PG_CHECK_HIP(hipMemcpyAsync(y, tmp, 8 * n, hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    return PG_OK;
+}
+
+__global__ void k_f32_to_float4(const float *in, int64_t m, float4 *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = make_float4(in[4 * i], in[4 * i + 1], in[4 * i + 2], in[4 * i + 3]);
+}
+
+__global__ void k_gather_pot(const float4 *in, const int32_t *eid, int64_t m, float4 *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[eid[i]];
+}
+
+template <class T>
+__global__ void k_bp_state_init(const float *p0, int64_t n, Vec2<T> *cur, Vec2<T> *nxt) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        cur[v] = Vec2<T>{(T)p0[2 * v], (T)p0[2 * v + 1]};
+        nxt[v] = Vec2<T>{T(0), T(0)};  // mmap'd zero pages (numa-BP.C:557)
+    }
+}
+
+template <class T>
+__global__ void k_bp_out(const Vec2<T> *a, const int32_t *perm, int64_t n, double *out) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const Vec2<T> x = a[perm[u]];
+        out[2 * u] = (double)x.x;
+        out[2 * u + 1] = (double)x.y;
+    }
+}
+
+template <class T>
+static int bp_run(pg_graph *g, int iters, const float4 *pot, const float2 *vp, const float *p0,
+                  uint8_t *f0, uint8_t *f1, double *prod, uint32_t flags, pg_stats *st) {
+    const int64_t n = g->n;
+    Vec2<T> *pa = nullptr, *pb = nullptr;
+    PG_TRY(workspace_t(g, sizeof(T) == 8 ? "bp_pa64" : "bp_pa32", (size_t)n, &pa));
+    PG_TRY(workspace_t(g, sizeof(T) == 8 ? "bp_pb64" : "bp_pb32", (size_t)n, &pb));
+    PG_LAUNCH_K(k_bp_state_init<T>, grid_for(n), p0, n, pa, pb);
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    Vec2<T> *cur = pa, *nxt = pb;
+    uint8_t *F = f0, *Fn = f1;
+    for (int t = 0; t < iters; t++) {
+        BpOp<T> op{F, Fn, pot, vp, cur, nxt};
+        PG_TRY(kt.begin());
+        PG_TRY(launch_pull(op, g->in_sched));
+        PG_TRY(kt.end());
+        std::swap(cur, nxt);  // swap(vertD_curr, vertD_next), numa-BP.C:303-304
+        std::swap(F, Fn);     // switchFrontier
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = iters;
+        st->dense_rounds = iters;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    if (!prod) return PG_OK;
+    double *tmp = nullptr;
+    PG_TRY(workspace_t(g, "bp_out", 2 * (size_t)n, &tmp));
+    PG_LAUNCH_K(k_bp_out<T>, grid_for(n), cur, g->perm, n, tmp);
+    PG_CHECK_HIP(hipMemcpyAsync(prod, tmp, 16 * n, hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    return PG_OK;
+}
+
+extern "C" int pg_bp(pg_graph *g, int iters, const float *vpot, const float *epot,
+                     const float *prod0, double *prod, uint32_t flags, pg_stats *st) {
+    PG_TRY(check_graph(g));
+    PG_REQUIRE(iters >= 0 && vpot && prod0, PG_ERR_ARG, "bad argument");
+    PG_REQUIRE(epot || g->bp_pot, PG_ERR_STATE,
+               "no edge potentials: pass epot or call pg_graph_bp_potentials");
+    const int64_t n = g->n, m = g->m;
+    float4 *pot = nullptr;
+    if (epot) {
+        float4 *tmp = nullptr;
+        PG_TRY(workspace_t(g, "bp_pot_in", (size_t)m + 1, &tmp));
+        PG_TRY(workspace_t(g, "bp_pot", (size_t)m + 1, &pot));
+        PG_CHECK_HIP(hipMemcpyAsync(tmp, epot, 16 * m, hipMemcpyHostToDevice, stream()));
+        PG_LAUNCH_K(k_gather_pot, grid_for(m), tmp, g->in_eid, m, pot);
+    } else {
+        pot = reinterpret_cast<float4 *>(g->bp_pot);
+    }
+    float2 *vp = nullptr;
+    float *p0 = nullptr;
+    uint8_t *f0 = nullptr, *f1 = nullptr;
+    PG_TRY(workspace_t(g, "bp_vpot", (size_t)n, &vp));
+    PG_TRY(workspace_t(g, "bp_p0", 2 * (size_t)n, &p0));
+    PG_TRY(workspace_t(g, "bp_f0", (size_t)n, &f0));
+    PG_TRY(workspace_t(g, "bp_f1", (size_t)n, &f1));
+    PG_TRY(from_input_order_f32(g, vpot, reinterpret_cast<float *>(vp), 2));
+    PG_TRY(from_input_order_f32(g, prod0, p0, 2));
+    PG_LAUNCH_K(k_fill_u8, grid_for(n), f0, (uint8_t)1, n);
+    if (flags & PG_BP_FP32)
+        return bp_run<float>(g, iters, pot, vp, p0, f0, f1, prod, flags, st);
+    return bp_run<double>(g, iters, pot, vp, p0, f0, f1, prod, flags, st);
+}
+
+extern "C" int pg_bfs(pg_graph *g, int64_t src, int64_t threshold, int32_t *level,
+                      int32_t *parent, uint32_t flags, pg_stats *st) {
+    PG_TRY(check_graph(g));
+    int32_t *lv = nullptr, *pr = nullptr;
+    PG_TRY(bfs_run(g, src, threshold, flags, st, &lv, &pr));
+    PG_TRY(to_input_order_i32(g, lv, level, false));
+    return to_input_order_i32(g, pr, parent, true);
+}
+
+extern "C" int pg_cc(pg_graph *g, int64_t threshold, int32_t *label, uint32_t flags,
+                     pg_stats *st) {
+    PG_TRY(check_graph(g));
+    int32_t *lb = nullptr;
+    PG_TRY(cc_run(g, threshold, flags, st, &lb));
+    return to_input_order_i32(g, lb, label, false);
+}
+
+extern "C" int pg_sssp(pg_graph *g, int64_t src, int64_t threshold, int32_t *dist, uint32_t flags,
+                       pg_stats *st) {
+    PG_TRY(check_graph(g));
+    int32_t *d = nullptr;
+    PG_TRY(sssp_run(g, src, threshold, flags, st, &d));
+    return to_input_order_i32(g, d, dist, false);
+}

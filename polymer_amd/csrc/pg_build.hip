@@ -1,0 +1,1342 @@
+// pg_build.hip -- library state, graph ingest and the HBM layout.
+//
+// Replaces the reference's NUMA layer (polymer.h:66-519: partitionByDegree,
+// graphFilter2Direction, graph*Hasher, mapDataArray) and its in-edge build
+// (IO-numa.h:145-183, a quickSort of m (dst,src) pairs) with one device
+// resident CSR + CSC:
+//   * vertices relabelled by in-degree, descending (ties by input id), so a
+//     pull over the CSC visits vertices in non-increasing degree order and
+//     the degree bins of the pull schedule are contiguous id ranges;
+//   * both directions sorted by 64-bit (vertex, neighbour) keys with a device
+//     radix sort; the payload keeps each edge's index in the input CSR so that
+//     edge data (weights, BP potentials, gradients) can be given and returned
+//     in the caller's order.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <fstream>
+#include <string>
+
+#include "pg_engine.h"
+#include "pg_internal.h"
+#include "pg_kernels_common.h"
+
+namespace pg {
+
+static thread_local char g_err[1024] = "";
+static hipStream_t g_stream = nullptr;
+static int g_device = -1;
+static bool g_init = false;
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+hipStream_t stream() { return g_stream; }
+bool initialised() { return g_init; }
+
+// ---- temporary device buffers (graph build only) ------------------------------
+// Frees wait for the library stream first: kernels enqueued earlier may still
+// read the buffer.
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) {
+            (void)hipStreamSynchronize(g_stream);
+            (void)hipFree(p);
+        }
+    }
+    int alloc(size_t bytes) {
+        if (p) {
+            PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        PG_CHECK_HIP(hipMalloc(&p, bytes ? bytes : 16));
+        return PG_OK;
+    }
+    template <class T>
+    T *as() const {
+        return reinterpret_cast<T *>(p);
+    }
+};
+
+int galloc(pg_graph *g, void **p, size_t bytes) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        *p = nullptr;
+        return e == hipErrorOutOfMemory ? PG_ERR_OOM : PG_ERR_HIP;
+    }
+    g->owned.push_back(*p);
+    return PG_OK;
+}
+
+int workspace(pg_graph *g, const char *name, size_t bytes, void **p) {
+    auto it = g->ws.find(name);
+    if (it != g->ws.end() && it->second.second >= bytes) {
+        *p = it->second.first;
+        return PG_OK;
+    }
+    if (it != g->ws.end()) {
+        PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+        PG_CHECK_HIP(hipFree(it->second.first));
+        g->ws.erase(it);
+    }
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        set_error("workspace %s: hipMalloc(%zu) failed: %s", name, bytes, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? PG_ERR_OOM : PG_ERR_HIP;
+    }
+    g->ws[name] = {q, bytes};
+    *p = q;
+    return PG_OK;
+}
+
+int Timer::start() {
+    if (!a) PG_CHECK_HIP(hipEventCreate(&a));
+    if (!b) PG_CHECK_HIP(hipEventCreate(&b));
+    PG_CHECK_HIP(hipEventRecord(a, g_stream));
+    return PG_OK;
+}
+int Timer::stop(double *ms) {
+    PG_CHECK_HIP(hipEventRecord(b, g_stream));
+    PG_CHECK_HIP(hipEventSynchronize(b));
+    float f = 0.f;
+    PG_CHECK_HIP(hipEventElapsedTime(&f, a, b));
+    *ms = f;
+    return PG_OK;
+}
+Timer::~Timer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+}
+
+int KernelTimer::begin() {
+    if (!on) return PG_OK;
+    hipEvent_t e;
+    PG_CHECK_HIP(hipEventCreate(&e));
+    ev.push_back(e);
+    PG_CHECK_HIP(hipEventRecord(e, g_stream));
+    return PG_OK;
+}
+int KernelTimer::end() { return begin(); }
+int KernelTimer::total(double *ms, int *launches) {
+    *ms = 0.0;
+    *launches = 0;
+    if (!on || ev.size() < 2) return PG_OK;
+    PG_CHECK_HIP(hipEventSynchronize(ev.back()));
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+        float f = 0.f;
+        PG_CHECK_HIP(hipEventElapsedTime(&f, ev[i], ev[i + 1]));
+        *ms += f;
+        (*launches)++;
+    }
+    return PG_OK;
+}
+KernelTimer::~KernelTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+}
+
+// ---- hipcub wrappers ----------------------------------------------------------
+static int sort_pairs(uint64_t *kin, uint64_t *kout, uint32_t *vin, uint32_t *vout, int64_t n,
+                      int end_bit) {
+    size_t tb = 0;
+    PG_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, n, 0,
+                                                    end_bit, g_stream));
+    DevBuf tmp;
+    PG_TRY(tmp.alloc(tb));
+    PG_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, kin, kout, vin, vout, n, 0,
+                                                    end_bit, g_stream));
+    return PG_OK;
+}
+
+static int sort_keys(uint64_t *kin, uint64_t *kout, int64_t n, int end_bit) {
+    size_t tb = 0;
+    PG_CHECK_HIP(
+        hipcub::DeviceRadixSort::SortKeys(nullptr, tb, kin, kout, n, 0, end_bit, g_stream));
+    DevBuf tmp;
+    PG_TRY(tmp.alloc(tb));
+    PG_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, kin, kout, n, 0, end_bit, g_stream));
+    return PG_OK;
+}
+
+template <class TI, class TO>
+static int exclusive_sum(const TI *in, TO *out, int64_t n) {
+    size_t tb = 0;
+    PG_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, g_stream));
+    DevBuf tmp;
+    PG_TRY(tmp.alloc(tb));
+    PG_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, n, g_stream));
+    return PG_OK;
+}
+
+template <class TI, class TO>
+static int inclusive_sum(const TI *in, TO *out, int64_t n) {
+    size_t tb = 0;
+    PG_CHECK_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out, n, g_stream));
+    DevBuf tmp;
+    PG_TRY(tmp.alloc(tb));
+    PG_CHECK_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, in, out, n, g_stream));
+    return PG_OK;
+}
+
+static int select_flagged(const uint64_t *in, const uint8_t *flags, uint64_t *out,
+                          unsigned long long *d_nsel, int64_t n) {
+    size_t tb = 0;
+    PG_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, in, flags, out, d_nsel, n, g_stream));
+    DevBuf tmp;
+    PG_TRY(tmp.alloc(tb));
+    PG_CHECK_HIP(hipcub::DeviceSelect::Flagged(tmp.p, tb, in, flags, out, d_nsel, n, g_stream));
+    return PG_OK;
+}
+
+int exclusive_scan_u32(pg_graph *, const uint32_t *in, uint32_t *out, int64_t n) {
+    return exclusive_sum(in, out, n);
+}
+
+static int bits_for(int64_t n) {
+    int b = 1;
+    while ((int64_t(1) << b) < n) b++;
+    return b;
+}
+
+#define PG_LAUNCH(kern, grid, ...)                                                     \
+    do {                                                                               \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, g_stream, __VA_ARGS__); \
+        PG_CHECK_HIP(hipGetLastError());                                               \
+    } while (0)
+
+// ---- build kernels -------------------------------------------------------------
+__global__ void k_rmat_keys(uint64_t base, int scale, uint32_t t0, uint32_t t1, uint32_t t2,
+                            int64_t count, uint64_t *keys, uint32_t *ctr) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t s = 0, d = 0;
+        for (int i = 0; i < scale; i++) {
+            const uint32_t r = (uint32_t)(dmix64(base + (uint64_t)k * 32ull + (uint64_t)i) >> 32);
+            const uint32_t q = r < t0 ? 0u : (r < t1 ? 1u : (r < t2 ? 2u : 3u));
+            s = (s << 1) | (q >> 1);
+            d = (d << 1) | (q & 1u);
+        }
+        keys[k] = (s == d) ? ~0ull : (((uint64_t)s << 32) | d);
+        ctr[k] = (uint32_t)k;
+    }
+}
+
+__global__ void k_first_by_ctr(const uint64_t *keys, const uint32_t *ctr, int64_t count,
+                               uint32_t *mark) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        if (k != ~0ull && (i == 0 || keys[i - 1] != k)) mark[ctr[i]] = 1;
+    }
+}
+
+__global__ void k_find_cutoff(const uint32_t *cnt, int64_t count, uint32_t target,
+                              int64_t *cutoff) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        if (cnt[k] == target && (k == 0 || cnt[k - 1] + 1 == target)) *cutoff = k;
+    }
+}
+
+__global__ void k_select_flags(const uint64_t *keys, const uint32_t *ctr, int64_t count,
+                               const int64_t *cutoff, uint8_t *sel) {
+    const int64_t cut = *cutoff;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        sel[i] = (k != ~0ull && (i == 0 || keys[i - 1] != k) && (int64_t)ctr[i] <= cut) ? 1 : 0;
+    }
+}
+
+__global__ void k_unique_flags(const uint64_t *keys, int64_t count, uint8_t *sel) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        const uint32_t s = (uint32_t)(k >> 32), d = (uint32_t)k;
+        sel[i] = (s != d && (i == 0 || keys[i - 1] != k)) ? 1 : 0;
+    }
+}
+
+__global__ void k_mirror(const uint64_t *in, int64_t m, uint64_t *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = in[i];
+        out[i] = k;
+        out[m + i] = (k << 32) | (k >> 32);
+    }
+}
+
+__global__ void k_keys_from_csr(const int32_t *off, const int32_t *adj, int64_t n,
+                                uint64_t *keys) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        for (int e = off[v]; e < off[v + 1]; e++) keys[e] = ((uint64_t)v << 32) | (uint32_t)adj[e];
+    }
+}
+
+__global__ void k_degrees(const uint64_t *keys, int64_t m, uint32_t *outdeg, uint32_t *indeg) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        atomicAdd(&outdeg[k >> 32], 1u);
+        atomicAdd(&indeg[(uint32_t)k], 1u);
+    }
+}
+
+__global__ void k_vertex_keys(const uint32_t *deg, int64_t n, uint64_t *vk) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        vk[v] = ((uint64_t)(0xFFFFFFFFu - deg[v]) << 32) | (uint32_t)v;
+}
+
+__global__ void k_perm_from_sorted(const uint64_t *vk, int64_t n, int32_t *iperm, int32_t *perm) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = (int32_t)(uint32_t)vk[i];
+        iperm[i] = v;
+        perm[v] = (int32_t)i;
+    }
+}
+
+// relabelled edge keys; dir 0: (src,dst), dir 1: (dst,src)
+__global__ void k_relabel_keys(const uint64_t *keys, int64_t m, const int32_t *perm, int dir,
+                               uint64_t *out, uint32_t *eidx) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        const uint32_t s = (uint32_t)perm[k >> 32], d = (uint32_t)perm[(uint32_t)k];
+        out[i] = dir == 0 ? (((uint64_t)s << 32) | d) : (((uint64_t)d << 32) | s);
+        eidx[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_permute_counts(const uint32_t *deg, const int32_t *perm, int64_t n,
+                                 uint32_t *out) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        out[v < n ? perm[v] : n] = v < n ? deg[v] : 0u;
+}
+
+__global__ void k_low32(const uint64_t *keys, int64_t m, int32_t *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (int32_t)(uint32_t)keys[i];
+}
+
+template <class T>
+__global__ void k_gather(const T *src, const int32_t *idx, int64_t m, T *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = src[idx[i]];
+}
+
+__global__ void k_pair_weights_i32(const uint64_t *keys, int64_t m, uint64_t seed, int32_t *w) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        w[i] = dweight_i32(seed, (int32_t)(k >> 32), (int32_t)(uint32_t)k);
+    }
+}
+
+__global__ void k_pair_values_f32(const uint64_t *keys, int64_t m, uint64_t seed, float *w) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        w[i] = dvalue_f32(seed, (int32_t)(k >> 32), (int32_t)(uint32_t)k);
+    }
+}
+
+__global__ void k_count_above(const int32_t *off, const int32_t *order, int64_t n,
+                              const int64_t *thr, int nthr, int64_t *out) {
+    // binary search per threshold over non-increasing degrees
+    const int j = threadIdx.x;
+    if (j >= nthr) return;
+    const int64_t T = thr[j];
+    int64_t lo = 0, hi = n;  // first position with deg <= T
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        const int v = order ? order[mid] : (int)mid;
+        const int64_t d = off[v + 1] - off[v];
+        if (d > T) lo = mid + 1; else hi = mid;
+    }
+    out[j] = lo;
+}
+
+__global__ void k_degree_of(const int32_t *off, const int32_t *order, int64_t count, int32_t *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = order ? order[i] : (int)i;
+        out[i] = off[v + 1] - off[v];
+    }
+}
+
+__global__ void k_out_order_keys(const int32_t *off, int64_t n, uint64_t *vk) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        vk[v] = ((uint64_t)(0xFFFFFFFFu - (uint32_t)(off[v + 1] - off[v])) << 32) | (uint32_t)v;
+}
+
+__global__ void k_bp_pot(const int32_t *in_off, const int32_t *in_adj, const int32_t *iperm,
+                         int64_t n, uint64_t seed, float lo, float hi, float *pot) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t dv = iperm[v];
+        for (int e = in_off[v]; e < in_off[v + 1]; e++) {
+            const int32_t sv = iperm[in_adj[e]];
+#pragma unroll
+            for (int k = 0; k < 4; k++) pot[4ll * e + k] = dbp_pot(seed, sv, dv, k, lo, hi);
+        }
+    }
+}
+
+// input-order transfers
+__global__ void k_to_input_i32(const int32_t *a, const int32_t *perm, const int32_t *iperm,
+                               int64_t n, int map_ids, int32_t *out) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        int32_t x = a[perm[u]];
+        if (map_ids && x >= 0) x = iperm[x];
+        out[u] = x;
+    }
+}
+
+__global__ void k_to_input_f32(const float *a, const int32_t *perm, int64_t n, int width,
+                               float *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * width;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = i / width, c = i % width;
+        out[i] = a[(int64_t)perm[u] * width + c];
+    }
+}
+
+__global__ void k_from_input_f32(const float *in, const int32_t *perm, int64_t n, int width,
+                                 float *a) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * width;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = i / width, c = i % width;
+        a[(int64_t)perm[u] * width + c] = in[i];
+    }
+}
+
+// ---- frontier conversions (bitmap <-> ordered list) -------------------------------
+// One 32-bit word per thread: popcounts scanned in LDS, block totals scanned
+// on the device, then every thread writes its ids in order -- an ordered
+// packIndex (polymer.h:552-567, utils.h:207-227).
+__global__ void k_word_counts(const uint32_t *bits, int64_t nwords, uint32_t *block_cnt) {
+    __shared__ uint32_t sm[kBlock / 64];
+    const int64_t w = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t c = w < nwords ? (uint32_t)__popc(bits[w]) : 0u;
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane_id() == 0) sm[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < kBlock / 64; i++) t += sm[i];
+        block_cnt[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_word_write(const uint32_t *bits, int64_t nwords, int64_t n,
+                             const uint32_t *block_off, int32_t *list,
+                             unsigned long long *count_out, int64_t nblocks) {
+    __shared__ uint32_t s_scan[kBlock + 1];
+    const int t = threadIdx.x;
+    const int64_t w = (int64_t)blockIdx.x * kBlock + t;
+    const uint32_t word = w < nwords ? bits[w] : 0u;
+    s_scan[t + 1] = (uint32_t)__popc(word);
+    if (t == 0) s_scan[0] = 0;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        uint32_t x = (t + 1 - o >= 1) ? s_scan[t + 1 - o] : 0u;
+        __syncthreads();
+        s_scan[t + 1] += x;
+        __syncthreads();
+    }
+    uint32_t pos = block_off[blockIdx.x] + s_scan[t];
+    uint32_t x = word;
+    while (x) {
+        const int b = __ffs(x) - 1;
+        x &= x - 1;
+        const int64_t v = w * 32 + b;
+        if (v < n) list[pos++] = (int32_t)v;
+    }
+    if (blockIdx.x == nblocks - 1 && t == kBlock - 1) *count_out = block_off[blockIdx.x] + s_scan[kBlock];
+}
+
+__global__ void k_list_to_bits(const int32_t *list, const unsigned long long *count, uint32_t *bits) {
+    const int64_t c = (int64_t)*count;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = list[i];
+        atomicOr(&bits[v >> 5], 1u << (v & 31));
+    }
+}
+
+__global__ void k_list_degree_sum(const int32_t *list, const unsigned long long *count,
+                                  const int32_t *off, unsigned long long *out) {
+    __shared__ unsigned long long sm[kBlock / 64];
+    const int64_t c = (int64_t)*count;
+    unsigned long long acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = list[i];
+        acc += (unsigned long long)(off[v + 1] - off[v]);
+    }
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane_id() == 0) sm[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kBlock / 64; i++) t += sm[i];
+        if (t) atomicAdd(out, t);
+    }
+}
+
+__global__ void k_bits_from_bools_input(const uint8_t *flags, const int32_t *perm, int64_t n,
+                                        uint32_t *bits) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (int64_t)gridDim.x * blockDim.x)
+        if (flags[u]) {
+            const int v = perm[u];
+            atomicOr(&bits[v >> 5], 1u << (v & 31));
+        }
+}
+
+__global__ void k_bits_from_ids_input(const int32_t *ids, int64_t k, const int32_t *perm,
+                                      uint32_t *bits) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = perm[ids[i]];
+        atomicOr(&bits[v >> 5], 1u << (v & 31));
+    }
+}
+
+__global__ void k_bits_to_input_bits(const uint32_t *bits, const int32_t *perm, int64_t n,
+                                     uint32_t *ibits) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (int64_t)gridDim.x * blockDim.x)
+        if (bit_test(bits, perm[u])) atomicOr(&ibits[u >> 5], 1u << (u & 31));
+}
+
+__global__ void k_fill_bits_all(uint32_t *bits, int64_t n) {
+    const int64_t nw = (n + 31) / 32;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t rem = n - w * 32;
+        bits[w] = rem >= 32 ? 0xFFFFFFFFu : ((1u << rem) - 1u);
+    }
+}
+
+int bits_to_list(pg_graph *g, const uint32_t *bits, int32_t *list, unsigned long long *d_count) {
+    const int64_t nwords = (g->n + 31) / 32;
+    const int64_t nb = (nwords + kBlock - 1) / kBlock;
+    uint32_t *bc = nullptr, *bo = nullptr;
+    PG_TRY(workspace_t(g, "compact_bc", (size_t)nb + 1, &bc));
+    PG_TRY(workspace_t(g, "compact_bo", (size_t)nb + 1, &bo));
+    PG_LAUNCH(k_word_counts, (unsigned)nb, bits, nwords, bc);
+    PG_TRY(exclusive_sum(bc, bo, nb));
+    PG_LAUNCH(k_word_write, (unsigned)nb, bits, nwords, g->n, bo, list, d_count, nb);
+    return PG_OK;
+}
+
+int list_to_bits(pg_graph *g, const int32_t *list, const unsigned long long *d_count,
+                 uint32_t *bits) {
+    PG_LAUNCH(k_list_to_bits, grid_for(g->n, 8192), list, d_count, bits);
+    return PG_OK;
+}
+
+int list_degree_sum(pg_graph *g, const int32_t *list, const unsigned long long *d_count,
+                    const int32_t *off, unsigned long long *d_out) {
+    PG_LAUNCH(k_list_degree_sum, grid_for(g->n, 4096), list, d_count, off, d_out);
+    return PG_OK;
+}
+
+int fill_bits_all(pg_graph *g, uint32_t *bits) {
+    PG_LAUNCH(k_fill_bits_all, grid_for((g->n + 31) / 32), bits, g->n);
+    return PG_OK;
+}
+
+// ---- input-order transfers ---------------------------------------------------------
+int to_input_order_i32(pg_graph *g, const int32_t *dev_internal, int32_t *host, bool map_ids) {
+    if (!host) return PG_OK;
+    int32_t *tmp = nullptr;
+    PG_TRY(workspace_t(g, "xfer", (size_t)g->n * 4, reinterpret_cast<float **>(&tmp)));
+    PG_LAUNCH(k_to_input_i32, grid_for(g->n), dev_internal, g->perm, g->iperm, g->n,
+              map_ids ? 1 : 0, tmp);
+    PG_CHECK_HIP(hipMemcpyAsync(host, tmp, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return PG_OK;
+}
+
+int to_input_order_f32(pg_graph *g, const float *dev_internal, float *host, int width) {
+    if (!host) return PG_OK;
+    float *tmp = nullptr;
+    PG_TRY(workspace_t(g, "xfer", (size_t)g->n * width, &tmp));
+    PG_LAUNCH(k_to_input_f32, grid_for(g->n * width), dev_internal, g->perm, g->n, width, tmp);
+    PG_CHECK_HIP(hipMemcpyAsync(host, tmp, sizeof(float) * g->n * width, hipMemcpyDeviceToHost,
+                                g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return PG_OK;
+}
+
+int from_input_order_f32(pg_graph *g, const float *host, float *dev_internal, int width) {
+    float *tmp = nullptr;
+    PG_TRY(workspace_t(g, "xfer", (size_t)g->n * width, &tmp));
+    PG_CHECK_HIP(hipMemcpyAsync(tmp, host, sizeof(float) * g->n * width, hipMemcpyHostToDevice,
+                                g_stream));
+    PG_LAUNCH(k_from_input_f32, grid_for(g->n * width), tmp, g->perm, g->n, width, dev_internal);
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return PG_OK;
+}
+
+int edges_from_input_f32(pg_graph *g, const float *host, bool csc, float *dev) {
+    float *tmp = nullptr;
+    PG_TRY(workspace_t(g, "xfer_e", (size_t)g->m, &tmp));
+    PG_CHECK_HIP(hipMemcpyAsync(tmp, host, sizeof(float) * g->m, hipMemcpyHostToDevice, g_stream));
+    PG_LAUNCH(k_gather<float>, grid_for(g->m), tmp, csc ? g->in_eid : g->out_eid, g->m, dev);
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return PG_OK;
+}
+
+__global__ void k_scatter_f32(const float *src, const int32_t *idx, int64_t m, float *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[idx[i]] = src[i];
+}
+
+int edges_to_input_f32(pg_graph *g, const float *dev, bool csc, float *host) {
+    if (!host) return PG_OK;
+    float *tmp = nullptr;
+    PG_TRY(workspace_t(g, "xfer_e", (size_t)g->m, &tmp));
+    PG_LAUNCH(k_scatter_f32, grid_for(g->m), dev, csc ? g->in_eid : g->out_eid, g->m, tmp);
+    PG_CHECK_HIP(hipMemcpyAsync(host, tmp, sizeof(float) * g->m, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return PG_OK;
+}
+
+// ---- schedule ---------------------------------------------------------------------
+static int free_schedule(PullSchedule &s) {
+    if (s.d_multi) (void)hipFree(s.d_multi);
+    if (s.d_partial) (void)hipFree(s.d_partial);
+    if (s.d_tickets) (void)hipFree(s.d_tickets);
+    s.d_multi = nullptr;
+    s.d_partial = nullptr;
+    s.d_tickets = nullptr;
+    s.built = false;
+    s.nblocks = 0;
+    return PG_OK;
+}
+
+int build_schedule(pg_graph *g, bool in_direction) {
+    PullSchedule &s = in_direction ? g->in_sched : g->out_sched;
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    free_schedule(s);
+    const int32_t *off = in_direction ? g->in_off : g->out_off;
+    const int32_t *adj = in_direction ? g->in_adj : g->out_adj;
+    const int32_t *order = in_direction ? nullptr : g->out_order;
+    const PullParams &P = s.params;
+    const int64_t n = g->n;
+    const int ipl = std::max(1, P.items_per_lane);
+    // class upper bounds, descending: multi | block | 64 | 32 | 16 | 8 | 4 | 2 | 1
+    const int gs[9] = {kModeMulti, kModeBlock, 64, 32, 16, 8, 4, 2, 1};
+    int64_t thr[8];
+    thr[0] = std::max<int64_t>(P.multi_threshold, P.block_threshold);
+    thr[1] = std::max<int64_t>(P.block_threshold, 32LL * ipl);
+    for (int j = 2; j < 8; j++) thr[j] = (int64_t)ipl << (7 - j);  // 32ipl,16,8,4,2,ipl
+    // thr[2] = 32*ipl ... thr[7] = ipl
+    DevBuf dthr, dcnt;
+    PG_TRY(dthr.alloc(sizeof(thr)));
+    PG_TRY(dcnt.alloc(sizeof(thr)));
+    PG_CHECK_HIP(hipMemcpyAsync(dthr.p, thr, sizeof(thr), hipMemcpyHostToDevice, g_stream));
+    hipLaunchKernelGGL(k_count_above, dim3(1), dim3(64), 0, g_stream, off, order, n,
+                       dthr.as<int64_t>(), 8, dcnt.as<int64_t>());
+    PG_CHECK_HIP(hipGetLastError());
+    int64_t cnt[8];
+    PG_CHECK_HIP(hipMemcpyAsync(cnt, dcnt.p, sizeof(cnt), hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    int64_t bounds[10];
+    bounds[0] = 0;
+    for (int j = 0; j < 8; j++) bounds[j + 1] = std::max(cnt[j], bounds[j]);
+    bounds[9] = n;
+    PullArgs &a = s.args;
+    a = PullArgs{};
+    a.off = off;
+    a.adj = adj;
+    a.vlist = order;
+    a.chunk = (int32_t)std::max<int64_t>(kBlock, P.chunk);
+    int nseg = 0;
+    int64_t blocks = 0;
+    std::vector<MultiItem> items;
+    for (int c = 0; c < 9; c++) {
+        const int64_t vb = bounds[c], ve = bounds[c + 1];
+        if (ve <= vb) continue;
+        PullSeg seg{(int32_t)vb, (int32_t)ve, gs[c], (int32_t)blocks};
+        if (gs[c] == kModeMulti) {
+            std::vector<int32_t> deg((size_t)(ve - vb));
+            DevBuf dd;
+            PG_TRY(dd.alloc(sizeof(int32_t) * deg.size()));
+            hipLaunchKernelGGL(k_degree_of, dim3(grid_for(ve)), dim3(kBlock), 0, g_stream, off,
+                               order, ve, dd.as<int32_t>());
+            PG_CHECK_HIP(hipGetLastError());
+            PG_CHECK_HIP(hipMemcpyAsync(deg.data(), dd.p, sizeof(int32_t) * deg.size(),
+                                        hipMemcpyDeviceToHost, g_stream));
+            PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+            for (int64_t p = vb; p < ve; p++) {
+                const int32_t nch = (int32_t)((deg[p - vb] + a.chunk - 1) / a.chunk);
+                const int32_t first = (int32_t)items.size();
+                for (int32_t ch = 0; ch < nch; ch++)
+                    items.push_back(MultiItem{(int32_t)p, ch, nch, first, (int32_t)(p - vb), {0, 0, 0}});
+            }
+            blocks += (int64_t)items.size();
+        } else if (gs[c] == kModeBlock) {
+            blocks += ve - vb;
+        } else {
+            blocks += ((ve - vb) * gs[c] + kBlock - 1) / kBlock;
+        }
+        if (nseg >= kMaxSegs) {
+            set_error("schedule: too many segments");
+            return PG_ERR_STATE;
+        }
+        a.seg[nseg++] = seg;
+    }
+    a.nseg = nseg;
+    if (!items.empty()) {
+        PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&s.d_multi), sizeof(MultiItem) * items.size()));
+        PG_CHECK_HIP(hipMemcpy(s.d_multi, items.data(), sizeof(MultiItem) * items.size(),
+                               hipMemcpyHostToDevice));
+        PG_CHECK_HIP(hipMalloc(&s.d_partial, 32 * items.size() + 64));
+        const int64_t nmv = bounds[1] - bounds[0];
+        PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&s.d_tickets), sizeof(unsigned) * (nmv + 1)));
+        PG_CHECK_HIP(hipMemset(s.d_tickets, 0, sizeof(unsigned) * (nmv + 1)));
+    }
+    a.multi = s.d_multi;
+    a.partial = s.d_partial;
+    a.tickets = s.d_tickets;
+    if (blocks > 0x7FFFFFFF) {
+        set_error("schedule: grid too large");
+        return PG_ERR_UNSUPPORTED;
+    }
+    s.nblocks = (int)blocks;
+    s.built = true;
+    return PG_OK;
+}
+
+int ensure_out_schedule(pg_graph *g) {
+    if (g->out_sched.built) return PG_OK;
+    if (!g->out_order) {
+        DevBuf vk, vk2;
+        PG_TRY(vk.alloc(sizeof(uint64_t) * g->n));
+        PG_TRY(vk2.alloc(sizeof(uint64_t) * g->n));
+        PG_LAUNCH(k_out_order_keys, grid_for(g->n), g->out_off, g->n, vk.as<uint64_t>());
+        PG_TRY(sort_keys(vk.as<uint64_t>(), vk2.as<uint64_t>(), g->n, 64));
+        PG_TRY(galloc_t(g, &g->out_order, g->n));
+        PG_LAUNCH(k_low32, grid_for(g->n), vk2.as<uint64_t>(), g->n, g->out_order);
+    }
+    return build_schedule(g, false);
+}
+
+// ---- graph construction ---------------------------------------------------------------
+// keys: input-id (src<<32|dst) pairs in input CSR order (sorted by src, and for
+// generated graphs by (src,dst)); m entries; w_input: device weights in the
+// same order or nullptr.
+static int build_graph(pg_graph *g, uint64_t *keys, int64_t m, const void *w_input, int w_dtype) {
+    const int64_t n = g->n;
+    g->m = m;
+    const int vb = bits_for(n + 1);
+    DevBuf d_out, d_in, d_cnt, d_vk, d_vk2, d_k2, d_e1, d_e2;
+    PG_TRY(d_out.alloc(sizeof(uint32_t) * (n + 1)));
+    PG_TRY(d_in.alloc(sizeof(uint32_t) * (n + 1)));
+    PG_CHECK_HIP(hipMemsetAsync(d_out.p, 0, sizeof(uint32_t) * (n + 1), g_stream));
+    PG_CHECK_HIP(hipMemsetAsync(d_in.p, 0, sizeof(uint32_t) * (n + 1), g_stream));
+    PG_LAUNCH(k_degrees, grid_for(m), keys, m, d_out.as<uint32_t>(), d_in.as<uint32_t>());
+    // relabel by in-degree, descending, ties by input id
+    PG_TRY(d_vk.alloc(sizeof(uint64_t) * n));
+    PG_TRY(d_vk2.alloc(sizeof(uint64_t) * n));
+    PG_LAUNCH(k_vertex_keys, grid_for(n), d_in.as<uint32_t>(), n, d_vk.as<uint64_t>());
+    PG_TRY(sort_keys(d_vk.as<uint64_t>(), d_vk2.as<uint64_t>(), n, 64));
+    PG_TRY(galloc_t(g, &g->perm, n));
+    PG_TRY(galloc_t(g, &g->iperm, n));
+    PG_LAUNCH(k_perm_from_sorted, grid_for(n), d_vk2.as<uint64_t>(), n, g->iperm, g->perm);
+    d_vk.alloc(8);
+    d_vk2.alloc(8);
+    // CSR in internal ids
+    PG_TRY(d_k2.alloc(sizeof(uint64_t) * m));
+    uint64_t *k_sorted = nullptr;
+    PG_TRY(d_e1.alloc(sizeof(uint32_t) * m));
+    PG_TRY(d_e2.alloc(sizeof(uint32_t) * m));
+    PG_TRY(d_cnt.alloc(sizeof(uint32_t) * (n + 1)));
+    DevBuf d_k3;
+    PG_TRY(d_k3.alloc(sizeof(uint64_t) * m));
+    k_sorted = d_k3.as<uint64_t>();
+    for (int dir = 0; dir < (g->sym ? 1 : 2); dir++) {
+        PG_LAUNCH(k_relabel_keys, grid_for(m), keys, m, g->perm, dir, d_k2.as<uint64_t>(),
+                  d_e1.as<uint32_t>());
+        PG_TRY(sort_pairs(d_k2.as<uint64_t>(), k_sorted, d_e1.as<uint32_t>(), d_e2.as<uint32_t>(), m,
+                          32 + vb));
+        int32_t *off = nullptr, *adj = nullptr, *eid = nullptr;
+        PG_TRY(galloc_t(g, &off, n + 1));
+        PG_TRY(galloc_t(g, &adj, m));
+        PG_TRY(galloc_t(g, &eid, m));
+        PG_LAUNCH(k_permute_counts, grid_for(n + 1),
+                  dir == 0 ? d_out.as<uint32_t>() : d_in.as<uint32_t>(), g->perm, n,
+                  d_cnt.as<uint32_t>());
+        PG_TRY(exclusive_sum(d_cnt.as<uint32_t>(), reinterpret_cast<uint32_t *>(off), n + 1));
+        PG_LAUNCH(k_low32, grid_for(m), k_sorted, m, adj);
+        PG_CHECK_HIP(hipMemcpyAsync(eid, d_e2.p, sizeof(int32_t) * m, hipMemcpyDeviceToDevice, g_stream));
+        if (dir == 0) {
+            g->out_off = off; g->out_adj = adj; g->out_eid = eid;
+        } else {
+            g->in_off = off; g->in_adj = adj; g->in_eid = eid;
+        }
+    }
+    if (g->sym) {
+        g->in_off = g->out_off; g->in_adj = g->out_adj; g->in_eid = g->out_eid;
+    }
+    // weights
+    g->w_dtype = w_dtype;
+    if (w_dtype == PG_W_I32) {
+        PG_TRY(galloc_t(g, &g->out_w_i32, m));
+        PG_LAUNCH(k_gather<int32_t>, grid_for(m), (const int32_t *)w_input, g->out_eid, m, g->out_w_i32);
+        if (g->sym) g->in_w_i32 = g->out_w_i32;
+        else {
+            PG_TRY(galloc_t(g, &g->in_w_i32, m));
+            PG_LAUNCH(k_gather<int32_t>, grid_for(m), (const int32_t *)w_input, g->in_eid, m, g->in_w_i32);
+        }
+    } else if (w_dtype == PG_W_F32) {
+        PG_TRY(galloc_t(g, &g->out_w_f32, m));
+        PG_LAUNCH(k_gather<float>, grid_for(m), (const float *)w_input, g->out_eid, m, g->out_w_f32);
+        if (g->sym) g->in_w_f32 = g->out_w_f32;
+        else {
+            PG_TRY(galloc_t(g, &g->in_w_f32, m));
+            PG_LAUNCH(k_gather<float>, grid_for(m), (const float *)w_input, g->in_eid, m, g->in_w_f32);
+        }
+    }
+    // max degrees
+    int32_t o0[2] = {0, 0};
+    if (n > 0) {
+        PG_CHECK_HIP(hipMemcpyAsync(o0, g->in_off, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, g_stream));
+    }
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    g->max_in_deg = o0[1] - o0[0];
+    PG_TRY(build_schedule(g, true));
+    return PG_OK;
+}
+
+static int new_graph(int64_t n, pg_graph **out) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(out, PG_ERR_ARG, "null output handle");
+    PG_REQUIRE(n > 0 && n < 0x7FFFFFFF, PG_ERR_ARG, "n out of range: %lld", (long long)n);
+    *out = new pg_graph();
+    (*out)->n = n;
+    return PG_OK;
+}
+
+static void delete_graph(pg_graph *g) {
+    if (!g) return;
+    (void)hipStreamSynchronize(g_stream);
+    free_schedule(g->in_sched);
+    free_schedule(g->out_sched);
+    for (void *p : g->owned) (void)hipFree(p);
+    for (auto &kv : g->ws) (void)hipFree(kv.second.first);
+    delete g;
+}
+
+int create_from_host(int64_t n, int64_t m, const int64_t *off, const int32_t *adj, const void *w,
+                     int w_dtype, uint32_t flags, pg_graph **out) {
+    PG_REQUIRE(m >= 0 && m < 0x7FFFFFFF, PG_ERR_ARG, "m out of range: %lld", (long long)m);
+    PG_REQUIRE(off && (adj || m == 0), PG_ERR_ARG, "null CSR arrays");
+    PG_REQUIRE(w_dtype == PG_W_NONE || w_dtype == PG_W_I32 || w_dtype == PG_W_F32, PG_ERR_ARG,
+               "bad weight dtype %d", w_dtype);
+    PG_REQUIRE(off[0] == 0 && off[n] == m, PG_ERR_ARG, "offsets must span [0, m]");
+    // host-side validation and keys (self-loop removal as IO-numa.h:225-268)
+    std::vector<uint64_t> keys;
+    std::vector<char> wbytes;
+    keys.reserve((size_t)m);
+    const size_t wsz = 4;
+    if (w_dtype != PG_W_NONE) wbytes.reserve((size_t)m * wsz);
+    for (int64_t v = 0; v < n; v++) {
+        PG_REQUIRE(off[v] <= off[v + 1], PG_ERR_ARG, "offsets not monotone at %lld", (long long)v);
+        for (int64_t e = off[v]; e < off[v + 1]; e++) {
+            const int32_t d = adj[e];
+            PG_REQUIRE(d >= 0 && d < n, PG_ERR_ARG, "edge %lld: neighbour %d out of range",
+                       (long long)e, d);
+            if ((flags & PG_GRAPH_SKIP_SELF_LOOPS) && d == v) continue;
+            keys.push_back(((uint64_t)v << 32) | (uint32_t)d);
+            if (w_dtype != PG_W_NONE) {
+                const char *src = reinterpret_cast<const char *>(w) + e * wsz;
+                wbytes.insert(wbytes.end(), src, src + wsz);
+            }
+        }
+    }
+    PG_REQUIRE(w_dtype == PG_W_NONE || w, PG_ERR_ARG, "weights requested but w is NULL");
+    PG_TRY(new_graph(n, out));
+    pg_graph *g = *out;
+    g->sym = (flags & PG_GRAPH_SYMMETRIC) != 0;
+    const int64_t mm = (int64_t)keys.size();
+    DevBuf dk, dw;
+    int r = dk.alloc(sizeof(uint64_t) * (mm + 1));
+    if (r == PG_OK && mm) {
+        hipError_t e = hipMemcpy(dk.p, keys.data(), sizeof(uint64_t) * mm, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { set_error("upload failed: %s", hipGetErrorString(e)); r = PG_ERR_HIP; }
+    }
+    if (r == PG_OK && w_dtype != PG_W_NONE) {
+        r = dw.alloc(wbytes.size() + 16);
+        if (r == PG_OK && mm) {
+            hipError_t e = hipMemcpy(dw.p, wbytes.data(), wbytes.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) { set_error("upload failed: %s", hipGetErrorString(e)); r = PG_ERR_HIP; }
+        }
+    }
+    if (r == PG_OK) r = build_graph(g, dk.as<uint64_t>(), mm, dw.p, w_dtype);
+    if (r != PG_OK) {
+        delete_graph(g);
+        *out = nullptr;
+    }
+    return r;
+}
+
+// device rMat: the first m distinct non-loop pairs of the counter sequence
+static int rmat_keys(int scale, int64_t m, double a, double b, double c, uint64_t seed,
+                     DevBuf &out_keys) {
+    uint32_t thr[3];
+    host_rmat_thresholds(a, b, c, thr);
+    const uint64_t base = host_mix64(seed ^ 0x5851F42D4C957F2Dull);
+    int64_t C = m + m / 8 + 1024;
+    for (;;) {
+        PG_REQUIRE(C < 0xFFFFFFFFll, PG_ERR_ARG, "rmat: too many duplicates for m=%lld", (long long)m);
+        DevBuf k1, k2, c1, c2, mark, cnt, cut, sel, nsel;
+        PG_TRY(k1.alloc(8 * C)); PG_TRY(k2.alloc(8 * C));
+        PG_TRY(c1.alloc(4 * C)); PG_TRY(c2.alloc(4 * C));
+        PG_LAUNCH(k_rmat_keys, grid_for(C), base, scale, thr[0], thr[1], thr[2], C,
+                  k1.as<uint64_t>(), c1.as<uint32_t>());
+        PG_TRY(sort_pairs(k1.as<uint64_t>(), k2.as<uint64_t>(), c1.as<uint32_t>(), c2.as<uint32_t>(),
+                          C, 64));
+        PG_TRY(mark.alloc(4 * C));
+        PG_CHECK_HIP(hipMemsetAsync(mark.p, 0, 4 * C, g_stream));
+        PG_LAUNCH(k_first_by_ctr, grid_for(C), k2.as<uint64_t>(), c2.as<uint32_t>(), C, mark.as<uint32_t>());
+        PG_TRY(cnt.alloc(4 * C));
+        PG_TRY(inclusive_sum(mark.as<uint32_t>(), cnt.as<uint32_t>(), C));
+        uint32_t total = 0;
+        PG_CHECK_HIP(hipMemcpyAsync(&total, cnt.as<uint32_t>() + (C - 1), 4, hipMemcpyDeviceToHost, g_stream));
+        PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+        if ((int64_t)total >= m) {
+            PG_TRY(cut.alloc(8));
+            int64_t minus1 = -1;
+            PG_CHECK_HIP(hipMemcpyAsync(cut.p, &minus1, 8, hipMemcpyHostToDevice, g_stream));
+            if (m > 0) PG_LAUNCH(k_find_cutoff, grid_for(C), cnt.as<uint32_t>(), C, (uint32_t)m, cut.as<int64_t>());
+            PG_TRY(sel.alloc(C));
+            PG_LAUNCH(k_select_flags, grid_for(C), k2.as<uint64_t>(), c2.as<uint32_t>(), C,
+                      cut.as<int64_t>(), sel.as<uint8_t>());
+            PG_TRY(out_keys.alloc(8 * (m + 1)));
+            PG_TRY(nsel.alloc(8));
+            PG_TRY(select_flagged(k2.as<uint64_t>(), sel.as<uint8_t>(), out_keys.as<uint64_t>(),
+                                  nsel.as<unsigned long long>(), C));
+            unsigned long long got = 0;
+            PG_CHECK_HIP(hipMemcpyAsync(&got, nsel.p, 8, hipMemcpyDeviceToHost, g_stream));
+            PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+            PG_REQUIRE((int64_t)got == m, PG_ERR_STATE, "rmat: selected %llu of %lld", got, (long long)m);
+            return PG_OK;
+        }
+        C = C + C / 2 + 1024;
+    }
+}
+
+static int symmetrize_keys(DevBuf &keys, int64_t m, int64_t *m_out) {
+    DevBuf k2, k3, sel, nsel;
+    PG_TRY(k2.alloc(8 * (2 * m + 1)));
+    PG_TRY(k3.alloc(8 * (2 * m + 1)));
+    PG_LAUNCH(k_mirror, grid_for(m), keys.as<uint64_t>(), m, k2.as<uint64_t>());
+    PG_TRY(sort_keys(k2.as<uint64_t>(), k3.as<uint64_t>(), 2 * m, 64));
+    PG_TRY(sel.alloc(2 * m + 1));
+    PG_LAUNCH(k_unique_flags, grid_for(2 * m), k3.as<uint64_t>(), 2 * m, sel.as<uint8_t>());
+    PG_TRY(keys.alloc(8 * (2 * m + 1)));
+    PG_TRY(nsel.alloc(8));
+    PG_TRY(select_flagged(k3.as<uint64_t>(), sel.as<uint8_t>(), keys.as<uint64_t>(),
+                          nsel.as<unsigned long long>(), 2 * m));
+    unsigned long long got = 0;
+    PG_CHECK_HIP(hipMemcpyAsync(&got, nsel.p, 8, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    *m_out = (int64_t)got;
+    return PG_OK;
+}
+
+int create_rmat(int scale, int64_t m, double a, double b, double c, uint64_t seed, uint32_t flags,
+                uint64_t wseed, pg_graph **out) {
+    PG_REQUIRE(scale >= 1 && scale <= 30, PG_ERR_ARG, "scale out of range: %d", scale);
+    PG_REQUIRE(m >= 0 && m < 0x40000000ll, PG_ERR_ARG, "m out of range");
+    PG_REQUIRE(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0, PG_ERR_ARG, "bad rmat probabilities");
+    PG_REQUIRE(!((flags & PG_GRAPH_WEIGHT_I32) && (flags & PG_GRAPH_VALUE_F32)), PG_ERR_ARG,
+               "choose one weight kind");
+    PG_TRY(new_graph(int64_t(1) << scale, out));
+    pg_graph *g = *out;
+    DevBuf keys, w;
+    int64_t mm = m;
+    int r = rmat_keys(scale, m, a, b, c, seed, keys);
+    if (r == PG_OK && (flags & PG_GRAPH_SYMMETRIZE)) {
+        r = symmetrize_keys(keys, m, &mm);
+        g->sym = true;
+    }
+    int wd = PG_W_NONE;
+    if (r == PG_OK && (flags & PG_GRAPH_WEIGHT_I32)) {
+        wd = PG_W_I32;
+        r = w.alloc(4 * (mm + 1));
+        if (r == PG_OK && mm) {
+            hipLaunchKernelGGL(k_pair_weights_i32, dim3(grid_for(mm)), dim3(kBlock), 0, g_stream,
+                               keys.as<uint64_t>(), mm, wseed, w.as<int32_t>());
+            if (hipGetLastError() != hipSuccess) { set_error("weights launch failed"); r = PG_ERR_HIP; }
+        }
+    } else if (r == PG_OK && (flags & PG_GRAPH_VALUE_F32)) {
+        wd = PG_W_F32;
+        r = w.alloc(4 * (mm + 1));
+        if (r == PG_OK && mm) {
+            hipLaunchKernelGGL(k_pair_values_f32, dim3(grid_for(mm)), dim3(kBlock), 0, g_stream,
+                               keys.as<uint64_t>(), mm, wseed, w.as<float>());
+            if (hipGetLastError() != hipSuccess) { set_error("values launch failed"); r = PG_ERR_HIP; }
+        }
+    }
+    if (r == PG_OK) r = build_graph(g, keys.as<uint64_t>(), mm, w.p, wd);
+    if (r != PG_OK) {
+        delete_graph(g);
+        *out = nullptr;
+    }
+    return r;
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+// ====================================================================================
+// C ABI
+// ====================================================================================
+extern "C" int pg_init(int device) {
+    if (g_init) {
+        PG_REQUIRE(device == g_device, PG_ERR_STATE, "already initialised on device %d", g_device);
+        return PG_OK;
+    }
+    int count = 0;
+    PG_CHECK_HIP(hipGetDeviceCount(&count));
+    PG_REQUIRE(device >= 0 && device < count, PG_ERR_ARG, "device %d not present (%d devices)",
+               device, count);
+    PG_CHECK_HIP(hipSetDevice(device));
+    PG_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+    g_init = true;
+    return PG_OK;
+}
+
+extern "C" int pg_finalize(void) {
+    if (!g_init) return PG_OK;
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    PG_CHECK_HIP(hipStreamDestroy(g_stream));
+    g_stream = nullptr;
+    g_init = false;
+    g_device = -1;
+    return PG_OK;
+}
+
+extern "C" int pg_last_error(char *buf, size_t len) {
+    if (!buf || len == 0) return PG_ERR_ARG;
+    snprintf(buf, len, "%s", g_err);
+    return PG_OK;
+}
+
+extern "C" int pg_device_name(char *buf, size_t len) {
+    PG_REQUIRE(g_init, PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(buf && len > 0, PG_ERR_ARG, "null buffer");
+    hipDeviceProp_t p;
+    PG_CHECK_HIP(hipGetDeviceProperties(&p, g_device));
+    snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return PG_OK;
+}
+
+extern "C" int pg_graph_create(int64_t n, int64_t m, const int64_t *off, const int32_t *adj,
+                               const void *w, int w_dtype, uint32_t flags, pg_graph **out) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(out, PG_ERR_ARG, "null output handle");
+    *out = nullptr;
+    PG_REQUIRE(n > 0 && n < 0x7FFFFFFF, PG_ERR_ARG, "n out of range: %lld", (long long)n);
+    return create_from_host(n, m, off, adj, w, w_dtype, flags, out);
+}
+
+extern "C" int pg_graph_rmat(int scale, int64_t m, double a, double b, double c, uint64_t seed,
+                             uint32_t flags, uint64_t wseed, pg_graph **out) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(out, PG_ERR_ARG, "null output handle");
+    *out = nullptr;
+    return create_rmat(scale, m, a, b, c, seed, flags, wseed, out);
+}
+
+extern "C" int pg_graph_load_adj(const char *path, uint32_t flags, pg_graph **out) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(path && out, PG_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::ifstream f(path, std::ios::in | std::ios::binary | std::ios::ate);
+    PG_REQUIRE(f.is_open(), PG_ERR_ARG, "Unable to open file: %s", path);  // IO-numa.h:75-78
+    const std::streamsize size = f.tellg();
+    f.seekg(0);
+    std::string text((size_t)size, '\0');
+    f.read(&text[0], size);
+    // whitespace-separated words (IO-numa.h:90-109)
+    const char *p = text.data(), *endp = p + text.size();
+    auto next_word = [&](const char **b, const char **e) -> bool {
+        while (p < endp && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == 0)) p++;
+        if (p >= endp) return false;
+        *b = p;
+        while (p < endp && !(*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == 0)) p++;
+        *e = p;
+        return true;
+    };
+    auto next_ll = [&](long long *x) -> bool {
+        const char *b, *e;
+        if (!next_word(&b, &e)) return false;
+        *x = strtoll(std::string(b, e).c_str(), nullptr, 10);
+        return true;
+    };
+    const char *b, *e;
+    PG_REQUIRE(next_word(&b, &e), PG_ERR_ARG, "Bad input file: %s", path);
+    const std::string head(b, e);
+    const bool weighted = head == "WeightedAdjacencyGraph";
+    PG_REQUIRE(weighted || head == "AdjacencyGraph", PG_ERR_ARG, "Bad input file: %s", path);
+    long long n = 0, m = 0;
+    PG_REQUIRE(next_ll(&n) && next_ll(&m) && n > 0 && m >= 0, PG_ERR_ARG, "Bad input file: %s", path);
+    std::vector<int64_t> off((size_t)n + 1);
+    std::vector<int32_t> adj((size_t)m), w(weighted ? (size_t)m : 0);
+    long long x;
+    for (long long i = 0; i < n; i++) {
+        PG_REQUIRE(next_ll(&x), PG_ERR_ARG, "Bad input file (offsets): %s", path);
+        off[i] = x;
+    }
+    off[n] = m;
+    for (long long i = 0; i < m; i++) {
+        PG_REQUIRE(next_ll(&x), PG_ERR_ARG, "Bad input file (edges): %s", path);
+        adj[i] = (int32_t)x;
+    }
+    for (long long i = 0; weighted && i < m; i++) {
+        PG_REQUIRE(next_ll(&x), PG_ERR_ARG, "Bad input file (weights): %s", path);
+        w[i] = (int32_t)x;
+    }
+    PG_REQUIRE(!next_word(&b, &e), PG_ERR_ARG, "Bad input file (length): %s", path);  // IO-numa.h:122
+    return create_from_host(n, m, off.data(), adj.data(), weighted ? w.data() : nullptr,
+                            weighted ? PG_W_I32 : PG_W_NONE, flags, out);
+}
+
+extern "C" int pg_graph_destroy(pg_graph *g) {
+    if (!g) return PG_OK;
+    delete_graph(g);
+    return PG_OK;
+}
+
+extern "C" int pg_graph_info(const pg_graph *g, int64_t *n, int64_t *m, int64_t *max_in_degree,
+                             int32_t *w_dtype) {
+    PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (max_in_degree) *max_in_degree = g->max_in_deg;
+    if (w_dtype) *w_dtype = g->w_dtype;
+    return PG_OK;
+}
+
+__global__ void k_download_keys(const int32_t *off, const int32_t *adj, const int32_t *iperm,
+                                int64_t n, uint64_t *keys, uint32_t *eidx) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = (uint32_t)iperm[v];
+        for (int e = off[v]; e < off[v + 1]; e++) {
+            keys[e] = (s << 32) | (uint32_t)iperm[adj[e]];
+            eidx[e] = (uint32_t)e;
+        }
+    }
+}
+
+extern "C" int pg_graph_download(const pg_graph *gc, int64_t *off, int32_t *adj, void *w) {
+    PG_REQUIRE(gc, PG_ERR_ARG, "null graph");
+    pg_graph *g = const_cast<pg_graph *>(gc);
+    const int64_t n = g->n, m = g->m;
+    DevBuf k1, k2, e1, e2, low, wbuf;
+    PG_TRY(k1.alloc(8 * (m + 1))); PG_TRY(k2.alloc(8 * (m + 1)));
+    PG_TRY(e1.alloc(4 * (m + 1))); PG_TRY(e2.alloc(4 * (m + 1)));
+    PG_LAUNCH(k_download_keys, grid_for(n), g->out_off, g->out_adj, g->iperm, n,
+              k1.as<uint64_t>(), e1.as<uint32_t>());
+    PG_TRY(sort_pairs(k1.as<uint64_t>(), k2.as<uint64_t>(), e1.as<uint32_t>(), e2.as<uint32_t>(), m,
+                      32 + bits_for(n + 1)));
+    std::vector<uint64_t> keys((size_t)m);
+    PG_CHECK_HIP(hipMemcpyAsync(keys.data(), k2.p, 8 * m, hipMemcpyDeviceToHost, g_stream));
+    if (w && g->w_dtype != PG_W_NONE) {
+        PG_TRY(wbuf.alloc(4 * (m + 1)));
+        if (g->w_dtype == PG_W_I32)
+            PG_LAUNCH(k_gather<int32_t>, grid_for(m), g->out_w_i32, (const int32_t *)e2.p, m, wbuf.as<int32_t>());
+        else
+            PG_LAUNCH(k_gather<float>, grid_for(m), g->out_w_f32, (const int32_t *)e2.p, m, wbuf.as<float>());
+        PG_CHECK_HIP(hipMemcpyAsync(w, wbuf.p, 4 * m, hipMemcpyDeviceToHost, g_stream));
+    }
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    if (off) {
+        for (int64_t i = 0; i <= n; i++) off[i] = 0;
+        for (int64_t e = 0; e < m; e++) off[(keys[e] >> 32) + 1]++;
+        for (int64_t i = 0; i < n; i++) off[i + 1] += off[i];
+    }
+    if (adj)
+        for (int64_t e = 0; e < m; e++) adj[e] = (int32_t)(uint32_t)keys[e];
+    return PG_OK;
+}
+
+extern "C" int pg_graph_set_schedule(pg_graph *g, int items_per_lane, int64_t block_threshold,
+                                     int64_t multi_threshold, int64_t chunk) {
+    PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    for (PullSchedule *s : {&g->in_sched, &g->out_sched}) {
+        if (items_per_lane > 0) s->params.items_per_lane = items_per_lane;
+        if (block_threshold > 0) s->params.block_threshold = block_threshold;
+        if (multi_threshold > 0) s->params.multi_threshold = multi_threshold;
+        if (chunk > 0) s->params.chunk = chunk;
+    }
+    PG_TRY(build_schedule(g, true));
+    if (g->out_sched.built) PG_TRY(build_schedule(g, false));
+    return PG_OK;
+}
+
+extern "C" int pg_graph_bp_potentials(pg_graph *g, uint64_t seed, float lo, float hi) {
+    PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    if (!g->bp_pot) PG_TRY(galloc_t(g, &g->bp_pot, 4 * (size_t)g->m));
+    PG_LAUNCH(k_bp_pot, grid_for(g->n), g->in_off, g->in_adj, g->iperm, g->n, seed, lo, hi, g->bp_pot);
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return PG_OK;
+}
+
+// ---- subsets -------------------------------------------------------------------------
+extern "C" int pg_subset_create(pg_graph *g, pg_subset **out) {
+    PG_REQUIRE(g && out, PG_ERR_ARG, "null argument");
+    pg_subset *s = new pg_subset();
+    s->g = g;
+    const int64_t nw = (g->n + 31) / 32;
+    hipError_t e1 = hipMalloc(reinterpret_cast<void **>(&s->bits), 4 * (nw + 1));
+    hipError_t e2 = hipMalloc(reinterpret_cast<void **>(&s->list), 4 * (g->n + 1));
+    hipError_t e3 = hipMalloc(reinterpret_cast<void **>(&s->counters), 64);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        if (s->bits) (void)hipFree(s->bits);
+        if (s->list) (void)hipFree(s->list);
+        if (s->counters) (void)hipFree(s->counters);
+        delete s;
+        set_error("subset allocation failed");
+        return PG_ERR_OOM;
+    }
+    *out = s;
+    return pg_subset_clear(s);
+}
+
+extern "C" int pg_subset_destroy(pg_subset *s) {
+    if (!s) return PG_OK;
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipFree(s->bits);
+    (void)hipFree(s->list);
+    (void)hipFree(s->counters);
+    delete s;
+    return PG_OK;
+}
+
+extern "C" int pg_subset_clear(pg_subset *s) {
+    PG_REQUIRE(s, PG_ERR_ARG, "null subset");
+    PG_CHECK_HIP(hipMemsetAsync(s->bits, 0, 4 * ((s->g->n + 31) / 32 + 1), g_stream));
+    PG_CHECK_HIP(hipMemsetAsync(s->counters, 0, 64, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return PG_OK;
+}
+
+static int subset_finish(pg_subset *s) {
+    // list + |F| from the bitmap, then sum of out-degrees (vertexCounter)
+    PG_CHECK_HIP(hipMemsetAsync(s->counters, 0, 64, g_stream));
+    PG_TRY(bits_to_list(s->g, s->bits, s->list, &s->counters[0]));
+    PG_TRY(list_degree_sum(s->g, s->list, &s->counters[0], s->g->out_off, &s->counters[1]));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return PG_OK;
+}
+
+extern "C" int pg_subset_fill(pg_subset *s) {
+    PG_REQUIRE(s, PG_ERR_ARG, "null subset");
+    PG_TRY(fill_bits_all(s->g, s->bits));
+    return subset_finish(s);
+}
+
+extern "C" int pg_subset_from_dense(pg_subset *s, const uint8_t *flags) {
+    PG_REQUIRE(s && flags, PG_ERR_ARG, "null argument");
+    pg_graph *g = s->g;
+    uint8_t *tmp = nullptr;
+    PG_TRY(workspace_t(g, "subset_in", (size_t)g->n, &tmp));
+    PG_CHECK_HIP(hipMemcpyAsync(tmp, flags, g->n, hipMemcpyHostToDevice, g_stream));
+    PG_CHECK_HIP(hipMemsetAsync(s->bits, 0, 4 * ((g->n + 31) / 32 + 1), g_stream));
+    PG_LAUNCH(k_bits_from_bools_input, grid_for(g->n), tmp, g->perm, g->n, s->bits);
+    return subset_finish(s);
+}
+
+extern "C" int pg_subset_from_sparse(pg_subset *s, const int32_t *ids, int64_t k) {
+    PG_REQUIRE(s && (ids || k == 0) && k >= 0, PG_ERR_ARG, "bad argument");
+    pg_graph *g = s->g;
+    for (int64_t i = 0; i < k; i++)
+        PG_REQUIRE(ids[i] >= 0 && ids[i] < g->n, PG_ERR_ARG, "id %d out of range", ids[i]);
+    int32_t *tmp = nullptr;
+    PG_TRY(workspace_t(g, "subset_ids", (size_t)k + 1, &tmp));
+    if (k) PG_CHECK_HIP(hipMemcpyAsync(tmp, ids, 4 * k, hipMemcpyHostToDevice, g_stream));
+    PG_CHECK_HIP(hipMemsetAsync(s->bits, 0, 4 * ((g->n + 31) / 32 + 1), g_stream));
+    if (k) PG_LAUNCH(k_bits_from_ids_input, grid_for(k), tmp, k, g->perm, s->bits);
+    return subset_finish(s);
+}
+
+extern "C" int pg_subset_to_dense(pg_subset *s, uint8_t *flags) {
+    PG_REQUIRE(s && flags, PG_ERR_ARG, "null argument");
+    pg_graph *g = s->g;
+    const int64_t nw = (g->n + 31) / 32;
+    uint32_t *ib = nullptr;
+    PG_TRY(workspace_t(g, "subset_ibits", (size_t)nw + 1, &ib));
+    PG_CHECK_HIP(hipMemsetAsync(ib, 0, 4 * (nw + 1), g_stream));
+    PG_LAUNCH(k_bits_to_input_bits, grid_for(g->n), s->bits, g->perm, g->n, ib);
+    std::vector<uint32_t> hb((size_t)nw);
+    PG_CHECK_HIP(hipMemcpyAsync(hb.data(), ib, 4 * nw, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    for (int64_t u = 0; u < g->n; u++) flags[u] = (hb[u >> 5] >> (u & 31)) & 1u;
+    return PG_OK;
+}
+
+extern "C" int pg_subset_to_sparse(pg_subset *s, int32_t *ids, int64_t cap, int64_t *k) {
+    PG_REQUIRE(s && k, PG_ERR_ARG, "null argument");
+    pg_graph *g = s->g;
+    const int64_t nw = (g->n + 31) / 32;
+    uint32_t *ib = nullptr;
+    int32_t *il = nullptr;
+    unsigned long long *cnt = nullptr;
+    PG_TRY(workspace_t(g, "subset_ibits", (size_t)nw + 1, &ib));
+    PG_TRY(workspace_t(g, "subset_ilist", (size_t)g->n + 1, &il));
+    PG_TRY(workspace_t(g, "subset_icnt", 2, &cnt));
+    PG_CHECK_HIP(hipMemsetAsync(ib, 0, 4 * (nw + 1), g_stream));
+    PG_LAUNCH(k_bits_to_input_bits, grid_for(g->n), s->bits, g->perm, g->n, ib);
+    PG_TRY(bits_to_list(g, ib, il, cnt));
+    unsigned long long c = 0;
+    PG_CHECK_HIP(hipMemcpyAsync(&c, cnt, 8, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    *k = (int64_t)c;
+    PG_REQUIRE(ids || c == 0 || cap == 0, PG_ERR_ARG, "null ids");
+    const int64_t ncopy = std::min<int64_t>((int64_t)c, cap);
+    if (ncopy > 0) {
+        PG_CHECK_HIP(hipMemcpy(ids, il, 4 * ncopy, hipMemcpyDeviceToHost));
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_subset_size(pg_subset *s, int64_t *count, int64_t *out_edges) {
+    PG_REQUIRE(s, PG_ERR_ARG, "null subset");
+    unsigned long long c[2];
+    PG_CHECK_HIP(hipMemcpyAsync(c, s->counters, 16, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    if (count) *count = (int64_t)c[0];
+    if (out_edges) *out_edges = (int64_t)c[1];
+    return PG_OK;
+}

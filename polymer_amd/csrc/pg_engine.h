@@ -1,0 +1,442 @@
+// pg_engine.h -- the device side of the edgeMap engine for CDNA4 (wave64).
+//
+//  * k_pull<Op>: dense pull (edgeMapDense / edgeMapDenseReduce, polymer.h:
+//    924-968, 1097-1168) over a degree-binned schedule: G lanes per vertex
+//    (G = 1..64) for light vertices, a 256-thread block per heavy vertex and
+//    several blocks (chunked, finalised by the last block to arrive, in chunk
+//    order) for the heaviest.  Neighbour lists are read coalesced by the lanes
+//    of a group; results are reduced with cross-lane shuffles.
+//  * k_push<Op>: push over a frontier (edgeMapSparseV3 polymer.h:1928-2003,
+//    edgeMapDenseForward polymer.h:970-1032): each block takes 256 sources,
+//    scans their out-degrees in LDS and spreads the block's edges evenly over
+//    its threads (one binary search in LDS per edge).
+//  * frontier helpers: bitmap test/set, wave-aggregated list append.
+//
+// An Op for k_push provides
+//     bool src_active(int s);     void push(int s, int d, int e);
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pg_internal.h"
+
+namespace pg {
+
+// ---- lane-level helpers ------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ float shfl_xor_t(float v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ int shfl_xor_t(int v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ unsigned shfl_xor_t(unsigned v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ unsigned long long shfl_xor_t(unsigned long long v, int m) {
+    return __shfl_xor(v, m, 64);
+}
+__device__ __forceinline__ double shfl_xor_t(double v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ float2 shfl_xor_t(float2 v, int m) {
+    return make_float2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+__device__ __forceinline__ float4 shfl_xor_t(float4 v, int m) {
+    return make_float4(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64), __shfl_xor(v.z, m, 64),
+                       __shfl_xor(v.w, m, 64));
+}
+
+template <int G, class Op, class Acc>
+__device__ __forceinline__ Acc group_reduce(const Op &op, Acc a) {
+#pragma unroll
+    for (int o = G / 2; o >= 1; o >>= 1) a = op.combine(a, shfl_xor_t(a, o));
+    return a;
+}
+
+template <int G>
+__device__ __forceinline__ bool group_any(bool p) {
+    unsigned long long b = __ballot(p);
+    if (G == 64) return b != 0ull;
+    const int base = lane_id() & ~(G - 1);
+    const unsigned long long mask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    return ((b >> base) & mask) != 0ull;
+}
+
+// atomics that bypass the non-coherent vector L1 (used for cross-block data)
+template <class T>
+__device__ __forceinline__ T load_agent(const T *p) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4/8-byte loads only");
+    if constexpr (sizeof(T) == 4) {
+        unsigned u = __hip_atomic_load(reinterpret_cast<const unsigned *>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_bit_cast(T, u);
+    } else {
+        unsigned long long u =
+            __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_bit_cast(T, u);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void store_agent(T *p, T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4/8-byte stores only");
+    if constexpr (sizeof(T) == 4) {
+        __hip_atomic_store(reinterpret_cast<unsigned *>(p), __builtin_bit_cast(unsigned, v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(p),
+                           __builtin_bit_cast(unsigned long long, v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---- frontier helpers ------------------------------------------------------------
+__device__ __forceinline__ bool bit_test(const uint32_t *bits, int v) {
+    return (bits[v >> 5] >> (v & 31)) & 1u;
+}
+// returns true if this call set the bit
+__device__ __forceinline__ bool bit_set_new(uint32_t *bits, int v) {
+    const uint32_t m = 1u << (v & 31);
+    return (atomicOr(&bits[v >> 5], m) & m) == 0u;
+}
+
+// Wave-aggregated append: the lanes with pred write val to list; one atomic
+// per wave on *counter.  Works under divergence: the ballot only sees the
+// lanes that reach it, and the base is read from the (active) leader lane.
+__device__ __forceinline__ void agg_append(int32_t *list, unsigned long long *counter, bool pred,
+                                           int32_t val) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int lane = lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) {
+        const unsigned long long below = mask & ((1ull << lane) - 1ull);
+        list[base + __popcll(below)] = val;
+    }
+}
+
+// ---- pull engine ------------------------------------------------------------------
+// An Op for k_pull provides
+//     using Acc;  using Ctx;  using Stat;
+//     static constexpr bool kEarlyExit, kStats;
+//     bool active(int v);               Ctx ctx(int v);
+//     Acc identity();                   Acc combine(Acc, Acc);
+//     Acc edge(int s, int e, const Ctx&);  bool found(Acc, const Ctx&);
+//     Stat finish(int v, Acc, const Ctx&);  void skip(int v);
+// finish() and skip() run once per vertex (one lane).  When kStats, the Stat
+// values are combined per block with Op::stat_reduce and published once per
+// block with Op::stat_commit(stat_out, value) (e.g. an atomic add or max).
+template <class T>
+__device__ __forceinline__ void store_partial(T *p, const T &v) {
+    if constexpr (sizeof(T) <= 8 && (sizeof(T) == 4 || sizeof(T) == 8)) {
+        store_agent(p, v);
+    } else {
+        static_assert(sizeof(T) % 4 == 0, "partials are whole words");
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(&v);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(p);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 4); i++) store_agent(&dst[i], src[i]);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T load_partial(const T *p) {
+    if constexpr (sizeof(T) == 4 || sizeof(T) == 8) {
+        return load_agent(p);
+    } else {
+        T v;
+        uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(p);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 4); i++) dst[i] = load_agent(&src[i]);
+        return v;
+    }
+}
+
+template <int G, class Op>
+__device__ __forceinline__ typename Op::Stat pull_group(const Op &op, const PullArgs &pa,
+                                                        const PullSeg &seg, int rb) {
+    using Acc = typename Op::Acc;
+    using Stat = typename Op::Stat;
+    const int t = threadIdx.x;
+    const int64_t pos = (int64_t)seg.v_begin + ((int64_t)rb * kBlock + t) / G;
+    if (pos >= seg.v_end) return Stat(0);  // uniform within the group
+    const int lane = t & (G - 1);
+    const int v = pa.vlist ? pa.vlist[pos] : (int)pos;
+    if (!op.active(v)) {
+        if (lane == 0) op.skip(v);
+        return Stat(0);
+    }
+    const typename Op::Ctx c = op.ctx(v);
+    const int beg = pa.off[v], end = pa.off[v + 1];
+    Acc acc = op.identity();
+    if constexpr (Op::kEarlyExit) {
+        for (int base = beg; base < end; base += G) {
+            const int e = base + lane;
+            if (e < end) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+            if (group_any<G>(op.found(acc, c))) break;
+        }
+    } else {
+        int e = beg + lane;
+        for (; e + 3 * G < end; e += 4 * G) {
+            const int s0 = pa.adj[e], s1 = pa.adj[e + G], s2 = pa.adj[e + 2 * G],
+                      s3 = pa.adj[e + 3 * G];
+            const Acc a0 = op.edge(s0, e, c), a1 = op.edge(s1, e + G, c),
+                      a2 = op.edge(s2, e + 2 * G, c), a3 = op.edge(s3, e + 3 * G, c);
+            acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
+        }
+        for (; e < end; e += G) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+    }
+    acc = group_reduce<G>(op, acc);
+    if (lane == 0) return op.finish(v, acc, c);
+    return Stat(0);
+}
+
+template <class Op>
+__device__ __forceinline__ typename Op::Acc block_reduce(const Op &op, typename Op::Acc a,
+                                                         typename Op::Acc *sm) {
+    a = group_reduce<64>(op, a);
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) sm[w] = a;
+    __syncthreads();
+    typename Op::Acc r = sm[0];
+#pragma unroll
+    for (int i = 1; i < kBlock / 64; i++) r = op.combine(r, sm[i]);
+    __syncthreads();
+    return r;
+}
+
+template <class Op>
+__device__ __forceinline__ typename Op::Stat pull_block(const Op &op, const PullArgs &pa,
+                                                        const PullSeg &seg, int rb) {
+    using Acc = typename Op::Acc;
+    using Stat = typename Op::Stat;
+    __shared__ Acc sm[kBlock / 64];
+    __shared__ int sflag;
+    const int64_t pos = (int64_t)seg.v_begin + rb;
+    const int v = pa.vlist ? pa.vlist[pos] : (int)pos;
+    if (!op.active(v)) {  // uniform over the block
+        if (threadIdx.x == 0) op.skip(v);
+        return Stat(0);
+    }
+    const typename Op::Ctx c = op.ctx(v);
+    const int beg = pa.off[v], end = pa.off[v + 1];
+    Acc acc = op.identity();
+    if constexpr (Op::kEarlyExit) {
+        if (threadIdx.x == 0) sflag = 0;
+        __syncthreads();
+        for (int base = beg; base < end; base += kBlock) {
+            const int e = base + (int)threadIdx.x;
+            if (e < end) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+            if (group_any<64>(op.found(acc, c)) && lane_id() == 0) sflag = 1;
+            __syncthreads();
+            const int f = sflag;
+            __syncthreads();
+            if (f) break;
+        }
+    } else {
+        int e = beg + (int)threadIdx.x;
+        for (; e + 3 * kBlock < end; e += 4 * kBlock) {
+            const int s0 = pa.adj[e], s1 = pa.adj[e + kBlock], s2 = pa.adj[e + 2 * kBlock],
+                      s3 = pa.adj[e + 3 * kBlock];
+            const Acc a0 = op.edge(s0, e, c), a1 = op.edge(s1, e + kBlock, c),
+                      a2 = op.edge(s2, e + 2 * kBlock, c), a3 = op.edge(s3, e + 3 * kBlock, c);
+            acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
+        }
+        for (; e < end; e += kBlock) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+    }
+    acc = block_reduce(op, acc, sm);
+    if (threadIdx.x == 0) return op.finish(v, acc, c);
+    return Stat(0);
+}
+
+template <class Op>
+__device__ __forceinline__ typename Op::Stat pull_multi(const Op &op, const PullArgs &pa,
+                                                        const PullSeg &seg, int rb) {
+    using Acc = typename Op::Acc;
+    using Stat = typename Op::Stat;
+    static_assert(sizeof(Acc) <= 32, "multi-block partials hold 32 bytes");
+    __shared__ Acc sm[kBlock / 64];
+    __shared__ int slast;
+    const MultiItem it = pa.multi[rb];
+    const int v = pa.vlist ? pa.vlist[it.pos] : it.pos;
+    if (!op.active(v)) {
+        if (threadIdx.x == 0 && it.chunk == 0) op.skip(v);
+        return Stat(0);
+    }
+    const typename Op::Ctx c = op.ctx(v);
+    const int vbeg = pa.off[v], vend = pa.off[v + 1];
+    const int beg = vbeg + it.chunk * pa.chunk;
+    const int end = min(vend, beg + pa.chunk);
+    Acc acc = op.identity();
+    int e = beg + (int)threadIdx.x;
+    for (; e + 3 * kBlock < end; e += 4 * kBlock) {
+        const int s0 = pa.adj[e], s1 = pa.adj[e + kBlock], s2 = pa.adj[e + 2 * kBlock],
+                  s3 = pa.adj[e + 3 * kBlock];
+        const Acc a0 = op.edge(s0, e, c), a1 = op.edge(s1, e + kBlock, c),
+                  a2 = op.edge(s2, e + 2 * kBlock, c), a3 = op.edge(s3, e + 3 * kBlock, c);
+        acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
+    }
+    for (; e < end; e += kBlock) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+    acc = block_reduce(op, acc, sm);
+    Acc *part = reinterpret_cast<Acc *>(pa.partial);
+    if (threadIdx.x == 0) {
+        store_partial(&part[rb], acc);
+        __threadfence();
+        const unsigned prev = atomicAdd(&pa.tickets[it.slot], 1u);
+        slast = (prev == (unsigned)(it.nchunks - 1));
+    }
+    __syncthreads();
+    Stat st = Stat(0);
+    if (slast && threadIdx.x == 0) {
+        __threadfence();
+        Acc tot = op.identity();
+        for (int k = 0; k < it.nchunks; k++) tot = op.combine(tot, load_partial(&part[it.first + k]));
+        pa.tickets[it.slot] = 0u;  // ready for the next launch
+        st = op.finish(v, tot, c);
+    }
+    return st;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_pull(Op op, PullArgs pa,
+                                                 typename Op::Stat *stat_out) {
+    using Stat = typename Op::Stat;
+    const int b = blockIdx.x;
+    int si = 0;
+    while (si + 1 < pa.nseg && b >= pa.seg[si + 1].block_begin) ++si;
+    const PullSeg seg = pa.seg[si];
+    const int rb = b - seg.block_begin;
+    Stat st;
+    switch (seg.g) {
+        case kModeMulti: st = pull_multi(op, pa, seg, rb); break;
+        case kModeBlock: st = pull_block(op, pa, seg, rb); break;
+        case 64: st = pull_group<64>(op, pa, seg, rb); break;
+        case 32: st = pull_group<32>(op, pa, seg, rb); break;
+        case 16: st = pull_group<16>(op, pa, seg, rb); break;
+        case 8: st = pull_group<8>(op, pa, seg, rb); break;
+        case 4: st = pull_group<4>(op, pa, seg, rb); break;
+        case 2: st = pull_group<2>(op, pa, seg, rb); break;
+        default: st = pull_group<1>(op, pa, seg, rb); break;
+    }
+    if constexpr (Op::kStats) {
+        __shared__ Stat ssm[kBlock / 64];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) st = Op::stat_reduce(st, shfl_xor_t(st, o));
+        if (lane_id() == 0) ssm[threadIdx.x >> 6] = st;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Stat tot = ssm[0];
+#pragma unroll
+            for (int i = 1; i < kBlock / 64; i++) tot = Op::stat_reduce(tot, ssm[i]);
+            if (tot != Stat(0)) Op::stat_commit(stat_out, tot);
+        }
+    }
+}
+
+template <class Op>
+inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *stat_out = nullptr) {
+    if (s.nblocks == 0) return PG_OK;
+    hipLaunchKernelGGL(k_pull<Op>, dim3(s.nblocks), dim3(kBlock), 0, stream(), op, s.args,
+                       stat_out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("k_pull launch failed: %s", hipGetErrorString(e));
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
+// ---- push engine ------------------------------------------------------------------
+// kDense: sources are the vertex ids [0, count) filtered by op.src_active;
+// otherwise the sources are list[0, count).
+template <class Op, bool kDense>
+__global__ __launch_bounds__(kBlock) void k_push(Op op, const int32_t *__restrict__ off,
+                                                 const int32_t *__restrict__ adj,
+                                                 const int32_t *__restrict__ list, int64_t count) {
+    __shared__ int32_t s_src[kBlock];
+    __shared__ int32_t s_beg[kBlock];
+    __shared__ uint32_t s_scan[kBlock + 1];
+    const int t = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + t;
+    int32_t s = -1;
+    uint32_t deg = 0;
+    int32_t beg = 0;
+    if (i < count) {
+        s = kDense ? (int32_t)i : list[i];
+        if (op.src_active(s)) {
+            beg = off[s];
+            deg = (uint32_t)(off[s + 1] - beg);
+        }
+    }
+    s_src[t] = s;
+    s_beg[t] = beg;
+    // block-wide inclusive scan of deg (Hillis-Steele over LDS)
+    s_scan[t + 1] = deg;
+    if (t == 0) s_scan[0] = 0;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        uint32_t x = (t + 1 - o >= 1) ? s_scan[t + 1 - o] : 0u;
+        __syncthreads();
+        s_scan[t + 1] += x;
+        __syncthreads();
+    }
+    const uint32_t total = s_scan[kBlock];
+    for (uint32_t k = t; k < total; k += kBlock) {
+        // find j with s_scan[j] <= k < s_scan[j+1]
+        int lo = 0, hi = kBlock - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_scan[mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        const int src = s_src[lo];
+        const int e = s_beg[lo] + (int)(k - s_scan[lo]);
+        op.push(src, adj[e], e);
+    }
+}
+
+template <class Op, bool kDense>
+inline int launch_push(const Op &op, const int32_t *off, const int32_t *adj, const int32_t *list,
+                       int64_t count) {
+    if (count <= 0) return PG_OK;
+    const int64_t nb = (count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((k_push<Op, kDense>), dim3((unsigned)nb), dim3(kBlock), 0, stream(), op,
+                       off, adj, list, count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("k_push launch failed: %s", hipGetErrorString(e));
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
+// ---- small element-wise kernels ------------------------------------------------------
+template <class T>
+__global__ void k_fill(T *p, T v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <class T>
+inline int fill(T *p, T v, int64_t n) {
+    if (n <= 0) return PG_OK;
+    int64_t nb = (n + kBlock - 1) / kBlock;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)nb), dim3(kBlock), 0, stream(), p, v, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("fill launch failed: %s", hipGetErrorString(e));
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
+inline unsigned grid_for(int64_t n, int64_t cap = 65536) {
+    int64_t nb = (n + kBlock - 1) / kBlock;
+    if (nb < 1) nb = 1;
+    if (nb > cap) nb = cap;
+    return (unsigned)nb;
+}
+
+}  // namespace pg

@@ -1,0 +1,203 @@
+// pg_internal.h -- host-side structures of libpolymer_gpu.so shared by the
+// translation units (graph build, applications, batched forms, adjoint).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "polymer_gpu.h"
+
+namespace pg {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(const char *fmt, ...);
+
+#define PG_CHECK_HIP(call)                                                       \
+    do {                                                                         \
+        hipError_t _e = (call);                                                  \
+        if (_e != hipSuccess) {                                                  \
+            ::pg::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                            __FILE__, __LINE__);                                 \
+            return _e == hipErrorOutOfMemory ? PG_ERR_OOM : PG_ERR_HIP;          \
+        }                                                                        \
+    } while (0)
+
+#define PG_TRY(call)                      \
+    do {                                  \
+        int _r = (call);                  \
+        if (_r != PG_OK) return _r;       \
+    } while (0)
+
+#define PG_REQUIRE(cond, code, ...)              \
+    do {                                         \
+        if (!(cond)) {                           \
+            ::pg::set_error(__VA_ARGS__);        \
+            return (code);                       \
+        }                                        \
+    } while (0)
+
+hipStream_t stream();
+bool initialised();
+
+// ---- pull schedule ----------------------------------------------------------
+// Vertices are visited in an order where the pulled degree is non-increasing
+// (the in-degree relabelling makes this the identity for the CSC; a vertex
+// list gives the order for the CSR).  Contiguous runs with the same
+// lanes-per-vertex form segments; blocks of 256 threads are laid out segment
+// after segment.
+constexpr int kBlock = 256;
+constexpr int kMaxSegs = 12;
+constexpr int kModeMulti = 0;    // several blocks per vertex
+constexpr int kModeBlock = 256;  // one block per vertex
+
+struct PullSeg {
+    int32_t v_begin, v_end;  // positions in the visiting order
+    int32_t g;               // lanes per vertex: 1..64, kModeBlock, kModeMulti
+    int32_t block_begin;
+};
+
+struct MultiItem {  // one block of a multi-block vertex
+    int32_t pos;         // position in the visiting order
+    int32_t chunk;       // chunk index within the vertex
+    int32_t nchunks;
+    int32_t first;       // first multi-block index of this vertex
+    int32_t slot;        // ticket slot of this vertex
+    int32_t pad[3];
+};
+
+struct PullArgs {
+    const int32_t *off;    // offsets of the pulled direction
+    const int32_t *adj;    // neighbour ids of the pulled direction
+    const int32_t *vlist;  // visiting order (nullptr = identity)
+    const MultiItem *multi;
+    void *partial;         // 32 bytes per multi block
+    unsigned *tickets;     // one per multi vertex, zero between launches
+    int32_t chunk;
+    int32_t nseg;
+    PullSeg seg[kMaxSegs];
+};
+
+struct PullParams {
+    int items_per_lane = 4;
+    int64_t block_threshold = 4096;   // in-degree above which one block per vertex
+    int64_t multi_threshold = 32768;  // above which several blocks per vertex
+    int64_t chunk = 8192;             // edges per block for multi-block vertices
+};
+
+struct PullSchedule {
+    PullArgs args{};
+    int nblocks = 0;
+    PullParams params;
+    MultiItem *d_multi = nullptr;
+    void *d_partial = nullptr;
+    unsigned *d_tickets = nullptr;
+    bool built = false;
+};
+
+// ---- graph ---------------------------------------------------------------------
+struct pg_graph_impl;
+
+}  // namespace pg
+
+struct pg_graph {
+    int64_t n = 0, m = 0;
+    bool sym = false;
+    int w_dtype = PG_W_NONE;
+    int64_t max_in_deg = 0, max_out_deg = 0;
+    // device arrays, internal ids (relabelled by in-degree, descending)
+    int32_t *out_off = nullptr, *out_adj = nullptr;
+    int32_t *in_off = nullptr, *in_adj = nullptr;  // alias out_* when sym
+    int32_t *perm = nullptr;   // input id -> internal id
+    int32_t *iperm = nullptr;  // internal id -> input id
+    int32_t *out_eid = nullptr, *in_eid = nullptr;  // -> input CSR edge index
+    int32_t *out_w_i32 = nullptr, *in_w_i32 = nullptr;
+    float *out_w_f32 = nullptr, *in_w_f32 = nullptr;
+    int32_t *out_order = nullptr;  // internal ids by out-degree, descending
+    float *bp_pot = nullptr;       // 4 per edge, CSC order
+    // many-graph form: per internal vertex (1-damping)/n_g factor 1/n_g
+    float *inv_n = nullptr;
+    int num_graphs = 1;
+    // schedules
+    pg::PullSchedule in_sched;   // pull over CSC (in-edges)
+    pg::PullSchedule out_sched;  // pull over CSR (out-edges)
+    // cached device workspace
+    std::map<std::string, std::pair<void *, size_t>> ws;
+    // saved state of the last weighted PageRank forward (for the adjoint)
+    bool wpr_valid = false;
+    double wpr_damping = 0.85;
+    // allocations owned by the graph
+    std::vector<void *> owned;
+};
+
+struct pg_subset {
+    pg_graph *g = nullptr;
+    uint32_t *bits = nullptr;  // n/32 words
+    int32_t *list = nullptr;   // n entries
+    unsigned long long *counters = nullptr;  // [0]=count [1]=out-edges
+};
+
+namespace pg {
+
+// device allocation helpers (owned by the graph)
+int galloc(pg_graph *g, void **p, size_t bytes);
+template <class T>
+int galloc_t(pg_graph *g, T **p, size_t count) {
+    return galloc(g, reinterpret_cast<void **>(p), count * sizeof(T) + 16);
+}
+// named workspace, grown on demand, kept until the graph is destroyed
+int workspace(pg_graph *g, const char *name, size_t bytes, void **p);
+template <class T>
+int workspace_t(pg_graph *g, const char *name, size_t count, T **p) {
+    return workspace(g, name, count * sizeof(T) + 16, reinterpret_cast<void **>(p));
+}
+
+int build_schedule(pg_graph *g, bool in_direction);
+int ensure_out_schedule(pg_graph *g);
+
+// device-wide helpers implemented in pg_build.hip (hipcub)
+int exclusive_scan_u32(pg_graph *g, const uint32_t *in, uint32_t *out, int64_t n);
+
+// event timing on the library stream
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    int start();
+    int stop(double *ms);
+    ~Timer();
+};
+
+// per-kernel timing used with PG_PROFILE_KERNELS
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    int begin();
+    int end();
+    int total(double *ms, int *launches);
+    ~KernelTimer();
+};
+
+// frontier conversions (pg_build.hip)
+int bits_to_list(pg_graph *g, const uint32_t *bits, int32_t *list, unsigned long long *d_count);
+int list_to_bits(pg_graph *g, const int32_t *list, const unsigned long long *d_count,
+                 uint32_t *bits);
+int list_degree_sum(pg_graph *g, const int32_t *list, const unsigned long long *d_count,
+                    const int32_t *off, unsigned long long *d_out);
+int fill_bits_all(pg_graph *g, uint32_t *bits);
+
+// edge arrays between the caller's CSR order and the internal CSR/CSC order
+int edges_from_input_f32(pg_graph *g, const float *host, bool csc, float *dev);
+int edges_to_input_f32(pg_graph *g, const float *dev, bool csc, float *host);
+
+int create_from_host(int64_t n, int64_t m, const int64_t *off, const int32_t *adj, const void *w,
+                     int w_dtype, uint32_t flags, pg_graph **out);
+
+// copy an internal-order array to a host array in input order
+int to_input_order_i32(pg_graph *g, const int32_t *dev_internal, int32_t *host, bool map_ids);
+int to_input_order_f32(pg_graph *g, const float *dev_internal, float *host, int width);
+int from_input_order_f32(pg_graph *g, const float *host, float *dev_internal, int width);
+
+}  // namespace pg
