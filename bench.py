@@ -70,8 +70,32 @@ def bfs_extra(pa, scale=22, m=1 << 26, nsrc=8):
             "bfs_sources": len(srcs)}
 
 
+def profile_run(args):
+    """Short run for rocprofv3: a 1 GiB device-to-device copy (torch, the
+    counter calibration of known size: 1 GiB read + 1 GiB written), then one
+    warm-up and one measured PageRank of --iters iterations."""
+    import polymer_amd as pa
+    import torch
+
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log("copy kernel: 1073741824 bytes read, 1073741824 bytes written")
+    pa.init(0)
+    g = pa.Graph.rmat(args.scale, args.edges, seed=1)
+    g.pagerank(args.iters, 0.85, out=False)
+    _, st = g.pagerank(args.iters, 0.85, out=False, profile=True)
+    log("pagerank %d iterations: %.3f ms, kernel avg %.1f us, edges %d" %
+        (args.iters, st.ms, st.kernel_ms / max(st.kernel_launches, 1) * 1e3, st.edges))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--profile", action="store_true",
+                    help="short run to be wrapped by rocprofv3 (no JSON line)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -85,6 +109,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.profile:
+        profile_run(args)
+        return
 
     import polymer_amd as pa
 

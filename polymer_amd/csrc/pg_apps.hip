@@ -74,7 +74,10 @@ struct PrOp {
         contrib_n[v] = 0.f;
     }
     __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
-    __device__ static void stat_commit(Stat *p, Stat v) { atomicAdd(p, v); }
+    // spread over kStatSlots addresses so blocks do not serialise on one atomic
+    __device__ static void stat_commit(Stat *p, Stat v) {
+        atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
+    }
 };
 
 __global__ void k_pr_init(int64_t n, const int32_t *out_off, const float *inv_n, float one_over_n,
@@ -115,19 +118,22 @@ static int pr_buffers(pg_graph *g, int iters, PrBuffers *B) {
     PG_TRY(workspace_t(g, "pr_c1", n, &B->c1));
     PG_TRY(workspace_t(g, "pr_f0", n, &B->f0));
     PG_TRY(workspace_t(g, "pr_f1", n, &B->f1));
-    PG_TRY(workspace_t(g, "pr_stats", (size_t)iters + 1, &B->stats));
+    PG_TRY(workspace_t(g, "pr_stats", ((size_t)iters + 1) * kStatSlots, &B->stats));
     return PG_OK;
 }
 
-// Runs the iterations; on return *result points at the buffer holding p_curr.
-int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stats *st,
-                 float **result) {
+// Enqueues the whole solve on the library stream (no host synchronisation, no
+// allocation once the workspace exists: safe inside a stream capture).  On return
+// *result points at the buffer that will hold p_curr.
+int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, float **result,
+                     unsigned long long **stats) {
     PrBuffers B;
     PG_TRY(pr_buffers(g, iters, &B));
     const int64_t n = g->n;
     PG_LAUNCH_K(k_pr_init, grid_for(n), n, g->out_off, g->inv_n, (float)(1.0 / (double)n), B.a,
                 B.b, B.f0, B.c0);
-    PG_LAUNCH_K(k_zero_u64, (unsigned)((iters + 1 + kBlock - 1) / kBlock), B.stats, iters + 1);
+    PG_LAUNCH_K(k_zero_u64, (unsigned)(((iters + 1) * kStatSlots + kBlock - 1) / kBlock), B.stats,
+                (iters + 1) * kStatSlots);
     PrOp op{};
     op.out_off = g->out_off;
     op.addc_v = nullptr;
@@ -140,10 +146,6 @@ int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stat
     }
     if (inv_n_addc)  // (1-d)/n_g per vertex
         PG_LAUNCH_K(k_scale, grid_for(n), g->inv_n, n, (float)(1.0 - damping), inv_n_addc);
-    KernelTimer kt;
-    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
-    Timer tm;
-    PG_TRY(tm.start());
     float *cur = B.a, *nxt = B.b, *c = B.c0, *cn = B.c1;
     uint8_t *F = B.f0, *Fn = B.f1;
     for (int t = 0; t < iters; t++) {
@@ -153,23 +155,40 @@ int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stat
         op.contrib_n = cn;
         op.cur = cur;
         op.nxt = nxt;
-        PG_TRY(kt.begin());
-        PG_TRY(launch_pull(op, g->in_sched, &B.stats[t]));
-        PG_TRY(kt.end());
+        if (kt) PG_TRY(kt->begin());
+        PG_TRY(launch_pull(op, g->in_sched, &B.stats[(size_t)t * kStatSlots]));
+        if (kt) PG_TRY(kt->end());
         std::swap(cur, nxt);
         std::swap(F, Fn);
         std::swap(c, cn);
     }
+    *result = cur;
+    if (stats) *stats = B.stats;
+    return PG_OK;
+}
+
+// Runs the iterations; on return *result points at the buffer holding p_curr.
+int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stats *st,
+                 float **result) {
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    unsigned long long *stats = nullptr;
+    PG_TRY(pagerank_enqueue(g, iters, damping, &kt, result, &stats));
     double ms = 0;
     PG_TRY(tm.stop(&ms));
-    *result = cur;
+    PrBuffers B;
+    B.stats = stats;
     if (st) {
-        std::vector<unsigned long long> hs((size_t)iters + 1);
-        PG_CHECK_HIP(hipMemcpy(hs.data(), B.stats, 8 * (iters + 1), hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> hs(((size_t)iters + 1) * kStatSlots), he((size_t)iters + 1);
+        PG_CHECK_HIP(hipMemcpy(hs.data(), B.stats, 8 * hs.size(), hipMemcpyDeviceToHost));
+        for (int t = 0; t <= iters; t++)
+            for (int k = 0; k < kStatSlots; k++) he[t] += hs[(size_t)t * kStatSlots + k];
         st->rounds = iters;
         st->dense_rounds = iters;
         int64_t e = iters > 0 ? g->m : 0;  // F_1 = V
-        for (int t = 0; t + 1 < iters; t++) e += (int64_t)hs[t];
+        for (int t = 0; t + 1 < iters; t++) e += (int64_t)he[t];
         st->edges = e;
         st->ms = ms;
         PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));

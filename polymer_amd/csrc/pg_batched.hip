@@ -1,25 +1,720 @@
-// pg_batched.hip -- batched forms (filled in below).
+// pg_batched.hip -- batched forms: k sources carried together, many graphs in one.
+//
+//  * multi-source BFS: one 64-bit frontier mask per vertex (bit c = source c),
+//    pulled with the degree-binned engine (OR of in-neighbour masks, early exit
+//    once every source has reached the vertex);
+//  * multi-source Bellman-Ford and personalized PageRank: k-wide value rows per
+//    vertex ([n][64]), one wavefront per (vertex, edge chunk) with lane c
+//    carrying column c, so every gathered row is one coalesced 256-byte read;
+//  * many graphs: one device graph, per-vertex (1-d)/n_g, the iterations of
+//    numa-PageRank.C captured in a hipGraph and replayed as one launch.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "pg_engine.h"
 #include "pg_internal.h"
 
-extern "C" int pg_msbfs(pg_graph *, const int64_t *, int, int32_t *, uint32_t, pg_stats *) {
-    return PG_ERR_UNSUPPORTED;
+namespace pg {
+
+int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stats *st,
+                 float **result);
+int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, float **result,
+                     unsigned long long **stats);
+
+#define PG_LAUNCH_B(kern, grid, ...)                                                  \
+    do {                                                                              \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream(), __VA_ARGS__); \
+        PG_CHECK_HIP(hipGetLastError());                                              \
+    } while (0)
+
+constexpr int kRow = 64;  // columns per row = lanes per wavefront
+
+static int map_sources(pg_graph *g, const int64_t *src, int k, std::vector<int32_t> *out) {
+    PG_REQUIRE(src && k >= 1 && k <= kRow, PG_ERR_ARG, "need 1..64 sources, got %d", k);
+    std::vector<int32_t> in((size_t)k);
+    for (int c = 0; c < k; c++) {
+        PG_REQUIRE(src[c] >= 0 && src[c] < g->n, PG_ERR_ARG, "source %lld out of range",
+                   (long long)src[c]);
+        in[c] = (int32_t)src[c];
+    }
+    // perm lookups on the host: one small copy per source
+    out->resize((size_t)k);
+    for (int c = 0; c < k; c++)
+        PG_CHECK_HIP(hipMemcpy(&(*out)[c], g->perm + in[c], 4, hipMemcpyDeviceToHost));
+    return PG_OK;
 }
-extern "C" int pg_mssssp(pg_graph *, const int64_t *, int, int32_t *, uint32_t, pg_stats *) {
-    return PG_ERR_UNSUPPORTED;
+
+// ======================================================================================
+// Row engine: item w (one wavefront) = a vertex, or one chunk of a heavy vertex.
+// Heavy vertices are the in-degree prefix [0, H) of the relabelled order; their chunks
+// write partial rows that the last chunk to finish combines in chunk order.
+// ======================================================================================
+struct RowPlan {
+    int32_t H = 0;          // heavy vertices
+    int32_t nheavy = 0;     // heavy items
+    int64_t nitems = 0;
+    int32_t chunk = 1024;
+    MultiItem *items = nullptr;  // device, nheavy entries (pos = vertex)
+    unsigned *tickets = nullptr;
+    void *partial = nullptr;     // nheavy * 64 * 4 bytes
+};
+
+static int row_plan(pg_graph *g, RowPlan *P) {
+    const char *key = "rowplan_items";
+    auto it = g->ws.find(key);
+    const int32_t CH = 1024;
+    P->chunk = CH;
+    // heavy prefix: in-degree > CH (degrees non-increasing in the internal order)
+    int64_t lo = 0, hi = g->n;
+    std::vector<int32_t> off2(2);
+    while (lo < hi) {  // first vertex with in-degree <= CH
+        const int64_t mid = (lo + hi) / 2;
+        PG_CHECK_HIP(hipMemcpy(off2.data(), g->in_off + mid, 8, hipMemcpyDeviceToHost));
+        if (off2[1] - off2[0] > CH) lo = mid + 1; else hi = mid;
+    }
+    P->H = (int32_t)lo;
+    std::vector<int32_t> offs((size_t)P->H + 1);
+    if (P->H > 0)
+        PG_CHECK_HIP(hipMemcpy(offs.data(), g->in_off, 4 * (P->H + 1), hipMemcpyDeviceToHost));
+    std::vector<MultiItem> items;
+    for (int32_t v = 0; v < P->H; v++) {
+        const int32_t nch = (offs[v + 1] - offs[v] + CH - 1) / CH;
+        const int32_t first = (int32_t)items.size();
+        for (int32_t c = 0; c < nch; c++) items.push_back(MultiItem{v, c, nch, first, v, {0, 0, 0}});
+    }
+    P->nheavy = (int32_t)items.size();
+    P->nitems = (int64_t)P->nheavy + (g->n - P->H);
+    const bool have = it != g->ws.end();
+    PG_TRY(workspace_t(g, key, items.size() + 1, &P->items));
+    PG_TRY(workspace_t(g, "rowplan_tickets", (size_t)P->H + 1, &P->tickets));
+    PG_TRY(workspace(g, "rowplan_partial", (size_t)P->nheavy * kRow * 4 + 64, &P->partial));
+    if (!have) {
+        if (!items.empty())
+            PG_CHECK_HIP(hipMemcpy(P->items, items.data(), sizeof(MultiItem) * items.size(),
+                                   hipMemcpyHostToDevice));
+        PG_CHECK_HIP(hipMemset(P->tickets, 0, 4 * ((size_t)P->H + 1)));
+    }
+    return PG_OK;
 }
-extern "C" int pg_ppr_batched(pg_graph *, const int64_t *, int, int, double, float *, uint32_t,
-                              pg_stats *) {
-    return PG_ERR_UNSUPPORTED;
+
+// RowOp: using T; T identity(); T combine(T,T);
+//        bool src_live(int s) (wave-uniform); T edge(int s, int e, int lane);
+//        Stat finish(int v, T acc, int lane) (all 64 lanes, convergent).
+template <class RowOp>
+__global__ __launch_bounds__(kBlock) void k_rows(RowOp op, const int32_t *__restrict__ off,
+                                                 const int32_t *__restrict__ adj, RowPlan P,
+                                                 unsigned long long *stat) {
+    using T = typename RowOp::T;
+    const int64_t w = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kRow;
+    const int lane = threadIdx.x & (kRow - 1);
+    if (w >= P.nitems) return;  // uniform per wavefront
+    int v, beg, end;
+    MultiItem it{};
+    const bool heavy = w < P.nheavy;
+    if (heavy) {
+        it = P.items[w];
+        v = it.pos;
+        beg = off[v] + it.chunk * P.chunk;
+        end = min(off[v + 1], beg + P.chunk);
+    } else {
+        v = P.H + (int)(w - P.nheavy);
+        beg = off[v];
+        end = off[v + 1];
+    }
+    if (!op.vertex_live(v)) {
+        if (!heavy || it.chunk == 0) op.skip(v, lane);
+        return;
+    }
+    T acc = op.identity();
+    int e = beg;
+    for (; e + 3 < end; e += 4) {
+        const int s0 = ld_stream(adj + e), s1 = ld_stream(adj + e + 1), s2 = ld_stream(adj + e + 2),
+                  s3 = ld_stream(adj + e + 3);
+        const T a0 = op.edge(s0, e, lane), a1 = op.edge(s1, e + 1, lane),
+                a2 = op.edge(s2, e + 2, lane), a3 = op.edge(s3, e + 3, lane);
+        acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
+    }
+    for (; e < end; e++) acc = op.combine(acc, op.edge(ld_stream(adj + e), e, lane));
+    unsigned long long st = 0;
+    if (heavy && it.nchunks > 1) {
+        T *part = reinterpret_cast<T *>(P.partial);
+        store_agent(&part[(int64_t)w * kRow + lane], acc);
+        __threadfence();
+        unsigned prev = 0;
+        if (lane == 0) prev = atomicAdd(&P.tickets[it.slot], 1u);
+        prev = __shfl(prev, 0, 64);
+        if (prev != (unsigned)(it.nchunks - 1)) return;
+        __threadfence();
+        T tot = op.identity();
+        for (int c = 0; c < it.nchunks; c++)
+            tot = op.combine(tot, load_agent(&part[(int64_t)(it.first + c) * kRow + lane]));
+        if (lane == 0) P.tickets[it.slot] = 0u;
+        st = op.finish(v, tot, lane);
+    } else {
+        st = op.finish(v, acc, lane);
+    }
+    if (RowOp::kStats && lane == 0 && st) atomicAdd(&stat[blockIdx.x & (kStatSlots - 1)], st);
 }
-extern "C" int pg_ppr(pg_graph *, int64_t, int, double, float *, uint32_t, pg_stats *) {
-    return PG_ERR_UNSUPPORTED;
+
+template <class RowOp>
+static int launch_rows(const RowOp &op, pg_graph *g, const RowPlan &P, unsigned long long *stat) {
+    const int64_t threads = P.nitems * kRow;
+    const int64_t nb = (threads + kBlock - 1) / kBlock;
+    if (nb == 0) return PG_OK;
+    hipLaunchKernelGGL(k_rows<RowOp>, dim3((unsigned)nb), dim3(kBlock), 0, stream(), op, g->in_off,
+                       g->in_adj, P, stat);
+    PG_CHECK_HIP(hipGetLastError());
+    return PG_OK;
 }
-extern "C" int pg_graph_multi(int, const int64_t *, int64_t, const int64_t *, const int32_t *,
-                              pg_graph **) {
-    return PG_ERR_UNSUPPORTED;
+
+// ---- personalized PageRank, k-wide rows ---------------------------------------------
+struct PprRowOp {
+    using T = float;
+    static constexpr bool kStats = false;
+    const float *C;  // contributions r/outdeg, [n][64]
+    float *Cn;
+    float *R;        // ranks, [n][64]
+    const int32_t *out_off;
+    const int32_t *src;  // internal source of each column (-1 unused)
+    float damp, tele;
+    __device__ bool vertex_live(int) const { return true; }
+    __device__ void skip(int, int) const {}
+    __device__ float identity() const { return 0.f; }
+    __device__ float combine(float a, float b) const { return a + b; }
+    __device__ float edge(int s, int, int lane) const { return C[(int64_t)s * kRow + lane]; }
+    __device__ unsigned long long finish(int v, float sum, int lane) const {
+        const float r = damp * sum + (src[lane] == v ? tele : 0.f);
+        const int od = out_off[v + 1] - out_off[v];
+        R[(int64_t)v * kRow + lane] = r;
+        Cn[(int64_t)v * kRow + lane] = od > 0 ? r / (float)od : 0.f;
+        return 0ull;
+    }
+};
+
+__global__ void k_ppr_rows_init(int64_t n, const int32_t *out_off, const int32_t *src, float *C) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * kRow;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = (int)(i / kRow), c = (int)(i % kRow);
+        const int od = out_off[v + 1] - out_off[v];
+        C[i] = (src[c] == v && od > 0) ? 1.f / (float)od : 0.f;  // r_0 = e_src
+    }
 }
-extern "C" int pg_pagerank_multigraph(pg_graph *, int, double, float *, uint32_t, pg_stats *) {
-    return PG_ERR_UNSUPPORTED;
+
+template <class T>
+__global__ void k_rows_to_input(const T *R, const int32_t *perm, int64_t n, int k, T *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * k;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = i / n, u = i % n;
+        out[i] = R[(int64_t)perm[u] * kRow + c];
+    }
 }
-extern "C" int pg_graph_autotune(pg_graph *, int32_t *, int64_t *) { return PG_ERR_UNSUPPORTED; }
+
+template <class T>
+static int rows_download(pg_graph *g, const T *R, int k, T *host) {
+    if (!host) return PG_OK;
+    T *tmp = nullptr;
+    PG_TRY(workspace_t(g, "rows_out", (size_t)g->n * k, &tmp));
+    PG_LAUNCH_B(k_rows_to_input<T>, grid_for(g->n * k), R, g->perm, g->n, k, tmp);
+    PG_CHECK_HIP(hipMemcpyAsync(host, tmp, sizeof(T) * g->n * k, hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    return PG_OK;
+}
+
+// ---- plain personalized PageRank (one vector, degree-binned pull) -----------------------
+struct PprOp {
+    using Acc = float;
+    using Ctx = int;
+    using Stat = unsigned long long;
+    static constexpr bool kEarlyExit = false;
+    static constexpr bool kStats = false;
+    const float *contrib;
+    float *contrib_n;
+    float *R;
+    const int32_t *out_off;
+    int32_t src;
+    float damp, tele;
+    __device__ bool active(int) const { return true; }
+    __device__ int ctx(int v) const { return v; }
+    __device__ float identity() const { return 0.f; }
+    __device__ float combine(float a, float b) const { return a + b; }
+    __device__ float edge(int s, int, const int &) const { return contrib[s]; }
+    __device__ bool found(float, const int &) const { return false; }
+    __device__ Stat finish(int v, float sum, const int &) const {
+        const float r = damp * sum + (v == src ? tele : 0.f);
+        const int od = out_off[v + 1] - out_off[v];
+        R[v] = r;
+        contrib_n[v] = od > 0 ? r / (float)od : 0.f;
+        return 0ull;
+    }
+    __device__ void skip(int) const {}
+};
+
+__global__ void k_ppr_init(int64_t n, const int32_t *out_off, int32_t src, float *C) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int od = out_off[v + 1] - out_off[v];
+        C[v] = (v == src && od > 0) ? 1.f / (float)od : 0.f;
+    }
+}
+
+// ---- multi-source BFS: 64-bit masks ---------------------------------------------------------
+struct MsBfsOp {
+    using Acc = unsigned long long;
+    using Ctx = unsigned long long;  // visited mask of the vertex
+    using Stat = unsigned long long;
+    static constexpr bool kEarlyExit = true;
+    static constexpr bool kStats = true;
+    const unsigned long long *F;
+    unsigned long long *Fn;
+    unsigned long long *vis;
+    int32_t *level;  // [k][n]
+    int64_t n;
+    unsigned long long full;
+    int32_t round;
+    __device__ bool active(int v) const { return vis[v] != full; }
+    __device__ unsigned long long ctx(int v) const { return vis[v]; }
+    __device__ Acc identity() const { return 0ull; }
+    __device__ Acc combine(Acc a, Acc b) const { return a | b; }
+    __device__ Acc edge(int s, int, const Ctx &) const { return F[s]; }
+    __device__ bool found(Acc a, const Ctx &c) const { return (a | c) == full; }
+    __device__ Stat finish(int v, Acc a, const Ctx &c) const {
+        const unsigned long long nw = a & ~c & full;
+        Fn[v] = nw;
+        if (nw) {
+            vis[v] = c | nw;
+            unsigned long long x = nw;
+            while (x) {
+                const int b = __ffsll((long long)x) - 1;
+                x &= x - 1;
+                level[(int64_t)b * n + v] = round;
+            }
+        }
+        return nw ? 1ull : 0ull;
+    }
+    __device__ void skip(int v) const { Fn[v] = 0ull; }
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
+    __device__ static void stat_commit(Stat *p, Stat v) {
+        atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
+    }
+};
+
+__global__ void k_msbfs_seed(const int32_t *src, int k, int64_t n, unsigned long long *F,
+                             unsigned long long *vis, int32_t *level) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int c = 0; c < k; c++) {
+            const int s = src[c];
+            F[s] |= 1ull << c;
+            vis[s] |= 1ull << c;
+            level[(int64_t)c * n + s] = 0;
+        }
+    }
+}
+
+template <class T>
+__global__ void k_level_to_input(const T *lev, const int32_t *perm, int64_t n, int k, T *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * k;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = i / n, u = i % n;
+        out[i] = lev[c * n + perm[u]];
+    }
+}
+
+// ---- multi-source Bellman-Ford: [n][64] int32 distances ------------------------------------
+struct MsSsspOp {
+    using T = int;
+    static constexpr bool kStats = true;
+    const unsigned long long *F;  // sources improved last round
+    unsigned long long *Fn;
+    int32_t *dist;  // [n][64]
+    const int32_t *w;  // CSC order
+    __device__ bool vertex_live(int) const { return true; }
+    __device__ void skip(int, int) const {}
+    __device__ int identity() const { return 0x7FFFFFFF; }
+    __device__ int combine(int a, int b) const { return min(a, b); }
+    __device__ int edge(int s, int e, int lane) const {
+        const unsigned long long m = F[s];  // the same for every lane
+        if (!((m >> lane) & 1ull)) return 0x7FFFFFFF;
+        return dist[(int64_t)s * kRow + lane] + w[e];
+    }
+    __device__ unsigned long long finish(int v, int best, int lane) const {
+        int32_t *dv = &dist[(int64_t)v * kRow + lane];
+        const bool imp = best < *dv;
+        if (imp) *dv = best;
+        const unsigned long long m = __ballot(imp);
+        if (lane == 0) Fn[v] = m;
+        return (unsigned long long)__popcll(m);
+    }
+};
+
+__global__ void k_mssssp_seed(const int32_t *src, int k, unsigned long long *F, int32_t *dist) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int c = 0; c < k; c++) {
+            const int s = src[c];
+            F[s] |= 1ull << c;
+            dist[(int64_t)s * kRow + c] = 0;
+        }
+    }
+}
+
+static int sum_slots(const unsigned long long *d, unsigned long long *out) {
+    unsigned long long h[kStatSlots];
+    PG_CHECK_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    unsigned long long s = 0;
+    for (int i = 0; i < kStatSlots; i++) s += h[i];
+    *out = s;
+    return PG_OK;
+}
+
+// ---- many graphs ------------------------------------------------------------------------------
+__global__ void k_inv_n_input(const int64_t *vbase, int ng, int64_t n, float *inv) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = ng - 1;  // graph of u: vbase[g] <= u < vbase[g+1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (vbase[mid] <= u) lo = mid; else hi = mid - 1;
+        }
+        inv[u] = (float)(1.0 / (double)(vbase[lo + 1] - vbase[lo]));
+    }
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+// ====================================================================================
+// C ABI
+// ====================================================================================
+static int check_g(pg_graph *g) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    return PG_OK;
+}
+
+extern "C" int pg_ppr(pg_graph *g, int64_t src, int iters, double damping, float *rank,
+                      uint32_t flags, pg_stats *st) {
+    PG_TRY(check_g(g));
+    PG_REQUIRE(iters >= 0, PG_ERR_ARG, "iters < 0");
+    std::vector<int32_t> s;
+    PG_TRY(map_sources(g, &src, 1, &s));
+    const int64_t n = g->n;
+    float *c0 = nullptr, *c1 = nullptr, *R = nullptr;
+    PG_TRY(workspace_t(g, "ppr_c0", (size_t)n, &c0));
+    PG_TRY(workspace_t(g, "ppr_c1", (size_t)n, &c1));
+    PG_TRY(workspace_t(g, "ppr_r", (size_t)n, &R));
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    PG_LAUNCH_B(k_ppr_init, grid_for(n), n, g->out_off, s[0], c0);
+    if (iters == 0) {  // r_0 = e_src
+        PG_TRY(fill(R, 0.f, n));
+        hipLaunchKernelGGL(k_fill<float>, dim3(1), dim3(1), 0, stream(), R + s[0], 1.f, (int64_t)1);
+        PG_CHECK_HIP(hipGetLastError());
+    }
+    float *c = c0, *cn = c1;
+    for (int t = 0; t < iters; t++) {
+        PprOp op{c, cn, R, g->out_off, s[0], (float)damping, (float)(1.0 - damping)};
+        PG_TRY(kt.begin());
+        PG_TRY(launch_pull(op, g->in_sched));
+        PG_TRY(kt.end());
+        std::swap(c, cn);
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = iters;
+        st->dense_rounds = iters;
+        st->edges = (int64_t)iters * g->m;
+        st->active = (int64_t)iters * n;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    return to_input_order_f32(g, R, rank, 1);
+}
+
+extern "C" int pg_ppr_batched(pg_graph *g, const int64_t *src, int k, int iters, double damping,
+                              float *rank, uint32_t flags, pg_stats *st) {
+    PG_TRY(check_g(g));
+    PG_REQUIRE(iters >= 0, PG_ERR_ARG, "iters < 0");
+    std::vector<int32_t> s;
+    PG_TRY(map_sources(g, src, k, &s));
+    s.resize(kRow, -1);
+    const int64_t n = g->n;
+    RowPlan P;
+    PG_TRY(row_plan(g, &P));
+    float *C0 = nullptr, *C1 = nullptr, *R = nullptr;
+    int32_t *dsrc = nullptr;
+    PG_TRY(workspace_t(g, "pprb_c0", (size_t)n * kRow, &C0));
+    PG_TRY(workspace_t(g, "pprb_c1", (size_t)n * kRow, &C1));
+    PG_TRY(workspace_t(g, "pprb_r", (size_t)n * kRow, &R));
+    PG_TRY(workspace_t(g, "pprb_src", (size_t)kRow, &dsrc));
+    PG_CHECK_HIP(hipMemcpyAsync(dsrc, s.data(), 4 * kRow, hipMemcpyHostToDevice, stream()));
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    PG_LAUNCH_B(k_ppr_rows_init, grid_for(n * kRow), n, g->out_off, dsrc, C0);
+    if (iters == 0) PG_TRY(fill(R, 0.f, n * kRow));
+    float *C = C0, *Cn = C1;
+    for (int t = 0; t < iters; t++) {
+        PprRowOp op{C, Cn, R, g->out_off, dsrc, (float)damping, (float)(1.0 - damping)};
+        PG_TRY(kt.begin());
+        PG_TRY(launch_rows(op, g, P, nullptr));
+        PG_TRY(kt.end());
+        std::swap(C, Cn);
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = iters;
+        st->dense_rounds = iters;
+        st->edges = (int64_t)iters * g->m * k;
+        st->active = (int64_t)iters * n * k;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    if (iters == 0 && rank) {  // r_0 = e_src for each column
+        for (int c = 0; c < k; c++)
+            for (int64_t u = 0; u < n; u++) rank[(int64_t)c * n + u] = (u == src[c]) ? 1.f : 0.f;
+        return PG_OK;
+    }
+    return rows_download(g, R, k, rank);
+}
+
+extern "C" int pg_msbfs(pg_graph *g, const int64_t *src, int k, int32_t *level, uint32_t flags,
+                        pg_stats *st) {
+    PG_TRY(check_g(g));
+    std::vector<int32_t> s;
+    PG_TRY(map_sources(g, src, k, &s));
+    const int64_t n = g->n;
+    unsigned long long *F0 = nullptr, *F1 = nullptr, *vis = nullptr, *cnt = nullptr;
+    int32_t *lev = nullptr, *dsrc = nullptr;
+    PG_TRY(workspace_t(g, "msbfs_f0", (size_t)n, &F0));
+    PG_TRY(workspace_t(g, "msbfs_f1", (size_t)n, &F1));
+    PG_TRY(workspace_t(g, "msbfs_vis", (size_t)n, &vis));
+    PG_TRY(workspace_t(g, "msbfs_cnt", (size_t)kStatSlots, &cnt));
+    PG_TRY(workspace_t(g, "msbfs_level", (size_t)n * k, &lev));
+    PG_TRY(workspace_t(g, "msbfs_src", (size_t)k, &dsrc));
+    PG_CHECK_HIP(hipMemcpyAsync(dsrc, s.data(), 4 * k, hipMemcpyHostToDevice, stream()));
+    const unsigned long long full = (k == 64) ? ~0ull : ((1ull << k) - 1ull);
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    PG_TRY(fill(F0, 0ull, n));
+    PG_TRY(fill(vis, 0ull, n));
+    PG_TRY(fill(lev, -1, n * k));
+    PG_LAUNCH_B(k_msbfs_seed, 1, dsrc, k, n, F0, vis, lev);
+    unsigned long long *F = F0, *Fn = F1;
+    int rounds = 0;
+    int64_t active = k;
+    while (true) {
+        rounds++;
+        PG_TRY(fill(cnt, 0ull, kStatSlots));
+        MsBfsOp op{F, Fn, vis, lev, n, full, rounds};
+        PG_TRY(kt.begin());
+        PG_TRY(launch_pull(op, g->in_sched, cnt));
+        PG_TRY(kt.end());
+        unsigned long long c = 0;
+        PG_TRY(sum_slots(cnt, &c));
+        std::swap(F, Fn);
+        if (c == 0) break;
+        active += (int64_t)c;
+        if (rounds > n + 2) break;
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = rounds;
+        st->dense_rounds = rounds;
+        st->active = active;
+        st->edges = (int64_t)rounds * g->m;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    if (!level) return PG_OK;
+    int32_t *tmp = nullptr;
+    PG_TRY(workspace_t(g, "rows_out_i32", (size_t)n * k, &tmp));
+    PG_LAUNCH_B(k_level_to_input<int32_t>, grid_for(n * k), lev, g->perm, n, k, tmp);
+    PG_CHECK_HIP(hipMemcpyAsync(level, tmp, 4 * n * k, hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    return PG_OK;
+}
+
+extern "C" int pg_mssssp(pg_graph *g, const int64_t *src, int k, int32_t *dist, uint32_t flags,
+                         pg_stats *st) {
+    PG_TRY(check_g(g));
+    PG_REQUIRE(g->w_dtype == PG_W_I32, PG_ERR_STATE, "SSSP needs int32 weights");
+    std::vector<int32_t> s;
+    PG_TRY(map_sources(g, src, k, &s));
+    const int64_t n = g->n;
+    RowPlan P;
+    PG_TRY(row_plan(g, &P));
+    unsigned long long *F0 = nullptr, *F1 = nullptr, *cnt = nullptr;
+    int32_t *D = nullptr, *dsrc = nullptr;
+    PG_TRY(workspace_t(g, "mssssp_f0", (size_t)n, &F0));
+    PG_TRY(workspace_t(g, "mssssp_f1", (size_t)n, &F1));
+    PG_TRY(workspace_t(g, "mssssp_cnt", (size_t)kStatSlots, &cnt));
+    PG_TRY(workspace_t(g, "mssssp_dist", (size_t)n * kRow, &D));
+    PG_TRY(workspace_t(g, "mssssp_src", (size_t)k, &dsrc));
+    PG_CHECK_HIP(hipMemcpyAsync(dsrc, s.data(), 4 * k, hipMemcpyHostToDevice, stream()));
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    PG_TRY(fill(F0, 0ull, n));
+    PG_TRY(fill(D, (int32_t)1073741823, n * kRow));
+    PG_LAUNCH_B(k_mssssp_seed, 1, dsrc, k, F0, D);
+    unsigned long long *F = F0, *Fn = F1;
+    int rounds = 0;
+    int64_t active = k;
+    while (true) {
+        rounds++;
+        PG_TRY(fill(cnt, 0ull, kStatSlots));
+        MsSsspOp op{F, Fn, D, g->in_w_i32};
+        PG_TRY(kt.begin());
+        PG_TRY(launch_rows(op, g, P, cnt));
+        PG_TRY(kt.end());
+        unsigned long long c = 0;
+        PG_TRY(sum_slots(cnt, &c));
+        std::swap(F, Fn);
+        if (c == 0) break;
+        active += (int64_t)c;
+        if (rounds > 4 * n + 8) break;
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = rounds;
+        st->dense_rounds = rounds;
+        st->active = active;
+        st->edges = (int64_t)rounds * g->m;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    return rows_download(g, D, k, dist);
+}
+
+extern "C" int pg_graph_multi(int num_graphs, const int64_t *vbase, int64_t m, const int64_t *off,
+                              const int32_t *adj, pg_graph **out) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(out && vbase && off && num_graphs >= 1, PG_ERR_ARG, "bad argument");
+    *out = nullptr;
+    PG_REQUIRE(vbase[0] == 0, PG_ERR_ARG, "vbase[0] must be 0");
+    for (int i = 0; i < num_graphs; i++)
+        PG_REQUIRE(vbase[i + 1] > vbase[i], PG_ERR_ARG, "graph %d is empty", i);
+    const int64_t n = vbase[num_graphs];
+    // every edge stays inside its graph
+    for (int i = 0; i < num_graphs; i++)
+        for (int64_t u = vbase[i]; u < vbase[i + 1]; u++)
+            for (int64_t e = off[u]; e < off[u + 1]; e++)
+                PG_REQUIRE(adj[e] >= vbase[i] && adj[e] < vbase[i + 1], PG_ERR_ARG,
+                           "edge %lld leaves graph %d", (long long)e, i);
+    PG_TRY(create_from_host(n, m, off, adj, nullptr, PG_W_NONE, 0, out));
+    pg_graph *g = *out;
+    g->num_graphs = num_graphs;
+    int r = galloc_t(g, &g->inv_n, (size_t)n);
+    int64_t *dvb = nullptr;
+    float *tmp = nullptr;
+    if (r == PG_OK) r = workspace_t(g, "multi_vbase", (size_t)num_graphs + 1, &dvb);
+    if (r == PG_OK) r = workspace_t(g, "multi_inv", (size_t)n, &tmp);
+    if (r == PG_OK) {
+        std::vector<float> inv_host((size_t)n);
+        for (int i = 0; i < num_graphs; i++)
+            for (int64_t u = vbase[i]; u < vbase[i + 1]; u++)
+                inv_host[u] = (float)(1.0 / (double)(vbase[i + 1] - vbase[i]));
+        r = from_input_order_f32(g, inv_host.data(), g->inv_n, 1);
+    }
+    if (r != PG_OK) {
+        pg_graph_destroy(g);
+        *out = nullptr;
+    }
+    return r;
+}
+
+extern "C" int pg_pagerank_multigraph(pg_graph *g, int iters, double damping, float *rank,
+                                      uint32_t flags, pg_stats *st) {
+    PG_TRY(check_g(g));
+    PG_REQUIRE(g->inv_n, PG_ERR_STATE, "not a many-graph handle (pg_graph_multi)");
+    PG_REQUIRE(iters >= 0, PG_ERR_ARG, "iters < 0");
+    if (flags & PG_PROFILE_KERNELS) {  // launch by launch, timed per kernel
+        float *res = nullptr;
+        PG_TRY(pagerank_run(g, iters, damping, flags, st, &res));
+        return to_input_order_f32(g, res, rank, 1);
+    }
+    // Capture the whole solve (init + iterations) once, replay it as one launch.
+    auto key = std::string("pr_graph_") + std::to_string(iters) + "_" + std::to_string(damping);
+    auto it = g->graph_exec.find(key);
+    hipGraphExec_t exec = nullptr;
+    float *res = nullptr;
+    if (it == g->graph_exec.end()) {
+        // a dry run sizes the workspace (no allocation may happen inside the capture)
+        PG_TRY(pagerank_run(g, iters, damping, 0, nullptr, &res));
+        PG_CHECK_HIP(hipStreamSynchronize(stream()));
+        PG_CHECK_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+        int r = pagerank_enqueue(g, iters, damping, nullptr, &res, nullptr);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(stream(), &graph);
+        if (r != PG_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r;
+        }
+        PG_REQUIRE(e == hipSuccess, PG_ERR_HIP, "capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PG_REQUIRE(e == hipSuccess, PG_ERR_HIP, "instantiate failed: %s", hipGetErrorString(e));
+        g->graph_exec[key] = {exec, res};
+    } else {
+        exec = it->second.first;
+        res = it->second.second;
+    }
+    Timer tm;
+    PG_TRY(tm.start());
+    PG_CHECK_HIP(hipGraphLaunch(exec, stream()));
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = iters;
+        st->dense_rounds = iters;
+        st->ms = ms;
+    }
+    return to_input_order_f32(g, res, rank, 1);
+}
+
+extern "C" int pg_graph_autotune(pg_graph *g, int32_t *items_per_lane, int64_t *block_threshold) {
+    PG_TRY(check_g(g));
+    struct Cand {
+        int ipl;
+        int64_t bt, mt, ch;
+    };
+    const Cand cands[] = {{4, 4096, 32768, 8192},   {8, 4096, 32768, 8192},
+                          {16, 4096, 32768, 16384}, {8, 16384, 65536, 16384},
+                          {16, 16384, 65536, 16384}, {32, 16384, 65536, 16384},
+                          {16, 8192, 65536, 16384},  {4, 16384, 131072, 32768}};
+    const PullParams saved = g->in_sched.params;
+    double best = 1e30;
+    Cand bestc{saved.items_per_lane, saved.block_threshold, saved.multi_threshold, saved.chunk};
+    const int iters = 3;
+    float *res = nullptr;
+    for (const Cand &c : cands) {
+        PG_TRY(pg_graph_set_schedule(g, c.ipl, c.bt, c.mt, c.ch));
+        PG_TRY(pagerank_run(g, iters, 0.85, 0, nullptr, &res));  // warm
+        pg_stats s{};
+        double tot = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            PG_TRY(pagerank_run(g, iters, 0.85, PG_PROFILE_KERNELS, &s, &res));
+            tot += s.kernel_ms;
+        }
+        if (tot < best) {
+            best = tot;
+            bestc = c;
+        }
+    }
+    PG_TRY(pg_graph_set_schedule(g, bestc.ipl, bestc.bt, bestc.mt, bestc.ch));
+    if (items_per_lane) *items_per_lane = bestc.ipl;
+    if (block_threshold) *block_threshold = bestc.bt;
+    return PG_OK;
+}

@@ -79,6 +79,16 @@ int galloc(pg_graph *g, void **p, size_t bytes) {
     return PG_OK;
 }
 
+// Captured solves keep raw device pointers: any buffer they could reference that
+// moves (a workspace grows, a schedule is rebuilt) drops them; they are captured
+// again on next use.
+void drop_captures(pg_graph *g) {
+    if (g->graph_exec.empty()) return;
+    (void)hipStreamSynchronize(g_stream);
+    for (auto &kv : g->graph_exec) (void)hipGraphExecDestroy(kv.second.first);
+    g->graph_exec.clear();
+}
+
 int workspace(pg_graph *g, const char *name, size_t bytes, void **p) {
     auto it = g->ws.find(name);
     if (it != g->ws.end() && it->second.second >= bytes) {
@@ -86,6 +96,7 @@ int workspace(pg_graph *g, const char *name, size_t bytes, void **p) {
         return PG_OK;
     }
     if (it != g->ws.end()) {
+        drop_captures(g);
         PG_CHECK_HIP(hipStreamSynchronize(g_stream));
         PG_CHECK_HIP(hipFree(it->second.first));
         g->ws.erase(it);
@@ -640,6 +651,7 @@ static int free_schedule(PullSchedule &s) {
 
 int build_schedule(pg_graph *g, bool in_direction) {
     PullSchedule &s = in_direction ? g->in_sched : g->out_sched;
+    drop_captures(g);
     PG_CHECK_HIP(hipStreamSynchronize(g_stream));
     free_schedule(s);
     const int32_t *off = in_direction ? g->in_off : g->out_off;
@@ -844,6 +856,7 @@ static int new_graph(int64_t n, pg_graph **out) {
 
 static void delete_graph(pg_graph *g) {
     if (!g) return;
+    drop_captures(g);
     (void)hipStreamSynchronize(g_stream);
     free_schedule(g->in_sched);
     free_schedule(g->out_sched);

@@ -153,6 +153,41 @@ __device__ __forceinline__ T load_partial(const T *p) {
     }
 }
 
+// Neighbour ids are read once per launch: non-temporal loads keep them from
+// evicting the gathered per-vertex values from L1/L2.
+__device__ __forceinline__ int ld_stream(const int32_t *p) { return __builtin_nontemporal_load(p); }
+
+// acc combined with op.edge over e, e+S, e+2S, ... < end; eight loads in flight
+template <int S, class Op>
+__device__ __forceinline__ typename Op::Acc edge_range(const Op &op, const int32_t *__restrict__ adj,
+                                                       int e, int end, const typename Op::Ctx &c,
+                                                       typename Op::Acc acc) {
+    using Acc = typename Op::Acc;
+    for (; e + 7 * S < end; e += 8 * S) {
+        int s[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s[k] = ld_stream(adj + e + k * S);
+        Acc a[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = op.edge(s[k], e + k * S, c);
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] = op.combine(a[k], a[k + 4]);
+        a[0] = op.combine(a[0], a[2]);
+        a[1] = op.combine(a[1], a[3]);
+        acc = op.combine(acc, op.combine(a[0], a[1]));
+    }
+    if (e + 3 * S < end) {
+        const int s0 = ld_stream(adj + e), s1 = ld_stream(adj + e + S),
+                  s2 = ld_stream(adj + e + 2 * S), s3 = ld_stream(adj + e + 3 * S);
+        const Acc a0 = op.edge(s0, e, c), a1 = op.edge(s1, e + S, c), a2 = op.edge(s2, e + 2 * S, c),
+                  a3 = op.edge(s3, e + 3 * S, c);
+        acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
+        e += 4 * S;
+    }
+    for (; e < end; e += S) acc = op.combine(acc, op.edge(ld_stream(adj + e), e, c));
+    return acc;
+}
+
 template <int G, class Op>
 __device__ __forceinline__ typename Op::Stat pull_group(const Op &op, const PullArgs &pa,
                                                         const PullSeg &seg, int rb) {
@@ -177,15 +212,7 @@ __device__ __forceinline__ typename Op::Stat pull_group(const Op &op, const Pull
             if (group_any<G>(op.found(acc, c))) break;
         }
     } else {
-        int e = beg + lane;
-        for (; e + 3 * G < end; e += 4 * G) {
-            const int s0 = pa.adj[e], s1 = pa.adj[e + G], s2 = pa.adj[e + 2 * G],
-                      s3 = pa.adj[e + 3 * G];
-            const Acc a0 = op.edge(s0, e, c), a1 = op.edge(s1, e + G, c),
-                      a2 = op.edge(s2, e + 2 * G, c), a3 = op.edge(s3, e + 3 * G, c);
-            acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
-        }
-        for (; e < end; e += G) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+        acc = edge_range<G>(op, pa.adj, beg + lane, end, c, acc);
     }
     acc = group_reduce<G>(op, acc);
     if (lane == 0) return op.finish(v, acc, c);
@@ -235,15 +262,7 @@ __device__ __forceinline__ typename Op::Stat pull_block(const Op &op, const Pull
             if (f) break;
         }
     } else {
-        int e = beg + (int)threadIdx.x;
-        for (; e + 3 * kBlock < end; e += 4 * kBlock) {
-            const int s0 = pa.adj[e], s1 = pa.adj[e + kBlock], s2 = pa.adj[e + 2 * kBlock],
-                      s3 = pa.adj[e + 3 * kBlock];
-            const Acc a0 = op.edge(s0, e, c), a1 = op.edge(s1, e + kBlock, c),
-                      a2 = op.edge(s2, e + 2 * kBlock, c), a3 = op.edge(s3, e + 3 * kBlock, c);
-            acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
-        }
-        for (; e < end; e += kBlock) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+        acc = edge_range<kBlock>(op, pa.adj, beg + (int)threadIdx.x, end, c, acc);
     }
     acc = block_reduce(op, acc, sm);
     if (threadIdx.x == 0) return op.finish(v, acc, c);
@@ -269,15 +288,7 @@ __device__ __forceinline__ typename Op::Stat pull_multi(const Op &op, const Pull
     const int beg = vbeg + it.chunk * pa.chunk;
     const int end = min(vend, beg + pa.chunk);
     Acc acc = op.identity();
-    int e = beg + (int)threadIdx.x;
-    for (; e + 3 * kBlock < end; e += 4 * kBlock) {
-        const int s0 = pa.adj[e], s1 = pa.adj[e + kBlock], s2 = pa.adj[e + 2 * kBlock],
-                  s3 = pa.adj[e + 3 * kBlock];
-        const Acc a0 = op.edge(s0, e, c), a1 = op.edge(s1, e + kBlock, c),
-                  a2 = op.edge(s2, e + 2 * kBlock, c), a3 = op.edge(s3, e + 3 * kBlock, c);
-        acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
-    }
-    for (; e < end; e += kBlock) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+    acc = edge_range<kBlock>(op, pa.adj, beg + (int)threadIdx.x, end, c, acc);
     acc = block_reduce(op, acc, sm);
     Acc *part = reinterpret_cast<Acc *>(pa.partial);
     if (threadIdx.x == 0) {

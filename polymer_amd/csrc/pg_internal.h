@@ -52,6 +52,7 @@ bool initialised();
 // after segment.
 constexpr int kBlock = 256;
 constexpr int kMaxSegs = 12;
+constexpr int kStatSlots = 64;  // per-launch statistic counters (spread atomics)
 constexpr int kModeMulti = 0;    // several blocks per vertex
 constexpr int kModeBlock = 256;  // one block per vertex
 
@@ -127,6 +128,8 @@ struct pg_graph {
     pg::PullSchedule out_sched;  // pull over CSR (out-edges)
     // cached device workspace
     std::map<std::string, std::pair<void *, size_t>> ws;
+    // captured solves (hipGraph executables) and the buffer holding their result
+    std::map<std::string, std::pair<hipGraphExec_t, float *>> graph_exec;
     // saved state of the last weighted PageRank forward (for the adjoint)
     bool wpr_valid = false;
     double wpr_damping = 0.85;

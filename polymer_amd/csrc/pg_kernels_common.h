@@ -1,8 +1,8 @@
 // pg_kernels_common.h -- deterministic synthetic-input functions shared by the
 // device generators.  They are functions of (seed, vertex pair) only, so a
-// graph's weights do not depend on edge order; the float ones round every
-// operation explicitly (__fmul_rn/__fadd_rn) so no FMA contraction changes
-// a value.
+// graph's weights do not depend on edge order; the float ones are compiled
+// with contraction off so every operation rounds as written (no fma), as the
+// oracle's C code does.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,9 +51,12 @@ __device__ __forceinline__ float dunit_f32(uint64_t seed, uint64_t i) {
 }
 
 __device__ __forceinline__ float drange_f32(uint64_t seed, uint64_t i, float lo, float hi) {
+    // __fmul_rn/__fadd_rn are plain * and + here: forbid their fusion into an fma
+#pragma clang fp contract(off)
     const float u = dunit_f32(seed, i);
-    const float span = __fsub_rn(hi, lo);
-    return __fadd_rn(lo, __fmul_rn(span, u));
+    const float span = hi - lo;
+    const float t = span * u;
+    return lo + t;
 }
 
 // BP edge potential W_e[k/2][k%2] for the edge s->d
