@@ -92,6 +92,68 @@ def profile_run(args):
         (args.iters, st.ms, st.kernel_ms / max(st.kernel_launches, 1) * 1e3, st.edges))
 
 
+def batched_extra(pa, scale=22, m=1 << 26, k=64, iters=20):
+    """Personalized PageRank with k one-hot teleports: one batched pass (k-wide rows)
+    against k plain single-vector runs on the same GPU, same tolerance."""
+    g = pa.Graph.rmat(scale, m, seed=1)
+    rng = np.random.default_rng(11)
+    src = rng.choice(g.n, k, replace=False)
+    src[0] = 0
+    g.ppr_batched(src, iters, out=False)  # workspace / warm-up
+    _, stb = g.ppr_batched(src, iters, out=False)
+    g.ppr(int(src[0]), iters, out=False)
+    plain_ms = 0.0
+    for s in src:
+        _, st = g.ppr(int(s), iters, out=False)
+        plain_ms += st.ms
+    return {"ppr64_scale22_batched_ms": round(stb.ms, 3),
+            "ppr64_scale22_plain_total_ms": round(plain_ms, 3),
+            "ppr64_batched_speedup": round(plain_ms / stb.ms, 2)}
+
+
+def adjoint_extra(pa, orc_free=True, scale=22, m=1 << 26):
+    """Weighted PageRank forward solve vs adjoint (fixed point + gradient pass)."""
+    g = pa.Graph.rmat(scale, m, seed=2, weights="f32", wseed=4)
+    off, adj, w = g.download()
+    src = np.repeat(np.arange(g.n), np.diff(off))
+    tot = np.bincount(src, weights=w.astype(np.float64), minlength=g.n)
+    wn = (w / tot[src]).astype(np.float32)
+    tele = np.full(g.n, 1.0 / g.n, np.float32)
+    gv = np.random.default_rng(0).standard_normal(g.n).astype(np.float32)
+    g.wpr_forward(wn, tele, 0.85, 200, 1e-7, out=False)
+    _, stf = g.wpr_forward(wn, tele, 0.85, 200, 1e-7, out=False)
+    g.wpr_adjoint(gv, 200, 1e-6 * float(np.abs(gv).sum()), out=False)
+    _, _, _, sta = g.wpr_adjoint(gv, 200, 1e-6 * float(np.abs(gv).sum()), out=False)
+    return {"wpr_scale22_forward_ms": round(stf.ms, 3), "wpr_forward_iters": stf.rounds,
+            "wpr_scale22_adjoint_ms": round(sta.ms, 3), "wpr_adjoint_iters": sta.rounds,
+            "adjoint_over_forward": round(sta.ms / stf.ms, 3)}
+
+
+def multigraph_extra(pa, count=512, iters=20):
+    """512 independent rMat graphs, n = 2^s with s uniform in [8, 16], average degree
+    16, PageRank of every graph in one captured launch."""
+    rng = np.random.default_rng(5)
+    offs, adjs, vb, base, ebase = [], [], [0], 0, 0
+    for i in range(count):
+        s = int(rng.integers(8, 17))
+        h = pa.Graph.rmat(s, 16 << s, seed=1000 + i)
+        off, adj, _ = h.download()
+        h.close()
+        offs.append(off[:-1] + ebase)
+        adjs.append(adj + base)
+        base += h.n
+        ebase += h.m
+        vb.append(base)
+    g = pa.Graph.multi(np.array(vb), np.concatenate(offs + [np.array([ebase])]),
+                       np.concatenate(adjs))
+    g.autotune()
+    g.pagerank_multigraph(iters, out=False)
+    _, st = g.pagerank_multigraph(iters, out=False)
+    return {"multigraph_512_n": g.n, "multigraph_512_m": g.m,
+            "multigraph_512_pagerank_ms": round(st.ms, 3),
+            "multigraph_512_gteps": round(g.m * iters / (st.ms / 1e3) / 1e9, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--profile", action="store_true",
@@ -120,6 +182,8 @@ def main():
     t0 = time.perf_counter()
     g = pa.Graph.rmat(args.scale, args.edges, seed=1)
     log("graph scale %d m %d built in %.1f s" % (args.scale, g.m, time.perf_counter() - t0))
+    ipl, bt = g.autotune()
+    log("autotuned pull schedule: %d items per lane, block threshold %d" % (ipl, bt))
 
     for _ in range(args.warmup):
         g.pagerank(args.iters, 0.85, out=False)
@@ -150,10 +214,14 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        try:
-            extra.update(bfs_extra(pa))
-        except Exception as e:  # report, do not hide
-            extra["bfs_error"] = str(e)
+        for name, fn in (("bfs", bfs_extra), ("batched", batched_extra),
+                         ("adjoint", adjoint_extra), ("multigraph", multigraph_extra)):
+            t1 = time.perf_counter()
+            try:
+                extra.update(fn(pa))
+            except Exception as e:  # report, do not hide
+                extra[name + "_error"] = str(e)
+            log("%s extra done in %.1f s" % (name, time.perf_counter() - t1))
 
     cpu = None
     if not args.no_cpu:

@@ -616,7 +616,7 @@ int bfs_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_stat
             PG_TRY(launch_pull(op, g->in_sched));
         } else {
             BfsPushOp op{N->bits, N->list, &N->ctr[0], level, parent, rounds};
-            PG_TRY((launch_push<BfsPushOp, false>(op, g->out_off, g->out_adj, F->list, cnt)));
+            PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
         }
         PG_TRY(kt.end());
         PG_TRY(frontier_stats(g, *N, &cnt, &edges));
@@ -672,7 +672,7 @@ int cc_run(pg_graph *g, int64_t threshold, uint32_t flags, pg_stats *st, int32_t
             PG_TRY(launch_pull(op, g->in_sched));
         } else {
             CcPushOp op{N->bits, N->list, &N->ctr[0], label, prev};
-            PG_TRY((launch_push<CcPushOp, false>(op, g->out_off, g->out_adj, F->list, cnt)));
+            PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
         }
         PG_TRY(kt.end());
         PG_TRY(frontier_stats(g, *N, &cnt, &edges));
@@ -723,13 +723,19 @@ int sssp_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_sta
         PG_TRY(clear_frontier(g, *N));
         const bool dense = cnt + edges >= thr;
         PG_TRY(kt.begin());
+        SsspPushOp<false> op{F->bits, N->bits, N->list, &N->ctr[0], dist, g->out_w_i32};
         if (dense) {
+            // edgeMapDenseForward visits the active vertices in id order: the
+            // bitmap compacted in order (packIndex), then the same balanced push
             dense_rounds++;
-            SsspPushOp<true> op{F->bits, N->bits, N->list, &N->ctr[0], dist, g->out_w_i32};
-            PG_TRY((launch_push<SsspPushOp<true>, true>(op, g->out_off, g->out_adj, nullptr, g->n)));
+            int32_t *ordered = nullptr;
+            unsigned long long *oc = nullptr;
+            PG_TRY(workspace_t(g, "sssp_ordered", (size_t)g->n + 1, &ordered));
+            PG_TRY(workspace_t(g, "sssp_ordered_n", 2, &oc));
+            PG_TRY(bits_to_list(g, F->bits, ordered, oc));
+            PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, ordered, cnt, edges));
         } else {
-            SsspPushOp<false> op{F->bits, N->bits, N->list, &N->ctr[0], dist, g->out_w_i32};
-            PG_TRY((launch_push<SsspPushOp<false>, false>(op, g->out_off, g->out_adj, F->list, cnt)));
+            PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
         }
         PG_TRY(kt.end());
         PG_TRY(frontier_stats(g, *N, &cnt, &edges));

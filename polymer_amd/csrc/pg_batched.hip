@@ -122,18 +122,22 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowOp op, const int32_t *__rest
         beg = off[v];
         end = off[v + 1];
     }
-    if (!op.vertex_live(v)) {
+    if (!op.vertex_live(v, heavy || end > beg)) {
         if (!heavy || it.chunk == 0) op.skip(v, lane);
         return;
     }
     T acc = op.identity();
     int e = beg;
-    for (; e + 3 < end; e += 4) {
-        const int s0 = ld_stream(adj + e), s1 = ld_stream(adj + e + 1), s2 = ld_stream(adj + e + 2),
-                  s3 = ld_stream(adj + e + 3);
-        const T a0 = op.edge(s0, e, lane), a1 = op.edge(s1, e + 1, lane),
-                a2 = op.edge(s2, e + 2, lane), a3 = op.edge(s3, e + 3, lane);
-        acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
+    for (; e + 7 < end; e += 8) {  // eight row gathers in flight per wavefront
+        int s[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s[k] = ld_stream(adj + e + k);
+        T a[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = op.edge(s[k], e + k, lane);
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] = op.combine(a[k], a[k + 4]);
+        acc = op.combine(acc, op.combine(op.combine(a[0], a[2]), op.combine(a[1], a[3])));
     }
     for (; e < end; e++) acc = op.combine(acc, op.edge(ld_stream(adj + e), e, lane));
     unsigned long long st = 0;
@@ -178,7 +182,12 @@ struct PprRowOp {
     const int32_t *out_off;
     const int32_t *src;  // internal source of each column (-1 unused)
     float damp, tele;
-    __device__ bool vertex_live(int) const { return true; }
+    bool write_r;  // ranks are stored by the last iteration only
+    bool steady;   // iteration >= 2: a vertex without in-edges no longer changes
+    // A vertex with no in-edge has r = (1-d)[v is a source] from iteration 1 on, so
+    // once both contribution buffers hold that value (iterations 1 and 2) it is
+    // skipped until the last iteration stores its rank.
+    __device__ bool vertex_live(int, bool has_in) const { return has_in || !steady || write_r; }
     __device__ void skip(int, int) const {}
     __device__ float identity() const { return 0.f; }
     __device__ float combine(float a, float b) const { return a + b; }
@@ -186,8 +195,9 @@ struct PprRowOp {
     __device__ unsigned long long finish(int v, float sum, int lane) const {
         const float r = damp * sum + (src[lane] == v ? tele : 0.f);
         const int od = out_off[v + 1] - out_off[v];
-        R[(int64_t)v * kRow + lane] = r;
-        Cn[(int64_t)v * kRow + lane] = od > 0 ? r / (float)od : 0.f;
+        if (write_r) R[(int64_t)v * kRow + lane] = r;
+        // a row without out-edges is never gathered
+        if (od > 0) Cn[(int64_t)v * kRow + lane] = r / (float)od;
         return 0ull;
     }
 };
@@ -234,7 +244,12 @@ struct PprOp {
     const int32_t *out_off;
     int32_t src;
     float damp, tele;
-    __device__ bool active(int) const { return true; }
+    bool write_r;  // ranks are stored by the last iteration only
+    bool steady;   // same skipping rule as PprRowOp
+    const int32_t *in_off;
+    __device__ bool active(int v) const {
+        return !steady || write_r || in_off[v + 1] > in_off[v];
+    }
     __device__ int ctx(int v) const { return v; }
     __device__ float identity() const { return 0.f; }
     __device__ float combine(float a, float b) const { return a + b; }
@@ -243,8 +258,8 @@ struct PprOp {
     __device__ Stat finish(int v, float sum, const int &) const {
         const float r = damp * sum + (v == src ? tele : 0.f);
         const int od = out_off[v + 1] - out_off[v];
-        R[v] = r;
-        contrib_n[v] = od > 0 ? r / (float)od : 0.f;
+        if (write_r) R[v] = r;
+        if (od > 0) contrib_n[v] = r / (float)od;
         return 0ull;
     }
     __device__ void skip(int) const {}
@@ -328,7 +343,7 @@ struct MsSsspOp {
     unsigned long long *Fn;
     int32_t *dist;  // [n][64]
     const int32_t *w;  // CSC order
-    __device__ bool vertex_live(int) const { return true; }
+    __device__ bool vertex_live(int, bool) const { return true; }
     __device__ void skip(int, int) const {}
     __device__ int identity() const { return 0x7FFFFFFF; }
     __device__ int combine(int a, int b) const { return min(a, b); }
@@ -416,7 +431,8 @@ extern "C" int pg_ppr(pg_graph *g, int64_t src, int iters, double damping, float
     }
     float *c = c0, *cn = c1;
     for (int t = 0; t < iters; t++) {
-        PprOp op{c, cn, R, g->out_off, s[0], (float)damping, (float)(1.0 - damping)};
+        PprOp op{c,     cn,   R, g->out_off, s[0], (float)damping, (float)(1.0 - damping),
+                 t == iters - 1, t >= 2, g->in_off};
         PG_TRY(kt.begin());
         PG_TRY(launch_pull(op, g->in_sched));
         PG_TRY(kt.end());
@@ -460,7 +476,8 @@ extern "C" int pg_ppr_batched(pg_graph *g, const int64_t *src, int k, int iters,
     if (iters == 0) PG_TRY(fill(R, 0.f, n * kRow));
     float *C = C0, *Cn = C1;
     for (int t = 0; t < iters; t++) {
-        PprRowOp op{C, Cn, R, g->out_off, dsrc, (float)damping, (float)(1.0 - damping)};
+        PprRowOp op{C, Cn, R, g->out_off, dsrc, (float)damping, (float)(1.0 - damping),
+                    t == iters - 1, t >= 2};
         PG_TRY(kt.begin());
         PG_TRY(launch_rows(op, g, P, nullptr));
         PG_TRY(kt.end());

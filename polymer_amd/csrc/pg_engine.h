@@ -52,10 +52,13 @@ __device__ __forceinline__ Acc group_reduce(const Op &op, Acc a) {
 template <int G>
 __device__ __forceinline__ bool group_any(bool p) {
     unsigned long long b = __ballot(p);
-    if (G == 64) return b != 0ull;
-    const int base = lane_id() & ~(G - 1);
-    const unsigned long long mask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
-    return ((b >> base) & mask) != 0ull;
+    if constexpr (G == 64) {
+        return b != 0ull;
+    } else {
+        const int base = lane_id() & ~(G - 1);
+        const unsigned long long mask = (1ull << G) - 1ull;
+        return ((b >> base) & mask) != 0ull;
+    }
 }
 
 // atomics that bypass the non-coherent vector L1 (used for cross-block data)
@@ -206,9 +209,20 @@ __device__ __forceinline__ typename Op::Stat pull_group(const Op &op, const Pull
     const int beg = pa.off[v], end = pa.off[v + 1];
     Acc acc = op.identity();
     if constexpr (Op::kEarlyExit) {
-        for (int base = beg; base < end; base += G) {
-            const int e = base + lane;
-            if (e < end) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+        // four chunks of G edges are loaded before each exit test, so four
+        // independent (neighbour id, neighbour state) load pairs are in flight;
+        // a hit in an earlier chunk still wins the min over edge positions
+        constexpr int U = 4;
+        for (int base = beg; base < end; base += U * G) {
+            int s[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int e = base + k * G + lane;
+                s[k] = e < end ? ld_stream(pa.adj + e) : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++)
+                if (s[k] >= 0) acc = op.combine(acc, op.edge(s[k], base + k * G + lane, c));
             if (group_any<G>(op.found(acc, c))) break;
         }
     } else {
@@ -416,6 +430,107 @@ inline int launch_push(const Op &op, const int32_t *off, const int32_t *adj, con
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_error("k_push launch failed: %s", hipGetErrorString(e));
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
+// Device-wide edge-balanced push: scan[j] is the exclusive prefix sum of the
+// out-degrees of list[0..count) (scan[count] = total).  Block b takes the edges
+// [b*kPushEdges, (b+1)*kPushEdges) of the frontier, whatever vertices they belong
+// to, so a hub's edges are spread over as many blocks as they fill.  The block's
+// window of list entries is staged in LDS; the rare window larger than LDS falls
+// back to binary searches in global memory.
+constexpr int kPushEdges = 2048;
+constexpr int kPushWin = 2048;
+
+__device__ __forceinline__ int64_t upper_bound_u32(const uint32_t *a, int64_t n, uint32_t x) {
+    int64_t lo = 0, hi = n;  // first index with a[i] > x
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_push_global(Op op, const int32_t *__restrict__ off,
+                                                        const int32_t *__restrict__ adj,
+                                                        const int32_t *__restrict__ list,
+                                                        const uint32_t *__restrict__ scan,
+                                                        int64_t count, uint32_t total) {
+    __shared__ uint32_t s_scan[kPushWin + 1];
+    __shared__ int32_t s_beg[kPushWin];
+    __shared__ int64_t s_j0, s_j1;
+    const uint32_t E0 = (uint32_t)blockIdx.x * kPushEdges;
+    const uint32_t E1 = min(total, E0 + (uint32_t)kPushEdges);
+    if (threadIdx.x == 0) {
+        s_j0 = upper_bound_u32(scan, count + 1, E0) - 1;
+        s_j1 = upper_bound_u32(scan, count + 1, E1 - 1) - 1;
+    }
+    __syncthreads();
+    const int64_t j0 = s_j0, nwin = s_j1 - s_j0 + 1;
+    if (nwin <= kPushWin) {
+        for (int64_t i = threadIdx.x; i < nwin; i += kBlock) {
+            s_scan[i] = scan[j0 + i];
+            s_beg[i] = off[list[j0 + i]];
+        }
+        __syncthreads();
+        for (uint32_t k = E0 + threadIdx.x; k < E1; k += kBlock) {
+            int lo = 0, hi = (int)nwin - 1;  // last i with s_scan[i] <= k
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_scan[mid] <= k) lo = mid; else hi = mid - 1;
+            }
+            const int e = s_beg[lo] + (int)(k - s_scan[lo]);
+            op.push(list[j0 + lo], adj[e], e);
+        }
+    } else {
+        for (uint32_t k = E0 + threadIdx.x; k < E1; k += kBlock) {
+            const int64_t j = upper_bound_u32(scan, count + 1, k) - 1;
+            const int s = list[j];
+            const int e = off[s] + (int)(k - scan[j]);
+            op.push(s, adj[e], e);
+        }
+    }
+}
+
+// deg[i] = out-degree of list[i] for i < count, deg[count] = 0
+static __global__ void k_list_degrees(const int32_t *list, int64_t count, const int32_t *off,
+                                      uint32_t *deg) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t d = 0;
+        if (i < count) {
+            const int v = list[i];
+            d = (uint32_t)(off[v + 1] - off[v]);
+        }
+        deg[i] = d;
+    }
+}
+
+// Push over list[0..count) whose out-degrees sum to total (host values).
+template <class Op>
+inline int launch_push_global(const Op &op, pg_graph *g, const int32_t *off, const int32_t *adj,
+                              const int32_t *list, int64_t count, int64_t total) {
+    if (count <= 0 || total <= 0) return PG_OK;
+    if (total >= 0xFFFFFFFFll) {
+        set_error("push: frontier edge count exceeds 32 bits");
+        return PG_ERR_UNSUPPORTED;
+    }
+    uint32_t *deg = nullptr, *scan = nullptr;
+    PG_TRY(workspace_t(g, "push_deg", (size_t)count + 1, &deg));
+    PG_TRY(workspace_t(g, "push_scan", (size_t)count + 1, &scan));
+    const int64_t dgrid = std::min<int64_t>((count + kBlock) / kBlock, 65536);
+    hipLaunchKernelGGL(k_list_degrees, dim3((unsigned)dgrid), dim3(kBlock), 0, stream(), list,
+                       count, off, deg);
+    PG_TRY(exclusive_scan_u32(g, deg, scan, count + 1));
+    const int64_t nb = (total + kPushEdges - 1) / kPushEdges;
+    hipLaunchKernelGGL(k_push_global<Op>, dim3((unsigned)nb), dim3(kBlock), 0, stream(), op, off,
+                       adj, list, scan, count, (uint32_t)total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("k_push_global launch failed: %s", hipGetErrorString(e));
         return PG_ERR_HIP;
     }
     return PG_OK;
