@@ -71,8 +71,9 @@ def bfs_extra(pa, scale=22, m=1 << 26, nsrc=8):
 
 
 def profile_run(args):
-    """Short run for rocprofv3: a 1 GiB device-to-device copy (torch, the
-    counter calibration of known size: 1 GiB read + 1 GiB written), then one
+    """Short run for rocprofv3: a plain streaming kernel of known size (torch's
+    elementwise negation of a 1 GiB fp32 tensor into another: exactly 1 GiB
+    read and 1 GiB written in one dispatch, the counter calibration), then one
     warm-up and one measured PageRank of --iters iterations."""
     import polymer_amd as pa
     import torch
@@ -81,15 +82,36 @@ def profile_run(args):
     y = torch.empty_like(x)
     x.fill_(1.0)
     torch.cuda.synchronize()
-    y.copy_(x)
+    torch.neg(x, out=y)
     torch.cuda.synchronize()
-    log("copy kernel: 1073741824 bytes read, 1073741824 bytes written")
+    log("calibration kernel (neg): 1073741824 bytes read, 1073741824 bytes written")
+    del x, y
     pa.init(0)
     g = pa.Graph.rmat(args.scale, args.edges, seed=1)
+    sched = [int(x) for x in args.schedule.split(",")]
+    g.set_tiles(sched[4], sched[5])
+    g.set_schedule(*sched[:4])
+    log("pull schedule (ipl, block, multi, chunk, tiles, tile edges):", g.get_schedule())
     g.pagerank(args.iters, 0.85, out=False)
     _, st = g.pagerank(args.iters, 0.85, out=False, profile=True)
-    log("pagerank %d iterations: %.3f ms, kernel avg %.1f us, edges %d" %
+    log("pagerank %d iterations: %.3f ms, per-iteration pull avg %.1f us, edges %d" %
         (args.iters, st.ms, st.kernel_ms / max(st.kernel_launches, 1) * 1e3, st.edges))
+
+
+def measured_traffic(schedule):
+    """HBM bytes per PageRank iteration from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it was taken with the same pull schedule."""
+    path = os.path.join(ROOT, "profiles", "pagerank_scale24_pmc.txt")
+    if not os.path.exists(path):
+        return None, "no pmc summary"
+    kv = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    if kv.get("schedule") != ",".join(str(x) for x in schedule):
+        return None, "pmc summary taken with schedule %s" % kv.get("schedule")
+    return int(float(kv["hbm_bytes_per_iteration"])), "profiles/pagerank_scale24_pmc.txt"
 
 
 def batched_extra(pa, scale=22, m=1 << 26, k=64, iters=20):
@@ -166,6 +188,8 @@ def main():
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--schedule", default="16,16384,65536,16384,1,262144",
+                    help="--profile only: ipl,block,multi,chunk,tiles,tile_edges")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -182,8 +206,9 @@ def main():
     t0 = time.perf_counter()
     g = pa.Graph.rmat(args.scale, args.edges, seed=1)
     log("graph scale %d m %d built in %.1f s" % (args.scale, g.m, time.perf_counter() - t0))
-    ipl, bt = g.autotune()
-    log("autotuned pull schedule: %d items per lane, block threshold %d" % (ipl, bt))
+    g.autotune()
+    schedule = g.get_schedule()
+    log("autotuned pull schedule (ipl, block, multi, chunk, tiles, tile edges):", schedule)
 
     for _ in range(args.warmup):
         g.pagerank(args.iters, 0.85, out=False)
@@ -201,16 +226,22 @@ def main():
     log("pagerank: %.3f ms/step wall, %.3f ms/step device, %.2f GTEPS" %
         (ms_per_step, dev_ms / args.steps, gteps))
 
-    # dominant kernel: the pull iteration, timed launch by launch with HIP events
+    # dominant kernel: the pull of one iteration (with tiles, the launches
+    # k_pull<PrOp> hot segments + k_pull_tiles<PrOp> + k_tile_finish<PrOp>),
+    # bracketed by HIP events on the library stream, iteration by iteration
     _, stp = g.pagerank(args.iters, 0.85, out=False, profile=True)
     avg_kernel_s = stp.kernel_ms / max(stp.kernel_launches, 1) / 1e3
     bytes_per_launch = pagerank_bytes(g.n, stp.edges, args.iters) / args.iters
     achieved = bytes_per_launch / avg_kernel_s / 1e9
+    traffic, traffic_src = measured_traffic(schedule)
+    kernels = ("k_pull<PrOp> (hot) + k_pull_tiles<PrOp> + k_tile_finish<PrOp>" if schedule[4]
+               else "k_pull<PrOp>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_pull<PrOp>", "avg_launch_us": round(avg_kernel_s * 1e6, 2),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": traffic_src,
+                "kernel": "pull iteration: " + kernels, "avg_launch_us": round(avg_kernel_s * 1e6, 2),
                 "bytes_per_launch": int(bytes_per_launch)}
-    log("kernel avg %.1f us, %.1f GB/s algorithmic" % (avg_kernel_s * 1e6, achieved))
+    log("pull iteration avg %.1f us, %.1f GB/s algorithmic" % (avg_kernel_s * 1e6, achieved))
 
     extra = {}
     if not args.no_extra:
@@ -244,7 +275,8 @@ def main():
                 "or duplicates",
         "config": {"workload": "pagerank rmat scale-%d m=%d, %d iterations per step, "
                                "damping 0.85" % (args.scale, g.m, args.iters),
-                   "n": g.n, "m": g.m, "iters": args.iters},
+                   "n": g.n, "m": g.m, "iters": args.iters,
+                   "schedule": ",".join(str(x) for x in schedule)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,

@@ -33,8 +33,8 @@ INF_DIST = 1073741823
 EXPORTS = [
     "pg_init", "pg_finalize", "pg_last_error", "pg_device_name",
     "pg_graph_create", "pg_graph_rmat", "pg_graph_load_adj", "pg_graph_destroy",
-    "pg_graph_info", "pg_graph_download", "pg_graph_set_schedule", "pg_graph_autotune",
-    "pg_graph_bp_potentials",
+    "pg_graph_info", "pg_graph_download", "pg_graph_set_schedule", "pg_graph_set_tiles",
+    "pg_graph_get_schedule", "pg_graph_autotune", "pg_graph_bp_potentials",
     "pg_subset_create", "pg_subset_destroy", "pg_subset_clear", "pg_subset_fill",
     "pg_subset_from_dense", "pg_subset_from_sparse", "pg_subset_to_dense",
     "pg_subset_to_sparse", "pg_subset_size",
@@ -95,6 +95,8 @@ def load():
         "pg_graph_info": [P, P, P, P, P],
         "pg_graph_download": [P, P, P, P],
         "pg_graph_set_schedule": [P, ctypes.c_int, i64, i64, i64],
+        "pg_graph_set_tiles": [P, ctypes.c_int, i64],
+        "pg_graph_get_schedule": [P, P, P, P, P, P, P],
         "pg_graph_autotune": [P, P, P],
         "pg_graph_bp_potentials": [P, u64, ctypes.c_float, ctypes.c_float],
         "pg_subset_create": [P, PP],
@@ -251,6 +253,19 @@ class Graph:
     def set_schedule(self, items_per_lane=0, block_threshold=0, multi_threshold=0, chunk=0):
         _check(load().pg_graph_set_schedule(self._h, items_per_lane, block_threshold,
                                             multi_threshold, chunk), "pg_graph_set_schedule")
+
+    def set_tiles(self, enable=-1, tile_edges=0):
+        """Source-sorted tiles for the PageRank pull of light vertices (on by default)."""
+        _check(load().pg_graph_set_tiles(self._h, int(enable), int(tile_edges)),
+               "pg_graph_set_tiles")
+
+    def get_schedule(self):
+        """(items_per_lane, block_threshold, multi_threshold, chunk, tiles, tile_edges)"""
+        i32, i64 = ctypes.c_int32, ctypes.c_int64
+        v = [i32(), i64(), i64(), i64(), i32(), i64()]
+        _check(load().pg_graph_get_schedule(self._h, *[ctypes.byref(x) for x in v]),
+               "pg_graph_get_schedule")
+        return tuple(x.value for x in v)
 
     def autotune(self):
         ipl, bt = ctypes.c_int32(), ctypes.c_int64()

@@ -78,6 +78,12 @@ struct PrOp {
     __device__ static void stat_commit(Stat *p, Stat v) {
         atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
     }
+    // Tiled pull: contributions in fixed point with 62 fraction bits.  Ranks of
+    // one graph sum to at most 1 (no teleport mass is created beyond (1-d)), so
+    // every per-vertex sum of contributions is <= 1 < 2^(64-62) and cannot
+    // overflow; the truncation of each term is < 2^-62 absolute.
+    __device__ unsigned long long tile_value(int s) const { return to_fixed<62>(contrib[s]); }
+    __device__ float tile_sum(unsigned long long q) const { return (float)((double)q * 0x1p-62); }
 };
 
 __global__ void k_pr_init(int64_t n, const int32_t *out_off, const float *inv_n, float one_over_n,
@@ -148,6 +154,10 @@ int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, fl
         PG_LAUNCH_K(k_scale, grid_for(n), g->inv_n, n, (float)(1.0 - damping), inv_n_addc);
     float *cur = B.a, *nxt = B.b, *c = B.c0, *cn = B.c1;
     uint8_t *F = B.f0, *Fn = B.f1;
+    // the tiles are built before any capture (pagerank_prepare); inside one only
+    // a built set is used
+    const bool tiled = g->use_tiles && g->tiles.built && g->tiles.light_begin == g->in_sched.light_begin &&
+                       g->tiles.light_end == g->in_sched.light_end;
     for (int t = 0; t < iters; t++) {
         op.F = F;
         op.Fn = Fn;
@@ -156,7 +166,10 @@ int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, fl
         op.cur = cur;
         op.nxt = nxt;
         if (kt) PG_TRY(kt->begin());
-        PG_TRY(launch_pull(op, g->in_sched, &B.stats[(size_t)t * kStatSlots]));
+        if (tiled)
+            PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, &B.stats[(size_t)t * kStatSlots]));
+        else
+            PG_TRY(launch_pull(op, g->in_sched, &B.stats[(size_t)t * kStatSlots]));
         if (kt) PG_TRY(kt->end());
         std::swap(cur, nxt);
         std::swap(F, Fn);
@@ -170,6 +183,7 @@ int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, fl
 // Runs the iterations; on return *result points at the buffer holding p_curr.
 int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stats *st,
                  float **result) {
+    if (g->use_tiles) PG_TRY(ensure_tiles(g));  // once per schedule, before any capture
     KernelTimer kt;
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
     Timer tm;

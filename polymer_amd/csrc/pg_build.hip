@@ -662,19 +662,20 @@ int build_schedule(pg_graph *g, bool in_direction) {
     const int ipl = std::max(1, P.items_per_lane);
     // class upper bounds, descending: multi | block | 64 | 32 | 16 | 8 | 4 | 2 | 1
     const int gs[9] = {kModeMulti, kModeBlock, 64, 32, 16, 8, 4, 2, 1};
-    int64_t thr[8];
+    int64_t thr[9];
     thr[0] = std::max<int64_t>(P.multi_threshold, P.block_threshold);
     thr[1] = std::max<int64_t>(P.block_threshold, 32LL * ipl);
     for (int j = 2; j < 8; j++) thr[j] = (int64_t)ipl << (7 - j);  // 32ipl,16,8,4,2,ipl
-    // thr[2] = 32*ipl ... thr[7] = ipl
+    // thr[2] = 32*ipl ... thr[7] = ipl; thr[8] = 0 counts the vertices with edges
+    thr[8] = 0;
     DevBuf dthr, dcnt;
     PG_TRY(dthr.alloc(sizeof(thr)));
     PG_TRY(dcnt.alloc(sizeof(thr)));
     PG_CHECK_HIP(hipMemcpyAsync(dthr.p, thr, sizeof(thr), hipMemcpyHostToDevice, g_stream));
     hipLaunchKernelGGL(k_count_above, dim3(1), dim3(64), 0, g_stream, off, order, n,
-                       dthr.as<int64_t>(), 8, dcnt.as<int64_t>());
+                       dthr.as<int64_t>(), 9, dcnt.as<int64_t>());
     PG_CHECK_HIP(hipGetLastError());
-    int64_t cnt[8];
+    int64_t cnt[9];
     PG_CHECK_HIP(hipMemcpyAsync(cnt, dcnt.p, sizeof(cnt), hipMemcpyDeviceToHost, g_stream));
     PG_CHECK_HIP(hipStreamSynchronize(g_stream));
     int64_t bounds[10];
@@ -740,7 +741,142 @@ int build_schedule(pg_graph *g, bool in_direction) {
         return PG_ERR_UNSUPPORTED;
     }
     s.nblocks = (int)blocks;
+    // hot schedule: the multi and block segments (laid out first, so their block
+    // numbers carry over) and the vertices without pulled edges
+    s.light_begin = bounds[2];
+    s.light_end = std::max(cnt[8], bounds[2]);
+    PullArgs &h = s.args_hot;
+    h = a;
+    h.nseg = 0;
+    int64_t hblocks = 0;
+    for (int i = 0; i < a.nseg; i++) {
+        if (a.seg[i].g != kModeMulti && a.seg[i].g != kModeBlock) break;
+        h.seg[h.nseg++] = a.seg[i];
+        hblocks = (i + 1 < a.nseg) ? a.seg[i + 1].block_begin : blocks;
+    }
+    if (s.light_end < n) {
+        h.seg[h.nseg++] = PullSeg{(int32_t)s.light_end, (int32_t)n, 1, (int32_t)hblocks};
+        hblocks += (n - s.light_end + kBlock - 1) / kBlock;
+    }
+    s.nblocks_hot = (int)hblocks;
     s.built = true;
+    return PG_OK;
+}
+
+// ---- source-sorted tiles ----------------------------------------------------------------
+void free_tiles(pg_graph *g) {
+    TileSet &t = g->tiles;
+    if (t.d_vbeg) (void)hipFree(t.d_vbeg);
+    if (t.d_parts) (void)hipFree(t.d_parts);
+    if (t.d_fin) (void)hipFree(t.d_fin);
+    if (t.d_src) (void)hipFree(t.d_src);
+    if (t.d_dst) (void)hipFree(t.d_dst);
+    if (t.d_partial) (void)hipFree(t.d_partial);
+    t = TileSet{};
+}
+
+// key = tile << 32 | source, value = destination offset within the tile, for the
+// CSC edges of the light positions [p0, p1)
+__global__ void k_tile_keys(const int32_t *off, const int32_t *adj, int64_t p0, int64_t p1,
+                            int64_t e0, uint64_t *keys, uint32_t *vals) {
+    for (int64_t p = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < p1;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)((p - p0) / kTileMaxV);
+        const uint32_t local = (uint32_t)((p - p0) % kTileMaxV);
+        for (int64_t e = off[p]; e < off[p + 1]; e++) {
+            keys[e - e0] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)adj[e];
+            vals[e - e0] = local;
+        }
+    }
+}
+
+__global__ void k_tile_split(const uint64_t *keys, const uint32_t *vals, int64_t m, int32_t *src,
+                             uint16_t *dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        src[i] = (int32_t)(uint32_t)keys[i];
+        dst[i] = (uint16_t)vals[i];
+    }
+}
+
+int ensure_tiles(pg_graph *g) {
+    PullSchedule &s = g->in_sched;
+    PG_REQUIRE(s.built, PG_ERR_STATE, "tiles: the pull schedule is not built");
+    TileSet &t = g->tiles;
+    if (t.built && t.light_begin == s.light_begin && t.light_end == s.light_end &&
+        t.tile_edges == g->tile_edges)
+        return PG_OK;
+    drop_captures(g);
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    free_tiles(g);
+    const int64_t p0 = s.light_begin, p1 = s.light_end;
+    PG_REQUIRE(p1 - p0 < ((int64_t)1 << 31), PG_ERR_UNSUPPORTED, "tiles: light range too large");
+    // tiles: kTileMaxV consecutive light positions; parts: about tile_edges edges
+    std::vector<int32_t> off_h((size_t)(p1 - p0 + 1));
+    if (p1 > p0)
+        PG_CHECK_HIP(hipMemcpy(off_h.data(), g->in_off + p0, sizeof(int32_t) * off_h.size(),
+                               hipMemcpyDeviceToHost));
+    std::vector<int32_t> vbeg;
+    std::vector<TilePart> parts;
+    std::vector<TileFin> fin;
+    int64_t slots = 0;  // partial elements
+    for (int64_t p = p0; p < p1; p += kTileMaxV) {
+        const int64_t q = std::min<int64_t>(p1, p + kTileMaxV);
+        const int64_t nv = q - p;
+        const int64_t e0 = (int64_t)off_h[p - p0] - off_h[0], e1 = (int64_t)off_h[q - p0] - off_h[0];
+        const int64_t pe = std::max<int64_t>(g->tile_edges, nv);
+        const int64_t np = std::max<int64_t>(1, (e1 - e0 + pe - 1) / pe);
+        const int32_t tile = (int32_t)vbeg.size();
+        for (int64_t k = 0; k < np; k++)
+            parts.push_back(TilePart{tile, 0, np > 1 ? slots + k * nv : -1,
+                                     e0 + (e1 - e0) * k / np, e0 + (e1 - e0) * (k + 1) / np});
+        if (np > 1) {
+            for (int64_t v0 = 0; v0 < nv; v0 += kBlock)
+                fin.push_back(TileFin{tile, (int32_t)np, (int32_t)v0, 0, slots});
+            slots += np * nv;
+        }
+        vbeg.push_back((int32_t)p);
+    }
+    const int64_t mt = p1 > p0 ? (int64_t)off_h[p1 - p0] - off_h[0] : 0;
+    vbeg.push_back((int32_t)p1);
+    const int nt = (int)vbeg.size() - 1;
+    PG_REQUIRE(parts.size() < 0x7FFFFFFF && fin.size() < 0x7FFFFFFF, PG_ERR_UNSUPPORTED,
+               "tiles: too many parts");
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_vbeg), sizeof(int32_t) * vbeg.size()));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_parts), sizeof(TilePart) * (parts.size() + 1)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_fin), sizeof(TileFin) * (fin.size() + 1)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_src), sizeof(int32_t) * (mt + 16)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_dst), sizeof(uint16_t) * (mt + 16)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_partial),
+                           sizeof(unsigned long long) * (slots + 1)));
+    PG_CHECK_HIP(hipMemcpy(t.d_vbeg, vbeg.data(), sizeof(int32_t) * vbeg.size(), hipMemcpyHostToDevice));
+    if (!parts.empty())
+        PG_CHECK_HIP(hipMemcpy(t.d_parts, parts.data(), sizeof(TilePart) * parts.size(),
+                               hipMemcpyHostToDevice));
+    if (!fin.empty())
+        PG_CHECK_HIP(hipMemcpy(t.d_fin, fin.data(), sizeof(TileFin) * fin.size(), hipMemcpyHostToDevice));
+    if (mt > 0) {
+        DevBuf k0, k1, v0, v1;
+        PG_TRY(k0.alloc(sizeof(uint64_t) * mt));
+        PG_TRY(k1.alloc(sizeof(uint64_t) * mt));
+        PG_TRY(v0.alloc(sizeof(uint32_t) * mt));
+        PG_TRY(v1.alloc(sizeof(uint32_t) * mt));
+        PG_LAUNCH(k_tile_keys, grid_for(p1 - p0), g->in_off, g->in_adj, p0, p1, (int64_t)off_h[0],
+                  k0.as<uint64_t>(), v0.as<uint32_t>());
+        PG_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(),
+                          mt, 32 + bits_for(nt)));
+        PG_LAUNCH(k_tile_split, grid_for(mt), k1.as<uint64_t>(), v1.as<uint32_t>(), mt, t.d_src,
+                  t.d_dst);
+        PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    }
+    t.args = TileArgs{t.d_vbeg,    t.d_parts,           t.d_fin,           t.d_src,
+                      t.d_dst,     t.d_partial,         (int32_t)parts.size(), (int32_t)fin.size()};
+    t.ntiles = nt;
+    t.m_tiled = mt;
+    t.tile_edges = g->tile_edges;
+    t.light_begin = p0;
+    t.light_end = p1;
+    t.built = true;
     return PG_OK;
 }
 
@@ -860,6 +996,7 @@ static void delete_graph(pg_graph *g) {
     (void)hipStreamSynchronize(g_stream);
     free_schedule(g->in_sched);
     free_schedule(g->out_sched);
+    free_tiles(g);
     for (void *p : g->owned) (void)hipFree(p);
     for (auto &kv : g->ws) (void)hipFree(kv.second.first);
     delete g;
@@ -1215,6 +1352,30 @@ extern "C" int pg_graph_set_schedule(pg_graph *g, int items_per_lane, int64_t bl
     }
     PG_TRY(build_schedule(g, true));
     if (g->out_sched.built) PG_TRY(build_schedule(g, false));
+    return PG_OK;
+}
+
+extern "C" int pg_graph_set_tiles(pg_graph *g, int enable, int64_t tile_edges) {
+    PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    PG_REQUIRE(tile_edges >= 0 && tile_edges <= (int64_t)1 << 30, PG_ERR_ARG,
+               "tile_edges out of range: %lld", (long long)tile_edges);
+    drop_captures(g);
+    if (enable >= 0) g->use_tiles = enable != 0;
+    if (tile_edges > 0) g->tile_edges = tile_edges;
+    return PG_OK;
+}
+
+extern "C" int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane,
+                                     int64_t *block_threshold, int64_t *multi_threshold,
+                                     int64_t *chunk, int32_t *tiles, int64_t *tile_edges) {
+    PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    const PullParams &p = g->in_sched.params;
+    if (items_per_lane) *items_per_lane = p.items_per_lane;
+    if (block_threshold) *block_threshold = p.block_threshold;
+    if (multi_threshold) *multi_threshold = p.multi_threshold;
+    if (chunk) *chunk = p.chunk;
+    if (tiles) *tiles = g->use_tiles ? 1 : 0;
+    if (tile_edges) *tile_edges = g->tile_edges;
     return PG_OK;
 }
 

@@ -372,6 +372,163 @@ inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *s
     return PG_OK;
 }
 
+// ---- tiled pull ---------------------------------------------------------------------
+// The light destinations are pulled tile by tile (TileArgs): a 1024-thread
+// workgroup streams its tile's (source, destination) pairs, sorted by source, and
+// adds the gathered values into per-destination accumulators in LDS.  An Op used
+// here also provides
+//     unsigned long long tile_value(int s);  // the gathered value as fixed point
+//     Acc tile_sum(unsigned long long q);     // an accumulated fixed-point sum
+// Integer addition is associative, so the result does not depend on the order in
+// which the LDS atomics land.  The per-vertex protocol (active / finish / skip,
+// statistics) is the one k_pull follows.
+__device__ __forceinline__ unsigned ld_stream_u16(const uint16_t *p) {
+    return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ unsigned long long ld_stream_u64(const unsigned long long *p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// x >= 0 (finite, at most 2^(63-F)) as an unsigned fixed-point number with F
+// fraction bits, truncated: the mantissa shifted by the exponent, no float->int64
+// conversion sequence.
+template <int F>
+__host__ __device__ __forceinline__ unsigned long long to_fixed(float x) {
+    const unsigned u = __builtin_bit_cast(unsigned, x);
+    const int ex = (int)(u >> 23);  // biased exponent (sign bit is 0)
+    const unsigned long long mant =
+        ex ? (unsigned long long)((u & 0x7FFFFFu) | 0x800000u) : (unsigned long long)(u & 0x7FFFFFu);
+    const int sh = (ex ? ex : 1) - 150 + F;  // x = mant * 2^(ex-150)
+    if (sh >= 0) return mant << sh;
+    return sh > -64 ? mant >> (-sh) : 0ull;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
+                                                             typename Op::Stat *stat_out) {
+    using Stat = typename Op::Stat;
+    __shared__ unsigned long long acc[kTileMaxV];
+    __shared__ Stat ssm[kTileThreads / 64];
+    const TilePart part = ta.parts[blockIdx.x];
+    const int t = part.tile;
+    const int vb = ta.vbeg[t];
+    const int nv = ta.vbeg[t + 1] - vb;  // <= kTileMaxV by construction
+    for (int i = threadIdx.x; i < nv; i += kTileThreads) acc[i] = 0ull;
+    __syncthreads();
+    {
+        const int32_t *__restrict__ src = ta.src + part.e0;
+        const uint16_t *__restrict__ dst = ta.dst + part.e0;
+        const int cnt = (int)(part.e1 - part.e0);
+        int e = threadIdx.x;
+        constexpr int U = 8;
+        for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
+            int s[U];
+            unsigned d[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                s[k] = ld_stream(src + e + k * kTileThreads);
+                d[k] = ld_stream_u16(dst + e + k * kTileThreads);
+            }
+            unsigned long long q[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) q[k] = op.tile_value(s[k]);
+#pragma unroll
+            for (int k = 0; k < U; k++) atomicAdd(&acc[d[k]], q[k]);
+        }
+        for (; e < cnt; e += kTileThreads)
+            atomicAdd(&acc[ld_stream_u16(dst + e)], op.tile_value(ld_stream(src + e)));
+    }
+    __syncthreads();
+    if (part.slot >= 0) {
+        // one of several parts: store the partial sums; k_tile_finish adds them up
+        unsigned long long *p = ta.partial + part.slot;
+        for (int i = threadIdx.x; i < nv; i += kTileThreads) p[i] = acc[i];
+        return;  // uniform over the block
+    }
+    Stat st = Stat(0);
+    for (int i = threadIdx.x; i < nv; i += kTileThreads) {
+        const int v = vb + i;
+        if (!op.active(v)) {
+            op.skip(v);
+            continue;
+        }
+        const typename Op::Ctx c = op.ctx(v);
+        st = Op::stat_reduce(st, op.finish(v, op.tile_sum(acc[i]), c));
+    }
+    if constexpr (Op::kStats) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) st = Op::stat_reduce(st, shfl_xor_t(st, o));
+        if (lane_id() == 0) ssm[threadIdx.x >> 6] = st;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Stat tot = ssm[0];
+#pragma unroll
+            for (int i = 1; i < kTileThreads / 64; i++) tot = Op::stat_reduce(tot, ssm[i]);
+            if (tot != Stat(0)) Op::stat_commit(stat_out, tot);
+        }
+    }
+}
+
+// Vertices of the tiles with several parts: one thread per vertex sums the
+// vertex's partial slots (coalesced across the block) and finalises.
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
+                                                        typename Op::Stat *stat_out) {
+    using Stat = typename Op::Stat;
+    __shared__ Stat ssm[kBlock / 64];
+    const TileFin f = ta.fin[blockIdx.x];
+    const int vb = ta.vbeg[f.tile];
+    const int nv = ta.vbeg[f.tile + 1] - vb;
+    const int i = f.v0 + (int)threadIdx.x;
+    Stat st = Stat(0);
+    if (i < nv) {
+        const int v = vb + i;
+        if (!op.active(v)) {
+            op.skip(v);
+        } else {
+            const unsigned long long *p = ta.partial + f.slot0 + i;
+            unsigned long long q = 0ull;
+            for (int k = 0; k < f.nparts; k++) q += ld_stream_u64(p + (int64_t)k * nv);
+            st = op.finish(v, op.tile_sum(q), op.ctx(v));
+        }
+    }
+    if constexpr (Op::kStats) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) st = Op::stat_reduce(st, shfl_xor_t(st, o));
+        if (lane_id() == 0) ssm[threadIdx.x >> 6] = st;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Stat tot = ssm[0];
+#pragma unroll
+            for (int w = 1; w < kBlock / 64; w++) tot = Op::stat_reduce(tot, ssm[w]);
+            if (tot != Stat(0)) Op::stat_commit(stat_out, tot);
+        }
+    }
+}
+
+// Pull over the CSC with the light range taken from the tiles: the hot schedule
+// (multi-block and block-per-vertex segments, zero-degree tail), the tile parts,
+// then the vertices of the tiles with several parts.
+template <class Op>
+inline int launch_pull_tiled(const Op &op, const PullSchedule &s, const TileSet &t,
+                             typename Op::Stat *stat_out = nullptr) {
+    if (s.nblocks_hot > 0)
+        hipLaunchKernelGGL(k_pull<Op>, dim3(s.nblocks_hot), dim3(kBlock), 0, stream(), op,
+                           s.args_hot, stat_out);
+    if (t.args.nparts > 0)
+        hipLaunchKernelGGL(k_pull_tiles<Op>, dim3(t.args.nparts), dim3(kTileThreads), 0, stream(),
+                           op, t.args, stat_out);
+    if (t.args.nfin > 0)
+        hipLaunchKernelGGL(k_tile_finish<Op>, dim3(t.args.nfin), dim3(kBlock), 0, stream(), op,
+                           t.args, stat_out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("tiled pull launch failed: %s", hipGetErrorString(e));
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
 // ---- push engine ------------------------------------------------------------------
 // kDense: sources are the vertex ids [0, count) filtered by op.src_active;
 // otherwise the sources are list[0, count).

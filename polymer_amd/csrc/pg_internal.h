@@ -90,6 +90,61 @@ struct PullParams {
     int64_t chunk = 8192;             // edges per block for multi-block vertices
 };
 
+// ---- source-sorted tiles --------------------------------------------------------
+// The light destinations of the CSC (in-degree in [1, block_threshold)) are cut
+// into tiles of kTileMaxV consecutive positions.  A tile stores its in-edges
+// sorted by source id, with the destination as a 16-bit offset from the tile's
+// first position, so the gathers of a wavefront walking the list hit few cache
+// lines.  A tile's list is split into parts of about tile_edges edges (at least
+// as many edges as the tile has vertices); one workgroup per part adds the
+// gathered values into per-destination accumulators in LDS.  A tile with one
+// part finalises its vertices in place; the parts of a larger tile store their
+// accumulators to a partial slot each, and a second launch (one thread per
+// vertex) sums the slots and finalises.
+constexpr int kTileMaxV = 8192;      // destinations per tile (64 KiB of LDS accumulators)
+constexpr int kTileThreads = 1024;   // threads per tile workgroup
+
+struct TilePart {
+    int32_t tile;
+    int32_t pad;
+    int64_t slot;    // element offset of this part's partial slot, -1 = single part
+    int64_t e0, e1;  // edge range in src / dst
+};
+
+struct TileFin {     // kBlock vertices of a tile with several parts
+    int32_t tile;
+    int32_t nparts;
+    int32_t v0;      // first vertex offset within the tile
+    int32_t pad;
+    int64_t slot0;   // offset of the tile's first partial slot (slots are nv apart)
+};
+
+struct TileArgs {
+    const int32_t *vbeg;         // ntiles + 1 positions (visiting order)
+    const TilePart *parts;       // one per workgroup
+    const TileFin *fin;          // one per finishing block
+    const int32_t *src;          // source ids, sorted within a tile
+    const uint16_t *dst;         // destination position - vbeg[tile]
+    unsigned long long *partial; // fixed-point partial sums
+    int32_t nparts;
+    int32_t nfin;
+};
+
+struct TileSet {
+    bool built = false;
+    int64_t light_begin = -1, light_end = -1;  // light range the tiles were cut for
+    int64_t tile_edges = 0;
+    TileArgs args{};
+    int32_t *d_vbeg = nullptr;
+    TilePart *d_parts = nullptr;
+    TileFin *d_fin = nullptr;
+    int32_t *d_src = nullptr;
+    uint16_t *d_dst = nullptr;
+    unsigned long long *d_partial = nullptr;
+    int32_t ntiles = 0;
+    int64_t m_tiled = 0;
+};
+
 struct PullSchedule {
     PullArgs args{};
     int nblocks = 0;
@@ -98,6 +153,11 @@ struct PullSchedule {
     void *d_partial = nullptr;
     unsigned *d_tickets = nullptr;
     bool built = false;
+    // the same schedule without the light range [light_begin, light_end), which
+    // the tiles cover (multi and block segments, then the zero-degree tail)
+    PullArgs args_hot{};
+    int nblocks_hot = 0;
+    int64_t light_begin = 0, light_end = 0;
 };
 
 // ---- graph ---------------------------------------------------------------------
@@ -126,6 +186,9 @@ struct pg_graph {
     // schedules
     pg::PullSchedule in_sched;   // pull over CSC (in-edges)
     pg::PullSchedule out_sched;  // pull over CSR (out-edges)
+    pg::TileSet tiles;           // source-sorted tiles of the CSC light range
+    bool use_tiles = true;       // PageRank-family solves use the tiles
+    int64_t tile_edges = 262144; // target edges per tile
     // cached device workspace
     std::map<std::string, std::pair<void *, size_t>> ws;
     // captured solves (hipGraph executables) and the buffer holding their result
@@ -161,6 +224,9 @@ int workspace_t(pg_graph *g, const char *name, size_t count, T **p) {
 
 int build_schedule(pg_graph *g, bool in_direction);
 int ensure_out_schedule(pg_graph *g);
+// builds (once per block threshold) the source-sorted tiles of the CSC light range
+int ensure_tiles(pg_graph *g);
+void free_tiles(pg_graph *g);
 
 // device-wide helpers implemented in pg_build.hip (hipcub)
 int exclusive_scan_u32(pg_graph *g, const uint32_t *in, uint32_t *out, int64_t n);

@@ -129,8 +129,36 @@ def test_pagerank_schedules_agree(pa, orc):
     for ipl, bt, mt, ch in [(1, 64, 128, 256), (2, 512, 1024, 512), (4, 4096, 32768, 8192),
                             (16, 100000, 200000, 8192), (8, 300, 600, 256)]:
         d.set_schedule(ipl, bt, mt, ch)
-        r, _ = d.pagerank(10, 0.85)
-        assert rel_l1(r, ref) < TOL, (ipl, bt, mt, ch)
+        for tiles in (1, 0):
+            d.set_tiles(tiles)
+            r, _ = d.pagerank(10, 0.85)
+            assert rel_l1(r, ref) < TOL, (ipl, bt, mt, ch, tiles)
+        d.set_tiles(1)
+
+
+def test_pagerank_tiles(pa, orc):
+    # source-sorted tiles: tiny tiles (a vertex larger than the target alone in its
+    # tile), default tiles, tiles capped by the vertex limit, and no tiles
+    say("case: pagerank tiles")
+    g = orc.rmat(16, 1 << 20, seed=8)
+    ref, ept = orc.pagerank(g, 20, 0.85)
+    d = pa.Graph.rmat(16, 1 << 20, seed=8)
+    d.set_schedule(0, 2048, 0, 0)
+    out = []
+    for tiles, te in ((1, 7), (1, 4096), (1, 262144), (1, 1 << 30), (0, 0)):
+        d.set_tiles(tiles, te)
+        r, st = d.pagerank(20, 0.85)
+        err = rel_l1(r, ref)
+        say("tiles %d edges %d: relL1 %.3e" % (tiles, te, err))
+        assert err < TOL, (tiles, te)
+        assert st.edges == int(ept.sum())
+        assert np.array_equal(r == 0, ref == 0)
+        out.append(r)
+    r2, _ = d.pagerank(20, 0.85)  # tiles off again: identical
+    assert np.array_equal(r2, out[-1])
+    d.set_tiles(1, 4096)
+    r3, _ = d.pagerank(20, 0.85)  # fixed-point sums: order independent, reproducible
+    assert np.array_equal(r3, out[1])
 
 
 # ---- SpMV -------------------------------------------------------------------------------
