@@ -122,6 +122,26 @@ int pg_graph_autotune(pg_graph *g, int32_t *items_per_lane, int64_t *block_thres
 /* Fills m edge values in CSR order of the input ids (pair hash, see oracle). */
 int pg_graph_bp_potentials(pg_graph *g, uint64_t seed, float lo, float hi);
 
+/* Device view of the graph in input ids for code compiled against the
+ * edgeMap/vertexMap header (polymer.h): CSR and CSC (graph.h:11-87
+ * getOutNeighbor / getInNeighbor), neighbour lists sorted by id, weights
+ * in the order of each adjacency array.  Built on first request and owned
+ * by the graph; all pointers are device pointers.  For a symmetric graph
+ * the in-arrays alias the out-arrays.  stream is the library's hipStream_t:
+ * work queued there is ordered with every library call. */
+typedef struct pg_view {
+    int64_t n, m;
+    int32_t symmetric, w_dtype;
+    const int32_t *out_off; /* n + 1 */
+    const int32_t *out_adj; /* m */
+    const int32_t *in_off;  /* n + 1 */
+    const int32_t *in_adj;  /* m */
+    const void *out_w;      /* m, or NULL */
+    const void *in_w;       /* m, or NULL */
+    void *stream;
+} pg_view;
+int pg_graph_view(pg_graph *g, pg_view *v);
+
 /* ---- vertex subsets (polymer.h:521-881 LocalFrontier / vertices) ------- */
 int pg_subset_create(pg_graph *g, pg_subset **out);
 int pg_subset_destroy(pg_subset *s);

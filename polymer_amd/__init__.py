@@ -34,7 +34,7 @@ EXPORTS = [
     "pg_init", "pg_finalize", "pg_last_error", "pg_device_name",
     "pg_graph_create", "pg_graph_rmat", "pg_graph_load_adj", "pg_graph_destroy",
     "pg_graph_info", "pg_graph_download", "pg_graph_set_schedule", "pg_graph_set_tiles",
-    "pg_graph_get_schedule", "pg_graph_autotune", "pg_graph_bp_potentials",
+    "pg_graph_get_schedule", "pg_graph_autotune", "pg_graph_bp_potentials", "pg_graph_view",
     "pg_subset_create", "pg_subset_destroy", "pg_subset_clear", "pg_subset_fill",
     "pg_subset_from_dense", "pg_subset_from_sparse", "pg_subset_to_dense",
     "pg_subset_to_sparse", "pg_subset_size",

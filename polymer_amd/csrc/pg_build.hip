@@ -1341,6 +1341,94 @@ extern "C" int pg_graph_download(const pg_graph *gc, int64_t *off, int32_t *adj,
     return PG_OK;
 }
 
+// off[v] = first i with (keys[i] >> 32) >= v, for v in [0, n]; keys sorted
+__global__ void k_offsets_from_sorted(const uint64_t *keys, int64_t m, int64_t n, int32_t *off) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t hi = i < m ? (int64_t)(keys[i] >> 32) : n;
+        const int64_t lo = i > 0 ? (int64_t)(keys[i - 1] >> 32) + 1 : 0;
+        for (int64_t v = lo; v <= hi; v++) off[v] = (int32_t)i;
+    }
+}
+
+__global__ void k_swap_halves(const uint64_t *keys, int64_t m, uint64_t *out, uint32_t *idx) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        out[i] = (k << 32) | (k >> 32);
+        idx[i] = (uint32_t)i;
+    }
+}
+
+extern "C" int pg_graph_view(pg_graph *g, pg_view *v) {
+    PG_REQUIRE(g && v, PG_ERR_ARG, "null argument");
+    const int64_t n = g->n, m = g->m;
+    const size_t wsz = 4;
+    if (!g->view_built) {
+        DevBuf k1, k2, e1, e2;
+        PG_TRY(k1.alloc(8 * (m + 1)));
+        PG_TRY(k2.alloc(8 * (m + 1)));
+        PG_TRY(e1.alloc(4 * (m + 1)));
+        PG_TRY(e2.alloc(4 * (m + 1)));
+        // CSR: (src, dst) in input ids, sorted; e2 = internal CSR edge of each entry
+        PG_LAUNCH(k_download_keys, grid_for(n), g->out_off, g->out_adj, g->iperm, n,
+                  k1.as<uint64_t>(), e1.as<uint32_t>());
+        PG_TRY(sort_pairs(k1.as<uint64_t>(), k2.as<uint64_t>(), e1.as<uint32_t>(), e2.as<uint32_t>(),
+                          m, 32 + bits_for(n + 1)));
+        PG_TRY(galloc_t(g, &g->v_out_off, n + 1));
+        PG_TRY(galloc_t(g, &g->v_out_adj, m + 1));
+        PG_LAUNCH(k_offsets_from_sorted, grid_for(m + 1), k2.as<uint64_t>(), m, n, g->v_out_off);
+        PG_LAUNCH(k_low32, grid_for(m), k2.as<uint64_t>(), m, g->v_out_adj);
+        if (g->w_dtype != PG_W_NONE) {
+            PG_TRY(galloc(g, &g->v_out_w, wsz * (m + 1)));
+            if (g->w_dtype == PG_W_I32)
+                PG_LAUNCH(k_gather<int32_t>, grid_for(m), g->out_w_i32, (const int32_t *)e2.p, m,
+                          static_cast<int32_t *>(g->v_out_w));
+            else
+                PG_LAUNCH(k_gather<float>, grid_for(m), g->out_w_f32, (const int32_t *)e2.p, m,
+                          static_cast<float *>(g->v_out_w));
+        }
+        if (g->sym) {
+            g->v_in_off = g->v_out_off;
+            g->v_in_adj = g->v_out_adj;
+            g->v_in_w = g->v_out_w;
+        } else {
+            // CSC: (dst, src) sorted; e2 = position of the entry in the view's CSR
+            PG_LAUNCH(k_swap_halves, grid_for(m), k2.as<uint64_t>(), m, k1.as<uint64_t>(),
+                      e1.as<uint32_t>());
+            PG_TRY(sort_pairs(k1.as<uint64_t>(), k2.as<uint64_t>(), e1.as<uint32_t>(),
+                              e2.as<uint32_t>(), m, 32 + bits_for(n + 1)));
+            PG_TRY(galloc_t(g, &g->v_in_off, n + 1));
+            PG_TRY(galloc_t(g, &g->v_in_adj, m + 1));
+            PG_LAUNCH(k_offsets_from_sorted, grid_for(m + 1), k2.as<uint64_t>(), m, n, g->v_in_off);
+            PG_LAUNCH(k_low32, grid_for(m), k2.as<uint64_t>(), m, g->v_in_adj);
+            if (g->w_dtype != PG_W_NONE) {
+                PG_TRY(galloc(g, &g->v_in_w, wsz * (m + 1)));
+                if (g->w_dtype == PG_W_I32)
+                    PG_LAUNCH(k_gather<int32_t>, grid_for(m), static_cast<const int32_t *>(g->v_out_w),
+                              (const int32_t *)e2.p, m, static_cast<int32_t *>(g->v_in_w));
+                else
+                    PG_LAUNCH(k_gather<float>, grid_for(m), static_cast<const float *>(g->v_out_w),
+                              (const int32_t *)e2.p, m, static_cast<float *>(g->v_in_w));
+            }
+        }
+        PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+        g->view_built = true;
+    }
+    v->n = n;
+    v->m = m;
+    v->symmetric = g->sym ? 1 : 0;
+    v->w_dtype = g->w_dtype;
+    v->out_off = g->v_out_off;
+    v->out_adj = g->v_out_adj;
+    v->in_off = g->v_in_off;
+    v->in_adj = g->v_in_adj;
+    v->out_w = g->v_out_w;
+    v->in_w = g->v_in_w;
+    v->stream = (void *)g_stream;
+    return PG_OK;
+}
+
 extern "C" int pg_graph_set_schedule(pg_graph *g, int items_per_lane, int64_t block_threshold,
                                      int64_t multi_threshold, int64_t chunk) {
     PG_REQUIRE(g, PG_ERR_ARG, "null graph");

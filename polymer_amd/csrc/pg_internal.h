@@ -189,6 +189,10 @@ struct pg_graph {
     pg::TileSet tiles;           // source-sorted tiles of the CSC light range
     bool use_tiles = true;       // PageRank-family solves use the tiles
     int64_t tile_edges = 262144; // target edges per tile
+    // input-id CSR / CSC for the header-level edgeMap API (built on request)
+    bool view_built = false;
+    int32_t *v_out_off = nullptr, *v_out_adj = nullptr, *v_in_off = nullptr, *v_in_adj = nullptr;
+    void *v_out_w = nullptr, *v_in_w = nullptr;
     // cached device workspace
     std::map<std::string, std::pair<void *, size_t>> ws;
     // captured solves (hipGraph executables) and the buffer holding their result
