@@ -60,13 +60,17 @@ def bfs_extra(pa, scale=22, m=1 << 26, nsrc=8):
     rng = np.random.default_rng(7)
     srcs = [0] + [int(x) for x in rng.integers(0, gs.n, nsrc)]
     gs.bfs(0, out=False)  # warm-up / workspace
-    tot_ms, tot_edges = 0.0, 0
+    tot_ms, tot_edges, tot_bytes = 0.0, 0, 0
     for s in srcs:
         _, _, st = gs.bfs(s, out=False)
         tot_ms += st.ms
         tot_edges += st.edges  # directed entries of the reached vertices
+        tot_bytes += 4 * st.edges + 16 * st.active  # 4 m_reached + 16 n_reached
+    gbs = tot_bytes / (tot_ms / 1e3) / 1e9
     return {"bfs_scale22_gteps_graph500": round(tot_edges / 2 / (tot_ms / 1e3) / 1e9, 3),
             "bfs_scale22_gteps_directed": round(tot_edges / (tot_ms / 1e3) / 1e9, 3),
+            "bfs_scale22_algorithmic_gbs": round(gbs, 1),
+            "bfs_scale22_hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
             "bfs_sources": len(srcs)}
 
 
@@ -89,9 +93,10 @@ def profile_run(args):
     pa.init(0)
     g = pa.Graph.rmat(args.scale, args.edges, seed=1)
     sched = [int(x) for x in args.schedule.split(",")]
-    g.set_tiles(sched[4], sched[5])
+    g.set_tiles(sched[4], sched[5], sched[6])
     g.set_schedule(*sched[:4])
-    log("pull schedule (ipl, block, multi, chunk, tiles, tile edges):", g.get_schedule())
+    log("pull schedule (ipl, block, multi, chunk, tiles, tile edges, cold from):",
+        g.get_schedule())
     g.pagerank(args.iters, 0.85, out=False)
     _, st = g.pagerank(args.iters, 0.85, out=False, profile=True)
     log("pagerank %d iterations: %.3f ms, per-iteration pull avg %.1f us, edges %d" %
@@ -188,8 +193,8 @@ def main():
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--schedule", default="16,16384,65536,16384,1,262144",
-                    help="--profile only: ipl,block,multi,chunk,tiles,tile_edges")
+    ap.add_argument("--schedule", default="16,16384,65536,16384,1,524288,2097152",
+                    help="--profile only: ipl,block,multi,chunk,tiles,tile_edges,cold_from")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -227,8 +232,9 @@ def main():
         (ms_per_step, dev_ms / args.steps, gteps))
 
     # dominant kernel: the pull of one iteration (with tiles, the launches
-    # k_pull<PrOp> hot segments + k_pull_tiles<PrOp> + k_tile_finish<PrOp>),
-    # bracketed by HIP events on the library stream, iteration by iteration
+    # k_cold_gather<PrOp> + k_pull<PrOp> hot segments + k_pull_tiles<PrOp> +
+    # k_tile_finish<PrOp>), bracketed by HIP events on the library stream,
+    # iteration by iteration
     _, stp = g.pagerank(args.iters, 0.85, out=False, profile=True)
     avg_kernel_s = stp.kernel_ms / max(stp.kernel_launches, 1) / 1e3
     bytes_per_launch = pagerank_bytes(g.n, stp.edges, args.iters) / args.iters
@@ -236,6 +242,8 @@ def main():
     traffic, traffic_src = measured_traffic(schedule)
     kernels = ("k_pull<PrOp> (hot) + k_pull_tiles<PrOp> + k_tile_finish<PrOp>" if schedule[4]
                else "k_pull<PrOp>")
+    if schedule[4] and schedule[6] >= 0:
+        kernels = "k_cold_gather<PrOp> + " + kernels
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": traffic_src,
@@ -243,7 +251,13 @@ def main():
                 "bytes_per_launch": int(bytes_per_launch)}
     log("pull iteration avg %.1f us, %.1f GB/s algorithmic" % (avg_kernel_s * 1e6, achieved))
 
-    extra = {}
+    # the same solve with the ranks copied to host memory in input order (the
+    # PCIe-inclusive rate; never the reported value)
+    g.pagerank(args.iters, 0.85, out=True)
+    t1 = time.perf_counter()
+    g.pagerank(args.iters, 0.85, out=True)
+    host_ms = (time.perf_counter() - t1) * 1e3
+    extra = {"pagerank_step_with_host_copy_ms": round(host_ms, 3)}
     if not args.no_extra:
         for name, fn in (("bfs", bfs_extra), ("batched", batched_extra),
                          ("adjoint", adjoint_extra), ("multigraph", multigraph_extra)):

@@ -82,7 +82,8 @@ struct PrOp {
     // one graph sum to at most 1 (no teleport mass is created beyond (1-d)), so
     // every per-vertex sum of contributions is <= 1 < 2^(64-62) and cannot
     // overflow; the truncation of each term is < 2^-62 absolute.
-    __device__ unsigned long long tile_value(int s) const { return to_fixed<62>(contrib[s]); }
+    __device__ float tile_raw(int s) const { return contrib[s]; }
+    __device__ unsigned long long tile_fix(float x) const { return to_fixed<62>(x); }
     __device__ float tile_sum(unsigned long long q) const { return (float)((double)q * 0x1p-62); }
 };
 

@@ -772,6 +772,10 @@ void free_tiles(pg_graph *g) {
     if (t.d_src) (void)hipFree(t.d_src);
     if (t.d_dst) (void)hipFree(t.d_dst);
     if (t.d_partial) (void)hipFree(t.d_partial);
+    if (t.d_cold_parts) (void)hipFree(t.d_cold_parts);
+    if (t.d_cold_pos) (void)hipFree(t.d_cold_pos);
+    if (t.d_cold_srcl) (void)hipFree(t.d_cold_srcl);
+    if (t.d_cold_val) (void)hipFree(t.d_cold_val);
     t = TileSet{};
 }
 
@@ -799,12 +803,136 @@ __global__ void k_tile_split(const uint64_t *keys, const uint32_t *vals, int64_t
     }
 }
 
+// first position in [tb[t], tb[t+1]) of src whose value is >= cold (src sorted there)
+__global__ void k_tile_cold_start(const int32_t *src, const int64_t *tb, int nt, int32_t cold,
+                                  int64_t *out) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+        int64_t lo = tb[t], hi = tb[t + 1];
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (src[mid] < cold) lo = mid + 1; else hi = mid;
+        }
+        out[t] = lo;
+    }
+}
+
+// key = chunk << 32 | cold value index, value = source - chunk's first source,
+// for every cold tile edge (tile found by binary search over tile edge starts)
+__global__ void k_cold_keys(const int32_t *src, int64_t mt, const int64_t *tb, int nt,
+                            const int64_t *cstart, const int64_t *cbase, int32_t cold,
+                            uint64_t *keys, uint32_t *vals) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mt;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nt - 1;  // last tile with tb[t] <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tb[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        if (i < cstart[lo]) continue;
+        const int64_t k = cbase[lo] + (i - cstart[lo]);
+        const uint32_t rel = (uint32_t)(src[i] - cold);
+        keys[k] = ((uint64_t)(rel / kColdChunk) << 32) | (uint64_t)k;
+        vals[k] = rel % kColdChunk;
+    }
+}
+
+__global__ void k_cold_split(const uint64_t *keys, const uint32_t *vals, int64_t m, uint32_t *pos,
+                             uint16_t *srcl, uint32_t *chunk) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        pos[i] = (uint32_t)keys[i];
+        srcl[i] = (uint16_t)vals[i];
+        chunk[i] = (uint32_t)(keys[i] >> 32);
+    }
+}
+
+// off[c] = first i with chunk[i] >= c, c in [0, nc]
+__global__ void k_chunk_offsets(const uint32_t *chunk, int64_t m, int64_t nc, int64_t *off) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t hi = i < m ? (int64_t)chunk[i] : nc;
+        const int64_t lo = i > 0 ? (int64_t)chunk[i - 1] + 1 : 0;
+        for (int64_t c = lo; c <= hi; c++) off[c] = i;
+    }
+}
+
+// Builds the cold pre-pass of the tiles: for each tile the first edge whose source
+// is cold (tile lists are sorted by source), the cold value array in tile order,
+// and the pre-pass lists grouped by source chunk.  Fills the cold fields of parts.
+static int build_cold(pg_graph *g, const std::vector<int32_t> &vbeg,
+                      const std::vector<int32_t> &off_h, std::vector<TilePart> &parts) {
+    TileSet &t = g->tiles;
+    const int nt = (int)vbeg.size() - 1;
+    const int64_t p0 = vbeg.front(), mt = (int64_t)off_h.back() - off_h[0];
+    const int32_t cold = (int32_t)g->tile_cold;
+    std::vector<int64_t> tb((size_t)nt + 1);
+    for (int k = 0; k <= nt; k++) tb[k] = (int64_t)off_h[vbeg[k] - p0] - off_h[0];
+    DevBuf dtb, dcs, dcb;
+    PG_TRY(dtb.alloc(sizeof(int64_t) * (nt + 1)));
+    PG_TRY(dcs.alloc(sizeof(int64_t) * (nt + 1)));
+    PG_TRY(dcb.alloc(sizeof(int64_t) * (nt + 1)));
+    PG_CHECK_HIP(hipMemcpy(dtb.p, tb.data(), sizeof(int64_t) * tb.size(), hipMemcpyHostToDevice));
+    PG_LAUNCH(k_tile_cold_start, grid_for(nt), t.d_src, dtb.as<int64_t>(), nt, cold, dcs.as<int64_t>());
+    std::vector<int64_t> cs((size_t)nt), cb((size_t)nt + 1);
+    // the library stream does not synchronise with the null stream: copy on it
+    PG_CHECK_HIP(hipMemcpyAsync(cs.data(), dcs.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    cb[0] = 0;
+    for (int k = 0; k < nt; k++) cb[k + 1] = cb[k] + (tb[k + 1] - cs[k]);
+    const int64_t mc = cb[nt];
+    PG_REQUIRE(mc < ((int64_t)1 << 32), PG_ERR_UNSUPPORTED, "tiles: too many cold edges");
+    for (auto &pt : parts) {
+        const int64_t c0 = cs[pt.tile];
+        pt.cold_e = std::min(std::max(c0, pt.e0), pt.e1);
+        pt.cold_v = cb[pt.tile] + (pt.cold_e - c0);
+    }
+    t.m_cold = mc;
+    if (mc == 0) return PG_OK;
+    PG_CHECK_HIP(hipMemcpy(dcb.p, cb.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice));
+    DevBuf k0, k1, v0, v1, ch;
+    PG_TRY(k0.alloc(sizeof(uint64_t) * mc));
+    PG_TRY(k1.alloc(sizeof(uint64_t) * mc));
+    PG_TRY(v0.alloc(sizeof(uint32_t) * mc));
+    PG_TRY(v1.alloc(sizeof(uint32_t) * mc));
+    PG_TRY(ch.alloc(sizeof(uint32_t) * mc));
+    PG_LAUNCH(k_cold_keys, grid_for(mt), t.d_src, mt, dtb.as<int64_t>(), nt, dcs.as<int64_t>(),
+              dcb.as<int64_t>(), cold, k0.as<uint64_t>(), v0.as<uint32_t>());
+    const int64_t nchunks = (g->n - cold + kColdChunk - 1) / kColdChunk;
+    PG_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), mc,
+                      32 + bits_for(nchunks + 1)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_cold_pos), sizeof(uint32_t) * (mc + 16)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_cold_srcl), sizeof(uint16_t) * (mc + 16)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_cold_val), sizeof(float) * (mc + 16)));
+    PG_LAUNCH(k_cold_split, grid_for(mc), k1.as<uint64_t>(), v1.as<uint32_t>(), mc, t.d_cold_pos,
+              t.d_cold_srcl, ch.as<uint32_t>());
+    DevBuf dco;
+    PG_TRY(dco.alloc(sizeof(int64_t) * (nchunks + 1)));
+    PG_LAUNCH(k_chunk_offsets, grid_for(mc + 1), ch.as<uint32_t>(), mc, nchunks, dco.as<int64_t>());
+    std::vector<int64_t> co((size_t)nchunks + 1);
+    PG_CHECK_HIP(hipMemcpyAsync(co.data(), dco.p, sizeof(int64_t) * co.size(), hipMemcpyDeviceToHost,
+                                g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    std::vector<ColdPart> cparts;
+    const int64_t per = 65536;  // pre-pass edges per workgroup
+    for (int64_t c = 0; c < nchunks; c++)
+        for (int64_t a = co[c]; a < co[c + 1]; a += per)
+            cparts.push_back(ColdPart{(int32_t)c, 0, a, std::min(co[c + 1], a + per)});
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_cold_parts),
+                           sizeof(ColdPart) * (cparts.size() + 1)));
+    if (!cparts.empty())
+        PG_CHECK_HIP(hipMemcpy(t.d_cold_parts, cparts.data(), sizeof(ColdPart) * cparts.size(),
+                               hipMemcpyHostToDevice));
+    t.cold = ColdArgs{t.d_cold_parts, t.d_cold_pos, t.d_cold_srcl, t.d_cold_val,
+                      (int32_t)cparts.size(), cold, g->n};
+    return PG_OK;
+}
+
 int ensure_tiles(pg_graph *g) {
     PullSchedule &s = g->in_sched;
     PG_REQUIRE(s.built, PG_ERR_STATE, "tiles: the pull schedule is not built");
     TileSet &t = g->tiles;
     if (t.built && t.light_begin == s.light_begin && t.light_end == s.light_end &&
-        t.tile_edges == g->tile_edges)
+        t.tile_edges == g->tile_edges && t.cold_begin == g->tile_cold)
         return PG_OK;
     drop_captures(g);
     PG_CHECK_HIP(hipStreamSynchronize(g_stream));
@@ -850,9 +978,6 @@ int ensure_tiles(pg_graph *g) {
     PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_partial),
                            sizeof(unsigned long long) * (slots + 1)));
     PG_CHECK_HIP(hipMemcpy(t.d_vbeg, vbeg.data(), sizeof(int32_t) * vbeg.size(), hipMemcpyHostToDevice));
-    if (!parts.empty())
-        PG_CHECK_HIP(hipMemcpy(t.d_parts, parts.data(), sizeof(TilePart) * parts.size(),
-                               hipMemcpyHostToDevice));
     if (!fin.empty())
         PG_CHECK_HIP(hipMemcpy(t.d_fin, fin.data(), sizeof(TileFin) * fin.size(), hipMemcpyHostToDevice));
     if (mt > 0) {
@@ -869,8 +994,20 @@ int ensure_tiles(pg_graph *g) {
                   t.d_dst);
         PG_CHECK_HIP(hipStreamSynchronize(g_stream));
     }
-    t.args = TileArgs{t.d_vbeg,    t.d_parts,           t.d_fin,           t.d_src,
-                      t.d_dst,     t.d_partial,         (int32_t)parts.size(), (int32_t)fin.size()};
+    // cold split: every part's first cold edge; the pre-pass lists by source chunk
+    for (auto &pt : parts) {
+        pt.cold_e = pt.e1;
+        pt.cold_v = 0;
+    }
+    t.cold = ColdArgs{};
+    t.cold_begin = g->tile_cold;
+    if (mt > 0 && g->tile_cold >= 0 && g->tile_cold < g->n) PG_TRY(build_cold(g, vbeg, off_h, parts));
+    if (!parts.empty())
+        PG_CHECK_HIP(hipMemcpy(t.d_parts, parts.data(), sizeof(TilePart) * parts.size(),
+                               hipMemcpyHostToDevice));
+    t.args = TileArgs{t.d_vbeg,   t.d_parts,       t.d_fin,
+                      t.d_src,    t.d_dst,         t.d_partial,
+                      t.d_cold_val, (int32_t)parts.size(), (int32_t)fin.size()};
     t.ntiles = nt;
     t.m_tiled = mt;
     t.tile_edges = g->tile_edges;
@@ -1443,19 +1580,22 @@ extern "C" int pg_graph_set_schedule(pg_graph *g, int items_per_lane, int64_t bl
     return PG_OK;
 }
 
-extern "C" int pg_graph_set_tiles(pg_graph *g, int enable, int64_t tile_edges) {
+extern "C" int pg_graph_set_tiles(pg_graph *g, int enable, int64_t tile_edges, int64_t cold_from) {
     PG_REQUIRE(g, PG_ERR_ARG, "null graph");
     PG_REQUIRE(tile_edges >= 0 && tile_edges <= (int64_t)1 << 30, PG_ERR_ARG,
                "tile_edges out of range: %lld", (long long)tile_edges);
+    PG_REQUIRE(cold_from >= -1, PG_ERR_ARG, "cold_from out of range: %lld", (long long)cold_from);
     drop_captures(g);
     if (enable >= 0) g->use_tiles = enable != 0;
     if (tile_edges > 0) g->tile_edges = tile_edges;
+    if (cold_from != 0) g->tile_cold = cold_from;
     return PG_OK;
 }
 
 extern "C" int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane,
                                      int64_t *block_threshold, int64_t *multi_threshold,
-                                     int64_t *chunk, int32_t *tiles, int64_t *tile_edges) {
+                                     int64_t *chunk, int32_t *tiles, int64_t *tile_edges,
+                                     int64_t *cold_from) {
     PG_REQUIRE(g, PG_ERR_ARG, "null graph");
     const PullParams &p = g->in_sched.params;
     if (items_per_lane) *items_per_lane = p.items_per_lane;
@@ -1464,6 +1604,7 @@ extern "C" int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane,
     if (chunk) *chunk = p.chunk;
     if (tiles) *tiles = g->use_tiles ? 1 : 0;
     if (tile_edges) *tile_edges = g->tile_edges;
+    if (cold_from) *cold_from = g->tile_cold;
     return PG_OK;
 }
 

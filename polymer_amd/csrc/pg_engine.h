@@ -377,8 +377,9 @@ inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *s
 // workgroup streams its tile's (source, destination) pairs, sorted by source, and
 // adds the gathered values into per-destination accumulators in LDS.  An Op used
 // here also provides
-//     unsigned long long tile_value(int s);  // the gathered value as fixed point
-//     Acc tile_sum(unsigned long long q);     // an accumulated fixed-point sum
+//     float tile_raw(int s);                      // the value gathered from source s
+//     unsigned long long tile_fix(float x);       // that value as fixed point
+//     Acc tile_sum(unsigned long long q);         // an accumulated fixed-point sum
 // Integer addition is associative, so the result does not depend on the order in
 // which the LDS atomics land.  The per-vertex protocol (active / finish / skip,
 // statistics) is the one k_pull follows.
@@ -415,12 +416,12 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
     const int nv = ta.vbeg[t + 1] - vb;  // <= kTileMaxV by construction
     for (int i = threadIdx.x; i < nv; i += kTileThreads) acc[i] = 0ull;
     __syncthreads();
-    {
+    constexpr int U = 8;
+    {  // warm sources: gathered
         const int32_t *__restrict__ src = ta.src + part.e0;
         const uint16_t *__restrict__ dst = ta.dst + part.e0;
-        const int cnt = (int)(part.e1 - part.e0);
+        const int cnt = (int)(part.cold_e - part.e0);
         int e = threadIdx.x;
-        constexpr int U = 8;
         for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
             int s[U];
             unsigned d[U];
@@ -431,12 +432,31 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
             }
             unsigned long long q[U];
 #pragma unroll
-            for (int k = 0; k < U; k++) q[k] = op.tile_value(s[k]);
+            for (int k = 0; k < U; k++) q[k] = op.tile_fix(op.tile_raw(s[k]));
 #pragma unroll
             for (int k = 0; k < U; k++) atomicAdd(&acc[d[k]], q[k]);
         }
         for (; e < cnt; e += kTileThreads)
-            atomicAdd(&acc[ld_stream_u16(dst + e)], op.tile_value(ld_stream(src + e)));
+            atomicAdd(&acc[ld_stream_u16(dst + e)], op.tile_fix(op.tile_raw(ld_stream(src + e))));
+    }
+    {  // cold sources: values streamed from the pre-pass
+        const float *__restrict__ val = ta.cold_val + part.cold_v;
+        const uint16_t *__restrict__ dst = ta.dst + part.cold_e;
+        const int cnt = (int)(part.e1 - part.cold_e);
+        int e = threadIdx.x;
+        for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
+            float x[U];
+            unsigned d[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                x[k] = __builtin_nontemporal_load(val + e + k * kTileThreads);
+                d[k] = ld_stream_u16(dst + e + k * kTileThreads);
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++) atomicAdd(&acc[d[k]], op.tile_fix(x[k]));
+        }
+        for (; e < cnt; e += kTileThreads)
+            atomicAdd(&acc[ld_stream_u16(dst + e)], op.tile_fix(__builtin_nontemporal_load(val + e)));
     }
     __syncthreads();
     if (part.slot >= 0) {
@@ -467,6 +487,36 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
             if (tot != Stat(0)) Op::stat_commit(stat_out, tot);
         }
     }
+}
+
+// Cold pre-pass: one workgroup per slice of a source chunk's list stages the
+// chunk's kColdChunk source values in LDS, then writes each listed edge's value
+// to its place in the cold value array (runs of consecutive places per tile).
+template <class Op>
+__global__ __launch_bounds__(kTileThreads) void k_cold_gather(Op op, ColdArgs ca) {
+    __shared__ float sv[kColdChunk];
+    const ColdPart cp = ca.parts[blockIdx.x];
+    const int64_t base = (int64_t)ca.cold_begin + (int64_t)cp.chunk * kColdChunk;
+    const int len = (int)min((int64_t)kColdChunk, ca.n - base);
+    for (int j = threadIdx.x; j < len; j += kTileThreads) sv[j] = op.tile_raw((int)(base + j));
+    __syncthreads();
+    const uint32_t *__restrict__ pos = ca.pos + cp.a;
+    const uint16_t *__restrict__ srcl = ca.srcl + cp.a;
+    const int cnt = (int)(cp.b - cp.a);
+    constexpr int U = 4;
+    int e = threadIdx.x;
+    for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
+        uint32_t p[U];
+        unsigned l[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            p[k] = __builtin_nontemporal_load(pos + e + k * kTileThreads);
+            l[k] = ld_stream_u16(srcl + e + k * kTileThreads);
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) ca.val[p[k]] = sv[l[k]];
+    }
+    for (; e < cnt; e += kTileThreads) ca.val[__builtin_nontemporal_load(pos + e)] = sv[ld_stream_u16(srcl + e)];
 }
 
 // Vertices of the tiles with several parts: one thread per vertex sums the
@@ -512,6 +562,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
 template <class Op>
 inline int launch_pull_tiled(const Op &op, const PullSchedule &s, const TileSet &t,
                              typename Op::Stat *stat_out = nullptr) {
+    if (t.cold.nparts > 0)
+        hipLaunchKernelGGL(k_cold_gather<Op>, dim3(t.cold.nparts), dim3(kTileThreads), 0, stream(),
+                           op, t.cold);
     if (s.nblocks_hot > 0)
         hipLaunchKernelGGL(k_pull<Op>, dim3(s.nblocks_hot), dim3(kBlock), 0, stream(), op,
                            s.args_hot, stat_out);

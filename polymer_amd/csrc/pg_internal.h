@@ -109,6 +109,31 @@ struct TilePart {
     int32_t pad;
     int64_t slot;    // element offset of this part's partial slot, -1 = single part
     int64_t e0, e1;  // edge range in src / dst
+    int64_t cold_e;  // first edge in [e0, e1] whose source is cold (e1 if none)
+    int64_t cold_v;  // index in the cold value array of edge cold_e
+};
+
+// Cold sources (internal id >= cold_begin: low out-degree, little reuse) are not
+// gathered by the tile kernel.  A pre-pass walks the cold edges grouped by source
+// chunk (kColdChunk consecutive sources staged in LDS) and writes each edge's
+// value into a cold value array laid out in tile order; the tile kernel streams
+// it.  A random line fetch per cold edge becomes about 10 streamed bytes.
+constexpr int kColdChunk = 16384;  // sources per chunk (64 KiB of LDS)
+
+struct ColdPart {
+    int32_t chunk;
+    int32_t pad;
+    int64_t a, b;  // range in the pre-pass lists
+};
+
+struct ColdArgs {
+    const ColdPart *parts;
+    const uint32_t *pos;    // index in the cold value array
+    const uint16_t *srcl;   // source - chunk's first source
+    float *val;             // cold values, tile order
+    int32_t nparts;
+    int32_t cold_begin;
+    int64_t n;
 };
 
 struct TileFin {     // kBlock vertices of a tile with several parts
@@ -126,6 +151,7 @@ struct TileArgs {
     const int32_t *src;          // source ids, sorted within a tile
     const uint16_t *dst;         // destination position - vbeg[tile]
     unsigned long long *partial; // fixed-point partial sums
+    const float *cold_val;       // values of the cold edges (written by the pre-pass)
     int32_t nparts;
     int32_t nfin;
 };
@@ -143,6 +169,14 @@ struct TileSet {
     unsigned long long *d_partial = nullptr;
     int32_t ntiles = 0;
     int64_t m_tiled = 0;
+    // cold split
+    int64_t cold_begin = -1;  // the setting the tiles were built for
+    ColdArgs cold{};
+    ColdPart *d_cold_parts = nullptr;
+    uint32_t *d_cold_pos = nullptr;
+    uint16_t *d_cold_srcl = nullptr;
+    float *d_cold_val = nullptr;
+    int64_t m_cold = 0;
 };
 
 struct PullSchedule {
@@ -189,6 +223,7 @@ struct pg_graph {
     pg::TileSet tiles;           // source-sorted tiles of the CSC light range
     bool use_tiles = true;       // PageRank-family solves use the tiles
     int64_t tile_edges = 262144; // target edges per tile
+    int64_t tile_cold = 1 << 20; // first cold source id of the tiles (<0: no cold split)
     // input-id CSR / CSC for the header-level edgeMap API (built on request)
     bool view_built = false;
     int32_t *v_out_off = nullptr, *v_out_adj = nullptr, *v_in_off = nullptr, *v_in_adj = nullptr;

@@ -145,20 +145,26 @@ def test_pagerank_tiles(pa, orc):
     d = pa.Graph.rmat(16, 1 << 20, seed=8)
     d.set_schedule(0, 2048, 0, 0)
     out = []
-    for tiles, te in ((1, 7), (1, 4096), (1, 262144), (1, 1 << 30), (0, 0)):
-        d.set_tiles(tiles, te)
+    # (tiles, edges per part, first cold source): the cold pre-pass from source 1000,
+    # 5000, 30000 and none
+    for tiles, te, cold in ((1, 7, -1), (1, 4096, -1), (1, 262144, -1), (1, 1 << 30, -1),
+                            (1, 4096, 1000), (1, 262144, 5000), (1, 7, 30000), (0, 0, -1)):
+        d.set_tiles(tiles, te, cold)
         r, st = d.pagerank(20, 0.85)
         err = rel_l1(r, ref)
-        say("tiles %d edges %d: relL1 %.3e" % (tiles, te, err))
-        assert err < TOL, (tiles, te)
+        say("tiles %d edges %d cold %d: relL1 %.3e" % (tiles, te, cold, err))
+        assert err < TOL, (tiles, te, cold)
         assert st.edges == int(ept.sum())
         assert np.array_equal(r == 0, ref == 0)
         out.append(r)
     r2, _ = d.pagerank(20, 0.85)  # tiles off again: identical
     assert np.array_equal(r2, out[-1])
-    d.set_tiles(1, 4096)
+    d.set_tiles(1, 4096, -1)
     r3, _ = d.pagerank(20, 0.85)  # fixed-point sums: order independent, reproducible
     assert np.array_equal(r3, out[1])
+    d.set_tiles(1, 4096, 1000)  # the cold values are the same fp32 numbers: same sums
+    r4, _ = d.pagerank(20, 0.85)
+    assert np.array_equal(r4, out[1])
 
 
 # ---- SpMV -------------------------------------------------------------------------------
