@@ -193,7 +193,7 @@ def main():
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--schedule", default="16,16384,65536,16384,1,524288,2097152",
+    ap.add_argument("--schedule", default="16,1073741824,1073741825,16384,1,524288,1048576",
                     help="--profile only: ipl,block,multi,chunk,tiles,tile_edges,cold_from")
     args = ap.parse_args()
     if args.gpus != 1:

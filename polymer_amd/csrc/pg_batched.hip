@@ -711,13 +711,13 @@ extern "C" int pg_graph_autotune(pg_graph *g, int32_t *items_per_lane, int64_t *
         int tiles;
         int64_t te, cold;
     };
-    const int64_t M = 1 << 20;
+    const int64_t M = 1 << 20, ALL = (int64_t)1 << 30;  // ALL: every vertex in tiles
     const Cand cands[] = {
-        {16, 16384, 65536, 16384, 1, 524288, 2 * M}, {16, 16384, 65536, 16384, 1, 524288, M},
-        {16, 65536, 131072, 16384, 1, 524288, 2 * M}, {16, 16384, 65536, 16384, 1, 262144, M},
-        {16, 4096, 32768, 16384, 1, 524288, 2 * M},  {16, 16384, 65536, 16384, 1, 262144, -1},
-        {16, 65536, 131072, 16384, 1, 262144, -1},   {8, 4096, 32768, 8192, 0, 0, -1},
-        {16, 16384, 65536, 16384, 0, 0, -1},         {32, 16384, 65536, 16384, 0, 0, -1}};
+        {16, ALL, ALL + 1, 16384, 1, 524288, 2 * M},  {16, 16384, 65536, 16384, 1, 524288, 2 * M},
+        {16, ALL, ALL + 1, 16384, 1, 524288, M},      {16, 262144, 524288, 16384, 1, 524288, 2 * M},
+        {16, 16384, 65536, 16384, 1, 262144, M},      {16, ALL, ALL + 1, 16384, 1, 262144, -1},
+        {16, 16384, 65536, 16384, 1, 262144, -1},     {8, 4096, 32768, 8192, 0, 0, -1},
+        {16, 16384, 65536, 16384, 0, 0, -1},          {32, 16384, 65536, 16384, 0, 0, -1}};
     const PullParams saved = g->in_sched.params;
     double best = 1e30;
     Cand bestc{saved.items_per_lane, saved.block_threshold, saved.multi_threshold,

@@ -754,6 +754,7 @@ int build_schedule(pg_graph *g, bool in_direction) {
         h.seg[h.nseg++] = a.seg[i];
         hblocks = (i + 1 < a.nseg) ? a.seg[i + 1].block_begin : blocks;
     }
+    s.nblocks_hot_body = (int)hblocks;
     if (s.light_end < n) {
         h.seg[h.nseg++] = PullSeg{(int32_t)s.light_end, (int32_t)n, 1, (int32_t)hblocks};
         hblocks += (n - s.light_end + kBlock - 1) / kBlock;
@@ -772,6 +773,9 @@ void free_tiles(pg_graph *g) {
     if (t.d_src) (void)hipFree(t.d_src);
     if (t.d_dst) (void)hipFree(t.d_dst);
     if (t.d_partial) (void)hipFree(t.d_partial);
+    if (t.d_packed) (void)hipFree(t.d_packed);
+    if (t.d_gbase) (void)hipFree(t.d_gbase);
+    if (t.d_esc) (void)hipFree(t.d_esc);
     if (t.d_cold_parts) (void)hipFree(t.d_cold_parts);
     if (t.d_cold_pos) (void)hipFree(t.d_cold_pos);
     if (t.d_cold_srcl) (void)hipFree(t.d_cold_srcl);
@@ -854,6 +858,74 @@ __global__ void k_chunk_offsets(const uint32_t *chunk, int64_t m, int64_t nc, in
         const int64_t lo = i > 0 ? (int64_t)chunk[i - 1] + 1 : 0;
         for (int64_t c = lo; c <= hi; c++) off[c] = i;
     }
+}
+
+// per 64-edge group: min and max of the warm sources (src < cold); empty -> min = max = 0
+__global__ void k_group_span(const int32_t *src, int64_t mt, int32_t cold, int32_t *gmin,
+                             uint8_t *wide) {
+    const int64_t ng = (mt + kPackGroup - 1) / kPackGroup;
+    for (int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < ng;
+         gi += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0x7FFFFFFF, hi = -1;
+        const int64_t e1 = min(mt, (gi + 1) * kPackGroup);
+        for (int64_t i = gi * kPackGroup; i < e1; i++) {
+            const int32_t s = src[i];
+            if (s < cold) {
+                lo = min(lo, s);
+                hi = max(hi, s);
+            }
+        }
+        if (hi < 0) lo = hi = 0;
+        gmin[gi] = lo;
+        wide[gi] = (int64_t)hi - lo >= ((int64_t)1 << (32 - kPackDstBits)) ? 1 : 0;
+    }
+}
+
+__global__ void k_tile_pack(const int32_t *src, const uint16_t *dst, int64_t mt, int32_t cold,
+                            const int32_t *gbase, uint32_t *packed, int32_t *esc) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mt;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t s = src[i], b = gbase[i / kPackGroup];
+        uint32_t w = dst[i];
+        if (s < cold) {
+            if (b >= 0)
+                w |= (uint32_t)(s - b) << kPackDstBits;
+            else
+                esc[(int64_t)(-b - 1) * kPackGroup + (i % kPackGroup)] = s;
+        }
+        packed[i] = w;
+    }
+}
+
+// Packs the tile edges (source offset + destination in one word, escape rows for
+// wide groups) and frees the separate source / destination arrays.
+static int pack_tiles(pg_graph *g, int64_t mt, int32_t cold) {
+    TileSet &t = g->tiles;
+    const int64_t ng = (mt + kPackGroup - 1) / kPackGroup;
+    DevBuf dmin, dwide;
+    PG_TRY(dmin.alloc(sizeof(int32_t) * (ng + 1)));
+    PG_TRY(dwide.alloc(ng + 1));
+    PG_LAUNCH(k_group_span, grid_for(ng), t.d_src, mt, cold, dmin.as<int32_t>(), dwide.as<uint8_t>());
+    std::vector<int32_t> base((size_t)ng);
+    std::vector<uint8_t> wide((size_t)ng);
+    PG_CHECK_HIP(hipMemcpyAsync(base.data(), dmin.p, sizeof(int32_t) * ng, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipMemcpyAsync(wide.data(), dwide.p, ng, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    int64_t nesc = 0;
+    for (int64_t gi = 0; gi < ng; gi++)
+        if (wide[gi]) base[gi] = -(int32_t)(++nesc);
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_gbase), sizeof(int32_t) * (ng + 1)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_esc), sizeof(int32_t) * (nesc * kPackGroup + 1)));
+    PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_packed), sizeof(uint32_t) * (mt + 16)));
+    PG_CHECK_HIP(hipMemcpyAsync(t.d_gbase, base.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice, g_stream));
+    PG_LAUNCH(k_tile_pack, grid_for(mt), t.d_src, t.d_dst, mt, cold, t.d_gbase, t.d_packed, t.d_esc);
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    (void)hipFree(t.d_src);
+    (void)hipFree(t.d_dst);
+    t.d_src = nullptr;
+    t.d_dst = nullptr;
+    t.n_esc = nesc;
+    return PG_OK;
 }
 
 // Builds the cold pre-pass of the tiles: for each tile the first edge whose source
@@ -1001,13 +1073,15 @@ int ensure_tiles(pg_graph *g) {
     }
     t.cold = ColdArgs{};
     t.cold_begin = g->tile_cold;
-    if (mt > 0 && g->tile_cold >= 0 && g->tile_cold < g->n) PG_TRY(build_cold(g, vbeg, off_h, parts));
+    const bool cold_split = mt > 0 && g->tile_cold >= 0 && g->tile_cold < g->n;
+    if (cold_split) PG_TRY(build_cold(g, vbeg, off_h, parts));
+    if (mt > 0) PG_TRY(pack_tiles(g, mt, cold_split ? (int32_t)g->tile_cold : (int32_t)g->n));
     if (!parts.empty())
         PG_CHECK_HIP(hipMemcpy(t.d_parts, parts.data(), sizeof(TilePart) * parts.size(),
                                hipMemcpyHostToDevice));
-    t.args = TileArgs{t.d_vbeg,   t.d_parts,       t.d_fin,
-                      t.d_src,    t.d_dst,         t.d_partial,
-                      t.d_cold_val, (int32_t)parts.size(), (int32_t)fin.size()};
+    t.args = TileArgs{t.d_vbeg,    t.d_parts,   t.d_fin,      t.d_packed,
+                      t.d_gbase,   t.d_esc,     t.d_partial,  t.d_cold_val,
+                      (int32_t)parts.size(),    (int32_t)fin.size()};
     t.ntiles = nt;
     t.m_tiled = mt;
     t.tile_edges = g->tile_edges;

@@ -417,31 +417,44 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
     for (int i = threadIdx.x; i < nv; i += kTileThreads) acc[i] = 0ull;
     __syncthreads();
     constexpr int U = 8;
+    constexpr unsigned kDstMask = (1u << kPackDstBits) - 1u;
+    static_assert(kTileMaxV == (1 << kPackDstBits), "destination offsets fill the low bits");
     {  // warm sources: gathered
-        const int32_t *__restrict__ src = ta.src + part.e0;
-        const uint16_t *__restrict__ dst = ta.dst + part.e0;
+        const int64_t e0 = part.e0;
         const int cnt = (int)(part.cold_e - part.e0);
         int e = threadIdx.x;
         for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
-            int s[U];
-            unsigned d[U];
+            unsigned w[U];
+            int b[U];
 #pragma unroll
             for (int k = 0; k < U; k++) {
-                s[k] = ld_stream(src + e + k * kTileThreads);
-                d[k] = ld_stream_u16(dst + e + k * kTileThreads);
+                const int64_t i = e0 + e + k * kTileThreads;
+                w[k] = __builtin_nontemporal_load(ta.packed + i);
+                b[k] = ta.gbase[i / kPackGroup];
             }
             unsigned long long q[U];
 #pragma unroll
-            for (int k = 0; k < U; k++) q[k] = op.tile_fix(op.tile_raw(s[k]));
+            for (int k = 0; k < U; k++) {
+                const int64_t i = e0 + e + k * kTileThreads;
+                const int s = b[k] >= 0 ? b[k] + (int)(w[k] >> kPackDstBits)
+                                        : ta.esc[(int64_t)(-b[k] - 1) * kPackGroup + (i % kPackGroup)];
+                q[k] = op.tile_fix(op.tile_raw(s));
+            }
 #pragma unroll
-            for (int k = 0; k < U; k++) atomicAdd(&acc[d[k]], q[k]);
+            for (int k = 0; k < U; k++) atomicAdd(&acc[w[k] & kDstMask], q[k]);
         }
-        for (; e < cnt; e += kTileThreads)
-            atomicAdd(&acc[ld_stream_u16(dst + e)], op.tile_fix(op.tile_raw(ld_stream(src + e))));
+        for (; e < cnt; e += kTileThreads) {
+            const int64_t i = e0 + e;
+            const unsigned w = __builtin_nontemporal_load(ta.packed + i);
+            const int b = ta.gbase[i / kPackGroup];
+            const int s = b >= 0 ? b + (int)(w >> kPackDstBits)
+                                 : ta.esc[(int64_t)(-b - 1) * kPackGroup + (i % kPackGroup)];
+            atomicAdd(&acc[w & kDstMask], op.tile_fix(op.tile_raw(s)));
+        }
     }
     {  // cold sources: values streamed from the pre-pass
         const float *__restrict__ val = ta.cold_val + part.cold_v;
-        const uint16_t *__restrict__ dst = ta.dst + part.cold_e;
+        const uint32_t *__restrict__ pk = ta.packed + part.cold_e;
         const int cnt = (int)(part.e1 - part.cold_e);
         int e = threadIdx.x;
         for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
@@ -450,13 +463,14 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
 #pragma unroll
             for (int k = 0; k < U; k++) {
                 x[k] = __builtin_nontemporal_load(val + e + k * kTileThreads);
-                d[k] = ld_stream_u16(dst + e + k * kTileThreads);
+                d[k] = __builtin_nontemporal_load(pk + e + k * kTileThreads) & kDstMask;
             }
 #pragma unroll
             for (int k = 0; k < U; k++) atomicAdd(&acc[d[k]], op.tile_fix(x[k]));
         }
         for (; e < cnt; e += kTileThreads)
-            atomicAdd(&acc[ld_stream_u16(dst + e)], op.tile_fix(__builtin_nontemporal_load(val + e)));
+            atomicAdd(&acc[__builtin_nontemporal_load(pk + e) & kDstMask],
+                      op.tile_fix(__builtin_nontemporal_load(val + e)));
     }
     __syncthreads();
     if (part.slot >= 0) {
@@ -559,14 +573,18 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
 // Pull over the CSC with the light range taken from the tiles: the hot schedule
 // (multi-block and block-per-vertex segments, zero-degree tail), the tile parts,
 // then the vertices of the tiles with several parts.
+// skip_tail: leave out the zero-in-degree vertices, for an Op whose results
+// there are already in place (PageRank from its third iteration: those vertices
+// left the frontier after the first, and both double buffers hold their zeros).
 template <class Op>
 inline int launch_pull_tiled(const Op &op, const PullSchedule &s, const TileSet &t,
-                             typename Op::Stat *stat_out = nullptr) {
+                             typename Op::Stat *stat_out = nullptr, bool skip_tail = false) {
     if (t.cold.nparts > 0)
         hipLaunchKernelGGL(k_cold_gather<Op>, dim3(t.cold.nparts), dim3(kTileThreads), 0, stream(),
                            op, t.cold);
-    if (s.nblocks_hot > 0)
-        hipLaunchKernelGGL(k_pull<Op>, dim3(s.nblocks_hot), dim3(kBlock), 0, stream(), op,
+    const int hot_blocks = skip_tail ? s.nblocks_hot_body : s.nblocks_hot;
+    if (hot_blocks > 0)
+        hipLaunchKernelGGL(k_pull<Op>, dim3(hot_blocks), dim3(kBlock), 0, stream(), op,
                            s.args_hot, stat_out);
     if (t.args.nparts > 0)
         hipLaunchKernelGGL(k_pull_tiles<Op>, dim3(t.args.nparts), dim3(kTileThreads), 0, stream(),

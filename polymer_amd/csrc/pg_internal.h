@@ -144,12 +144,20 @@ struct TileFin {     // kBlock vertices of a tile with several parts
     int64_t slot0;   // offset of the tile's first partial slot (slots are nv apart)
 };
 
+// A tile edge is one packed word: destination offset in the low 13 bits and, for a
+// warm edge, the source as an offset from its 64-edge group's base in the high 19
+// bits.  A group whose warm sources span 2^19 or more (at a tile boundary, or in a
+// sparse stretch) has base -(k+1) and keeps its sources in escape row k.
+constexpr int kPackDstBits = 13;  // kTileMaxV = 2^13
+constexpr int kPackGroup = 64;
+
 struct TileArgs {
     const int32_t *vbeg;         // ntiles + 1 positions (visiting order)
     const TilePart *parts;       // one per workgroup
     const TileFin *fin;          // one per finishing block
-    const int32_t *src;          // source ids, sorted within a tile
-    const uint16_t *dst;         // destination position - vbeg[tile]
+    const uint32_t *packed;      // one word per tile edge
+    const int32_t *gbase;        // per 64-edge group: source base, or -(escape row + 1)
+    const int32_t *esc;          // escape rows of 64 source ids
     unsigned long long *partial; // fixed-point partial sums
     const float *cold_val;       // values of the cold edges (written by the pre-pass)
     int32_t nparts;
@@ -164,8 +172,12 @@ struct TileSet {
     int32_t *d_vbeg = nullptr;
     TilePart *d_parts = nullptr;
     TileFin *d_fin = nullptr;
-    int32_t *d_src = nullptr;
-    uint16_t *d_dst = nullptr;
+    int32_t *d_src = nullptr;    // build only (freed once packed)
+    uint16_t *d_dst = nullptr;   // build only
+    uint32_t *d_packed = nullptr;
+    int32_t *d_gbase = nullptr;
+    int32_t *d_esc = nullptr;
+    int64_t n_esc = 0;
     unsigned long long *d_partial = nullptr;
     int32_t ntiles = 0;
     int64_t m_tiled = 0;
@@ -191,6 +203,7 @@ struct PullSchedule {
     // the tiles cover (multi and block segments, then the zero-degree tail)
     PullArgs args_hot{};
     int nblocks_hot = 0;
+    int nblocks_hot_body = 0;  // the same without the zero-degree tail (its last segment)
     int64_t light_begin = 0, light_end = 0;
 };
 
