@@ -109,14 +109,17 @@ def measured_traffic(schedule):
     path = os.path.join(ROOT, "profiles", "pagerank_scale24_pmc.txt")
     if not os.path.exists(path):
         return None, "no pmc summary"
-    kv = {}
+    want = ",".join(str(x) for x in schedule)
+    cur, found = None, {}
     for line in open(path):
-        if ":" in line and not line.startswith("#"):
-            k, v = line.split(":", 1)
-            kv[k.strip()] = v.strip()
-    if kv.get("schedule") != ",".join(str(x) for x in schedule):
-        return None, "pmc summary taken with schedule %s" % kv.get("schedule")
-    return int(float(kv["hbm_bytes_per_iteration"])), "profiles/pagerank_scale24_pmc.txt"
+        line = line.strip()
+        if line.startswith("[schedule ") and line.endswith("]"):
+            cur = line[len("[schedule "):-1]
+        elif line.startswith("hbm_bytes_per_iteration:") and cur:
+            found[cur] = int(float(line.split(":", 1)[1]))
+    if want not in found:
+        return None, "pmc summary has schedules %s only" % sorted(found)
+    return found[want], "profiles/pagerank_scale24_pmc.txt [schedule %s]" % want
 
 
 def batched_extra(pa, scale=22, m=1 << 26, k=64, iters=20):

@@ -416,7 +416,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
     const int nv = ta.vbeg[t + 1] - vb;  // <= kTileMaxV by construction
     for (int i = threadIdx.x; i < nv; i += kTileThreads) acc[i] = 0ull;
     __syncthreads();
-    constexpr int U = 8;
+    constexpr int U = 4;  // measured on MI355X: 4 edges in flight per lane beat 2, 3, 8, 16
     constexpr unsigned kDstMask = (1u << kPackDstBits) - 1u;
     static_assert(kTileMaxV == (1 << kPackDstBits), "destination offsets fill the low bits");
     {  // warm sources: gathered
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(kTileThreads) void k_cold_gather(Op op, ColdArgs ca
     const uint32_t *__restrict__ pos = ca.pos + cp.a;
     const uint16_t *__restrict__ srcl = ca.srcl + cp.a;
     const int cnt = (int)(cp.b - cp.a);
-    constexpr int U = 4;
+    constexpr int U = 8;
     int e = threadIdx.x;
     for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
         uint32_t p[U];
