@@ -388,16 +388,15 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
 // in-neighbour found in the frontier becomes the parent, then break); sparse rounds push
 // top-down from the frontier list (edgeMapSparseV3).  Levels are the round numbers.
 // ======================================================================================
+// The frontier of round r is {v : level[v] == r - 1}: the pull tests that instead of
+// a bitmap, and writes nothing shared; the next frontier's list is compacted from
+// the levels afterwards (one counter per wave for millions of hits would serialise).
 struct BfsPullOp {
     using Acc = int;
     using Ctx = int;
     using Stat = unsigned long long;
     static constexpr bool kEarlyExit = true;
     static constexpr bool kStats = false;
-    const uint32_t *F;
-    uint32_t *Fn;
-    int32_t *list_n;
-    unsigned long long *cnt_n;
     int32_t *level, *parent;
     const int32_t *in_adj;
     int32_t round;
@@ -406,39 +405,51 @@ struct BfsPullOp {
     __device__ int identity() const { return 0x7FFFFFFF; }
     __device__ int combine(int a, int b) const { return min(a, b); }
     __device__ int edge(int s, int e, const int &) const {
-        return bit_test(F, s) ? e : 0x7FFFFFFF;
+        return level[s] == round - 1 ? e : 0x7FFFFFFF;
     }
     __device__ bool found(int a, const int &) const { return a != 0x7FFFFFFF; }
     __device__ Stat finish(int v, int a, const int &) const {
-        const bool hit = a != 0x7FFFFFFF;
-        if (hit) {
+        if (a != 0x7FFFFFFF) {
             parent[v] = in_adj[a];
             level[v] = round;
-            atomicOr(&Fn[v >> 5], 1u << (v & 31));
         }
-        agg_append(list_n, cnt_n, hit, v);
         return 0ull;
     }
     __device__ void skip(int) const {}
 };
 
+// append: push the winners to the next list (small rounds); otherwise the list
+// is compacted from the levels after the round
 struct BfsPushOp {
-    uint32_t *Fn;
     int32_t *list_n;
     unsigned long long *cnt_n;
     int32_t *level, *parent;
     int32_t round;
+    bool append;
     __device__ bool src_active(int) const { return true; }
     __device__ void push(int s, int d, int) const {
         bool won = false;
         if (level[d] < 0 && atomicCAS(&parent[d], -1, s) == -1) {
             level[d] = round;
-            atomicOr(&Fn[d >> 5], 1u << (d & 31));
             won = true;
         }
-        agg_append(list_n, cnt_n, won, d);
+        if (append) agg_append(list_n, cnt_n, won, d);
     }
 };
+
+// bits of {v : level[v] == round}, one ballot per 64 vertices (no atomics)
+__global__ void k_level_bits(const int32_t *level, int64_t n, int32_t round, uint32_t *bits) {
+    const int64_t nwords = (n + 31) / 32;
+    const int64_t npad = (n + 63) / 64 * 64;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < npad;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const bool p = v < n && level[v] == round;
+        const unsigned long long m = __ballot(p);
+        const int lane = threadIdx.x & 63;
+        if ((lane == 0 || lane == 32) && (v >> 5) < nwords)
+            bits[v >> 5] = lane == 0 ? (uint32_t)m : (uint32_t)(m >> 32);
+    }
+}
 
 // ======================================================================================
 // Connected components (numa-Components.C:55-119 CC_F / CC_Vertex_F, edgeMapCustom
@@ -455,8 +466,6 @@ struct CcPullOp {
     static constexpr bool kStats = false;
     const uint32_t *F;
     uint32_t *Fn;
-    int32_t *list_n;
-    unsigned long long *cnt_n;
     int32_t *label;
     __device__ bool active(int) const { return true; }
     __device__ int ctx(int v) const { return v; }
@@ -467,12 +476,10 @@ struct CcPullOp {
     }
     __device__ bool found(int, const int &) const { return false; }
     __device__ Stat finish(int v, int mn, const int &) const {
-        const bool dec = mn < label[v];
-        if (dec) {
+        if (mn < label[v]) {  // the next list is compacted from Fn after the round
             label[v] = mn;
             atomicOr(&Fn[v >> 5], 1u << (v & 31));
         }
-        agg_append(list_n, cnt_n, dec, v);
         return 0ull;
     }
     __device__ void skip(int) const {}
@@ -484,6 +491,7 @@ struct CcPushOp {
     unsigned long long *cnt_n;
     int32_t *label;
     const int32_t *prev;
+    bool append;  // else the next list is compacted from Fn after the round
     __device__ bool src_active(int) const { return true; }
     __device__ void push(int s, int d, int) const {
         const int ls = label[s];
@@ -493,7 +501,7 @@ struct CcPushOp {
             act = ls < old && old == prev[d];
             if (act) atomicOr(&Fn[d >> 5], 1u << (d & 31));
         }
-        agg_append(list_n, cnt_n, act, d);
+        if (append) agg_append(list_n, cnt_n, act, d);
     }
 };
 
@@ -531,6 +539,7 @@ struct SsspPushOp {
     unsigned long long *cnt_n;
     int32_t *dist;
     const int32_t *w;  // CSR order
+    bool append;       // else the next list is compacted from Fn after the round
     __device__ bool src_active(int s) const { return kDense ? bit_test(F, s) : true; }
     __device__ void push(int s, int d, int e) const {
         const int nd = dist[s] + w[e];
@@ -539,7 +548,7 @@ struct SsspPushOp {
             const int old = atomicMin(&dist[d], nd);
             if (nd < old) act = bit_set_new(Fn, d);
         }
-        agg_append(list_n, cnt_n, act, d);
+        if (append) agg_append(list_n, cnt_n, act, d);
     }
 };
 
@@ -626,13 +635,19 @@ int bfs_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_stat
         PG_TRY(clear_frontier(g, *N));
         const bool dense = cnt + edges >= thr;  // P9: >=
         PG_TRY(kt.begin());
+        // a round that may find many vertices builds the next list by compaction
+        const bool compact = dense || edges >= g->n / 64;
         if (dense) {
             dense_rounds++;
-            BfsPullOp op{F->bits, N->bits, N->list, &N->ctr[0], level, parent, g->in_adj, rounds};
+            BfsPullOp op{level, parent, g->in_adj, rounds};
             PG_TRY(launch_pull(op, g->in_sched));
         } else {
-            BfsPushOp op{N->bits, N->list, &N->ctr[0], level, parent, rounds};
+            BfsPushOp op{N->list, &N->ctr[0], level, parent, rounds, !compact};
             PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
+        }
+        if (compact) {
+            PG_LAUNCH_K(k_level_bits, grid_for(g->n), level, g->n, (int32_t)rounds, N->bits);
+            PG_TRY(bits_to_list(g, N->bits, N->list, &N->ctr[0]));
         }
         PG_TRY(kt.end());
         PG_TRY(frontier_stats(g, *N, &cnt, &edges));
@@ -682,14 +697,17 @@ int cc_run(pg_graph *g, int64_t threshold, uint32_t flags, pg_stats *st, int32_t
         PG_TRY(clear_frontier(g, *N));
         const bool dense = cnt + edges >= thr;
         PG_TRY(kt.begin());
+        // a round that may activate many vertices builds the next list by compaction
+        const bool compact = dense || edges >= g->n / 64;
         if (dense) {
             dense_rounds++;
-            CcPullOp op{F->bits, N->bits, N->list, &N->ctr[0], label};
+            CcPullOp op{F->bits, N->bits, label};
             PG_TRY(launch_pull(op, g->in_sched));
         } else {
-            CcPushOp op{N->bits, N->list, &N->ctr[0], label, prev};
+            CcPushOp op{N->bits, N->list, &N->ctr[0], label, prev, !compact};
             PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
         }
+        if (compact) PG_TRY(bits_to_list(g, N->bits, N->list, &N->ctr[0]));
         PG_TRY(kt.end());
         PG_TRY(frontier_stats(g, *N, &cnt, &edges));
         std::swap(F, N);
@@ -739,7 +757,9 @@ int sssp_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_sta
         PG_TRY(clear_frontier(g, *N));
         const bool dense = cnt + edges >= thr;
         PG_TRY(kt.begin());
-        SsspPushOp<false> op{F->bits, N->bits, N->list, &N->ctr[0], dist, g->out_w_i32};
+        // a round that may activate many vertices builds the next list by compaction
+        const bool compact = dense || edges >= g->n / 64;
+        SsspPushOp<false> op{F->bits, N->bits, N->list, &N->ctr[0], dist, g->out_w_i32, !compact};
         if (dense) {
             // edgeMapDenseForward visits the active vertices in id order: the
             // bitmap compacted in order (packIndex), then the same balanced push
@@ -753,6 +773,7 @@ int sssp_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_sta
         } else {
             PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
         }
+        if (compact) PG_TRY(bits_to_list(g, N->bits, N->list, &N->ctr[0]));
         PG_TRY(kt.end());
         PG_TRY(frontier_stats(g, *N, &cnt, &edges));
         std::swap(F, N);
