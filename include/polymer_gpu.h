@@ -115,6 +115,12 @@ int pg_graph_set_schedule(pg_graph *g, int items_per_lane, int64_t block_thresho
  * 2^20), -1 = none, 0 keeps.  Schedule tuning only (the reference partitions
  * by NUMA node instead, polymer.h:131-234); results agree to fp32 rounding. */
 int pg_graph_set_tiles(pg_graph *g, int enable, int64_t tile_edges, int64_t cold_from);
+/* Sizes of the tile structures of the last PageRank-family solve (built on
+ * first use): tiles, workgroup parts, tiled in-edges, cold in-edges served by
+ * the pre-pass, 64-edge groups whose sources are stored unpacked.  Any pointer
+ * may be NULL; all zero before the first tiled solve. */
+int pg_graph_tile_stats(const pg_graph *g, int64_t *tiles, int64_t *parts, int64_t *tiled_edges,
+                        int64_t *cold_edges, int64_t *escape_groups);
 /* The current pull schedule and tile setting (any pointer may be NULL). */
 int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane, int64_t *block_threshold,
                           int64_t *multi_threshold, int64_t *chunk, int32_t *tiles,

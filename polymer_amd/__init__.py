@@ -34,7 +34,8 @@ EXPORTS = [
     "pg_init", "pg_finalize", "pg_last_error", "pg_device_name",
     "pg_graph_create", "pg_graph_rmat", "pg_graph_load_adj", "pg_graph_destroy",
     "pg_graph_info", "pg_graph_download", "pg_graph_set_schedule", "pg_graph_set_tiles",
-    "pg_graph_get_schedule", "pg_graph_autotune", "pg_graph_bp_potentials", "pg_graph_view",
+    "pg_graph_get_schedule", "pg_graph_tile_stats", "pg_graph_autotune",
+    "pg_graph_bp_potentials", "pg_graph_view",
     "pg_subset_create", "pg_subset_destroy", "pg_subset_clear", "pg_subset_fill",
     "pg_subset_from_dense", "pg_subset_from_sparse", "pg_subset_to_dense",
     "pg_subset_to_sparse", "pg_subset_size",
@@ -97,6 +98,7 @@ def load():
         "pg_graph_set_schedule": [P, ctypes.c_int, i64, i64, i64],
         "pg_graph_set_tiles": [P, ctypes.c_int, i64, i64],
         "pg_graph_get_schedule": [P, P, P, P, P, P, P, P],
+        "pg_graph_tile_stats": [P, P, P, P, P, P],
         "pg_graph_autotune": [P, P, P],
         "pg_graph_bp_potentials": [P, u64, ctypes.c_float, ctypes.c_float],
         "pg_subset_create": [P, PP],
@@ -268,6 +270,14 @@ class Graph:
         _check(load().pg_graph_get_schedule(self._h, *[ctypes.byref(x) for x in v]),
                "pg_graph_get_schedule")
         return tuple(x.value for x in v)
+
+    def tile_stats(self):
+        """dict: tiles, parts, tiled_edges, cold_edges, escape_groups"""
+        v = [ctypes.c_int64() for _ in range(5)]
+        _check(load().pg_graph_tile_stats(self._h, *[ctypes.byref(x) for x in v]),
+               "pg_graph_tile_stats")
+        keys = ("tiles", "parts", "tiled_edges", "cold_edges", "escape_groups")
+        return dict(zip(keys, (x.value for x in v)))
 
     def autotune(self):
         ipl, bt = ctypes.c_int32(), ctypes.c_int64()

@@ -1666,6 +1666,19 @@ extern "C" int pg_graph_set_tiles(pg_graph *g, int enable, int64_t tile_edges, i
     return PG_OK;
 }
 
+extern "C" int pg_graph_tile_stats(const pg_graph *g, int64_t *tiles, int64_t *parts,
+                                   int64_t *tiled_edges, int64_t *cold_edges,
+                                   int64_t *escape_groups) {
+    PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    const TileSet &t = g->tiles;
+    if (tiles) *tiles = t.built ? t.ntiles : 0;
+    if (parts) *parts = t.built ? t.args.nparts : 0;
+    if (tiled_edges) *tiled_edges = t.built ? t.m_tiled : 0;
+    if (cold_edges) *cold_edges = t.built ? t.m_cold : 0;
+    if (escape_groups) *escape_groups = t.built ? t.n_esc : 0;
+    return PG_OK;
+}
+
 extern "C" int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane,
                                      int64_t *block_threshold, int64_t *multi_threshold,
                                      int64_t *chunk, int32_t *tiles, int64_t *tile_edges,

@@ -167,6 +167,28 @@ def test_pagerank_tiles(pa, orc):
     assert np.array_equal(r4, out[1])
 
 
+def test_pagerank_tiles_escapes(pa, orc):
+    # n = 2^20 > 2^19: 64-edge groups that straddle two tiles span more source ids
+    # than a packed offset holds and keep their sources unpacked (escape rows)
+    say("case: pagerank tiles with escape groups")
+    g = orc.rmat(20, 1 << 22, seed=12)
+    ref, _ = orc.pagerank(g, 10, 0.85)
+    d = pa.Graph.rmat(20, 1 << 22, seed=12)
+    d.set_schedule(16, 1 << 30, (1 << 30) + 1, 16384)  # every vertex in tiles
+    for cold in (-1, 300000):
+        d.set_tiles(1, 65536, cold)
+        r, _ = d.pagerank(10, 0.85)
+        ts = d.tile_stats()
+        err = rel_l1(r, ref)
+        say("cold %d: %s relL1 %.3e" % (cold, ts, err))
+        assert err < TOL
+        assert ts["tiled_edges"] == g.m
+        if cold < 0:
+            assert ts["escape_groups"] > 0 and ts["cold_edges"] == 0
+        else:
+            assert ts["cold_edges"] > 0
+
+
 # ---- SpMV -------------------------------------------------------------------------------
 def test_spmv_integer_weights_vs_reference_binary(pa, orc, pr1):
     say("case: spmv integer weights vs reference golden")
