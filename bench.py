@@ -74,6 +74,47 @@ def bfs_extra(pa, scale=22, m=1 << 26, nsrc=8):
             "bfs_sources": len(srcs)}
 
 
+def traversal_extra(pa, scale=22, m=1 << 26):
+    """Connected components on the symmetrised scale-22 graph; Bellman-Ford (int32
+    weights U[1,255]) from vertex 0 (the max-degree vertex) and 8 seeded sources."""
+    gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True, weights="i32", wseed=7)
+    gs.cc(out=False)
+    _, stc = gs.cc(out=False)
+    rng = np.random.default_rng(7)
+    srcs = [0] + [int(x) for x in rng.integers(0, gs.n, 8)]
+    gs.sssp(0, out=False)
+    tot_ms, tot_edges = 0.0, 0
+    for s in srcs:
+        _, st = gs.sssp(s, out=False)
+        tot_ms += st.ms
+        tot_edges += st.edges
+    return {"cc_scale22_ms": round(stc.ms, 3), "cc_rounds": stc.rounds,
+            "sssp_scale22_ms_per_source": round(tot_ms / len(srcs), 3),
+            "sssp_scale22_gteps_relaxed": round(tot_edges / (tot_ms / 1e3) / 1e9, 2)}
+
+
+def value_extra(pa, g, scale=24, m=1 << 28):
+    """SpMV (20 iterations, fp32 values U(0,1], x0 U[0,1)) and BP (10 iterations,
+    potentials U[0.1,1]) on the scale-24 graph, with their algorithmic bandwidth
+    (SpMV 12m + 12n, BP 20 E_t + 28n bytes per iteration)."""
+    out = {}
+    gw = pa.Graph.rmat(scale, m, seed=1, weights="f32", wseed=3)
+    x0 = np.random.default_rng(3).random(gw.n).astype(np.float32)
+    gw.spmv(x0, 20, out=False)
+    _, st = gw.spmv(x0, 20, out=False)
+    out["spmv_scale24_ms"] = round(st.ms, 3)
+    out["spmv_scale24_algorithmic_gbs"] = round((12 * gw.m + 12 * gw.n) * 20 / (st.ms / 1e3) / 1e9, 1)
+    gw.close()
+    g.bp_potentials(9)
+    vpot = np.random.default_rng(8).uniform(0.1, 1.0, 2 * g.n).astype(np.float32)
+    prod0 = np.ones(2 * g.n, np.float32)
+    g.bp(vpot, prod0, 10, out=False)
+    _, st = g.bp(vpot, prod0, 10, out=False)
+    out["bp_scale24_ms"] = round(st.ms, 3)
+    out["bp_scale24_algorithmic_gbs"] = round((20 * st.edges + 28 * g.n * 10) / (st.ms / 1e3) / 1e9, 1)
+    return out
+
+
 def profile_run(args):
     """Short run for rocprofv3: a plain streaming kernel of known size (torch's
     elementwise negation of a 1 GiB fp32 tensor into another: exactly 1 GiB
@@ -93,9 +134,9 @@ def profile_run(args):
     pa.init(0)
     g = pa.Graph.rmat(args.scale, args.edges, seed=1)
     sched = [int(x) for x in args.schedule.split(",")]
-    g.set_tiles(sched[4], sched[5], sched[6])
     g.set_schedule(*sched[:4])
-    log("pull schedule (ipl, block, multi, chunk, tiles, tile edges, cold from):",
+    g.set_tiles(sched[4], sched[6], sched[7], max_degree=sched[5])
+    log("pull schedule (ipl, block, multi, chunk, tiles, tile max degree, tile edges, cold from):",
         g.get_schedule())
     g.pagerank(args.iters, 0.85, out=False)
     _, st = g.pagerank(args.iters, 0.85, out=False, profile=True)
@@ -196,8 +237,9 @@ def main():
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--schedule", default="16,1073741824,1073741825,16384,1,524288,1048576",
-                    help="--profile only: ipl,block,multi,chunk,tiles,tile_edges,cold_from")
+    ap.add_argument("--schedule", default="16,16384,65536,16384,1,1073741824,524288,2097152",
+                    help="--profile only: ipl,block,multi,chunk,tiles,tile_max_degree,"
+                         "tile_edges,cold_from")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -245,7 +287,7 @@ def main():
     traffic, traffic_src = measured_traffic(schedule)
     kernels = ("k_pull<PrOp> (hot) + k_pull_tiles<PrOp> + k_tile_finish<PrOp>" if schedule[4]
                else "k_pull<PrOp>")
-    if schedule[4] and schedule[6] >= 0:
+    if schedule[4] and schedule[7] >= 0:
         kernels = "k_cold_gather<PrOp> + " + kernels
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -262,7 +304,8 @@ def main():
     host_ms = (time.perf_counter() - t1) * 1e3
     extra = {"pagerank_step_with_host_copy_ms": round(host_ms, 3)}
     if not args.no_extra:
-        for name, fn in (("bfs", bfs_extra), ("batched", batched_extra),
+        for name, fn in (("bfs", bfs_extra), ("traversal", traversal_extra),
+                         ("value", lambda p: value_extra(p, g)), ("batched", batched_extra),
                          ("adjoint", adjoint_extra), ("multigraph", multigraph_extra)):
             t1 = time.perf_counter()
             try:

@@ -107,14 +107,16 @@ int pg_graph_download(const pg_graph *g, int64_t *off, int32_t *adj, void *w);
 /* Pull-schedule tuning: items per lane, block/multi thresholds.  0 = keep. */
 int pg_graph_set_schedule(pg_graph *g, int items_per_lane, int64_t block_threshold,
                           int64_t multi_threshold, int64_t chunk);
-/* Source-sorted tiles for the PageRank-family pull over the light vertices
- * (in-degree below the block threshold): enable = 1 on (the default), 0 off,
- * negative keeps; tile_edges = target edges per workgroup, 0 keeps (default
- * 262144); cold_from = first internal source id whose values a pre-pass
- * streams into tile order instead of the tile kernel gathering them (default
- * 2^20), -1 = none, 0 keeps.  Schedule tuning only (the reference partitions
- * by NUMA node instead, polymer.h:131-234); results agree to fp32 rounding. */
-int pg_graph_set_tiles(pg_graph *g, int enable, int64_t tile_edges, int64_t cold_from);
+/* Source-sorted tiles for the PageRank-family pull: enable = 1 on (the
+ * default), 0 off, negative keeps; max_degree = in-degree up to which a vertex
+ * is put in a tile (heavier ones keep the block pull), 0 keeps (default 2^30:
+ * all); tile_edges = target edges per workgroup, 0 keeps (default 524288);
+ * cold_from = first internal source id whose values a pre-pass streams into
+ * tile order instead of the tile kernel gathering them (default 2^21), -1 =
+ * none, 0 keeps.  Schedule tuning only (the reference partitions by NUMA node
+ * instead, polymer.h:131-234); results agree to fp32 rounding. */
+int pg_graph_set_tiles(pg_graph *g, int enable, int64_t max_degree, int64_t tile_edges,
+                       int64_t cold_from);
 /* Sizes of the tile structures of the last PageRank-family solve (built on
  * first use): tiles, workgroup parts, tiled in-edges, cold in-edges served by
  * the pre-pass, 64-edge groups whose sources are stored unpacked.  Any pointer
@@ -124,7 +126,7 @@ int pg_graph_tile_stats(const pg_graph *g, int64_t *tiles, int64_t *parts, int64
 /* The current pull schedule and tile setting (any pointer may be NULL). */
 int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane, int64_t *block_threshold,
                           int64_t *multi_threshold, int64_t *chunk, int32_t *tiles,
-                          int64_t *tile_edges, int64_t *cold_from);
+                          int64_t *max_degree, int64_t *tile_edges, int64_t *cold_from);
 /* Times candidate schedules on PageRank iterations and keeps the fastest. */
 int pg_graph_autotune(pg_graph *g, int32_t *items_per_lane, int64_t *block_threshold);
 /* Fills m edge values in CSR order of the input ids (pair hash, see oracle). */

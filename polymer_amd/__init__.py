@@ -96,8 +96,8 @@ def load():
         "pg_graph_info": [P, P, P, P, P],
         "pg_graph_download": [P, P, P, P],
         "pg_graph_set_schedule": [P, ctypes.c_int, i64, i64, i64],
-        "pg_graph_set_tiles": [P, ctypes.c_int, i64, i64],
-        "pg_graph_get_schedule": [P, P, P, P, P, P, P, P],
+        "pg_graph_set_tiles": [P, ctypes.c_int, i64, i64, i64],
+        "pg_graph_get_schedule": [P, P, P, P, P, P, P, P, P],
         "pg_graph_tile_stats": [P, P, P, P, P, P],
         "pg_graph_autotune": [P, P, P],
         "pg_graph_bp_potentials": [P, u64, ctypes.c_float, ctypes.c_float],
@@ -256,17 +256,18 @@ class Graph:
         _check(load().pg_graph_set_schedule(self._h, items_per_lane, block_threshold,
                                             multi_threshold, chunk), "pg_graph_set_schedule")
 
-    def set_tiles(self, enable=-1, tile_edges=0, cold_from=0):
-        """Source-sorted tiles for the PageRank pull of light vertices (on by default);
-        cold_from: first source id served by the cold pre-pass (-1 none, 0 keep)."""
-        _check(load().pg_graph_set_tiles(self._h, int(enable), int(tile_edges), int(cold_from)),
-               "pg_graph_set_tiles")
+    def set_tiles(self, enable=-1, tile_edges=0, cold_from=0, max_degree=0):
+        """Source-sorted tiles for the PageRank pull (on by default); max_degree: the
+        largest in-degree put in a tile; cold_from: first source id served by the cold
+        pre-pass (-1 none).  0 keeps a setting."""
+        _check(load().pg_graph_set_tiles(self._h, int(enable), int(max_degree), int(tile_edges),
+                                         int(cold_from)), "pg_graph_set_tiles")
 
     def get_schedule(self):
-        """(items_per_lane, block_threshold, multi_threshold, chunk, tiles, tile_edges,
-        cold_from)"""
+        """(items_per_lane, block_threshold, multi_threshold, chunk, tiles, max_degree,
+        tile_edges, cold_from)"""
         i32, i64 = ctypes.c_int32, ctypes.c_int64
-        v = [i32(), i64(), i64(), i64(), i32(), i64(), i64()]
+        v = [i32(), i64(), i64(), i64(), i32(), i64(), i64(), i64()]
         _check(load().pg_graph_get_schedule(self._h, *[ctypes.byref(x) for x in v]),
                "pg_graph_get_schedule")
         return tuple(x.value for x in v)

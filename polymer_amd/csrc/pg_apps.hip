@@ -345,13 +345,14 @@ struct BpOp {
     using Ctx = Vec2<T>;  // q_l = phi_d[l] * P_cur[d][l]
     using Stat = unsigned long long;
     static constexpr bool kEarlyExit = false;
-    static constexpr bool kStats = false;
+    static constexpr bool kStats = true;  // sum of out-degrees of the next frontier
     const uint8_t *F;
     uint8_t *Fn;
     const float4 *pot;   // CSC order, W_e[0][0], W_e[0][1], W_e[1][0], W_e[1][1]
     const float2 *vpot;  // phi
     const Vec2<T> *cur;
     Vec2<T> *nxt;
+    const int32_t *out_off;
     __device__ bool active(int v) const { return F[v] != 0; }
     __device__ Ctx ctx(int v) const {
         const float2 phi = vpot[v];
@@ -371,9 +372,13 @@ struct BpOp {
     __device__ Stat finish(int v, Acc a, const Ctx &) const {
         nxt[v] = Vec2<T>{a.p0, a.p1};
         Fn[v] = a.any ? 1 : 0;
-        return 0ull;
+        return a.any ? (Stat)(out_off[v + 1] - out_off[v]) : 0ull;
     }
     __device__ void skip(int v) const { Fn[v] = 0; }
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
+    __device__ static void stat_commit(Stat *p, Stat v) {
+        atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
+    }
 };
 
 __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
@@ -933,10 +938,14 @@ static int bp_run(pg_graph *g, int iters, const float4 *pot, const float2 *vp, c
     PG_TRY(tm.start());
     Vec2<T> *cur = pa, *nxt = pb;
     uint8_t *F = f0, *Fn = f1;
+    unsigned long long *stats = nullptr;  // per iteration: sum of out-degrees of F_{t+1}
+    PG_TRY(workspace_t(g, "bp_stats", ((size_t)iters + 1) * kStatSlots, &stats));
+    PG_LAUNCH_K(k_zero_u64, (unsigned)(((iters + 1) * kStatSlots + kBlock - 1) / kBlock), stats,
+                (iters + 1) * kStatSlots);
     for (int t = 0; t < iters; t++) {
-        BpOp<T> op{F, Fn, pot, vp, cur, nxt};
+        BpOp<T> op{F, Fn, pot, vp, cur, nxt, g->out_off};
         PG_TRY(kt.begin());
-        PG_TRY(launch_pull(op, g->in_sched));
+        PG_TRY(launch_pull(op, g->in_sched, &stats[(size_t)t * kStatSlots]));
         PG_TRY(kt.end());
         std::swap(cur, nxt);  // swap(vertD_curr, vertD_next), numa-BP.C:303-304
         std::swap(F, Fn);     // switchFrontier
@@ -944,8 +953,14 @@ static int bp_run(pg_graph *g, int iters, const float4 *pot, const float2 *vp, c
     double ms = 0;
     PG_TRY(tm.stop(&ms));
     if (st) {
+        std::vector<unsigned long long> hs(((size_t)iters + 1) * kStatSlots);
+        PG_CHECK_HIP(hipMemcpy(hs.data(), stats, 8 * hs.size(), hipMemcpyDeviceToHost));
+        int64_t e = iters > 0 ? g->m : 0;  // F_0 = V
+        for (int t = 0; t + 1 < iters; t++)
+            for (int k = 0; k < kStatSlots; k++) e += (int64_t)hs[(size_t)t * kStatSlots + k];
         st->rounds = iters;
         st->dense_rounds = iters;
+        st->edges = e;
         st->ms = ms;
         PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
     }

@@ -704,29 +704,35 @@ extern "C" int pg_pagerank_multigraph(pg_graph *g, int iters, double damping, fl
 extern "C" int pg_graph_autotune(pg_graph *g, int32_t *items_per_lane, int64_t *block_threshold) {
     PG_TRY(check_g(g));
     // candidates: lanes per light vertex, block / multi-block thresholds, multi-block
-    // chunk, tiles (0 = off), edges per tile part, first cold source (-1 = none)
+    // chunk (the pull every app uses), tiles (0 = off), the largest in-degree put
+    // in a tile, edges per tile part, first cold source (-1 = none)
     struct Cand {
         int ipl;
         int64_t bt, mt, ch;
         int tiles;
-        int64_t te, cold;
+        int64_t maxdeg, te, cold;
     };
     const int64_t M = 1 << 20, ALL = (int64_t)1 << 30;  // ALL: every vertex in tiles
     const Cand cands[] = {
-        {16, ALL, ALL + 1, 16384, 1, 524288, 2 * M},  {16, 16384, 65536, 16384, 1, 524288, 2 * M},
-        {16, ALL, ALL + 1, 16384, 1, 524288, M},      {16, 262144, 524288, 16384, 1, 524288, 2 * M},
-        {16, 16384, 65536, 16384, 1, 262144, M},      {16, ALL, ALL + 1, 16384, 1, 262144, -1},
-        {16, 16384, 65536, 16384, 1, 262144, -1},     {8, 4096, 32768, 8192, 0, 0, -1},
-        {16, 16384, 65536, 16384, 0, 0, -1},          {32, 16384, 65536, 16384, 0, 0, -1}};
+        {16, 16384, 65536, 16384, 1, ALL, 524288, 2 * M},
+        {16, 16384, 65536, 16384, 1, ALL, 524288, M},
+        {16, 16384, 65536, 16384, 1, 262144, 524288, 2 * M},
+        {16, 16384, 65536, 16384, 1, 16384, 524288, 2 * M},
+        {16, 16384, 65536, 16384, 1, ALL, 262144, M},
+        {16, 16384, 65536, 16384, 1, ALL, 262144, -1},
+        {16, 16384, 65536, 16384, 1, 16384, 262144, -1},
+        {8, 4096, 32768, 8192, 0, ALL, 0, -1},
+        {16, 16384, 65536, 16384, 0, ALL, 0, -1},
+        {32, 16384, 65536, 16384, 0, ALL, 0, -1}};
     const PullParams saved = g->in_sched.params;
     double best = 1e30;
-    Cand bestc{saved.items_per_lane, saved.block_threshold, saved.multi_threshold,
-               saved.chunk, g->use_tiles ? 1 : 0, g->tile_edges, g->tile_cold};
+    Cand bestc{saved.items_per_lane, saved.block_threshold, saved.multi_threshold, saved.chunk,
+               g->use_tiles ? 1 : 0, g->tile_max_degree, g->tile_edges, g->tile_cold};
     const int iters = 3;
     float *res = nullptr;
     for (const Cand &c : cands) {
-        PG_TRY(pg_graph_set_tiles(g, c.tiles, c.te, c.cold));
         PG_TRY(pg_graph_set_schedule(g, c.ipl, c.bt, c.mt, c.ch));
+        PG_TRY(pg_graph_set_tiles(g, c.tiles, c.maxdeg, c.te, c.cold));
         PG_TRY(pagerank_run(g, iters, 0.85, 0, nullptr, &res));  // warm
         pg_stats s{};
         double tot = 0;
@@ -739,8 +745,8 @@ extern "C" int pg_graph_autotune(pg_graph *g, int32_t *items_per_lane, int64_t *
             bestc = c;
         }
     }
-    PG_TRY(pg_graph_set_tiles(g, bestc.tiles, bestc.te, bestc.cold));
     PG_TRY(pg_graph_set_schedule(g, bestc.ipl, bestc.bt, bestc.mt, bestc.ch));
+    PG_TRY(pg_graph_set_tiles(g, bestc.tiles, bestc.maxdeg, bestc.te, bestc.cold));
     if (items_per_lane) *items_per_lane = bestc.ipl;
     if (block_threshold) *block_threshold = bestc.bt;
     return PG_OK;

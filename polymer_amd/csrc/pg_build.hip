@@ -662,20 +662,22 @@ int build_schedule(pg_graph *g, bool in_direction) {
     const int ipl = std::max(1, P.items_per_lane);
     // class upper bounds, descending: multi | block | 64 | 32 | 16 | 8 | 4 | 2 | 1
     const int gs[9] = {kModeMulti, kModeBlock, 64, 32, 16, 8, 4, 2, 1};
-    int64_t thr[9];
+    int64_t thr[10];
     thr[0] = std::max<int64_t>(P.multi_threshold, P.block_threshold);
     thr[1] = std::max<int64_t>(P.block_threshold, 32LL * ipl);
     for (int j = 2; j < 8; j++) thr[j] = (int64_t)ipl << (7 - j);  // 32ipl,16,8,4,2,ipl
-    // thr[2] = 32*ipl ... thr[7] = ipl; thr[8] = 0 counts the vertices with edges
+    // thr[2] = 32*ipl ... thr[7] = ipl; thr[8] = 0 counts the vertices with edges;
+    // thr[9]: the in-degree above which a vertex is not put in a tile
     thr[8] = 0;
+    thr[9] = std::max<int64_t>(0, g->tile_max_degree);
     DevBuf dthr, dcnt;
     PG_TRY(dthr.alloc(sizeof(thr)));
     PG_TRY(dcnt.alloc(sizeof(thr)));
     PG_CHECK_HIP(hipMemcpyAsync(dthr.p, thr, sizeof(thr), hipMemcpyHostToDevice, g_stream));
     hipLaunchKernelGGL(k_count_above, dim3(1), dim3(64), 0, g_stream, off, order, n,
-                       dthr.as<int64_t>(), 9, dcnt.as<int64_t>());
+                       dthr.as<int64_t>(), 10, dcnt.as<int64_t>());
     PG_CHECK_HIP(hipGetLastError());
-    int64_t cnt[9];
+    int64_t cnt[10];
     PG_CHECK_HIP(hipMemcpyAsync(cnt, dcnt.p, sizeof(cnt), hipMemcpyDeviceToHost, g_stream));
     PG_CHECK_HIP(hipStreamSynchronize(g_stream));
     int64_t bounds[10];
@@ -741,18 +743,30 @@ int build_schedule(pg_graph *g, bool in_direction) {
         return PG_ERR_UNSUPPORTED;
     }
     s.nblocks = (int)blocks;
-    // hot schedule: the multi and block segments (laid out first, so their block
-    // numbers carry over) and the vertices without pulled edges
-    s.light_begin = bounds[2];
-    s.light_end = std::max(cnt[8], bounds[2]);
+    // hot schedule for the tiled pull: the segments above clipped to the positions
+    // whose in-degree exceeds the tile limit (a clipped multi-block segment keeps
+    // its first items, which are in position order), then the vertices without
+    // pulled edges
+    s.light_begin = std::min(cnt[9], std::max(cnt[8], (int64_t)0));
+    s.light_end = std::max(cnt[8], s.light_begin);
     PullArgs &h = s.args_hot;
     h = a;
     h.nseg = 0;
     int64_t hblocks = 0;
-    for (int i = 0; i < a.nseg; i++) {
-        if (a.seg[i].g != kModeMulti && a.seg[i].g != kModeBlock) break;
-        h.seg[h.nseg++] = a.seg[i];
-        hblocks = (i + 1 < a.nseg) ? a.seg[i + 1].block_begin : blocks;
+    for (int i = 0; i < a.nseg && a.seg[i].v_begin < s.light_begin; i++) {
+        PullSeg sg = a.seg[i];
+        sg.v_end = (int32_t)std::min<int64_t>(sg.v_end, s.light_begin);
+        sg.block_begin = (int32_t)hblocks;
+        if (sg.g == kModeMulti) {
+            int64_t k = 0;
+            while (k < (int64_t)items.size() && items[k].pos < sg.v_end) k++;
+            hblocks += k;
+        } else if (sg.g == kModeBlock) {
+            hblocks += sg.v_end - sg.v_begin;
+        } else {
+            hblocks += ((int64_t)(sg.v_end - sg.v_begin) * sg.g + kBlock - 1) / kBlock;
+        }
+        h.seg[h.nseg++] = sg;
     }
     s.nblocks_hot_body = (int)hblocks;
     if (s.light_end < n) {
@@ -1654,8 +1668,10 @@ extern "C" int pg_graph_set_schedule(pg_graph *g, int items_per_lane, int64_t bl
     return PG_OK;
 }
 
-extern "C" int pg_graph_set_tiles(pg_graph *g, int enable, int64_t tile_edges, int64_t cold_from) {
+extern "C" int pg_graph_set_tiles(pg_graph *g, int enable, int64_t max_degree, int64_t tile_edges,
+                                  int64_t cold_from) {
     PG_REQUIRE(g, PG_ERR_ARG, "null graph");
+    PG_REQUIRE(max_degree >= 0, PG_ERR_ARG, "max_degree out of range: %lld", (long long)max_degree);
     PG_REQUIRE(tile_edges >= 0 && tile_edges <= (int64_t)1 << 30, PG_ERR_ARG,
                "tile_edges out of range: %lld", (long long)tile_edges);
     PG_REQUIRE(cold_from >= -1, PG_ERR_ARG, "cold_from out of range: %lld", (long long)cold_from);
@@ -1663,6 +1679,10 @@ extern "C" int pg_graph_set_tiles(pg_graph *g, int enable, int64_t tile_edges, i
     if (enable >= 0) g->use_tiles = enable != 0;
     if (tile_edges > 0) g->tile_edges = tile_edges;
     if (cold_from != 0) g->tile_cold = cold_from;
+    if (max_degree > 0 && max_degree != g->tile_max_degree) {
+        g->tile_max_degree = max_degree;
+        PG_TRY(build_schedule(g, true));  // the hot / light split moves
+    }
     return PG_OK;
 }
 
@@ -1681,8 +1701,8 @@ extern "C" int pg_graph_tile_stats(const pg_graph *g, int64_t *tiles, int64_t *p
 
 extern "C" int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane,
                                      int64_t *block_threshold, int64_t *multi_threshold,
-                                     int64_t *chunk, int32_t *tiles, int64_t *tile_edges,
-                                     int64_t *cold_from) {
+                                     int64_t *chunk, int32_t *tiles, int64_t *max_degree,
+                                     int64_t *tile_edges, int64_t *cold_from) {
     PG_REQUIRE(g, PG_ERR_ARG, "null graph");
     const PullParams &p = g->in_sched.params;
     if (items_per_lane) *items_per_lane = p.items_per_lane;
@@ -1690,6 +1710,7 @@ extern "C" int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane,
     if (multi_threshold) *multi_threshold = p.multi_threshold;
     if (chunk) *chunk = p.chunk;
     if (tiles) *tiles = g->use_tiles ? 1 : 0;
+    if (max_degree) *max_degree = g->tile_max_degree;
     if (tile_edges) *tile_edges = g->tile_edges;
     if (cold_from) *cold_from = g->tile_cold;
     return PG_OK;

@@ -84,10 +84,10 @@ struct PullArgs {
 };
 
 struct PullParams {
-    int items_per_lane = 4;
-    int64_t block_threshold = 4096;   // in-degree above which one block per vertex
-    int64_t multi_threshold = 32768;  // above which several blocks per vertex
-    int64_t chunk = 8192;             // edges per block for multi-block vertices
+    int items_per_lane = 16;
+    int64_t block_threshold = 16384;  // in-degree above which one block per vertex
+    int64_t multi_threshold = 65536;  // above which several blocks per vertex
+    int64_t chunk = 16384;            // edges per block for multi-block vertices
 };
 
 // ---- source-sorted tiles --------------------------------------------------------
@@ -235,8 +235,9 @@ struct pg_graph {
     pg::PullSchedule out_sched;  // pull over CSR (out-edges)
     pg::TileSet tiles;           // source-sorted tiles of the CSC light range
     bool use_tiles = true;       // PageRank-family solves use the tiles
-    int64_t tile_edges = 262144; // target edges per tile
-    int64_t tile_cold = 1 << 20; // first cold source id of the tiles (<0: no cold split)
+    int64_t tile_max_degree = (int64_t)1 << 30;  // heavier vertices stay in the hot pull
+    int64_t tile_edges = 524288; // target edges per tile part
+    int64_t tile_cold = 2 << 20; // first cold source id of the tiles (<0: no cold split)
     // input-id CSR / CSC for the header-level edgeMap API (built on request)
     bool view_built = false;
     int32_t *v_out_off = nullptr, *v_out_adj = nullptr, *v_in_off = nullptr, *v_in_adj = nullptr;

@@ -129,11 +129,13 @@ def test_pagerank_schedules_agree(pa, orc):
     for ipl, bt, mt, ch in [(1, 64, 128, 256), (2, 512, 1024, 512), (4, 4096, 32768, 8192),
                             (16, 100000, 200000, 8192), (8, 300, 600, 256)]:
         d.set_schedule(ipl, bt, mt, ch)
-        for tiles in (1, 0):
-            d.set_tiles(tiles)
+        # tiles for every vertex, for the light ones only (the multi-block and block
+        # segments stay), cut inside the lanes-per-vertex classes, and no tiles
+        for tiles, maxdeg in ((1, 1 << 30), (1, bt), (1, 100), (0, 0)):
+            d.set_tiles(tiles, max_degree=maxdeg)
             r, _ = d.pagerank(10, 0.85)
-            assert rel_l1(r, ref) < TOL, (ipl, bt, mt, ch, tiles)
-        d.set_tiles(1)
+            assert rel_l1(r, ref) < TOL, (ipl, bt, mt, ch, tiles, maxdeg)
+        d.set_tiles(1, max_degree=1 << 30)
 
 
 def test_pagerank_tiles(pa, orc):
@@ -144,6 +146,7 @@ def test_pagerank_tiles(pa, orc):
     ref, ept = orc.pagerank(g, 20, 0.85)
     d = pa.Graph.rmat(16, 1 << 20, seed=8)
     d.set_schedule(0, 2048, 0, 0)
+    d.set_tiles(max_degree=2048)  # vertices above in-degree 2048 keep the block pull
     out = []
     # (tiles, edges per part, first cold source): the cold pre-pass from source 1000,
     # 5000, 30000 and none
@@ -174,9 +177,8 @@ def test_pagerank_tiles_escapes(pa, orc):
     g = orc.rmat(20, 1 << 22, seed=12)
     ref, _ = orc.pagerank(g, 10, 0.85)
     d = pa.Graph.rmat(20, 1 << 22, seed=12)
-    d.set_schedule(16, 1 << 30, (1 << 30) + 1, 16384)  # every vertex in tiles
     for cold in (-1, 300000):
-        d.set_tiles(1, 65536, cold)
+        d.set_tiles(1, 65536, cold, max_degree=1 << 30)  # every vertex in tiles
         r, _ = d.pagerank(10, 0.85)
         ts = d.tile_stats()
         err = rel_l1(r, ref)
@@ -263,8 +265,18 @@ def test_bp(pa, orc):
     got2, _ = d.bp(vpot, prod0, 10)
     assert np.array_equal(got2, got10, equal_nan=True)
     # the reference's own initial state (zero products)
-    z, _ = d.bp(vpot, np.zeros(2 * g.n, np.float32), 3, epot=epot)
+    z, st = d.bp(vpot, np.zeros(2 * g.n, np.float32), 3, epot=epot)
     assert np.array_equal(z, orc.bp(g, vpot, epot, np.zeros(2 * g.n), 3))
+    # edges = sum over rounds of the out-degrees of F_t, F_0 = V, F_{t+1} = N_out(F_t)
+    outdeg = np.diff(g.off)
+    src_of = np.repeat(np.arange(g.n), outdeg)
+    F, e = np.ones(g.n, bool), 0
+    for _ in range(3):
+        e += int(outdeg[F].sum())
+        nxt = np.zeros(g.n, bool)
+        nxt[g.adj[F[src_of]]] = True
+        F = nxt
+    assert st.edges == e
 
 
 # ---- BFS --------------------------------------------------------------------------------
