@@ -74,6 +74,33 @@ def bfs_extra(pa, scale=22, m=1 << 26, nsrc=8):
             "bfs_sources": len(srcs)}
 
 
+def bfs24_extra(pa, scale=24, m=1 << 28, nsrc=8):
+    """BFS on the symmetrised scale-24 rMat (2^29 directed entries) from vertex 0 and
+    seeded random sources with at least one edge (Graph500's rule: an isolated draw
+    is skipped), direction-optimising."""
+    gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True)
+    rng = np.random.default_rng(9)
+    gs.bfs(0, out=False)
+    tot_ms, tot_edges, tot_bytes, used, draws = 0.0, 0, 0, 0, 0
+    s = 0
+    while used < nsrc and draws < 20 * nsrc:
+        _, _, st = gs.bfs(s, out=False)
+        if st.active > 1:
+            tot_ms += st.ms
+            tot_edges += st.edges
+            tot_bytes += 4 * st.edges + 16 * st.active
+            used += 1
+        s = int(rng.integers(0, gs.n))
+        draws += 1
+    gs.close()
+    gbs = tot_bytes / (tot_ms / 1e3) / 1e9
+    return {"bfs_scale24_gteps_graph500": round(tot_edges / 2 / (tot_ms / 1e3) / 1e9, 2),
+            "bfs_scale24_ms_per_source": round(tot_ms / used, 3),
+            "bfs_scale24_algorithmic_gbs": round(gbs, 1),
+            "bfs_scale24_hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bfs_scale24_sources": used}
+
+
 def traversal_extra(pa, scale=22, m=1 << 26):
     """Connected components on the symmetrised scale-22 graph; Bellman-Ford (int32
     weights U[1,255]) from vertex 0 (the max-degree vertex) and 8 seeded sources."""
@@ -304,7 +331,7 @@ def main():
     host_ms = (time.perf_counter() - t1) * 1e3
     extra = {"pagerank_step_with_host_copy_ms": round(host_ms, 3)}
     if not args.no_extra:
-        for name, fn in (("bfs", bfs_extra), ("traversal", traversal_extra),
+        for name, fn in (("bfs", bfs_extra), ("bfs24", bfs24_extra), ("traversal", traversal_extra),
                          ("value", lambda p: value_extra(p, g)), ("batched", batched_extra),
                          ("adjoint", adjoint_extra), ("multigraph", multigraph_extra)):
             t1 = time.perf_counter()

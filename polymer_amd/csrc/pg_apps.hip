@@ -393,67 +393,393 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
 // in-neighbour found in the frontier becomes the parent, then break); sparse rounds push
 // top-down from the frontier list (edgeMapSparseV3).  Levels are the round numbers.
 // ======================================================================================
-// The frontier of round r is {v : level[v] == r - 1}: the pull tests that instead of
-// a bitmap, and writes nothing shared; the next frontier's list is compacted from
-// the levels afterwards (one counter per wave for millions of hits would serialise).
-struct BfsPullOp {
-    using Acc = int;
-    using Ctx = int;
-    using Stat = unsigned long long;
-    static constexpr bool kEarlyExit = true;
-    static constexpr bool kStats = false;
-    int32_t *level, *parent;
-    const int32_t *in_adj;
-    int32_t round;
-    __device__ bool active(int v) const { return level[v] < 0; }
-    __device__ int ctx(int v) const { return v; }
-    __device__ int identity() const { return 0x7FFFFFFF; }
-    __device__ int combine(int a, int b) const { return min(a, b); }
-    __device__ int edge(int s, int e, const int &) const {
-        return level[s] == round - 1 ? e : 0x7FFFFFFF;
-    }
-    __device__ bool found(int a, const int &) const { return a != 0x7FFFFFFF; }
-    __device__ Stat finish(int v, int a, const int &) const {
-        if (a != 0x7FFFFFFF) {
-            parent[v] = in_adj[a];
-            level[v] = round;
-        }
-        return 0ull;
-    }
-    __device__ void skip(int) const {}
-};
+// The frontier of round r is {v : level[v] == r - 1}, held as a bitmap (n/8 bytes,
+// small enough to stay in L2 while a bottom-up round probes it).
+//
+// Bottom-up round.  On rMat graphs almost every vertex found by a bottom-up round
+// finds its parent at the first or second in-neighbour probed (the in-lists start
+// with the highest-degree sources, which the frontier holds), so the round is
+// per-vertex overhead, not edge traffic; one workgroup per 256 vertices made it
+// dispatch-bound.  Here each wave walks chunks of 64 consecutive vertices with a
+// grid-sized stride, the in-list probed in order so that the lowest hit position
+// wins (the first frontier in-neighbour in in-list order).  The round's winners are
+// also the next frontier: the chunk's ballot is its two bitmap words, and their
+// number and out-degree sum go to spread counters.  Vertices without in-edges (the
+// zero-degree tail of the in-degree order) are not visited.
+// Almost every hit is the first in-neighbour, so that one is kept for every vertex
+// in a separate array (first_in, 4 bytes per vertex, -1 without in-edges): the
+// common case reads it, the visited bits and the out-offsets with coalesced loads
+// and probes one bitmap word.  Only a miss reads the vertex's offsets and the rest
+// of its in-list.  A wave takes J chunks of 64 at once and issues each stage's
+// loads for all of them before waiting.
+constexpr int kBuChunks = 4;
+constexpr int kBuLaneScan = 64;
+__global__ void k_first_in(const int32_t *in_off, const int32_t *in_adj, int64_t n,
+                           int32_t *first_in) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        first_in[v] = in_off[v] < in_off[v + 1] ? in_adj[in_off[v]] : -1;
+}
 
-// append: push the winners to the next list (small rounds); otherwise the list
-// is compacted from the levels after the round
+template <int J>
+__global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(
+    const int32_t *__restrict__ in_off, const int32_t *__restrict__ in_adj,
+    const int32_t *__restrict__ first_in, const int32_t *__restrict__ out_off,
+    const uint32_t *__restrict__ F, uint32_t *__restrict__ V, int32_t *level, int32_t *parent,
+    int32_t round, int64_t nbody, uint32_t *__restrict__ Fn, unsigned long long *slots) {
+    __shared__ unsigned long long sc[kBlock / 64], sd[kBlock / 64];
+    const int lane = lane_id();
+    const int64_t nchunks = (nbody + 64 * J - 1) / (64 * J);
+    const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+    unsigned long long cnt = 0, deg = 0;
+    for (int64_t ch = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); ch < nchunks;
+         ch += nwaves) {
+        // visited word of this lane's vertex (all ones past the body)
+        uint32_t vw[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int64_t v = (ch * J + j) * 64 + lane;
+            vw[j] = v < nbody ? V[v >> 5] : 0xFFFFFFFFu;
+        }
+        int ps[J], od[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int64_t v = (ch * J + j) * 64 + lane;
+            const bool open = !((vw[j] >> (lane & 31)) & 1u);
+            ps[j] = open ? first_in[v] : -1;
+            od[j] = open ? out_off[v + 1] - out_off[v] : 0;
+        }
+        bool miss[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const bool open = ps[j] >= 0;
+            const bool h = open && bit_test(F, ps[j]);
+            miss[j] = open && !h;
+            if (!h) ps[j] = -1;
+        }
+        // a miss scans the rest of its in-list: four ids per step by its own lane
+        // (all lanes of all J chunks stepping together) while at most kBuLaneScan
+        // remain, else by the whole wave, 256 per step, one vertex at a time (an
+        // edge-balanced scan of all the wave's misses measured slower: its prefix
+        // sums cost every chunk, and it cannot stop a list at its first hit)
+        int cur[J], end[J];
+        bool act[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int64_t v = (ch * J + j) * 64 + lane;
+            cur[j] = miss[j] ? in_off[v] + 1 : 0;
+            end[j] = miss[j] ? in_off[v + 1] : 0;
+            act[j] = miss[j] && end[j] > cur[j] && end[j] - cur[j] <= kBuLaneScan;
+        }
+        while (true) {
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < J; j++) any = any || act[j];
+            if (!__ballot(any)) break;
+#pragma unroll
+            for (int j = 0; j < J; j++) {
+                if (!act[j]) continue;
+                const int k4 = min(end[j] - cur[j], 4);
+                int s[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) s[k] = k < k4 ? ld_stream(in_adj + cur[j] + k) : -1;
+#pragma unroll
+                for (int k = 3; k >= 0; k--)
+                    if (s[k] >= 0 && bit_test(F, s[k])) ps[j] = s[k];
+                cur[j] += 4;
+                act[j] = ps[j] < 0 && cur[j] < end[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            unsigned long long rest = __ballot(miss[j] && end[j] - cur[j] > kBuLaneScan);
+            while (rest) {
+                const int l = __ffsll((long long)rest) - 1;
+                rest &= rest - 1;
+                const int b = __shfl(cur[j], l, 64), e = __shfl(end[j], l, 64);
+                for (int base = b; base < e; base += 256) {
+                    int sv[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int ee = base + 64 * k + lane;
+                        sv[k] = ee < e ? ld_stream(in_adj + ee) : -1;
+                    }
+                    unsigned long long hm[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) hm[k] = __ballot(sv[k] >= 0 && bit_test(F, sv[k]));
+                    int k = 0;
+                    while (k < 4 && !hm[k]) k++;
+                    if (k < 4) {
+                        const int sf = __shfl(sv[k], __ffsll((long long)hm[k]) - 1, 64);
+                        if (lane == l) ps[j] = sf;
+                        break;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int64_t c64 = ch * J + j;
+            const int64_t v = c64 * 64 + lane;
+            const bool won = ps[j] >= 0;
+            if (won) {
+                parent[v] = ps[j];
+                level[v] = round;
+                cnt++;
+                deg += (unsigned long long)od[j];
+            }
+            // the chunk's two words belong to this wave: the next frontier's are
+            // written whole (no clearing needed), the visited ones gain the winners
+            const unsigned long long wm = __ballot(won);
+            if (c64 * 64 < nbody && (lane & 31) == 0) {
+                const uint32_t w = lane == 0 ? (uint32_t)wm : (uint32_t)(wm >> 32);
+                Fn[2 * c64 + (lane >> 5)] = w;
+                if (w) V[2 * c64 + (lane >> 5)] = vw[j] | w;
+            }
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, 64);
+        deg += __shfl_xor(deg, o, 64);
+    }
+    if (lane == 0) {
+        sc[threadIdx.x >> 6] = cnt;
+        sd[threadIdx.x >> 6] = deg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = 0, d = 0;
+        for (int i = 0; i < kBlock / 64; i++) {
+            c += sc[i];
+            d += sd[i];
+        }
+        if (c) {
+            unsigned long long *p = slots + 2 * (blockIdx.x & 63);
+            atomicAdd(&p[0], c);
+            atomicAdd(&p[1], d);
+        }
+    }
+}
+
+// append: push the winners to the next list and bitmap (small rounds); otherwise
+// both are taken from the levels after the round
+// The claim is a CAS on the level (the reference CASes the parent from -1; which
+// frontier vertex wins is unspecified either way), so parents need no
+// initialisation: the parent of an unreached vertex is set to -1 on output.  The
+// visited bitmap V (n/8 bytes, L2-resident) screens the destinations first, so a
+// destination already reached costs a cached bit test, not a gather from the
+// 4-byte levels.  A winner sets its bits in V and in the next frontier bits_n.
+// This is the edge-balanced push of a large frontier list (its statistics are
+// summed from bits_n afterwards); small frontiers are pushed from the bitmap by
+// k_bfs_push_bits.  V holds exactly the vertices reached before this round, so a
+// clear bit means the destination's level is this round whoever writes it: the
+// claim is plain stores and returnless ORs, no CAS whose result each lane would
+// wait for.  Concurrent claimers of one destination all store the same level and
+// one of their (equally valid) parents survives.
 struct BfsPushOp {
-    int32_t *list_n;
-    unsigned long long *cnt_n;
+    uint32_t *bits_n, *V;
     int32_t *level, *parent;
     int32_t round;
-    bool append;
     __device__ bool src_active(int) const { return true; }
     __device__ void push(int s, int d, int) const {
-        bool won = false;
-        if (level[d] < 0 && atomicCAS(&parent[d], -1, s) == -1) {
-            level[d] = round;
-            won = true;
-        }
-        if (append) agg_append(list_n, cnt_n, won, d);
+        if (bit_test(V, d)) return;
+        level[d] = round;
+        parent[d] = s;
+        const uint32_t b = 1u << (d & 31);
+        atomicOr(&V[d >> 5], b);
+        atomicOr(&bits_n[d >> 5], b);
     }
 };
 
-// bits of {v : level[v] == round}, one ballot per 64 vertices (no atomics)
-__global__ void k_level_bits(const int32_t *level, int64_t n, int32_t round, uint32_t *bits) {
-    const int64_t nwords = (n + 31) / 32;
-    const int64_t npad = (n + 63) / 64 * 64;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < npad;
-         v += (int64_t)gridDim.x * blockDim.x) {
-        const bool p = v < n && level[v] == round;
-        const unsigned long long m = __ballot(p);
-        const int lane = threadIdx.x & 63;
-        if ((lane == 0 || lane == 32) && (v >> 5) < nwords)
-            bits[v >> 5] = lane == 0 ? (uint32_t)m : (uint32_t)(m >> 32);
+// BFS counters: one region per round, cleared once per kBfsRegions rounds rather
+// than every round.  Words 0..127 of a region are 64 spread (count, out-degree)
+// pairs of the next frontier, word 129 the length of a list compacted from the
+// current frontier's bitmap, words 130 and 131 the lengths of a small push's
+// queues.
+constexpr int kBfsRegion = 256;
+constexpr int kBfsRegions = 64;
+// direction policy of the default threshold (the top-down/bottom-up switch of
+// direction-optimising BFS with the switch points as ratios)
+static int64_t kBfsAlpha = 14, kBfsBeta = 24;
+
+static int read_region(const unsigned long long *rs, int64_t *count, int64_t *edges) {
+    unsigned long long h[128];
+    PG_CHECK_HIP(hipMemcpyAsync(h, rs, sizeof(h), hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    unsigned long long c = 0, d = 0;
+    for (int k = 0; k < 64; k++) {
+        c += h[2 * k];
+        d += h[2 * k + 1];
     }
+    *count = (int64_t)c;
+    *edges = (int64_t)d;
+    return PG_OK;
+}
+
+// Top-down round over the frontier bitmap, for small frontiers: no list and no
+// scan.  A wave takes 64 bitmap words, a lane one word; every pass of the
+// wave-uniform loop takes the next set bit of each lane's word.  A vertex of
+// out-degree at most kBfsLaneDeg is pushed by its own lane; a larger one is queued
+// (qbig up to kBfsWaveDeg, qhuge above) for k_bfs_push_queue, which gives a queued
+// vertex a whole wave and a huge one the whole grid, so a hub in a small frontier
+// is not one lane's (or one wave's) serial loop.  The claim is the one of
+// BfsPushOp; the round's statistics are summed from Fn afterwards.
+constexpr int kBfsLaneDeg = 8;
+constexpr int kBfsWaveDeg = 1024;
+
+__device__ __forceinline__ void bfs_claim(int v, int d, uint32_t *V, int32_t *level,
+                                          int32_t *parent, uint32_t *Fn, int32_t round) {
+    if (bit_test(V, d)) return;
+    level[d] = round;
+    parent[d] = v;
+    const uint32_t b = 1u << (d & 31);
+    atomicOr(&V[d >> 5], b);
+    atomicOr(&Fn[d >> 5], b);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfs_push_bits(
+    const uint32_t *__restrict__ F, int64_t nw, const int32_t *__restrict__ out_off,
+    const int32_t *__restrict__ out_adj, uint32_t *V, int32_t *level, int32_t *parent,
+    uint32_t *Fn, int32_t round, int32_t *qbig, int32_t *qhuge, unsigned long long *qctr) {
+    const int lane = lane_id();
+    const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+    for (int64_t base = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 64;
+         base < nw; base += nwaves * 64) {
+        uint32_t x = base + lane < nw ? F[base + lane] : 0u;
+        while (__ballot(x != 0u)) {
+            int v = -1, deg = 0;
+            if (x) {
+                v = (int)((base + lane) * 32 + __ffs(x) - 1);
+                x &= x - 1;
+                const int beg = out_off[v];
+                deg = out_off[v + 1] - beg;
+                if (deg <= kBfsLaneDeg) {
+                    for (int e = beg; e < beg + deg; e++)
+                        bfs_claim(v, out_adj[e], V, level, parent, Fn, round);
+                    v = -1;
+                }
+            }
+            agg_append(qbig, &qctr[0], v >= 0 && deg <= kBfsWaveDeg, v);
+            agg_append(qhuge, &qctr[1], v >= 0 && deg > kBfsWaveDeg, v);
+        }
+    }
+}
+
+// the queued vertices of k_bfs_push_bits: the huge ones edge-balanced over the
+// whole grid (batches of kBlock vertices whose degree prefix every block forms in
+// LDS; edge k of a batch goes to thread k mod grid size), every big one to one
+// wave, four 64-edge slices in flight per step
+__global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
+    const int32_t *__restrict__ qbig, const int32_t *__restrict__ qhuge,
+    const unsigned long long *qctr, const int32_t *__restrict__ out_off,
+    const int32_t *__restrict__ out_adj, uint32_t *V, int32_t *level, int32_t *parent,
+    uint32_t *Fn, int32_t round) {
+    __shared__ int32_t s_v[kBlock], s_beg[kBlock];
+    __shared__ int64_t s_pre[kBlock + 1];
+    const int64_t nh = (int64_t)qctr[1], nb = (int64_t)qctr[0];
+    const int64_t gt = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t gn = (int64_t)gridDim.x * kBlock;
+    for (int64_t b0 = 0; b0 < nh; b0 += kBlock) {
+        const int nbat = (int)min<int64_t>(kBlock, nh - b0);
+        if ((int)threadIdx.x < nbat) {
+            const int v = qhuge[b0 + threadIdx.x];
+            s_v[threadIdx.x] = v;
+            s_beg[threadIdx.x] = out_off[v];
+            s_pre[threadIdx.x + 1] = out_off[v + 1] - out_off[v];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_pre[0] = 0;
+            for (int j = 1; j <= nbat; j++) s_pre[j] += s_pre[j - 1];
+        }
+        __syncthreads();
+        const int64_t tot = s_pre[nbat];
+        for (int64_t k = gt; k < tot; k += gn) {
+            int lo = 0, hi = nbat - 1;  // last j with s_pre[j] <= k
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= k) lo = mid; else hi = mid - 1;
+            }
+            const int e = s_beg[lo] + (int)(k - s_pre[lo]);
+            bfs_claim(s_v[lo], out_adj[e], V, level, parent, Fn, round);
+        }
+        __syncthreads();
+    }
+    const int lane = lane_id();
+    const int64_t nwaves = gn / 64;
+    for (int64_t i = gt / 64; i < nb; i += nwaves) {
+        const int v = qbig[i];
+        const int beg = out_off[v], end = out_off[v + 1];
+        for (int base = beg; base < end; base += 256) {
+            int d[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int e = base + 64 * k + lane;
+                d[k] = e < end ? out_adj[e] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (d[k] >= 0) bfs_claim(v, d[k], V, level, parent, Fn, round);
+        }
+    }
+}
+
+__global__ void k_bfs_seed(int32_t src, int32_t *level, int32_t *parent, uint32_t *F,
+                           uint32_t *V, int32_t *list, unsigned long long *ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        level[src] = 0;
+        parent[src] = src;
+        F[src >> 5] |= 1u << (src & 31);
+        V[src >> 5] |= 1u << (src & 31);
+        list[0] = src;
+        ctr[0] = 1ull;
+    }
+}
+
+// size and out-degree sum of the frontier in bits, added to slots[2k] and
+// slots[2k + 1], k = block mod 64 (after a push that set the bits)
+__global__ void k_bits_stats(const uint32_t *__restrict__ bits, int64_t n,
+                             const int32_t *__restrict__ off, unsigned long long *slots) {
+    __shared__ unsigned long long sc[kBlock / 64], sd[kBlock / 64];
+    const int64_t nw = (n + 31) / 32;
+    unsigned long long cnt = 0, deg = 0;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t x = bits[w];
+        cnt += __popc(x);
+        while (x) {
+            const int64_t v = w * 32 + __ffs(x) - 1;
+            x &= x - 1;
+            deg += (unsigned long long)(off[v + 1] - off[v]);
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, 64);
+        deg += __shfl_xor(deg, o, 64);
+    }
+    if (lane_id() == 0) {
+        sc[threadIdx.x >> 6] = cnt;
+        sd[threadIdx.x >> 6] = deg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = 0, d = 0;
+        for (int i = 0; i < kBlock / 64; i++) {
+            c += sc[i];
+            d += sd[i];
+        }
+        if (c) {
+            unsigned long long *p = slots + 2 * (blockIdx.x & 63);
+            atomicAdd(&p[0], c);
+            atomicAdd(&p[1], d);
+        }
+    }
+}
+
+__global__ void k_parent_mask(const int32_t *level, int64_t n, int32_t *parent) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        if (level[v] < 0) parent[v] = -1;
 }
 
 // ======================================================================================
@@ -558,10 +884,15 @@ struct SsspPushOp {
 };
 
 // ---- traversal state ---------------------------------------------------------------
+// ctr[0] = count, ctr[1] = out-edges; ctr[4 + 2k], ctr[5 + 2k] (k < kCtrSlots) are
+// spread partial sums of the same, added by k_level_frontier's blocks (thousands of
+// atomics on one address serialise)
+constexpr int kCtrSlots = 64;
+constexpr int kCtrWords = 4 + 2 * kCtrSlots;
 struct Frontier {
     uint32_t *bits;
     int32_t *list;
-    unsigned long long *ctr;  // [0] = count, [1] = out-edges
+    unsigned long long *ctr;
 };
 
 static int frontiers(pg_graph *g, Frontier *a, Frontier *b) {
@@ -570,14 +901,30 @@ static int frontiers(pg_graph *g, Frontier *a, Frontier *b) {
     PG_TRY(workspace_t(g, "fr_bits1", nw, &b->bits));
     PG_TRY(workspace_t(g, "fr_list0", (size_t)g->n + 1, &a->list));
     PG_TRY(workspace_t(g, "fr_list1", (size_t)g->n + 1, &b->list));
-    PG_TRY(workspace_t(g, "fr_ctr0", 4, &a->ctr));
-    PG_TRY(workspace_t(g, "fr_ctr1", 4, &b->ctr));
+    PG_TRY(workspace_t(g, "fr_ctr0", kCtrWords, &a->ctr));
+    PG_TRY(workspace_t(g, "fr_ctr1", kCtrWords, &b->ctr));
     return PG_OK;
 }
 
-static int clear_frontier(pg_graph *g, Frontier &f) {
-    PG_CHECK_HIP(hipMemsetAsync(f.bits, 0, 4 * ((g->n + 31) / 32 + 1), stream()));
-    PG_CHECK_HIP(hipMemsetAsync(f.ctr, 0, 32, stream()));
+static int clear_frontier(pg_graph *g, Frontier &f, bool bits = true) {
+    if (bits) PG_CHECK_HIP(hipMemsetAsync(f.bits, 0, 4 * ((g->n + 31) / 32 + 1), stream()));
+    PG_CHECK_HIP(hipMemsetAsync(f.ctr, 0, 8 * kCtrWords, stream()));
+    return PG_OK;
+}
+
+// |F| and the sum of its out-degrees (ctr[0..1] plus the slots), read back for the
+// direction switch
+static int read_counts(Frontier &f, int64_t *count, int64_t *edges) {
+    unsigned long long h[kCtrWords];
+    PG_CHECK_HIP(hipMemcpyAsync(h, f.ctr, sizeof(h), hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    unsigned long long c = h[0], d = h[1];
+    for (int k = 0; k < kCtrSlots; k++) {
+        c += h[4 + 2 * k];
+        d += h[5 + 2 * k];
+    }
+    *count = (int64_t)c;
+    *edges = (int64_t)d;
     return PG_OK;
 }
 
@@ -614,48 +961,91 @@ int bfs_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_stat
             int32_t **level_out, int32_t **parent_out) {
     int32_t s_int;
     PG_TRY(lookup_internal(g, src, &s_int));
-    const int64_t thr = resolve_threshold(threshold, g->m, 20);
-    int32_t *level = nullptr, *parent = nullptr;
+    int32_t *level = nullptr, *parent = nullptr, *first_in = nullptr;
+    uint32_t *V = nullptr;
+    unsigned long long *slots = nullptr;
+    const int64_t nw = (g->n + 31) / 32 + 1;
     PG_TRY(workspace_t(g, "bfs_level", (size_t)g->n, &level));
     PG_TRY(workspace_t(g, "bfs_parent", (size_t)g->n, &parent));
+    PG_TRY(workspace_t(g, "bfs_visited", (size_t)nw, &V));
+    PG_TRY(workspace_t(g, "bfs_slots", (size_t)kBfsRegion * kBfsRegions, &slots));
+    // a small push's queues: its frontier has fewer than n/64 edges, so fewer than
+    // n/(64 (kBfsLaneDeg + 1)) vertices of out-degree above kBfsLaneDeg, and fewer
+    // than n/(64 kBfsWaveDeg) + 1 above kBfsWaveDeg
+    int32_t *qbig = nullptr, *qhuge = nullptr;
+    PG_TRY(workspace_t(g, "bfs_qbig", (size_t)(g->n / (64 * (kBfsLaneDeg + 1)) + 64), &qbig));
+    PG_TRY(workspace_t(g, "bfs_qhuge", (size_t)(g->n / (64 * (int64_t)kBfsWaveDeg) + 64), &qhuge));
+    const bool have_first = g->ws.find("bfs_first_in") != g->ws.end();
+    PG_TRY(workspace_t(g, "bfs_first_in", (size_t)g->n, &first_in));
+    if (!have_first) PG_LAUNCH_K(k_first_in, grid_for(g->n), g->in_off, g->in_adj, g->n, first_in);
+    int32_t sdeg[2];  // the source's out-offsets: the first frontier's statistics
+    PG_CHECK_HIP(hipMemcpy(sdeg, g->out_off + s_int, 8, hipMemcpyDeviceToHost));
     Frontier A, B;
     PG_TRY(frontiers(g, &A, &B));
+    const int64_t nbody = g->in_sched.body_end;
     Timer tm;
     KernelTimer kt;
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
     PG_TRY(tm.start());
     PG_TRY(fill(level, -1, g->n));
-    PG_TRY(fill(parent, -1, g->n));
-    PG_TRY(clear_frontier(g, A));
-    PG_LAUNCH_K(k_seed_source, 1, s_int, level, parent, (int32_t *)nullptr, A.bits, A.list, A.ctr);
-    int64_t cnt = 1, edges = 0;
-    PG_TRY(frontier_stats(g, A, &cnt, &edges));
+    PG_CHECK_HIP(hipMemsetAsync(V, 0, 4 * nw, stream()));
+    PG_CHECK_HIP(hipMemsetAsync(A.bits, 0, 4 * nw, stream()));
+    PG_CHECK_HIP(hipMemsetAsync(B.bits, 0, 4 * nw, stream()));
+    PG_CHECK_HIP(hipMemsetAsync(slots, 0, 8 * kBfsRegion * kBfsRegions, stream()));
+    PG_LAUNCH_K(k_bfs_seed, 1, s_int, level, parent, A.bits, V, A.list, A.ctr);
+    int64_t cnt = 1, edges = sdeg[1] - sdeg[0];
     int rounds = 0, dense_rounds = 0;
     int64_t tot_active = 0, tot_edges = 0;
     Frontier *F = &A, *N = &B;
+    bool f_list = true;  // F->list holds the frontier (F->bits always does)
+    bool bu = false;     // the default policy's current direction
     while (cnt > 0) {
         rounds++;
         tot_active += cnt;
         tot_edges += edges;
-        PG_TRY(clear_frontier(g, *N));
-        const bool dense = cnt + edges >= thr;  // P9: >=
+        bool dense;
+        if (threshold >= 0) {
+            dense = cnt + edges >= threshold;  // P9: >=
+        } else {
+            // bottom-up once the frontier's edges exceed 1/alpha of the edges not yet
+            // reached; back to top-down when it holds fewer than n/beta vertices
+            const int64_t unexplored = std::max<int64_t>(g->m - tot_edges, 0);
+            bu = bu ? cnt >= g->n / kBfsBeta : edges > unexplored / kBfsAlpha;
+            dense = bu;
+        }
+        // region of the spread counters and the list counter for this round
+        if ((rounds - 1) % kBfsRegions == 0 && rounds > 1)
+            PG_CHECK_HIP(hipMemsetAsync(slots, 0, 8 * kBfsRegion * kBfsRegions, stream()));
+        unsigned long long *rs = slots + (size_t)((rounds - 1) % kBfsRegions) * kBfsRegion;
+        // a bottom-up round writes every word of the next bitmap, a push sets bits
+        // in a cleared one.  A small frontier is pushed from its bitmap, a large one
+        // edge-balanced from a list.
+        const bool bitpush = !dense && edges < g->n / 64;
         PG_TRY(kt.begin());
-        // a round that may find many vertices builds the next list by compaction
-        const bool compact = dense || edges >= g->n / 64;
         if (dense) {
             dense_rounds++;
-            BfsPullOp op{level, parent, g->in_adj, rounds};
-            PG_TRY(launch_pull(op, g->in_sched));
+            PG_LAUNCH_K(k_bfs_bottom_up<kBuChunks>, grid_for(nbody / (4 * kBuChunks), 2048),
+                        g->in_off, g->in_adj, first_in, g->out_off, F->bits, V, level, parent,
+                        (int32_t)rounds, nbody, N->bits, rs);
         } else {
-            BfsPushOp op{N->list, &N->ctr[0], level, parent, rounds, !compact};
-            PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
-        }
-        if (compact) {
-            PG_LAUNCH_K(k_level_bits, grid_for(g->n), level, g->n, (int32_t)rounds, N->bits);
-            PG_TRY(bits_to_list(g, N->bits, N->list, &N->ctr[0]));
+            PG_CHECK_HIP(hipMemsetAsync(N->bits, 0, 4 * nw, stream()));
+            if (bitpush) {
+                PG_LAUNCH_K(k_bfs_push_bits, grid_for(nw, 2048), F->bits, (g->n + 31) / 32,
+                            g->out_off, g->out_adj, V, level, parent, N->bits, (int32_t)rounds,
+                            qbig, qhuge, rs + 130);
+                PG_LAUNCH_K(k_bfs_push_queue, 1024, qbig, qhuge, rs + 130, g->out_off, g->out_adj,
+                            V, level, parent, N->bits, (int32_t)rounds);
+            } else {
+                if (!f_list) PG_TRY(bits_to_list(g, F->bits, F->list, rs + 129));
+                BfsPushOp op{N->bits, V, level, parent, (int32_t)rounds};
+                PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
+            }
+            PG_LAUNCH_K(k_bits_stats, grid_for((g->n + 31) / 32, 2048), N->bits, g->n,
+                        g->out_off, rs);
         }
         PG_TRY(kt.end());
-        PG_TRY(frontier_stats(g, *N, &cnt, &edges));
+        PG_TRY(read_region(rs, &cnt, &edges));
+        f_list = false;  // only the source's list is built
         std::swap(F, N);
     }
     double ms = 0;
@@ -1010,6 +1400,7 @@ extern "C" int pg_bfs(pg_graph *g, int64_t src, int64_t threshold, int32_t *leve
     PG_TRY(check_graph(g));
     int32_t *lv = nullptr, *pr = nullptr;
     PG_TRY(bfs_run(g, src, threshold, flags, st, &lv, &pr));
+    if (parent) PG_LAUNCH_K(k_parent_mask, grid_for(g->n), lv, g->n, pr);
     PG_TRY(to_input_order_i32(g, lv, level, false));
     return to_input_order_i32(g, pr, parent, true);
 }

@@ -210,8 +210,19 @@ static int select_flagged(const uint64_t *in, const uint8_t *flags, uint64_t *ou
     return PG_OK;
 }
 
-int exclusive_scan_u32(pg_graph *, const uint32_t *in, uint32_t *out, int64_t n) {
-    return exclusive_sum(in, out, n);
+// Scans inside traversal rounds keep their temporary storage in a graph workspace:
+// a hipFree per call would wait for the device every round.
+int exclusive_scan_u32(pg_graph *g, const uint32_t *in, uint32_t *out, int64_t n) {
+    size_t tb = 0, tmax = 0;
+    PG_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, g_stream));
+    // sized for the longest scan of this graph (n + 1 items) on the first call
+    PG_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmax, in, out,
+                                                  std::max<int64_t>(n, g->n + 1), g_stream));
+    tb = std::max(tb, tmax);
+    void *tmp = nullptr;
+    PG_TRY(workspace(g, "scan_tmp", tb, &tmp));
+    PG_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, g_stream));
+    return PG_OK;
 }
 
 static int bits_for(int64_t n) {
@@ -556,7 +567,7 @@ int bits_to_list(pg_graph *g, const uint32_t *bits, int32_t *list, unsigned long
     PG_TRY(workspace_t(g, "compact_bc", (size_t)nb + 1, &bc));
     PG_TRY(workspace_t(g, "compact_bo", (size_t)nb + 1, &bo));
     PG_LAUNCH(k_word_counts, (unsigned)nb, bits, nwords, bc);
-    PG_TRY(exclusive_sum(bc, bo, nb));
+    PG_TRY(exclusive_scan_u32(g, bc, bo, nb));
     PG_LAUNCH(k_word_write, (unsigned)nb, bits, nwords, g->n, bo, list, d_count, nb);
     return PG_OK;
 }
@@ -646,6 +657,8 @@ static int free_schedule(PullSchedule &s) {
     s.d_tickets = nullptr;
     s.built = false;
     s.nblocks = 0;
+    s.nblocks_body = 0;
+    s.body_end = 0;
     return PG_OK;
 }
 
@@ -743,6 +756,24 @@ int build_schedule(pg_graph *g, bool in_direction) {
         return PG_ERR_UNSUPPORTED;
     }
     s.nblocks = (int)blocks;
+    // the zero-degree tail ends the last segment: a pull whose vertices without
+    // edges have nothing to do launches only the blocks before it
+    {
+        const int64_t body = std::max<int64_t>(cnt[8], 0);
+        int64_t bb = 0;
+        for (int i = 0; i < a.nseg; i++) {
+            const PullSeg &sg = a.seg[i];
+            if (sg.v_begin >= body) break;
+            if (sg.v_end <= body || sg.g == kModeMulti || sg.g == kModeBlock) {
+                bb = i + 1 < a.nseg ? a.seg[i + 1].block_begin : blocks;
+            } else {
+                bb = sg.block_begin + ((body - sg.v_begin) * sg.g + kBlock - 1) / kBlock;
+                break;
+            }
+        }
+        s.nblocks_body = (int)bb;
+        s.body_end = body;
+    }
     // hot schedule for the tiled pull: the segments above clipped to the positions
     // whose in-degree exceeds the tile limit (a clipped multi-block segment keeps
     // its first items, which are in position order), then the vertices without

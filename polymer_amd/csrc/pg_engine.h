@@ -211,7 +211,9 @@ __device__ __forceinline__ typename Op::Stat pull_group(const Op &op, const Pull
     if constexpr (Op::kEarlyExit) {
         // four chunks of G edges are loaded before each exit test, so four
         // independent (neighbour id, neighbour state) load pairs are in flight;
-        // a hit in an earlier chunk still wins the min over edge positions
+        // a hit in an earlier chunk still wins the min over edge positions (eight
+        // or sixteen chunks per test measured slower: every lane then issues all
+        // its loads, and the loads of a G = 1 group each touch 64 cache lines)
         constexpr int U = 4;
         for (int base = beg; base < end; base += U * G) {
             int s[U];
@@ -359,11 +361,14 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, PullArgs pa,
     }
 }
 
+// body_only: skip the vertices without edges (for an Op whose finish() and skip()
+// do nothing to a vertex with no active in-neighbour)
 template <class Op>
-inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *stat_out = nullptr) {
-    if (s.nblocks == 0) return PG_OK;
-    hipLaunchKernelGGL(k_pull<Op>, dim3(s.nblocks), dim3(kBlock), 0, stream(), op, s.args,
-                       stat_out);
+inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *stat_out = nullptr,
+                       bool body_only = false) {
+    const int nb = body_only ? s.nblocks_body : s.nblocks;
+    if (nb == 0) return PG_OK;
+    hipLaunchKernelGGL(k_pull<Op>, dim3(nb), dim3(kBlock), 0, stream(), op, s.args, stat_out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_error("k_pull launch failed: %s", hipGetErrorString(e));
@@ -665,12 +670,16 @@ inline int launch_push(const Op &op, const int32_t *off, const int32_t *adj, con
 
 // Device-wide edge-balanced push: scan[j] is the exclusive prefix sum of the
 // out-degrees of list[0..count) (scan[count] = total).  Block b takes the edges
-// [b*kPushEdges, (b+1)*kPushEdges) of the frontier, whatever vertices they belong
-// to, so a hub's edges are spread over as many blocks as they fill.  The block's
-// window of list entries is staged in LDS; the rare window larger than LDS falls
-// back to binary searches in global memory.
+// [b*epb, (b+1)*epb) of the frontier, whatever vertices they belong to, so a hub's
+// edges are spread over as many blocks as they fill.  epb is kPushEdges for large
+// frontiers and shrinks (to one edge per lane) until the grid has kPushMinBlocks
+// blocks: a push is a chain of dependent accesses per edge, and a few hundred
+// blocks of eight-edge chains leave most of the chip idle.  The block's window of
+// list entries is staged in LDS; the rare window larger than LDS falls back to
+// binary searches in global memory.
 constexpr int kPushEdges = 2048;
 constexpr int kPushWin = 2048;
+constexpr int64_t kPushMinBlocks = 2048;
 
 __device__ __forceinline__ int64_t upper_bound_u32(const uint32_t *a, int64_t n, uint32_t x) {
     int64_t lo = 0, hi = n;  // first index with a[i] > x
@@ -686,12 +695,13 @@ __global__ __launch_bounds__(kBlock) void k_push_global(Op op, const int32_t *__
                                                         const int32_t *__restrict__ adj,
                                                         const int32_t *__restrict__ list,
                                                         const uint32_t *__restrict__ scan,
-                                                        int64_t count, uint32_t total) {
+                                                        int64_t count, uint32_t total,
+                                                        uint32_t epb) {
     __shared__ uint32_t s_scan[kPushWin + 1];
     __shared__ int32_t s_beg[kPushWin];
     __shared__ int64_t s_j0, s_j1;
-    const uint32_t E0 = (uint32_t)blockIdx.x * kPushEdges;
-    const uint32_t E1 = min(total, E0 + (uint32_t)kPushEdges);
+    const uint32_t E0 = (uint32_t)blockIdx.x * epb;
+    const uint32_t E1 = min(total, E0 + epb);
     if (threadIdx.x == 0) {
         s_j0 = upper_bound_u32(scan, count + 1, E0) - 1;
         s_j1 = upper_bound_u32(scan, count + 1, E1 - 1) - 1;
@@ -746,16 +756,19 @@ inline int launch_push_global(const Op &op, pg_graph *g, const int32_t *off, con
         set_error("push: frontier edge count exceeds 32 bits");
         return PG_ERR_UNSUPPORTED;
     }
+    // sized for the largest list once: growing a workspace frees and allocates
     uint32_t *deg = nullptr, *scan = nullptr;
-    PG_TRY(workspace_t(g, "push_deg", (size_t)count + 1, &deg));
-    PG_TRY(workspace_t(g, "push_scan", (size_t)count + 1, &scan));
+    PG_TRY(workspace_t(g, "push_deg", (size_t)g->n + 1, &deg));
+    PG_TRY(workspace_t(g, "push_scan", (size_t)g->n + 1, &scan));
     const int64_t dgrid = std::min<int64_t>((count + kBlock) / kBlock, 65536);
     hipLaunchKernelGGL(k_list_degrees, dim3((unsigned)dgrid), dim3(kBlock), 0, stream(), list,
                        count, off, deg);
     PG_TRY(exclusive_scan_u32(g, deg, scan, count + 1));
-    const int64_t nb = (total + kPushEdges - 1) / kPushEdges;
+    const int64_t epb = std::min<int64_t>(
+        kPushEdges, std::max<int64_t>(kBlock, (total + kPushMinBlocks - 1) / kPushMinBlocks));
+    const int64_t nb = (total + epb - 1) / epb;
     hipLaunchKernelGGL(k_push_global<Op>, dim3((unsigned)nb), dim3(kBlock), 0, stream(), op, off,
-                       adj, list, scan, count, (uint32_t)total);
+                       adj, list, scan, count, (uint32_t)total, (uint32_t)epb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_error("k_push_global launch failed: %s", hipGetErrorString(e));

@@ -194,6 +194,8 @@ struct TileSet {
 struct PullSchedule {
     PullArgs args{};
     int nblocks = 0;
+    int nblocks_body = 0;  // the blocks that cover the positions with edges
+    int64_t body_end = 0;  // the positions with edges are [0, body_end)
     PullParams params;
     MultiItem *d_multi = nullptr;
     void *d_partial = nullptr;
