@@ -74,24 +74,31 @@ def bfs_extra(pa, scale=22, m=1 << 26, nsrc=8):
             "bfs_sources": len(srcs)}
 
 
-def bfs24_extra(pa, scale=24, m=1 << 28, nsrc=8):
-    """BFS on the symmetrised scale-24 rMat (2^29 directed entries) from vertex 0 and
-    seeded random sources with at least one edge (Graph500's rule: an isolated draw
-    is skipped), direction-optimising."""
-    gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True)
+def bfs24_runs(gs, nsrc=8):
+    """BFS from vertex 0 and seeded random sources with at least one edge
+    (Graph500's rule: an isolated draw is skipped); returns the stats of each."""
     rng = np.random.default_rng(9)
     gs.bfs(0, out=False)
-    tot_ms, tot_edges, tot_bytes, used, draws = 0.0, 0, 0, 0, 0
-    s = 0
-    while used < nsrc and draws < 20 * nsrc:
+    runs, draws, s = [], 0, 0
+    while len(runs) < nsrc and draws < 20 * nsrc:
         _, _, st = gs.bfs(s, out=False)
         if st.active > 1:
-            tot_ms += st.ms
-            tot_edges += st.edges
-            tot_bytes += 4 * st.edges + 16 * st.active
-            used += 1
+            runs.append((s, st))
         s = int(rng.integers(0, gs.n))
         draws += 1
+    return runs
+
+
+def bfs24_extra(pa, scale=24, m=1 << 28, nsrc=8):
+    """BFS on the symmetrised scale-24 rMat (2^29 directed entries), direction-
+    optimising, from the sources of bfs24_runs."""
+    gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True)
+    tot_ms, tot_edges, tot_bytes, used = 0.0, 0, 0, 0
+    for _, st in bfs24_runs(gs, nsrc):
+        tot_ms += st.ms
+        tot_edges += st.edges
+        tot_bytes += 4 * st.edges + 16 * st.active
+        used += 1
     gs.close()
     gbs = tot_bytes / (tot_ms / 1e3) / 1e9
     return {"bfs_scale24_gteps_graph500": round(tot_edges / 2 / (tot_ms / 1e3) / 1e9, 2),
@@ -169,6 +176,30 @@ def profile_run(args):
     _, st = g.pagerank(args.iters, 0.85, out=False, profile=True)
     log("pagerank %d iterations: %.3f ms, per-iteration pull avg %.1f us, edges %d" %
         (args.iters, st.ms, st.kernel_ms / max(st.kernel_launches, 1) * 1e3, st.edges))
+
+
+def profile_bfs(args):
+    """Short run for rocprofv3: the calibration kernel of profile_run, then the
+    searches of bfs24_extra on the symmetrised scale-24 rMat (the source selection,
+    whose isolated draws are searches too), then the eight sources once more: the
+    last eight searches of the run are the measured ones."""
+    import polymer_amd as pa
+    import torch
+
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    log("calibration kernel (neg): 1073741824 bytes read, 1073741824 bytes written")
+    del x, y
+    pa.init(0)
+    gs = pa.Graph.rmat(args.scale, args.edges, seed=1, symmetrize=True)
+    for s, _ in bfs24_runs(gs):
+        _, _, st = gs.bfs(s, out=False)
+        log("bfs source %d: %.3f ms, %d rounds (%d bottom-up), %d edges, %d reached" %
+            (s, st.ms, st.rounds, st.dense_rounds, st.edges, st.active))
 
 
 def measured_traffic(schedule):
@@ -256,6 +287,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--profile", action="store_true",
                     help="short run to be wrapped by rocprofv3 (no JSON line)")
+    ap.add_argument("--profile-bfs", action="store_true",
+                    help="the same for the scale-24 BFS sources of the extras")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -274,6 +307,9 @@ def main():
         sys.exit(2)
     if args.profile:
         profile_run(args)
+        return
+    if args.profile_bfs:
+        profile_bfs(args)
         return
 
     import polymer_amd as pa

@@ -186,8 +186,11 @@ int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint32_t flags, 
  * the messages are kept in fp64 unless PG_BP_FP32 is set in flags. */
 int pg_bp(pg_graph *g, int iters, const float *vpot, const float *epot, const float *prod0,
           double *prod, uint32_t flags, pg_stats *st);
-/* numa-BFS.C:258-296.  threshold: < 0 -> m/20 (direction optimising),
- * 0 -> every round dense (the reference's setting, numa-BFS.C:268).
+/* numa-BFS.C:258-296.  threshold: < 0 -> direction optimising (bottom-up once
+ * the frontier's out-degree sum exceeds 1/14 of the edges not yet reached,
+ * top-down again when the frontier holds fewer than n/24 vertices);
+ * 0 -> every round dense (the reference's setting, numa-BFS.C:268);
+ * > 0 -> dense iff |F| + sum of out-degrees >= threshold (polymer.h:2247).
  * level/parent: host n (level -1 / parent -1 when unreached). */
 int pg_bfs(pg_graph *g, int64_t src, int64_t threshold, int32_t *level, int32_t *parent,
            uint32_t flags, pg_stats *st);
