@@ -60,19 +60,29 @@ struct RowPlan {
     void *partial = nullptr;     // nheavy * 64 * 4 bytes
 };
 
-static int row_plan(pg_graph *g, RowPlan *P) {
-    const char *key = "rowplan_items";
-    auto it = g->ws.find(key);
-    const int32_t CH = 1024;
-    P->chunk = CH;
-    // heavy prefix: in-degree > CH (degrees non-increasing in the internal order)
+// first vertex of in-degree <= d (degrees are non-increasing in the internal order)
+static int first_in_degree_at_most(pg_graph *g, int64_t d, int64_t *out) {
     int64_t lo = 0, hi = g->n;
-    std::vector<int32_t> off2(2);
-    while (lo < hi) {  // first vertex with in-degree <= CH
+    int32_t off2[2];
+    while (lo < hi) {
         const int64_t mid = (lo + hi) / 2;
-        PG_CHECK_HIP(hipMemcpy(off2.data(), g->in_off + mid, 8, hipMemcpyDeviceToHost));
-        if (off2[1] - off2[0] > CH) lo = mid + 1; else hi = mid;
+        PG_CHECK_HIP(hipMemcpy(off2, g->in_off + mid, 8, hipMemcpyDeviceToHost));
+        if (off2[1] - off2[0] > d) lo = mid + 1; else hi = mid;
     }
+    *out = lo;
+    return PG_OK;
+}
+
+// heavy prefix: in-degree > heavy_deg, cut into chunks of `chunk` edges; the plan's
+// device arrays are cached in workspaces named after `name`
+static int row_plan(pg_graph *g, RowPlan *P, const std::string &name = "rowplan",
+                    int32_t heavy_deg = 1024, int32_t chunk = 1024) {
+    const std::string key = name + "_items";
+    auto it = g->ws.find(key);
+    const int32_t CH = chunk;
+    P->chunk = CH;
+    int64_t lo = 0;
+    PG_TRY(first_in_degree_at_most(g, heavy_deg, &lo));
     P->H = (int32_t)lo;
     std::vector<int32_t> offs((size_t)P->H + 1);
     if (P->H > 0)
@@ -86,9 +96,9 @@ static int row_plan(pg_graph *g, RowPlan *P) {
     P->nheavy = (int32_t)items.size();
     P->nitems = (int64_t)P->nheavy + (g->n - P->H);
     const bool have = it != g->ws.end();
-    PG_TRY(workspace_t(g, key, items.size() + 1, &P->items));
-    PG_TRY(workspace_t(g, "rowplan_tickets", (size_t)P->H + 1, &P->tickets));
-    PG_TRY(workspace(g, "rowplan_partial", (size_t)P->nheavy * kRow * 4 + 64, &P->partial));
+    PG_TRY(workspace_t(g, key.c_str(), items.size() + 1, &P->items));
+    PG_TRY(workspace_t(g, (name + "_tickets").c_str(), (size_t)P->H + 1, &P->tickets));
+    PG_TRY(workspace(g, (name + "_partial").c_str(), (size_t)P->nheavy * kRow * 4 + 64, &P->partial));
     if (!have) {
         if (!items.empty())
             PG_CHECK_HIP(hipMemcpy(P->items, items.data(), sizeof(MultiItem) * items.size(),
@@ -150,8 +160,18 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowOp op, const int32_t *__rest
         prev = __shfl(prev, 0, 64);
         if (prev != (unsigned)(it.nchunks - 1)) return;
         __threadfence();
+        // chunk order; eight partials loaded before they are combined
         T tot = op.identity();
-        for (int c = 0; c < it.nchunks; c++)
+        int c = 0;
+        for (; c + 7 < it.nchunks; c += 8) {
+            T pv[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                pv[k] = load_agent(&part[(int64_t)(it.first + c + k) * kRow + lane]);
+#pragma unroll
+            for (int k = 0; k < 8; k++) tot = op.combine(tot, pv[k]);
+        }
+        for (; c < it.nchunks; c++)
             tot = op.combine(tot, load_agent(&part[(int64_t)(it.first + c) * kRow + lane]));
         if (lane == 0) P.tickets[it.slot] = 0u;
         st = op.finish(v, tot, lane);
@@ -201,6 +221,192 @@ struct PprRowOp {
         return 0ull;
     }
 };
+
+// in-degree classes of the batched PPR iteration (measured on the scale-22 graph:
+// 39.5 ms for 20 iterations at these, 46 ms at 4096 / 2048 / 256, 66 ms with a
+// heavy limit of 65536)
+constexpr int32_t kPprHeavy = 8192, kPprChunk = 4096, kPprLight = 512;
+
+// r = d·sum + (1-d)[v is the column's source] for the lane's four columns; the rank
+// row on the last iteration, the contribution row r/outdeg when v has out-edges
+__device__ __forceinline__ void ppr_row_finish(const PprRowOp &op, int64_t v, int l, float4 acc,
+                                               const int sc[4]) {
+    float4 r;
+    r.x = op.damp * acc.x + (sc[0] == (int)v ? op.tele : 0.f);
+    r.y = op.damp * acc.y + (sc[1] == (int)v ? op.tele : 0.f);
+    r.z = op.damp * acc.z + (sc[2] == (int)v ? op.tele : 0.f);
+    r.w = op.damp * acc.w + (sc[3] == (int)v ? op.tele : 0.f);
+    if (op.write_r) reinterpret_cast<float4 *>(op.R)[v * (kRow / 4) + l] = r;
+    const int od = op.out_off[v + 1] - op.out_off[v];
+    if (od > 0) {
+        const float f = (float)od;
+        reinterpret_cast<float4 *>(op.Cn)[v * (kRow / 4) + l] =
+            make_float4(r.x / f, r.y / f, r.z / f, r.w / f);
+    }
+}
+
+// The light vertices [H, n) of a batched PPR iteration, four per wavefront: a
+// 16-lane group per vertex and four columns (a float4) per lane, so one 16-byte
+// load per lane gathers four whole 256-byte rows, eight edges of each group in
+// flight.  With a wavefront per vertex and one column per lane, most of the
+// 2^22 one-wavefront items of the scale-22 graph were dispatch and per-vertex
+// overhead around a few gathers.  The grid is a fixed set of groups striding over
+// the vertices (consecutive groups take consecutive vertices, whose in-degrees are
+// alike in the in-degree order, so a wavefront's groups finish together).  The
+// in-degree prefix [0, H) goes to k_ppr_rows_wave.
+__global__ __launch_bounds__(kBlock) void k_ppr_rows4(PprRowOp op, const int32_t *__restrict__ in_off,
+                                                      const int32_t *__restrict__ in_adj, int32_t H,
+                                                      int64_t n) {
+    const int l = threadIdx.x & 15;
+    const int64_t ngroups = (int64_t)gridDim.x * (kBlock / 16);
+    const float4 *__restrict__ C4 = reinterpret_cast<const float4 *>(op.C);
+    int sc[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) sc[j] = op.src[4 * l + j];
+    for (int64_t v = H + (int64_t)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4); v < n;
+         v += ngroups) {
+        const int beg = in_off[v], end = in_off[v + 1];
+        if (!op.vertex_live((int)v, end > beg)) continue;
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        int e = beg;
+        for (; e + 7 < end; e += 8) {
+            int s[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) s[k] = ld_stream(in_adj + e + k);
+            float4 a[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+            acc.x += ((a[0].x + a[1].x) + (a[2].x + a[3].x)) + ((a[4].x + a[5].x) + (a[6].x + a[7].x));
+            acc.y += ((a[0].y + a[1].y) + (a[2].y + a[3].y)) + ((a[4].y + a[5].y) + (a[6].y + a[7].y));
+            acc.z += ((a[0].z + a[1].z) + (a[2].z + a[3].z)) + ((a[4].z + a[5].z) + (a[6].z + a[7].z));
+            acc.w += ((a[0].w + a[1].w) + (a[2].w + a[3].w)) + ((a[4].w + a[5].w) + (a[6].w + a[7].w));
+        }
+        for (; e + 3 < end; e += 4) {
+            int s[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) s[k] = ld_stream(in_adj + e + k);
+            float4 a[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+            acc.x += (a[0].x + a[1].x) + (a[2].x + a[3].x);
+            acc.y += (a[0].y + a[1].y) + (a[2].y + a[3].y);
+            acc.z += (a[0].z + a[1].z) + (a[2].z + a[3].z);
+            acc.w += (a[0].w + a[1].w) + (a[2].w + a[3].w);
+        }
+        for (; e < end; e++) {
+            const float4 a = C4[(int64_t)ld_stream(in_adj + e) * (kRow / 4) + l];
+            acc.x += a.x;
+            acc.y += a.y;
+            acc.z += a.z;
+            acc.w += a.w;
+        }
+        ppr_row_finish(op, v, l, acc, sc);  // a row without out-edges is never gathered
+    }
+}
+
+// The heavier vertices of a batched PPR iteration, one wavefront per item in the
+// same float4 layout: item w < P.nheavy is a chunk of a heavy vertex (in-degree
+// above the plan's threshold), item P.nheavy + i is the medium vertex P.H + i.  The
+// four 16-lane groups take every fourth edge (four gathers in flight each) and their
+// sums meet by two cross-group shuffles; a multi-chunk vertex is finished by its
+// last chunk to arrive, which adds the chunks' partial rows in chunk order.
+__global__ __launch_bounds__(kBlock) void k_ppr_rows_wave(PprRowOp op,
+                                                          const int32_t *__restrict__ in_off,
+                                                          const int32_t *__restrict__ in_adj,
+                                                          RowPlan P, int64_t nmedium) {
+    const int64_t w = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64;
+    if (w >= P.nheavy + nmedium) return;  // uniform per wavefront
+    const int lane = threadIdx.x & 63, q = lane >> 4, l = lane & 15;
+    const bool heavy = w < P.nheavy;
+    MultiItem it{};
+    int v, beg, end;
+    if (heavy) {
+        it = P.items[w];
+        v = it.pos;
+        beg = in_off[v] + it.chunk * P.chunk;
+        end = min(in_off[v + 1], beg + P.chunk);
+    } else {
+        v = P.H + (int)(w - P.nheavy);
+        beg = in_off[v];
+        end = in_off[v + 1];
+    }
+    const float4 *__restrict__ C4 = reinterpret_cast<const float4 *>(op.C);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int e = beg + q;
+    for (; e + 12 < end; e += 16) {
+        int s[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = ld_stream(in_adj + e + 4 * k);
+        float4 a[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+        acc.x += (a[0].x + a[1].x) + (a[2].x + a[3].x);
+        acc.y += (a[0].y + a[1].y) + (a[2].y + a[3].y);
+        acc.z += (a[0].z + a[1].z) + (a[2].z + a[3].z);
+        acc.w += (a[0].w + a[1].w) + (a[2].w + a[3].w);
+    }
+    for (; e < end; e += 4) {
+        const float4 a = C4[(int64_t)ld_stream(in_adj + e) * (kRow / 4) + l];
+        acc.x += a.x;
+        acc.y += a.y;
+        acc.z += a.z;
+        acc.w += a.w;
+    }
+#pragma unroll
+    for (int o = 16; o <= 32; o <<= 1) {
+        acc.x += __shfl_xor(acc.x, o, 64);
+        acc.y += __shfl_xor(acc.y, o, 64);
+        acc.z += __shfl_xor(acc.z, o, 64);
+        acc.w += __shfl_xor(acc.w, o, 64);
+    }
+    int sc[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) sc[j] = op.src[4 * l + j];
+    if (!heavy || it.nchunks == 1) {
+        if (q == 0) ppr_row_finish(op, v, l, acc, sc);
+        return;
+    }
+    float *part = reinterpret_cast<float *>(P.partial);
+    if (q == 0) {
+        store_agent(&part[w * kRow + 4 * l], acc.x);
+        store_agent(&part[w * kRow + 4 * l + 1], acc.y);
+        store_agent(&part[w * kRow + 4 * l + 2], acc.z);
+        store_agent(&part[w * kRow + 4 * l + 3], acc.w);
+    }
+    __threadfence();
+    unsigned prev = 0;
+    if (lane == 0) prev = atomicAdd(&P.tickets[it.slot], 1u);
+    prev = __shfl(prev, 0, 64);
+    if (prev != (unsigned)(it.nchunks - 1)) return;
+    __threadfence();
+    // the four groups load the partials of every fourth chunk, then the sums are
+    // added in chunk order by group 0 (a shuffle per chunk)
+    float4 tot = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c0 = 0; c0 < it.nchunks; c0 += 4) {
+        const int c = c0 + q;
+        float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (c < it.nchunks) {
+            const float *pc = &part[(int64_t)(it.first + c) * kRow + 4 * l];
+            pv.x = load_agent(pc);
+            pv.y = load_agent(pc + 1);
+            pv.z = load_agent(pc + 2);
+            pv.w = load_agent(pc + 3);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float x = __shfl(pv.x, 16 * k + l, 64), y = __shfl(pv.y, 16 * k + l, 64),
+                        z = __shfl(pv.z, 16 * k + l, 64), ww = __shfl(pv.w, 16 * k + l, 64);
+            if (c0 + k < it.nchunks) {
+                tot.x += x;
+                tot.y += y;
+                tot.z += z;
+                tot.w += ww;
+            }
+        }
+    }
+    if (lane == 0) P.tickets[it.slot] = 0u;  // ready for the next launch
+    if (q == 0) ppr_row_finish(op, v, l, tot, sc);
+}
 
 __global__ void k_ppr_rows_init(int64_t n, const int32_t *out_off, const int32_t *src, float *C) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * kRow;
@@ -459,8 +665,14 @@ extern "C" int pg_ppr_batched(pg_graph *g, const int64_t *src, int k, int iters,
     PG_TRY(map_sources(g, src, k, &s));
     s.resize(kRow, -1);
     const int64_t n = g->n;
+    // heavy: in-degree > kPprHeavy, chunks of kPprChunk (k_ppr_rows_wave, several
+    // wavefronts per vertex); medium: in-degree in (kPprLight, kPprHeavy] (one
+    // wavefront each); light: the rest (a 16-lane group each, k_ppr_rows4)
     RowPlan P;
-    PG_TRY(row_plan(g, &P));
+    PG_TRY(row_plan(g, &P, "pprplan", kPprHeavy, kPprChunk));
+    int64_t M = 0;
+    PG_TRY(first_in_degree_at_most(g, kPprLight, &M));
+    M = std::max<int64_t>(M, P.H);
     float *C0 = nullptr, *C1 = nullptr, *R = nullptr;
     int32_t *dsrc = nullptr;
     PG_TRY(workspace_t(g, "pprb_c0", (size_t)n * kRow, &C0));
@@ -479,7 +691,13 @@ extern "C" int pg_ppr_batched(pg_graph *g, const int64_t *src, int k, int iters,
         PprRowOp op{C, Cn, R, g->out_off, dsrc, (float)damping, (float)(1.0 - damping),
                     t == iters - 1, t >= 2};
         PG_TRY(kt.begin());
-        PG_TRY(launch_rows(op, g, P, nullptr));
+        const int64_t nwave = P.nheavy + (M - P.H);
+        if (nwave > 0)
+            PG_LAUNCH_B(k_ppr_rows_wave, (nwave * 64 + kBlock - 1) / kBlock, op, g->in_off,
+                        g->in_adj, P, M - P.H);
+        if (n > M)
+            PG_LAUNCH_B(k_ppr_rows4, grid_for((n - M) * 16, 4096), op, g->in_off, g->in_adj,
+                        (int32_t)M, n);
         PG_TRY(kt.end());
         std::swap(C, Cn);
     }
