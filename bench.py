@@ -137,7 +137,9 @@ def value_extra(pa, g, scale=24, m=1 << 28):
     gw.spmv(x0, 20, out=False)
     _, st = gw.spmv(x0, 20, out=False)
     out["spmv_scale24_ms"] = round(st.ms, 3)
-    out["spmv_scale24_algorithmic_gbs"] = round((12 * gw.m + 12 * gw.n) * 20 / (st.ms / 1e3) / 1e9, 1)
+    gbs = (12 * gw.m + 12 * gw.n) * 20 / (st.ms / 1e3) / 1e9
+    out["spmv_scale24_algorithmic_gbs"] = round(gbs, 1)
+    out["spmv_scale24_hbm_frac"] = round(gbs / HBM_PEAK_GBS, 4)
     gw.close()
     g.bp_potentials(9)
     vpot = np.random.default_rng(8).uniform(0.1, 1.0, 2 * g.n).astype(np.float32)

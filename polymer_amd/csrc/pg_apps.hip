@@ -82,6 +82,7 @@ struct PrOp {
     // one graph sum to at most 1 (no teleport mass is created beyond (1-d)), so
     // every per-vertex sum of contributions is <= 1 < 2^(64-62) and cannot
     // overflow; the truncation of each term is < 2^-62 absolute.
+    static constexpr bool kTileWeighted = false;
     __device__ float tile_raw(int s) const { return contrib[s]; }
     __device__ unsigned long long tile_fix(float x) const { return to_fixed<62>(x); }
     __device__ float tile_sum(unsigned long long q) const { return (float)((double)q * 0x1p-62); }
@@ -232,7 +233,23 @@ struct SpmvOp {
     const float *x;
     const float *w;  // CSC order
     float *y;
-    const int32_t *shift;  // this iteration's exponent shift
+    const int32_t *shift;   // this iteration's exponent shift
+    const int32_t *tscale;  // tiled pull: exponent k of this iteration's fixed point
+    // Tiled pull: a term x[s]·w_e is scaled by 2^k and accumulated as a signed
+    // (two's complement) 64-bit fixed-point number with 32 fraction bits.  k is
+    // chosen from max|x| and the largest in-weight sum W (k_spmv_tscale) so that
+    // max|x|·W·2^k <= 2^28: no sum or partial sum reaches 2^61.  Each term is
+    // truncated by < 2^-32 in scaled units, i.e. < 2^-60 of the largest possible
+    // sum; the sum itself is converted to fp32 once.
+    static constexpr bool kTileWeighted = true;
+    __device__ float tile_raw(int s) const { return x[s]; }
+    __device__ unsigned long long tile_fix(float p) const {
+        const unsigned long long q = to_fixed<32>(ldexpf(fabsf(p), *tscale));
+        return p < 0.f ? (unsigned long long)(-(long long)q) : q;
+    }
+    __device__ float tile_sum(unsigned long long q) const {
+        return ldexpf((float)(long long)q, -32 - *tscale);
+    }
     __device__ bool active(int) const { return true; }
     __device__ int ctx(int v) const { return v; }
     __device__ float identity() const { return 0.f; }
@@ -259,6 +276,33 @@ __global__ void k_spmv_shift(const unsigned *maxbits, int32_t *shift_next, long 
         *shift_next = s;
         total[0] += this_shift;
     }
+}
+
+// the tile scale exponent of an iteration whose input has max |x| = *maxbits (as
+// float bits) on a graph whose largest in-weight sum is at most *wmax_bits
+__global__ void k_spmv_tscale(const unsigned *maxbits, const unsigned *wmax_bits, int32_t *k) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const float mx = __uint_as_float(*maxbits), wm = __uint_as_float(*wmax_bits);
+        int s = 0;
+        if (mx > 0.f && isfinite(mx) && wm > 0.f && isfinite(wm))
+            s = 28 - (ilogbf(mx) + 1) - (ilogbf(wm) + 1);
+        *k = s;
+    }
+}
+
+// largest sum of |w| over a vertex's in-edges (fp32 sums, times 1.5 for their
+// rounding), as float bits in *out (non-negative floats order as integers)
+__global__ void k_in_weight_sum_max(const int32_t *in_off, const float *w, int64_t n,
+                                    unsigned *out) {
+    unsigned m = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int e = in_off[v]; e < in_off[v + 1]; e++) s += fabsf(w[e]);
+        m = max(m, __float_as_uint(s * 1.5f));
+    }
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned)__shfl_xor(m, o, 64));
+    if (lane_id() == 0) atomicMax(out, m);
 }
 
 __global__ void k_absmax_bits(const float *x, int64_t n, unsigned *out) {
@@ -1242,6 +1286,27 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
     hipLaunchKernelGGL(k_spmv_shift, dim3(1), dim3(64), 0, stream(), maxbits, shift, total, 0);
     PG_CHECK_HIP(hipGetLastError());
     PG_LAUNCH_K(k_ldexp_inplace, grid_for(n), xa, n, shift);
+    // tiled pull (the fixed-point tiles of the PageRank family, with the weights in
+    // tile order): the largest in-weight sum once per graph, max |x0| as scaled, and
+    // a fixed-point exponent per iteration
+    const bool tiled = g->use_tiles;
+    int32_t *tscale = nullptr;
+    unsigned *wmax = nullptr, *x0max = nullptr;
+    if (tiled) {
+        PG_TRY(ensure_tiles(g));
+        PG_REQUIRE(g->tiles.m_tiled == 0 || g->tiles.d_wt, PG_ERR_STATE,
+                   "tiles carry no weights");
+        const bool have_w = g->ws.find("spmv_wmax") != g->ws.end();
+        PG_TRY(workspace_t(g, "spmv_wmax", 2, &wmax));
+        if (!have_w) {
+            PG_TRY(fill(wmax, 0u, 2));
+            PG_LAUNCH_K(k_in_weight_sum_max, grid_for(n, 4096), g->in_off, w, n, wmax);
+        }
+        PG_TRY(workspace_t(g, "spmv_tscale", (size_t)iters + 2, &tscale));
+        PG_TRY(workspace_t(g, "spmv_x0max", 2, &x0max));
+        PG_TRY(fill(x0max, 0u, 2));
+        PG_LAUNCH_K(k_absmax_bits, grid_for(n, 2048), xa, n, x0max);
+    }
     KernelTimer kt;
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
     Timer tm;
@@ -1249,9 +1314,16 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
     float *cur = xa, *nxt = xb;
     for (int t = 0; t < iters; t++) {
         // iteration t scales its output by 2^-shift[t+1]; shift[t+1] comes from max(x_t)
-        SpmvOp op{cur, w, nxt, shift + t + 1};
+        SpmvOp op{cur, w, nxt, shift + t + 1, tscale ? tscale + t : nullptr};
         PG_TRY(kt.begin());
-        PG_TRY(launch_pull(op, g->in_sched, maxbits + t + 1));
+        if (tiled) {
+            hipLaunchKernelGGL(k_spmv_tscale, dim3(1), dim3(64), 0, stream(),
+                               t == 0 ? x0max : maxbits + t, wmax, tscale + t);
+            // the zero-in-degree tail holds zeros in both buffers from iteration 2
+            PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, maxbits + t + 1, t >= 2));
+        } else {
+            PG_TRY(launch_pull(op, g->in_sched, maxbits + t + 1));
+        }
         PG_TRY(kt.end());
         hipLaunchKernelGGL(k_spmv_shift, dim3(1), dim3(64), 0, stream(), maxbits + t + 1,
                            shift + t + 2, total, 0);

@@ -825,30 +825,39 @@ void free_tiles(pg_graph *g) {
     if (t.d_cold_pos) (void)hipFree(t.d_cold_pos);
     if (t.d_cold_srcl) (void)hipFree(t.d_cold_srcl);
     if (t.d_cold_val) (void)hipFree(t.d_cold_val);
+    if (t.d_wt) (void)hipFree(t.d_wt);
     t = TileSet{};
 }
 
-// key = tile << 32 | source, value = destination offset within the tile, for the
-// CSC edges of the light positions [p0, p1)
+// key = tile << 32 | source, value = the edge's index relative to e0, for the CSC
+// edges of the light positions [p0, p1); dloc[index] = destination offset within
+// the tile
 __global__ void k_tile_keys(const int32_t *off, const int32_t *adj, int64_t p0, int64_t p1,
-                            int64_t e0, uint64_t *keys, uint32_t *vals) {
+                            int64_t e0, uint64_t *keys, uint32_t *vals, uint16_t *dloc) {
     for (int64_t p = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < p1;
          p += (int64_t)gridDim.x * blockDim.x) {
         const int t = (int)((p - p0) / kTileMaxV);
-        const uint32_t local = (uint32_t)((p - p0) % kTileMaxV);
+        const uint16_t local = (uint16_t)((p - p0) % kTileMaxV);
         for (int64_t e = off[p]; e < off[p + 1]; e++) {
             keys[e - e0] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)adj[e];
-            vals[e - e0] = local;
+            vals[e - e0] = (uint32_t)(e - e0);
+            dloc[e - e0] = local;
         }
     }
 }
 
-__global__ void k_tile_split(const uint64_t *keys, const uint32_t *vals, int64_t m, int32_t *src,
-                             uint16_t *dst) {
+// tile-order sources and destinations, and the weights (as fp32) of a weighted
+// graph: the sorted values are CSC edge indices, so every edge keeps its own
+// weight whatever its source or duplicates
+__global__ void k_tile_split(const uint64_t *keys, const uint32_t *vals, int64_t m,
+                             const uint16_t *dloc, int64_t e0, const float *wf,
+                             const int32_t *wi, int32_t *src, uint16_t *dst, float *wt) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
          i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t ei = vals[i];
         src[i] = (int32_t)(uint32_t)keys[i];
-        dst[i] = (uint16_t)vals[i];
+        dst[i] = dloc[ei];
+        if (wt) wt[i] = wf ? wf[e0 + ei] : (float)wi[e0 + ei];
     }
 }
 
@@ -1097,18 +1106,26 @@ int ensure_tiles(pg_graph *g) {
     PG_CHECK_HIP(hipMemcpy(t.d_vbeg, vbeg.data(), sizeof(int32_t) * vbeg.size(), hipMemcpyHostToDevice));
     if (!fin.empty())
         PG_CHECK_HIP(hipMemcpy(t.d_fin, fin.data(), sizeof(TileFin) * fin.size(), hipMemcpyHostToDevice));
+    const bool weighted = g->w_dtype == PG_W_F32 || g->w_dtype == PG_W_I32;
+    if (weighted && mt > 0)
+        PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_wt), sizeof(float) * (mt + 16)));
     if (mt > 0) {
-        DevBuf k0, k1, v0, v1;
+        PG_REQUIRE(mt < ((int64_t)1 << 32), PG_ERR_UNSUPPORTED, "tiles: too many light edges");
+        DevBuf k0, k1, v0, v1, dl;
         PG_TRY(k0.alloc(sizeof(uint64_t) * mt));
         PG_TRY(k1.alloc(sizeof(uint64_t) * mt));
         PG_TRY(v0.alloc(sizeof(uint32_t) * mt));
         PG_TRY(v1.alloc(sizeof(uint32_t) * mt));
+        PG_TRY(dl.alloc(sizeof(uint16_t) * mt));
         PG_LAUNCH(k_tile_keys, grid_for(p1 - p0), g->in_off, g->in_adj, p0, p1, (int64_t)off_h[0],
-                  k0.as<uint64_t>(), v0.as<uint32_t>());
+                  k0.as<uint64_t>(), v0.as<uint32_t>(), dl.as<uint16_t>());
         PG_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(),
                           mt, 32 + bits_for(nt)));
-        PG_LAUNCH(k_tile_split, grid_for(mt), k1.as<uint64_t>(), v1.as<uint32_t>(), mt, t.d_src,
-                  t.d_dst);
+        PG_LAUNCH(k_tile_split, grid_for(mt), k1.as<uint64_t>(), v1.as<uint32_t>(), mt,
+                  dl.as<uint16_t>(), (int64_t)off_h[0],
+                  g->w_dtype == PG_W_F32 ? g->in_w_f32 : (const float *)nullptr,
+                  g->w_dtype == PG_W_I32 ? g->in_w_i32 : (const int32_t *)nullptr, t.d_src,
+                  t.d_dst, t.d_wt);
         PG_CHECK_HIP(hipStreamSynchronize(g_stream));
     }
     // cold split: every part's first cold edge; the pre-pass lists by source chunk
@@ -1126,7 +1143,7 @@ int ensure_tiles(pg_graph *g) {
                                hipMemcpyHostToDevice));
     t.args = TileArgs{t.d_vbeg,    t.d_parts,   t.d_fin,      t.d_packed,
                       t.d_gbase,   t.d_esc,     t.d_partial,  t.d_cold_val,
-                      (int32_t)parts.size(),    (int32_t)fin.size()};
+                      (int32_t)parts.size(),    (int32_t)fin.size(), t.d_wt};
     t.ntiles = nt;
     t.m_tiled = mt;
     t.tile_edges = g->tile_edges;

@@ -383,8 +383,10 @@ inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *s
 // adds the gathered values into per-destination accumulators in LDS.  An Op used
 // here also provides
 //     float tile_raw(int s);                      // the value gathered from source s
-//     unsigned long long tile_fix(float x);       // that value as fixed point
+//     unsigned long long tile_fix(float x);       // an edge's term as fixed point
 //     Acc tile_sum(unsigned long long q);         // an accumulated fixed-point sum
+//     static constexpr bool kTileWeighted;        // term = tile_raw(s) * w(edge)
+// (weights from TileArgs::wt, the graph's edge weights in tile order).
 // Integer addition is associative, so the result does not depend on the order in
 // which the LDS atomics land.  The per-vertex protocol (active / finish / skip,
 // statistics) is the one k_pull follows.
@@ -407,6 +409,15 @@ __host__ __device__ __forceinline__ unsigned long long to_fixed(float x) {
     const int sh = (ex ? ex : 1) - 150 + F;  // x = mant * 2^(ex-150)
     if (sh >= 0) return mant << sh;
     return sh > -64 ? mant >> (-sh) : 0ull;
+}
+
+// the fixed-point term of tile edge i whose source value is x: x itself, or x times
+// the edge's weight (Op::kTileWeighted, weights streamed in tile order)
+template <class Op>
+__device__ __forceinline__ unsigned long long tile_term(const Op &op, float x, const float *wt,
+                                                        int64_t i) {
+    if constexpr (Op::kTileWeighted) return op.tile_fix(x * __builtin_nontemporal_load(wt + i));
+    else return op.tile_fix(x);
 }
 
 template <class Op>
@@ -443,7 +454,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                 const int64_t i = e0 + e + k * kTileThreads;
                 const int s = b[k] >= 0 ? b[k] + (int)(w[k] >> kPackDstBits)
                                         : ta.esc[(int64_t)(-b[k] - 1) * kPackGroup + (i % kPackGroup)];
-                q[k] = op.tile_fix(op.tile_raw(s));
+                q[k] = tile_term(op, op.tile_raw(s), ta.wt, i);
             }
 #pragma unroll
             for (int k = 0; k < U; k++) atomicAdd(&acc[w[k] & kDstMask], q[k]);
@@ -454,7 +465,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
             const int b = ta.gbase[i / kPackGroup];
             const int s = b >= 0 ? b + (int)(w >> kPackDstBits)
                                  : ta.esc[(int64_t)(-b - 1) * kPackGroup + (i % kPackGroup)];
-            atomicAdd(&acc[w & kDstMask], op.tile_fix(op.tile_raw(s)));
+            atomicAdd(&acc[w & kDstMask], tile_term(op, op.tile_raw(s), ta.wt, i));
         }
     }
     {  // cold sources: values streamed from the pre-pass
@@ -471,11 +482,12 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                 d[k] = __builtin_nontemporal_load(pk + e + k * kTileThreads) & kDstMask;
             }
 #pragma unroll
-            for (int k = 0; k < U; k++) atomicAdd(&acc[d[k]], op.tile_fix(x[k]));
+            for (int k = 0; k < U; k++)
+                atomicAdd(&acc[d[k]], tile_term(op, x[k], ta.wt, part.cold_e + e + k * kTileThreads));
         }
         for (; e < cnt; e += kTileThreads)
             atomicAdd(&acc[__builtin_nontemporal_load(pk + e) & kDstMask],
-                      op.tile_fix(__builtin_nontemporal_load(val + e)));
+                      tile_term(op, __builtin_nontemporal_load(val + e), ta.wt, part.cold_e + e));
     }
     __syncthreads();
     if (part.slot >= 0) {

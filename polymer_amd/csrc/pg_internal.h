@@ -162,6 +162,7 @@ struct TileArgs {
     const float *cold_val;       // values of the cold edges (written by the pre-pass)
     int32_t nparts;
     int32_t nfin;
+    const float *wt;             // edge weights in tile order (weighted graphs), or null
 };
 
 struct TileSet {
@@ -178,6 +179,7 @@ struct TileSet {
     int32_t *d_gbase = nullptr;
     int32_t *d_esc = nullptr;
     int64_t n_esc = 0;
+    float *d_wt = nullptr;  // edge weights in tile order (weighted graphs only)
     unsigned long long *d_partial = nullptr;
     int32_t ntiles = 0;
     int64_t m_tiled = 0;
@@ -236,7 +238,7 @@ struct pg_graph {
     pg::PullSchedule in_sched;   // pull over CSC (in-edges)
     pg::PullSchedule out_sched;  // pull over CSR (out-edges)
     pg::TileSet tiles;           // source-sorted tiles of the CSC light range
-    bool use_tiles = true;       // PageRank-family solves use the tiles
+    bool use_tiles = true;       // PageRank-family and SpMV solves use the tiles
     int64_t tile_max_degree = (int64_t)1 << 30;  // heavier vertices stay in the hot pull
     int64_t tile_edges = 524288; // target edges per tile part
     int64_t tile_cold = 2 << 20; // first cold source id of the tiles (<0: no cold split)

@@ -213,6 +213,58 @@ def test_spmv_f32_values(pa, orc):
     assert rel_l1(y, ref) < TOL
 
 
+def test_spmv_tiles_signed(pa, orc):
+    """SpMV through the fixed-point tiles with signed weights and values (two's
+    complement sums), every tile layout of the PageRank tile test, against the fp64
+    oracle and against the untiled fp32 pull; tiled results are reproducible."""
+    say("case: spmv tiles, signed weights and values")
+    g = orc.rmat(14, 1 << 18, seed=8)
+    rng = np.random.default_rng(17)
+    w = rng.standard_normal(len(g.adj)).astype(np.float32)
+    x0 = rng.standard_normal(g.n).astype(np.float32)
+    ref = orc.spmv(g, w.astype(np.float64), x0.astype(np.float64), 6)
+    d = pa.Graph.from_csr(g.n, g.off, g.adj, w=w)
+    out = {}
+    for tiles, te, cold in ((1, 7, -1), (1, 4096, -1), (1, 1 << 30, -1), (1, 4096, 1000),
+                            (1, 262144, 5000), (1, 7, 30000), (0, 0, -1)):
+        d.set_tiles(tiles, te, cold, max_degree=1 << 30)
+        y, _ = d.spmv(x0, 6)
+        err = rel_l1(y, ref)
+        say("tiles %d edges %d cold %d: relL1 %.3e" % (tiles, te, cold, err))
+        assert err < TOL, (tiles, te, cold)
+        out[(tiles, te, cold)] = y
+    d.set_tiles(1, 4096, -1, max_degree=1 << 30)
+    y2, _ = d.spmv(x0, 6)  # integer sums: independent of the order the atomics land in
+    assert np.array_equal(y2, out[(1, 4096, -1)])
+    d.set_tiles(1, 4096, 1000, max_degree=1 << 30)  # cold values are the same fp32 numbers
+    y3, _ = d.spmv(x0, 6)
+    assert np.array_equal(y3, out[(1, 4096, -1)])
+
+
+def test_spmv_tiles_escapes_and_hot(pa, orc):
+    # n = 2^20: packed groups with escape rows; also a low tile in-degree limit, so
+    # that the heaviest destinations are pulled by k_pull beside the tiles
+    say("case: spmv tiles with escape groups and hot vertices")
+    g = orc.rmat(20, 1 << 22, seed=12)
+    w = orc.values_f32(g, 4)
+    x0 = orc.vertex_f32(g.n, 9)
+    ref = orc.spmv(g, w.astype(np.float64), x0.astype(np.float64), 8)
+    d = pa.Graph.rmat(20, 1 << 22, seed=12, weights="f32", wseed=4)
+    for maxdeg, cold in ((1 << 30, -1), (1 << 30, 300000), (64, 300000)):
+        d.set_tiles(1, 65536, cold, max_degree=maxdeg)
+        y, _ = d.spmv(x0, 8)
+        ts = d.tile_stats()
+        err = rel_l1(y, ref)
+        say("max degree %d cold %d: %s relL1 %.3e" % (maxdeg, cold, ts, err))
+        assert err < TOL
+        if maxdeg == 1 << 30:
+            assert ts["tiled_edges"] == g.m
+            if cold < 0:
+                assert ts["escape_groups"] > 0 and ts["cold_edges"] == 0
+            else:
+                assert ts["cold_edges"] > 0
+
+
 # ---- BP ---------------------------------------------------------------------------------
 def bp_compare(got, ref):
     """The reference's BP multiplies unnormalised messages (numa-BP.C:112-125):
