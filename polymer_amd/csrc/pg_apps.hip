@@ -397,6 +397,7 @@ struct BpOp {
     const Vec2<T> *cur;
     Vec2<T> *nxt;
     const int32_t *out_off;
+    const uint32_t *Fb;  // F as a bitmap: the per-edge source test reads n/8 bytes, not n
     __device__ bool active(int v) const { return F[v] != 0; }
     __device__ Ctx ctx(int v) const {
         const float2 phi = vpot[v];
@@ -408,7 +409,7 @@ struct BpOp {
         return Acc{a.p0 * b.p0, a.p1 * b.p1, a.any | b.any, 0};
     }
     __device__ Acc edge(int s, int e, const Ctx &q) const {
-        if (!F[s]) return identity();
+        if (!bit_test(Fb, s)) return identity();
         const float4 w = pot[e];
         return Acc{(T)w.x * q.x + (T)w.y * q.y, (T)w.z * q.x + (T)w.w * q.y, 1, 0};
     }
@@ -424,6 +425,18 @@ struct BpOp {
         atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
     }
 };
+
+// bits[w] = the 32 flags f[32w .. 32w+31] (zero past n)
+__global__ void k_flags_to_bits(const uint8_t *f, int64_t n, uint32_t *bits) {
+    const int64_t nw = (n + 31) / 32;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t x = 0;
+        const int64_t v0 = 32 * w;
+        for (int j = 0; j < 32 && v0 + j < n; j++) x |= (uint32_t)(f[v0 + j] != 0) << j;
+        bits[w] = x;
+    }
+}
 
 __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1404,9 +1417,12 @@ static int bp_run(pg_graph *g, int iters, const float4 *pot, const float2 *vp, c
     PG_TRY(workspace_t(g, "bp_stats", ((size_t)iters + 1) * kStatSlots, &stats));
     PG_LAUNCH_K(k_zero_u64, (unsigned)(((iters + 1) * kStatSlots + kBlock - 1) / kBlock), stats,
                 (iters + 1) * kStatSlots);
+    uint32_t *Fb = nullptr;
+    PG_TRY(workspace_t(g, "bp_fbits", (size_t)(n + 31) / 32 + 1, &Fb));
     for (int t = 0; t < iters; t++) {
-        BpOp<T> op{F, Fn, pot, vp, cur, nxt, g->out_off};
+        BpOp<T> op{F, Fn, pot, vp, cur, nxt, g->out_off, Fb};
         PG_TRY(kt.begin());
+        PG_LAUNCH_K(k_flags_to_bits, grid_for((n + 31) / 32), F, n, Fb);
         PG_TRY(launch_pull(op, g->in_sched, &stats[(size_t)t * kStatSlots]));
         PG_TRY(kt.end());
         std::swap(cur, nxt);  // swap(vertD_curr, vertD_next), numa-BP.C:303-304
