@@ -127,6 +127,37 @@ def traversal_extra(pa, scale=22, m=1 << 26):
             "sssp_scale22_gteps_relaxed": round(tot_edges / (tot_ms / 1e3) / 1e9, 2)}
 
 
+def multisource_extra(pa, scale=22, m=1 << 26, k=64):
+    """64 BFS and 64 Bellman-Ford sources on the symmetrised scale-22 graph in one
+    batched solve each (a 64-bit source mask per vertex), against the same 64
+    sources run one by one."""
+    gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True, weights="i32", wseed=7)
+    rng = np.random.default_rng(13)
+    src = rng.choice(gs.n, k, replace=False)
+    src[0] = 0
+    out = {}
+    gs.msbfs(src, out=False)
+    _, st = gs.msbfs(src, out=False)
+    one = 0.0
+    for s in src:
+        _, _, s1 = gs.bfs(int(s), out=False)
+        one += s1.ms
+    out["msbfs64_scale22_ms"] = round(st.ms, 3)
+    out["bfs64_one_by_one_ms"] = round(one, 3)
+    out["msbfs64_speedup"] = round(one / st.ms, 2)
+    gs.mssssp(src, out=False)
+    _, st = gs.mssssp(src, out=False)
+    one = 0.0
+    for s in src:
+        _, s1 = gs.sssp(int(s), out=False)
+        one += s1.ms
+    out["mssssp64_scale22_ms"] = round(st.ms, 3)
+    out["sssp64_one_by_one_ms"] = round(one, 3)
+    out["mssssp64_speedup"] = round(one / st.ms, 2)
+    gs.close()
+    return out
+
+
 def value_extra(pa, g, scale=24, m=1 << 28):
     """SpMV (20 iterations, fp32 values U(0,1], x0 U[0,1)) and BP (10 iterations,
     potentials U[0.1,1]) on the scale-24 graph, with their algorithmic bandwidth
@@ -370,6 +401,7 @@ def main():
     extra = {"pagerank_step_with_host_copy_ms": round(host_ms, 3)}
     if not args.no_extra:
         for name, fn in (("bfs", bfs_extra), ("bfs24", bfs24_extra), ("traversal", traversal_extra),
+                         ("multisource", multisource_extra),
                          ("value", lambda p: value_extra(p, g)), ("batched", batched_extra),
                          ("adjoint", adjoint_extra), ("multigraph", multigraph_extra)):
             t1 = time.perf_counter()

@@ -10,11 +10,20 @@
 // The output is the result array as raw little-endian binary.
 #pragma once
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
+
+// wall-clock milliseconds: a driver times its iteration loop between two device
+// synchronisations and prints "time_ms <t>"
+inline double wall_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
 
 struct AppArgs {
     int scale = -1;

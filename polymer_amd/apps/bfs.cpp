@@ -63,6 +63,8 @@ int main(int argc, char **argv) {
 
     intT round = 0, dense_rounds = 0;
     long long reached = 1;
+    polymerSync();
+    const double t_start = wall_ms();
     while (!Frontier->isEmpty()) {
         round++;
         edgeMap(G, Frontier, BFS_F{Parents}, next, (intT)threshold, DENSE);
@@ -75,11 +77,13 @@ int main(int argc, char **argv) {
         reached += Frontier->numNonzeros();
     }
     polymerSync();
+    const double t_loop = wall_ms() - t_start;
     std::vector<intT> out((size_t)2 * n);
     toHost(out.data(), Parents, (size_t)n);
     toHost(out.data() + n, Levels, (size_t)n);
     write_output(a.out, out);
     std::printf("bfs: n %d m %ld source %d reached %lld rounds %d dense %d\n", n, G.m, src, reached,
                 round, dense_rounds);
+    std::printf("time_ms %.3f\n", t_loop);
     return 0;
 }

@@ -68,6 +68,8 @@ int main(int argc, char **argv) {
     fillLocalFrontier(current);
     vertexMap(Frontier, Fill_F{p_curr, 1.0 / (double)n}, 0);
     long long edges = 0;
+    polymerSync();
+    const double t_start = wall_ms();
     for (int t = 0; t < a.rounds; t++) {
         vertexCounter(G, Frontier->getFrontier(0), 0, 0, 1);
         edges += Frontier->getEdgeStat();
@@ -79,9 +81,11 @@ int main(int argc, char **argv) {
         switchFrontier(0, Frontier, next);
     }
     polymerSync();
+    const double t_loop = wall_ms() - t_start;
     std::vector<double> r((size_t)n);
     toHost(r.data(), p_curr, (size_t)n);
     write_output(a.out, r);
     std::printf("pagerank: n %d m %ld rounds %d edges %lld\n", n, G.m, a.rounds, edges);
+    std::printf("time_ms %.3f\n", t_loop);
     return 0;
 }

@@ -987,13 +987,8 @@ static int read_counts(Frontier &f, int64_t *count, int64_t *edges) {
 
 // |F_next| and the sum of its out-degrees, read back for the direction switch
 static int frontier_stats(pg_graph *g, Frontier &f, int64_t *count, int64_t *edges) {
-    PG_TRY(list_degree_sum(g, f.list, &f.ctr[0], g->out_off, &f.ctr[1]));
-    unsigned long long h[2];
-    PG_CHECK_HIP(hipMemcpyAsync(h, f.ctr, 16, hipMemcpyDeviceToHost, stream()));
-    PG_CHECK_HIP(hipStreamSynchronize(stream()));
-    *count = (int64_t)h[0];
-    *edges = (int64_t)h[1];
-    return PG_OK;
+    PG_TRY(list_degree_sum(g, f.list, &f.ctr[0], g->out_off, f.ctr + 5, 2));
+    return read_counts(f, count, edges);
 }
 
 __global__ void k_seed_source(int32_t src, int32_t *level, int32_t *parent, int32_t *dist,

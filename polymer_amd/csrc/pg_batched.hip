@@ -4,8 +4,9 @@
 //    pulled with the degree-binned engine (OR of in-neighbour masks, early exit
 //    once every source has reached the vertex);
 //  * multi-source Bellman-Ford and personalized PageRank: k-wide value rows per
-//    vertex ([n][64]), one wavefront per (vertex, edge chunk) with lane c
-//    carrying column c, so every gathered row is one coalesced 256-byte read;
+//    vertex ([n][64]), a 16-lane group per light vertex and a wavefront per heavier
+//    vertex or edge chunk, four columns per lane, so every gathered row is one
+//    coalesced 256-byte read;
 //  * many graphs: one device graph, per-vertex (1-d)/n_g, the iterations of
 //    numa-PageRank.C captured in a hipGraph and replayed as one launch.
 #include <algorithm>
@@ -46,9 +47,9 @@ static int map_sources(pg_graph *g, const int64_t *src, int k, std::vector<int32
 }
 
 // ======================================================================================
-// Row engine: item w (one wavefront) = a vertex, or one chunk of a heavy vertex.
-// Heavy vertices are the in-degree prefix [0, H) of the relabelled order; their chunks
-// write partial rows that the last chunk to finish combines in chunk order.
+// Row plans: the heavy vertices are the in-degree prefix [0, H) of the relabelled
+// order, cut into items of `chunk` in-edges (one wavefront each); the rest are
+// taken by vertex.
 // ======================================================================================
 struct RowPlan {
     int32_t H = 0;          // heavy vertices
@@ -108,94 +109,8 @@ static int row_plan(pg_graph *g, RowPlan *P, const std::string &name = "rowplan"
     return PG_OK;
 }
 
-// RowOp: using T; T identity(); T combine(T,T);
-//        bool src_live(int s) (wave-uniform); T edge(int s, int e, int lane);
-//        Stat finish(int v, T acc, int lane) (all 64 lanes, convergent).
-template <class RowOp>
-__global__ __launch_bounds__(kBlock) void k_rows(RowOp op, const int32_t *__restrict__ off,
-                                                 const int32_t *__restrict__ adj, RowPlan P,
-                                                 unsigned long long *stat) {
-    using T = typename RowOp::T;
-    const int64_t w = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kRow;
-    const int lane = threadIdx.x & (kRow - 1);
-    if (w >= P.nitems) return;  // uniform per wavefront
-    int v, beg, end;
-    MultiItem it{};
-    const bool heavy = w < P.nheavy;
-    if (heavy) {
-        it = P.items[w];
-        v = it.pos;
-        beg = off[v] + it.chunk * P.chunk;
-        end = min(off[v + 1], beg + P.chunk);
-    } else {
-        v = P.H + (int)(w - P.nheavy);
-        beg = off[v];
-        end = off[v + 1];
-    }
-    if (!op.vertex_live(v, heavy || end > beg)) {
-        if (!heavy || it.chunk == 0) op.skip(v, lane);
-        return;
-    }
-    T acc = op.identity();
-    int e = beg;
-    for (; e + 7 < end; e += 8) {  // eight row gathers in flight per wavefront
-        int s[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) s[k] = ld_stream(adj + e + k);
-        T a[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) a[k] = op.edge(s[k], e + k, lane);
-#pragma unroll
-        for (int k = 0; k < 4; k++) a[k] = op.combine(a[k], a[k + 4]);
-        acc = op.combine(acc, op.combine(op.combine(a[0], a[2]), op.combine(a[1], a[3])));
-    }
-    for (; e < end; e++) acc = op.combine(acc, op.edge(ld_stream(adj + e), e, lane));
-    unsigned long long st = 0;
-    if (heavy && it.nchunks > 1) {
-        T *part = reinterpret_cast<T *>(P.partial);
-        store_agent(&part[(int64_t)w * kRow + lane], acc);
-        __threadfence();
-        unsigned prev = 0;
-        if (lane == 0) prev = atomicAdd(&P.tickets[it.slot], 1u);
-        prev = __shfl(prev, 0, 64);
-        if (prev != (unsigned)(it.nchunks - 1)) return;
-        __threadfence();
-        // chunk order; eight partials loaded before they are combined
-        T tot = op.identity();
-        int c = 0;
-        for (; c + 7 < it.nchunks; c += 8) {
-            T pv[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                pv[k] = load_agent(&part[(int64_t)(it.first + c + k) * kRow + lane]);
-#pragma unroll
-            for (int k = 0; k < 8; k++) tot = op.combine(tot, pv[k]);
-        }
-        for (; c < it.nchunks; c++)
-            tot = op.combine(tot, load_agent(&part[(int64_t)(it.first + c) * kRow + lane]));
-        if (lane == 0) P.tickets[it.slot] = 0u;
-        st = op.finish(v, tot, lane);
-    } else {
-        st = op.finish(v, acc, lane);
-    }
-    if (RowOp::kStats && lane == 0 && st) atomicAdd(&stat[blockIdx.x & (kStatSlots - 1)], st);
-}
-
-template <class RowOp>
-static int launch_rows(const RowOp &op, pg_graph *g, const RowPlan &P, unsigned long long *stat) {
-    const int64_t threads = P.nitems * kRow;
-    const int64_t nb = (threads + kBlock - 1) / kBlock;
-    if (nb == 0) return PG_OK;
-    hipLaunchKernelGGL(k_rows<RowOp>, dim3((unsigned)nb), dim3(kBlock), 0, stream(), op, g->in_off,
-                       g->in_adj, P, stat);
-    PG_CHECK_HIP(hipGetLastError());
-    return PG_OK;
-}
-
 // ---- personalized PageRank, k-wide rows ---------------------------------------------
 struct PprRowOp {
-    using T = float;
-    static constexpr bool kStats = false;
     const float *C;  // contributions r/outdeg, [n][64]
     float *Cn;
     float *R;        // ranks, [n][64]
@@ -208,18 +123,6 @@ struct PprRowOp {
     // once both contribution buffers hold that value (iterations 1 and 2) it is
     // skipped until the last iteration stores its rank.
     __device__ bool vertex_live(int, bool has_in) const { return has_in || !steady || write_r; }
-    __device__ void skip(int, int) const {}
-    __device__ float identity() const { return 0.f; }
-    __device__ float combine(float a, float b) const { return a + b; }
-    __device__ float edge(int s, int, int lane) const { return C[(int64_t)s * kRow + lane]; }
-    __device__ unsigned long long finish(int v, float sum, int lane) const {
-        const float r = damp * sum + (src[lane] == v ? tele : 0.f);
-        const int od = out_off[v + 1] - out_off[v];
-        if (write_r) R[(int64_t)v * kRow + lane] = r;
-        // a row without out-edges is never gathered
-        if (od > 0) Cn[(int64_t)v * kRow + lane] = r / (float)od;
-        return 0ull;
-    }
 };
 
 // in-degree classes of the batched PPR iteration (measured on the scale-22 graph:
@@ -542,31 +445,161 @@ __global__ void k_level_to_input(const T *lev, const int32_t *perm, int64_t n, i
 }
 
 // ---- multi-source Bellman-Ford: [n][64] int32 distances ------------------------------------
-struct MsSsspOp {
-    using T = int;
-    static constexpr bool kStats = true;
-    const unsigned long long *F;  // sources improved last round
-    unsigned long long *Fn;
-    int32_t *dist;  // [n][64]
-    const int32_t *w;  // CSC order
-    __device__ bool vertex_live(int, bool) const { return true; }
-    __device__ void skip(int, int) const {}
-    __device__ int identity() const { return 0x7FFFFFFF; }
-    __device__ int combine(int a, int b) const { return min(a, b); }
-    __device__ int edge(int s, int e, int lane) const {
-        const unsigned long long m = F[s];  // the same for every lane
-        if (!((m >> lane) & 1ull)) return 0x7FFFFFFF;
-        return dist[(int64_t)s * kRow + lane] + w[e];
+// ---- multi-source Bellman-Ford, grouped -------------------------------------------------
+// A round relaxes, for every vertex v and source column c, dist[v][c] with
+// dist[s][c] + w over the in-edges (s, v) whose source improved column c in the
+// previous round (F[s] bit c).  Distances only decrease and every relaxation uses
+// a value that was a true path length when read, so the fixed point is the exact
+// shortest distances whatever order the updates land in (a source read before or
+// after its own update this round is relaxed again next round if it improved).
+// Layout as the batched PPR rows: a 16-lane group per vertex, four columns (an
+// int4) per lane.  A bitmap of the vertices with any active column (n/8 bytes,
+// L2-resident) is tested per edge before the 64-bit mask and the row are read, so
+// a round with few active vertices streams little beyond the in-lists; a vertex
+// with no candidate reads nothing of its own row.  The improvements of the round
+// are counted per column into spread counters (the stop test).
+constexpr int32_t kMsLight = 256, kMsChunk = 2048;
+
+__device__ __forceinline__ void ms_relax4(int4 &best, unsigned nib, int4 ds, int we) {
+    if (nib & 1u) best.x = min(best.x, ds.x + we);
+    if (nib & 2u) best.y = min(best.y, ds.y + we);
+    if (nib & 4u) best.z = min(best.z, ds.z + we);
+    if (nib & 8u) best.w = min(best.w, ds.w + we);
+}
+
+// OR of the 16 lanes' (nibble << 4 l) over a 16-lane group
+__device__ __forceinline__ unsigned long long group16_mask(unsigned nib, int l) {
+    unsigned long long m = (unsigned long long)nib << (4 * l);
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) m |= __shfl_xor(m, o, 64);
+    return m;
+}
+
+__global__ void k_mask_any_bits(const unsigned long long *F, int64_t n, uint32_t *bits) {
+    const int64_t nw = (n + 31) / 32;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t x = 0;
+        const int64_t v0 = 32 * w;
+        for (int j = 0; j < 32 && v0 + j < n; j++) x |= (uint32_t)(F[v0 + j] != 0ull) << j;
+        bits[w] = x;
     }
-    __device__ unsigned long long finish(int v, int best, int lane) const {
-        int32_t *dv = &dist[(int64_t)v * kRow + lane];
-        const bool imp = best < *dv;
-        if (imp) *dv = best;
-        const unsigned long long m = __ballot(imp);
-        if (lane == 0) Fn[v] = m;
-        return (unsigned long long)__popcll(m);
+}
+
+// the light vertices [H, n): plain read-compare-write of the vertex's own row,
+// Fn[v] written for every vertex
+__global__ __launch_bounds__(kBlock) void k_mssssp_light(
+    const int32_t *__restrict__ in_off, const int32_t *__restrict__ in_adj,
+    const int32_t *__restrict__ w, const uint32_t *__restrict__ Fany,
+    const unsigned long long *__restrict__ F, unsigned long long *Fn, int32_t *dist, int32_t H,
+    int64_t n, unsigned long long *slots) {
+    __shared__ unsigned long long sc[kBlock / 64];
+    const int l = threadIdx.x & 15;
+    const int64_t ngroups = (int64_t)gridDim.x * (kBlock / 16);
+    int4 *D4 = reinterpret_cast<int4 *>(dist);
+    unsigned long long cnt = 0;
+    for (int64_t v = H + (int64_t)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4); v < n;
+         v += ngroups) {
+        const int beg = in_off[v], end = in_off[v + 1];
+        const int inf = 0x7FFFFFFF;
+        int4 best = make_int4(inf, inf, inf, inf);
+        for (int e = beg; e < end; e += 4) {  // four edges' loads in flight
+            int s[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) s[k] = e + k < end ? ld_stream(in_adj + e + k) : -1;
+            bool a[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) a[k] = s[k] >= 0 && bit_test(Fany, s[k]);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (!a[k]) continue;  // uniform over the group
+                const unsigned nib = (unsigned)(F[s[k]] >> (4 * l)) & 0xFu;
+                if (nib) ms_relax4(best, nib, D4[(int64_t)s[k] * (kRow / 4) + l], w[e + k]);
+            }
+        }
+        unsigned imp = 0;
+        if (best.x != inf || best.y != inf || best.z != inf || best.w != inf) {
+            int4 cur = D4[v * (kRow / 4) + l];
+            imp = (unsigned)(best.x < cur.x) | (unsigned)(best.y < cur.y) << 1 |
+                  (unsigned)(best.z < cur.z) << 2 | (unsigned)(best.w < cur.w) << 3;
+            if (imp) {
+                cur.x = min(cur.x, best.x);
+                cur.y = min(cur.y, best.y);
+                cur.z = min(cur.z, best.z);
+                cur.w = min(cur.w, best.w);
+                D4[v * (kRow / 4) + l] = cur;
+            }
+        }
+        cnt += __popc(imp);
+        const unsigned long long m = group16_mask(imp, l);
+        if (l == 0) Fn[v] = m;
     }
-};
+    for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (lane_id() == 0) sc[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = 0;
+        for (int i = 0; i < kBlock / 64; i++) c += sc[i];
+        if (c) atomicAdd(&slots[blockIdx.x & (kStatSlots - 1)], c);
+    }
+}
+
+// the heavy vertices [0, H) in chunks of P.chunk in-edges, a wavefront per chunk:
+// the four groups interleave the chunk's edges and meet by shuffles; a chunk
+// lowers the vertex's row with atomicMin and ORs the columns it improved into
+// Fn[v] (cleared for [0, H) before the launch)
+__global__ __launch_bounds__(kBlock) void k_mssssp_heavy(
+    const int32_t *__restrict__ in_off, const int32_t *__restrict__ in_adj,
+    const int32_t *__restrict__ w, const uint32_t *__restrict__ Fany,
+    const unsigned long long *__restrict__ F, unsigned long long *Fn, int32_t *dist, RowPlan P,
+    unsigned long long *slots) {
+    const int64_t wv = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64;
+    if (wv >= P.nheavy) return;  // uniform per wavefront
+    const int lane = threadIdx.x & 63, q = lane >> 4, l = lane & 15;
+    const MultiItem it = P.items[wv];
+    const int v = it.pos;
+    const int beg = in_off[v] + it.chunk * P.chunk;
+    const int end = min(in_off[v + 1], beg + P.chunk);
+    const int4 *D4 = reinterpret_cast<const int4 *>(dist);
+    const int inf = 0x7FFFFFFF;
+    int4 best = make_int4(inf, inf, inf, inf);
+    for (int e0 = beg + q; e0 < end; e0 += 16) {  // four edges' loads in flight
+        int s[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = e0 + 4 * k < end ? ld_stream(in_adj + e0 + 4 * k) : -1;
+        bool a[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] = s[k] >= 0 && bit_test(Fany, s[k]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!a[k]) continue;
+            const unsigned nib = (unsigned)(F[s[k]] >> (4 * l)) & 0xFu;
+            if (nib) ms_relax4(best, nib, D4[(int64_t)s[k] * (kRow / 4) + l], w[e0 + 4 * k]);
+        }
+    }
+#pragma unroll
+    for (int o = 16; o <= 32; o <<= 1) {
+        best.x = min(best.x, __shfl_xor(best.x, o, 64));
+        best.y = min(best.y, __shfl_xor(best.y, o, 64));
+        best.z = min(best.z, __shfl_xor(best.z, o, 64));
+        best.w = min(best.w, __shfl_xor(best.w, o, 64));
+    }
+    unsigned imp = 0;
+    if (q == 0) {
+        int32_t *dv = dist + (int64_t)v * kRow + 4 * l;
+        if (best.x != inf && best.x < atomicMin(dv + 0, best.x)) imp |= 1u;
+        if (best.y != inf && best.y < atomicMin(dv + 1, best.y)) imp |= 2u;
+        if (best.z != inf && best.z < atomicMin(dv + 2, best.z)) imp |= 4u;
+        if (best.w != inf && best.w < atomicMin(dv + 3, best.w)) imp |= 8u;
+    }
+    const unsigned long long m = group16_mask(imp, l);  // group 0 holds the chunk's mask
+    if (lane == 0 && m) {
+        // count the columns this chunk activated first (several chunks of a vertex
+        // may improve one column)
+        const unsigned long long fresh = m & ~atomicOr(&Fn[v], m);
+        if (fresh) atomicAdd(&slots[blockIdx.x & (kStatSlots - 1)], (unsigned long long)__popcll(fresh));
+    }
+}
 
 __global__ void k_mssssp_seed(const int32_t *src, int k, unsigned long long *F, int32_t *dist) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -786,10 +819,12 @@ extern "C" int pg_mssssp(pg_graph *g, const int64_t *src, int k, int32_t *dist, 
     std::vector<int32_t> s;
     PG_TRY(map_sources(g, src, k, &s));
     const int64_t n = g->n;
-    RowPlan P;
-    PG_TRY(row_plan(g, &P));
+    RowPlan P;  // heavy: in-degree > kMsLight, chunks of kMsChunk
+    PG_TRY(row_plan(g, &P, "mssplan", kMsLight, kMsChunk));
     unsigned long long *F0 = nullptr, *F1 = nullptr, *cnt = nullptr;
     int32_t *D = nullptr, *dsrc = nullptr;
+    uint32_t *Fany = nullptr;
+    PG_TRY(workspace_t(g, "mssssp_fany", (size_t)(n + 31) / 32 + 1, &Fany));
     PG_TRY(workspace_t(g, "mssssp_f0", (size_t)n, &F0));
     PG_TRY(workspace_t(g, "mssssp_f1", (size_t)n, &F1));
     PG_TRY(workspace_t(g, "mssssp_cnt", (size_t)kStatSlots, &cnt));
@@ -809,9 +844,16 @@ extern "C" int pg_mssssp(pg_graph *g, const int64_t *src, int k, int32_t *dist, 
     while (true) {
         rounds++;
         PG_TRY(fill(cnt, 0ull, kStatSlots));
-        MsSsspOp op{F, Fn, D, g->in_w_i32};
         PG_TRY(kt.begin());
-        PG_TRY(launch_rows(op, g, P, cnt));
+        PG_LAUNCH_B(k_mask_any_bits, grid_for((n + 31) / 32), F, n, Fany);
+        if (P.nheavy > 0) {
+            PG_CHECK_HIP(hipMemsetAsync(Fn, 0, 8 * (size_t)P.H, stream()));
+            PG_LAUNCH_B(k_mssssp_heavy, ((int64_t)P.nheavy * 64 + kBlock - 1) / kBlock, g->in_off,
+                        g->in_adj, g->in_w_i32, Fany, F, Fn, D, P, cnt);
+        }
+        if (n > P.H)
+            PG_LAUNCH_B(k_mssssp_light, grid_for((n - P.H) * 16, 4096), g->in_off, g->in_adj,
+                        g->in_w_i32, Fany, F, Fn, D, P.H, n, cnt);
         PG_TRY(kt.end());
         unsigned long long c = 0;
         PG_TRY(sum_slots(cnt, &c));

@@ -505,8 +505,10 @@ __global__ void k_list_to_bits(const int32_t *list, const unsigned long long *co
     }
 }
 
+// out[stride * (block mod 64)] += the block's sum: a few thousand blocks adding to
+// one word serialise (tens of microseconds on MI355X); spread == false adds to out[0]
 __global__ void k_list_degree_sum(const int32_t *list, const unsigned long long *count,
-                                  const int32_t *off, unsigned long long *out) {
+                                  const int32_t *off, unsigned long long *out, int stride) {
     __shared__ unsigned long long sm[kBlock / 64];
     const int64_t c = (int64_t)*count;
     unsigned long long acc = 0;
@@ -521,7 +523,7 @@ __global__ void k_list_degree_sum(const int32_t *list, const unsigned long long 
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
         for (int i = 0; i < kBlock / 64; i++) t += sm[i];
-        if (t) atomicAdd(out, t);
+        if (t) atomicAdd(&out[(int64_t)stride * (blockIdx.x & 63)], t);
     }
 }
 
@@ -579,8 +581,8 @@ int list_to_bits(pg_graph *g, const int32_t *list, const unsigned long long *d_c
 }
 
 int list_degree_sum(pg_graph *g, const int32_t *list, const unsigned long long *d_count,
-                    const int32_t *off, unsigned long long *d_out) {
-    PG_LAUNCH(k_list_degree_sum, grid_for(g->n, 4096), list, d_count, off, d_out);
+                    const int32_t *off, unsigned long long *d_out, int stride) {
+    PG_LAUNCH(k_list_degree_sum, grid_for(g->n, 4096), list, d_count, off, d_out, stride);
     return PG_OK;
 }
 

@@ -310,8 +310,10 @@ struct KernelTimer {
 int bits_to_list(pg_graph *g, const uint32_t *bits, int32_t *list, unsigned long long *d_count);
 int list_to_bits(pg_graph *g, const int32_t *list, const unsigned long long *d_count,
                  uint32_t *bits);
+// adds the out-degrees of list[0..*d_count) to d_out[stride * k], k < 64 (stride 0:
+// all to d_out[0])
 int list_degree_sum(pg_graph *g, const int32_t *list, const unsigned long long *d_count,
-                    const int32_t *off, unsigned long long *d_out);
+                    const int32_t *off, unsigned long long *d_out, int stride = 0);
 int fill_bits_all(pg_graph *g, uint32_t *bits);
 
 // edge arrays between the caller's CSR order and the internal CSR/CSC order
