@@ -158,6 +158,25 @@ def multisource_extra(pa, scale=22, m=1 << 26, k=64):
     return out
 
 
+def dropin_extra(pa):
+    """The drop-in path: the PageRank and BFS drivers written against
+    include/polymer.h (the reference's functors and iteration structure: fp64
+    values, atomic pushes), on the same scale-24 graphs; their loop times."""
+    import subprocess
+    out = {}
+    bindir = os.path.join(ROOT, "polymer_amd", "apps", "bin")
+    runs = (("dropin_pagerank_scale24_ms", ["pagerank", "-rmat", "24", str(1 << 28), "1",
+                                            "-rounds", "20"]),
+            ("dropin_bfs_scale24_ms", ["bfs", "-rmat", "24", str(1 << 28), "1", "-s", "-r", "0"]))
+    for key, cmd in runs:
+        r = subprocess.run([os.path.join(bindir, cmd[0])] + cmd[1:], capture_output=True,
+                           text=True, timeout=300, check=True)
+        for line in r.stdout.splitlines():
+            if line.startswith("time_ms "):
+                out[key] = round(float(line.split()[1]), 3)
+    return out
+
+
 def value_extra(pa, g, scale=24, m=1 << 28):
     """SpMV (20 iterations, fp32 values U(0,1], x0 U[0,1)) and BP (10 iterations,
     potentials U[0.1,1]) on the scale-24 graph, with their algorithmic bandwidth
@@ -403,7 +422,8 @@ def main():
         for name, fn in (("bfs", bfs_extra), ("bfs24", bfs24_extra), ("traversal", traversal_extra),
                          ("multisource", multisource_extra),
                          ("value", lambda p: value_extra(p, g)), ("batched", batched_extra),
-                         ("adjoint", adjoint_extra), ("multigraph", multigraph_extra)):
+                         ("adjoint", adjoint_extra), ("multigraph", multigraph_extra),
+                         ("dropin", dropin_extra)):
             t1 = time.perf_counter()
             try:
                 extra.update(fn(pa))

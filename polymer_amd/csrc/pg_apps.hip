@@ -471,6 +471,9 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
 // loads for all of them before waiting.
 constexpr int kBuChunks = 4;
 constexpr int kBuLaneScan = 64;
+// a round's direction, chosen on the device (k_bfs_round_begin): every kernel of a
+// round is enqueued and those of the other direction return at once
+constexpr unsigned long long kBfsModeDone = 0, kBfsModeBU = 1, kBfsModeTD = 2;
 __global__ void k_first_in(const int32_t *in_off, const int32_t *in_adj, int64_t n,
                            int32_t *first_in) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
@@ -478,17 +481,47 @@ __global__ void k_first_in(const int32_t *in_off, const int32_t *in_adj, int64_t
         first_in[v] = in_off[v] < in_off[v + 1] ? in_adj[in_off[v]] : -1;
 }
 
+// the statistics of a round's claims (count, out-degree sum), summed over the
+// block and added to slot pair blockIdx mod 64 of the round's region; every thread
+// of the block calls it
+__device__ __forceinline__ void bfs_flush_stats(unsigned long long cnt, unsigned long long deg,
+                                                unsigned long long *slots) {
+    __shared__ unsigned long long sc[kBlock / 64], sd[kBlock / 64];
+    for (int o = 32; o >= 1; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, 64);
+        deg += __shfl_xor(deg, o, 64);
+    }
+    if (lane_id() == 0) {
+        sc[threadIdx.x >> 6] = cnt;
+        sd[threadIdx.x >> 6] = deg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = 0, d = 0;
+        for (int i = 0; i < kBlock / 64; i++) {
+            c += sc[i];
+            d += sd[i];
+        }
+        if (c) {
+            unsigned long long *p = slots + 2 * (blockIdx.x & 63);
+            atomicAdd(&p[0], c);
+            atomicAdd(&p[1], d);
+        }
+    }
+}
+
+// (the body of a bottom-up round, wave-uniform; the winners' count and out-degree
+// sum accumulate in cnt and deg)
 template <int J>
-__global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(
+__device__ __forceinline__ void bfs_bottom_up(
     const int32_t *__restrict__ in_off, const int32_t *__restrict__ in_adj,
     const int32_t *__restrict__ first_in, const int32_t *__restrict__ out_off,
     const uint32_t *__restrict__ F, uint32_t *__restrict__ V, int32_t *level, int32_t *parent,
-    int32_t round, int64_t nbody, uint32_t *__restrict__ Fn, unsigned long long *slots) {
-    __shared__ unsigned long long sc[kBlock / 64], sd[kBlock / 64];
+    int32_t round, int64_t nbody, uint32_t *__restrict__ Fn, unsigned long long &cnt,
+    unsigned long long &deg) {
     const int lane = lane_id();
     const int64_t nchunks = (nbody + 64 * J - 1) / (64 * J);
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
-    unsigned long long cnt = 0, deg = 0;
     for (int64_t ch = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); ch < nchunks;
          ch += nwaves) {
         // visited word of this lane's vertex (all ones past the body)
@@ -586,7 +619,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(
                 deg += (unsigned long long)od[j];
             }
             // the chunk's two words belong to this wave: the next frontier's are
-            // written whole (no clearing needed), the visited ones gain the winners
+            // written whole, the visited ones gain the winners
             const unsigned long long wm = __ballot(won);
             if (c64 * 64 < nbody && (lane & 31) == 0) {
                 const uint32_t w = lane == 0 ? (uint32_t)wm : (uint32_t)(wm >> 32);
@@ -595,131 +628,190 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(
             }
         }
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-        cnt += __shfl_xor(cnt, o, 64);
-        deg += __shfl_xor(deg, o, 64);
-    }
-    if (lane == 0) {
-        sc[threadIdx.x >> 6] = cnt;
-        sd[threadIdx.x >> 6] = deg;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long c = 0, d = 0;
-        for (int i = 0; i < kBlock / 64; i++) {
-            c += sc[i];
-            d += sd[i];
-        }
-        if (c) {
-            unsigned long long *p = slots + 2 * (blockIdx.x & 63);
-            atomicAdd(&p[0], c);
-            atomicAdd(&p[1], d);
-        }
-    }
 }
 
-// append: push the winners to the next list and bitmap (small rounds); otherwise
-// both are taken from the levels after the round
-// The claim is a CAS on the level (the reference CASes the parent from -1; which
-// frontier vertex wins is unspecified either way), so parents need no
-// initialisation: the parent of an unreached vertex is set to -1 on output.  The
-// visited bitmap V (n/8 bytes, L2-resident) screens the destinations first, so a
-// destination already reached costs a cached bit test, not a gather from the
-// 4-byte levels.  A winner sets its bits in V and in the next frontier bits_n.
-// This is the edge-balanced push of a large frontier list (its statistics are
-// summed from bits_n afterwards); small frontiers are pushed from the bitmap by
-// k_bfs_push_bits.  V holds exactly the vertices reached before this round, so a
-// clear bit means the destination's level is this round whoever writes it: the
-// claim is plain stores and returnless ORs, no CAS whose result each lane would
-// wait for.  Concurrent claimers of one destination all store the same level and
-// one of their (equally valid) parents survives.
-struct BfsPushOp {
-    uint32_t *bits_n, *V;
-    int32_t *level, *parent;
-    int32_t round;
-    __device__ bool src_active(int) const { return true; }
-    __device__ void push(int s, int d, int) const {
-        if (bit_test(V, d)) return;
-        level[d] = round;
-        parent[d] = s;
-        const uint32_t b = 1u << (d & 31);
-        atomicOr(&V[d >> 5], b);
-        atomicOr(&bits_n[d >> 5], b);
-    }
-};
-
-// BFS counters: one region per round, cleared once per kBfsRegions rounds rather
-// than every round.  Words 0..127 of a region are 64 spread (count, out-degree)
-// pairs of the next frontier, word 129 the length of a list compacted from the
-// current frontier's bitmap, words 130 and 131 the lengths of a small push's
-// queues.
+// BFS counters and control: one region of kBfsRegion words per round (round r uses
+// region r mod kBfsRegions; region 0 holds the source's statistics).  Words 0..127
+// are 64 spread (count, out-degree) pairs of the frontier the round produces, 130
+// and 131 the lengths of a top-down round's queues, kBfsCtlMode.. the round's
+// direction and the running state after its decision (the direction policy's
+// flag, totals, round counts, done).
 constexpr int kBfsRegion = 256;
 constexpr int kBfsRegions = 64;
+constexpr int kBfsCtlMode = 132, kBfsCtlBu = 133, kBfsCtlEdges = 134, kBfsCtlActive = 135,
+              kBfsCtlRounds = 136, kBfsCtlDense = 137, kBfsCtlDone = 138, kBfsCtlEnd = 139;
+// rounds per enqueued (and captured) batch: the first batch of a search runs as
+// many rounds as the graph's last search took (kBfsFirstBatch before any), the
+// later ones kBfsAhead each; at most kBfsMaxBatch
+constexpr int kBfsAhead = 4, kBfsFirstBatch = 8, kBfsMaxBatch = 32;
 // direction policy of the default threshold (the top-down/bottom-up switch of
 // direction-optimising BFS with the switch points as ratios)
-static int64_t kBfsAlpha = 14, kBfsBeta = 24;
+constexpr int64_t kBfsAlpha = 14, kBfsBeta = 24;
 
-static int read_region(const unsigned long long *rs, int64_t *count, int64_t *edges) {
-    unsigned long long h[128];
-    PG_CHECK_HIP(hipMemcpyAsync(h, rs, sizeof(h), hipMemcpyDeviceToHost, stream()));
-    PG_CHECK_HIP(hipStreamSynchronize(stream()));
-    unsigned long long c = 0, d = 0;
-    for (int k = 0; k < 64; k++) {
-        c += h[2 * k];
-        d += h[2 * k + 1];
+// A batch of rounds is enqueued (captured once, replayed) with no host decision
+// in between.  Each kernel finds its round as base[0] + k + 1 (base[0] = rounds of
+// the batches before, advanced by k_bfs_advance at the end of a batch; k = the
+// round's place in the batch) and its counters in the round's region.  Three
+// frontier bitmaps rotate: round q reads F(q), writes N(q) and clears C(q), the
+// bitmap round q+1 writes (it held the frontier of round q-1).
+struct BfsRound {
+    unsigned long long *regions;
+    const unsigned long long *base;
+    uint32_t *bits0, *bits1, *bits2;  // round 1 reads bits0 (the source)
+    int k;
+    __device__ int64_t r() const { return (int64_t)base[0] + k + 1; }
+    __device__ unsigned long long *region(int64_t q) const {
+        return regions + (size_t)(q % kBfsRegions) * kBfsRegion;
     }
-    *count = (int64_t)c;
-    *edges = (int64_t)d;
-    return PG_OK;
+    __device__ uint32_t *pick(int64_t i) const {
+        return i == 0 ? bits0 : (i == 1 ? bits1 : bits2);
+    }
+    __device__ uint32_t *F(int64_t q) const { return pick((q + 2) % 3); }
+    __device__ uint32_t *N(int64_t q) const { return pick(q % 3); }
+    __device__ uint32_t *C(int64_t q) const { return pick((q + 1) % 3); }
+};
+
+// A top-down claim: V (n/8 bytes, L2-resident) screens out the vertices reached
+// before; an OR on V's word that returns it picks the one claimer of a destination,
+// which stores level and parent, sets the destination's bit in the next frontier
+// (a returnless OR) and counts it with its out-degree for the round's statistics.
+// Which frontier vertex becomes the parent is unspecified, as in the reference,
+// whose claim is a CAS of the parent from -1.  Parents need no initialisation: the
+// parent of an unreached vertex is set to -1 on output.
+__device__ __forceinline__ void bfs_claim(int v, int d, uint32_t *V, int32_t *level,
+                                          int32_t *parent, uint32_t *Fn, int32_t round,
+                                          const int32_t *__restrict__ out_off,
+                                          unsigned long long &cnt, unsigned long long &deg) {
+    const uint32_t b = 1u << (d & 31);
+    if (V[d >> 5] & b) return;
+    if (atomicOr(&V[d >> 5], b) & b) return;
+    level[d] = round;
+    parent[d] = v;
+    atomicOr(&Fn[d >> 5], b);
+    cnt++;
+    deg += (unsigned long long)(out_off[d + 1] - out_off[d]);
 }
 
-// Top-down round over the frontier bitmap, for small frontiers: no list and no
-// scan.  A wave takes 64 bitmap words, a lane one word; every pass of the
+// Top-down round over the frontier bitmap: no list and no scan, so the grid does
+// not depend on the frontier and the round can be enqueued before its frontier is
+// known.  A wave takes 64 bitmap words, a lane one word; every pass of the
 // wave-uniform loop takes the next set bit of each lane's word.  A vertex of
 // out-degree at most kBfsLaneDeg is pushed by its own lane; a larger one is queued
 // (qbig up to kBfsWaveDeg, qhuge above) for k_bfs_push_queue, which gives a queued
-// vertex a whole wave and a huge one the whole grid, so a hub in a small frontier
-// is not one lane's (or one wave's) serial loop.  The claim is the one of
-// BfsPushOp; the round's statistics are summed from Fn afterwards.
+// vertex a whole wave and a huge one the whole grid, edge-balanced, so a hub in the
+// frontier is not one lane's (or one wave's) serial loop.
 constexpr int kBfsLaneDeg = 8;
 constexpr int kBfsWaveDeg = 1024;
 
-__device__ __forceinline__ void bfs_claim(int v, int d, uint32_t *V, int32_t *level,
-                                          int32_t *parent, uint32_t *Fn, int32_t round) {
-    if (bit_test(V, d)) return;
-    level[d] = round;
-    parent[d] = v;
-    const uint32_t b = 1u << (d & 31);
-    atomicOr(&V[d >> 5], b);
-    atomicOr(&Fn[d >> 5], b);
-}
-
-__global__ __launch_bounds__(kBlock) void k_bfs_push_bits(
+__device__ __forceinline__ void bfs_push_bits(
     const uint32_t *__restrict__ F, int64_t nw, const int32_t *__restrict__ out_off,
     const int32_t *__restrict__ out_adj, uint32_t *V, int32_t *level, int32_t *parent,
-    uint32_t *Fn, int32_t round, int32_t *qbig, int32_t *qhuge, unsigned long long *qctr) {
+    uint32_t *Fn, int32_t round, int32_t *qbig, int32_t *qhuge, unsigned long long *qctr,
+    unsigned long long &cnt, unsigned long long &deg) {
     const int lane = lane_id();
     const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
     for (int64_t base = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 64;
          base < nw; base += nwaves * 64) {
         uint32_t x = base + lane < nw ? F[base + lane] : 0u;
         while (__ballot(x != 0u)) {
-            int v = -1, deg = 0;
+            int v = -1, dg = 0;
             if (x) {
                 v = (int)((base + lane) * 32 + __ffs(x) - 1);
                 x &= x - 1;
                 const int beg = out_off[v];
-                deg = out_off[v + 1] - beg;
-                if (deg <= kBfsLaneDeg) {
-                    for (int e = beg; e < beg + deg; e++)
-                        bfs_claim(v, out_adj[e], V, level, parent, Fn, round);
+                dg = out_off[v + 1] - beg;
+                if (dg <= kBfsLaneDeg) {
+                    for (int e = beg; e < beg + dg; e++)
+                        bfs_claim(v, out_adj[e], V, level, parent, Fn, round, out_off, cnt, deg);
                     v = -1;
                 }
             }
-            agg_append(qbig, &qctr[0], v >= 0 && deg <= kBfsWaveDeg, v);
-            agg_append(qhuge, &qctr[1], v >= 0 && deg > kBfsWaveDeg, v);
+            agg_append(qbig, &qctr[0], v >= 0 && dg <= kBfsWaveDeg, v);
+            agg_append(qhuge, &qctr[1], v >= 0 && dg > kBfsWaveDeg, v);
         }
     }
+}
+
+// Round r (its first kernel; k_bfs_push_queue finishes a top-down round).  Every
+// block takes the round's decision from region r-1 (the frontier's size c and
+// out-degree sum e, the running state): done when c = 0 or already done; else
+// threshold >= 0: bottom-up iff c + e >= threshold (P9); threshold < 0: bottom-up
+// once e exceeds 1/alpha of the edges not yet reached, top-down again when the
+// frontier holds fewer than n/beta vertices.  Block 0 writes the decision and the
+// new state into region r and clears the counters of region r+1 (no kernel of
+// round r touches it).  Then the grid clears C(r) and runs the round: bottom-up
+// over the unvisited body, or the top-down scan of the frontier bitmap.
+template <int J>
+__global__ __launch_bounds__(kBlock) void k_bfs_round(
+    BfsRound R, const int32_t *__restrict__ in_off, const int32_t *__restrict__ in_adj,
+    const int32_t *__restrict__ first_in, const int32_t *__restrict__ out_off,
+    const int32_t *__restrict__ out_adj, uint32_t *V, int32_t *level, int32_t *parent,
+    int32_t *qbig, int32_t *qhuge, int64_t nbody, int64_t nbits, int64_t threshold, int64_t m,
+    int64_t n, int64_t alpha, int64_t beta) {
+    __shared__ unsigned long long s_mode;
+    const int64_t r = R.r();
+    const unsigned long long *prev = R.region(r - 1);
+    unsigned long long *cur = R.region(r);
+    if (threadIdx.x < 64) {
+        const int t = threadIdx.x;
+        unsigned long long c = prev[2 * t], e = prev[2 * t + 1];
+        for (int o = 32; o >= 1; o >>= 1) {
+            c += __shfl_xor(c, o, 64);
+            e += __shfl_xor(e, o, 64);
+        }
+        if (t == 0) {
+            unsigned long long st[kBfsCtlEnd - kBfsCtlBu];
+            for (int i = 0; i < kBfsCtlEnd - kBfsCtlBu; i++) st[i] = prev[kBfsCtlBu + i];
+            unsigned long long mode = kBfsModeDone;
+            if (!st[kBfsCtlDone - kBfsCtlBu] && c != 0) {
+                st[kBfsCtlRounds - kBfsCtlBu] += 1;
+                st[kBfsCtlActive - kBfsCtlBu] += c;
+                const unsigned long long tot = st[kBfsCtlEdges - kBfsCtlBu] + e;
+                st[kBfsCtlEdges - kBfsCtlBu] = tot;
+                bool dense;
+                if (threshold >= 0) {
+                    dense = (int64_t)(c + e) >= threshold;
+                } else {
+                    const int64_t unexplored = m > (int64_t)tot ? m - (int64_t)tot : 0;
+                    dense = st[kBfsCtlBu - kBfsCtlBu] ? (int64_t)c >= n / beta
+                                                       : (int64_t)e > unexplored / alpha;
+                    st[kBfsCtlBu - kBfsCtlBu] = dense ? 1 : 0;
+                }
+                if (dense) st[kBfsCtlDense - kBfsCtlBu] += 1;
+                mode = dense ? kBfsModeBU : kBfsModeTD;
+            } else {
+                st[kBfsCtlDone - kBfsCtlBu] = 1;
+            }
+            s_mode = mode;
+            if (blockIdx.x == 0) {
+                cur[kBfsCtlMode] = mode;
+                for (int i = 0; i < kBfsCtlEnd - kBfsCtlBu; i++) cur[kBfsCtlBu + i] = st[i];
+            }
+        }
+    }
+    if (blockIdx.x == 0) {
+        unsigned long long *nx = R.region(r + 1);
+        for (int i = threadIdx.x; i < kBfsCtlMode; i += blockDim.x) nx[i] = 0ull;
+    }
+    __syncthreads();
+    const unsigned long long mode = s_mode;
+    if (mode == kBfsModeDone) return;  // block-uniform
+    uint32_t *Cb = R.C(r);
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w <= nbits;
+         w += (int64_t)gridDim.x * blockDim.x)
+        Cb[w] = 0u;
+    unsigned long long cnt = 0, deg = 0;
+    if (mode == kBfsModeBU)
+        bfs_bottom_up<J>(in_off, in_adj, first_in, out_off, R.F(r), V, level, parent, (int32_t)r,
+                         nbody, R.N(r), cnt, deg);
+    else
+        bfs_push_bits(R.F(r), nbits, out_off, out_adj, V, level, parent, R.N(r), (int32_t)r, qbig,
+                      qhuge, cur + 130, cnt, deg);
+    bfs_flush_stats(cnt, deg, cur);
+}
+
+__global__ void k_bfs_advance(unsigned long long *base, int64_t rounds) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) base[0] += (unsigned long long)rounds;
 }
 
 // the queued vertices of k_bfs_push_bits: the huge ones edge-balanced over the
@@ -727,12 +819,18 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push_bits(
 // LDS; edge k of a batch goes to thread k mod grid size), every big one to one
 // wave, four 64-edge slices in flight per step
 __global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
-    const int32_t *__restrict__ qbig, const int32_t *__restrict__ qhuge,
-    const unsigned long long *qctr, const int32_t *__restrict__ out_off,
-    const int32_t *__restrict__ out_adj, uint32_t *V, int32_t *level, int32_t *parent,
-    uint32_t *Fn, int32_t round) {
+    BfsRound R, const int32_t *__restrict__ qbig, const int32_t *__restrict__ qhuge,
+    const int32_t *__restrict__ out_off, const int32_t *__restrict__ out_adj, uint32_t *V,
+    int32_t *level, int32_t *parent) {
     __shared__ int32_t s_v[kBlock], s_beg[kBlock];
     __shared__ int64_t s_pre[kBlock + 1];
+    const int64_t rr = R.r();
+    unsigned long long *cur = R.region(rr);
+    if (cur[kBfsCtlMode] != kBfsModeTD) return;  // block-uniform
+    uint32_t *Fn = R.N(rr);
+    const int32_t round = (int32_t)rr;
+    const unsigned long long *qctr = cur + 130;
+    unsigned long long cnt = 0, deg = 0;
     const int64_t nh = (int64_t)qctr[1], nb = (int64_t)qctr[0];
     const int64_t gt = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t gn = (int64_t)gridDim.x * kBlock;
@@ -758,7 +856,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
                 if (s_pre[mid] <= k) lo = mid; else hi = mid - 1;
             }
             const int e = s_beg[lo] + (int)(k - s_pre[lo]);
-            bfs_claim(s_v[lo], out_adj[e], V, level, parent, Fn, round);
+            bfs_claim(s_v[lo], out_adj[e], V, level, parent, Fn, round, out_off, cnt, deg);
         }
         __syncthreads();
     }
@@ -776,60 +874,41 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
             }
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (d[k] >= 0) bfs_claim(v, d[k], V, level, parent, Fn, round);
+                if (d[k] >= 0)
+                    bfs_claim(v, d[k], V, level, parent, Fn, round, out_off, cnt, deg);
         }
     }
+    bfs_flush_stats(cnt, deg, cur);
 }
 
-__global__ void k_bfs_seed(int32_t src, int32_t *level, int32_t *parent, uint32_t *F,
-                           uint32_t *V, int32_t *list, unsigned long long *ctr) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        level[src] = 0;
+// A search's starting state in one pass: levels -1 but the source's 0, the source
+// its own parent, the visited bitmap and bitmap 0 (round 1's frontier) holding the
+// source alone, bitmaps 1 and 2 clear, every region clear but the source's
+// statistics in region 0, the round base 0.  (Each word is written once, with its
+// final value, so no ordering among threads is needed.)
+__global__ void k_bfs_init(int32_t src, int64_t n, int64_t nw, const int32_t *out_off,
+                           int32_t *level, int32_t *parent, uint32_t *V, uint32_t *b0,
+                           uint32_t *b1, uint32_t *b2, unsigned long long *regions,
+                           unsigned long long *base) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = gt; v < n; v += gs) level[v] = v == src ? 0 : -1;
+    const int64_t sw = src >> 5;
+    const uint32_t sb = 1u << (src & 31);
+    for (int64_t w = gt; w < nw; w += gs) {
+        const uint32_t x = w == sw ? sb : 0u;
+        V[w] = x;
+        b0[w] = x;
+        b1[w] = 0u;
+        b2[w] = 0u;
+    }
+    for (int64_t i = gt; i < (int64_t)kBfsRegion * kBfsRegions; i += gs)
+        regions[i] = i == 0 ? 1ull
+                            : (i == 1 ? (unsigned long long)(out_off[src + 1] - out_off[src]) : 0ull);
+    if (gt == 0) {
         parent[src] = src;
-        F[src >> 5] |= 1u << (src & 31);
-        V[src >> 5] |= 1u << (src & 31);
-        list[0] = src;
-        ctr[0] = 1ull;
-    }
-}
-
-// size and out-degree sum of the frontier in bits, added to slots[2k] and
-// slots[2k + 1], k = block mod 64 (after a push that set the bits)
-__global__ void k_bits_stats(const uint32_t *__restrict__ bits, int64_t n,
-                             const int32_t *__restrict__ off, unsigned long long *slots) {
-    __shared__ unsigned long long sc[kBlock / 64], sd[kBlock / 64];
-    const int64_t nw = (n + 31) / 32;
-    unsigned long long cnt = 0, deg = 0;
-    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
-         w += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t x = bits[w];
-        cnt += __popc(x);
-        while (x) {
-            const int64_t v = w * 32 + __ffs(x) - 1;
-            x &= x - 1;
-            deg += (unsigned long long)(off[v + 1] - off[v]);
-        }
-    }
-    for (int o = 32; o >= 1; o >>= 1) {
-        cnt += __shfl_xor(cnt, o, 64);
-        deg += __shfl_xor(deg, o, 64);
-    }
-    if (lane_id() == 0) {
-        sc[threadIdx.x >> 6] = cnt;
-        sd[threadIdx.x >> 6] = deg;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long c = 0, d = 0;
-        for (int i = 0; i < kBlock / 64; i++) {
-            c += sc[i];
-            d += sd[i];
-        }
-        if (c) {
-            unsigned long long *p = slots + 2 * (blockIdx.x & 63);
-            atomicAdd(&p[0], c);
-            atomicAdd(&p[1], d);
-        }
+        base[0] = 0ull;
+        base[1] = 0ull;
     }
 }
 
@@ -1021,92 +1100,113 @@ int bfs_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_stat
     PG_TRY(workspace_t(g, "bfs_parent", (size_t)g->n, &parent));
     PG_TRY(workspace_t(g, "bfs_visited", (size_t)nw, &V));
     PG_TRY(workspace_t(g, "bfs_slots", (size_t)kBfsRegion * kBfsRegions, &slots));
-    // a small push's queues: its frontier has fewer than n/64 edges, so fewer than
-    // n/(64 (kBfsLaneDeg + 1)) vertices of out-degree above kBfsLaneDeg, and fewer
-    // than n/(64 kBfsWaveDeg) + 1 above kBfsWaveDeg
+    unsigned long long *base = nullptr;  // rounds of the batches launched before
+    PG_TRY(workspace_t(g, "bfs_base", 2, &base));
+    // top-down queues: at most n vertices of out-degree above kBfsLaneDeg, at most
+    // m / kBfsWaveDeg above kBfsWaveDeg
     int32_t *qbig = nullptr, *qhuge = nullptr;
-    PG_TRY(workspace_t(g, "bfs_qbig", (size_t)(g->n / (64 * (kBfsLaneDeg + 1)) + 64), &qbig));
-    PG_TRY(workspace_t(g, "bfs_qhuge", (size_t)(g->n / (64 * (int64_t)kBfsWaveDeg) + 64), &qhuge));
+    PG_TRY(workspace_t(g, "bfs_qbig", (size_t)g->n + 64, &qbig));
+    PG_TRY(workspace_t(g, "bfs_qhuge", (size_t)(g->m / kBfsWaveDeg + 64), &qhuge));
     const bool have_first = g->ws.find("bfs_first_in") != g->ws.end();
     PG_TRY(workspace_t(g, "bfs_first_in", (size_t)g->n, &first_in));
     if (!have_first) PG_LAUNCH_K(k_first_in, grid_for(g->n), g->in_off, g->in_adj, g->n, first_in);
-    int32_t sdeg[2];  // the source's out-offsets: the first frontier's statistics
-    PG_CHECK_HIP(hipMemcpy(sdeg, g->out_off + s_int, 8, hipMemcpyDeviceToHost));
     Frontier A, B;
     PG_TRY(frontiers(g, &A, &B));
     const int64_t nbody = g->in_sched.body_end;
+    const int64_t nbits = (g->n + 31) / 32;
+    uint32_t *bits2 = nullptr;
+    PG_TRY(workspace_t(g, "bfs_bits2", (size_t)nw, &bits2));
+    const unsigned grid_round = grid_for(std::max<int64_t>(nbody / (4 * kBuChunks), nbits), 2048);
+    // A batch of L rounds (k_bfs_round decides and runs a bottom-up round or a
+    // top-down round's bitmap scan; k_bfs_push_queue finishes a top-down round),
+    // then the base advances.  No host decision inside: the host reads the state
+    // once per batch (rounds after the last frontier only record "done").
+    auto enqueue_batch = [&](int L) -> int {
+        for (int k = 0; k < L; k++) {
+            const BfsRound R{slots, base, A.bits, B.bits, bits2, k};
+            PG_LAUNCH_K(k_bfs_round<kBuChunks>, grid_round, R, g->in_off, g->in_adj, first_in,
+                        g->out_off, g->out_adj, V, level, parent, qbig, qhuge, nbody, nbits,
+                        threshold, g->m, g->n, kBfsAlpha, kBfsBeta);
+            PG_LAUNCH_K(k_bfs_push_queue, 1024, R, qbig, qhuge, g->out_off, g->out_adj, V, level,
+                        parent);
+        }
+        hipLaunchKernelGGL(k_bfs_advance, dim3(1), dim3(64), 0, stream(), base, (int64_t)L);
+        PG_CHECK_HIP(hipGetLastError());
+        return PG_OK;
+    };
+    const bool profile = (flags & PG_PROFILE_KERNELS) != 0;
+    // the batch of L rounds, captured once per graph, threshold and L, replayed
+    auto batch_exec = [&](int L, hipGraphExec_t *out) -> int {
+        const std::string key = "bfs_batch_" + std::to_string(threshold) + "_" + std::to_string(L);
+        auto it = g->graph_exec.find(key);
+        if (it != g->graph_exec.end()) {
+            *out = it->second.first;
+            return PG_OK;
+        }
+        PG_CHECK_HIP(hipStreamSynchronize(stream()));
+        PG_CHECK_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_batch(L);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(stream(), &graph);
+        if (rc != PG_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        PG_REQUIRE(e == hipSuccess, PG_ERR_HIP, "bfs capture failed: %s", hipGetErrorString(e));
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PG_REQUIRE(e == hipSuccess, PG_ERR_HIP, "bfs instantiate failed: %s", hipGetErrorString(e));
+        g->graph_exec[key] = {exec, nullptr};
+        *out = exec;
+        return PG_OK;
+    };
+    const int first = std::min(kBfsMaxBatch, g->bfs_rounds_hint > 0 ? g->bfs_rounds_hint
+                                                                      : kBfsFirstBatch);
+    hipGraphExec_t exec_first = nullptr, exec_next = nullptr;
+    if (!profile) {
+        PG_TRY(batch_exec(first, &exec_first));
+        PG_TRY(batch_exec(kBfsAhead, &exec_next));
+    }
     Timer tm;
     KernelTimer kt;
-    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    kt.on = profile;
     PG_TRY(tm.start());
-    PG_TRY(fill(level, -1, g->n));
-    PG_CHECK_HIP(hipMemsetAsync(V, 0, 4 * nw, stream()));
-    PG_CHECK_HIP(hipMemsetAsync(A.bits, 0, 4 * nw, stream()));
-    PG_CHECK_HIP(hipMemsetAsync(B.bits, 0, 4 * nw, stream()));
-    PG_CHECK_HIP(hipMemsetAsync(slots, 0, 8 * kBfsRegion * kBfsRegions, stream()));
-    PG_LAUNCH_K(k_bfs_seed, 1, s_int, level, parent, A.bits, V, A.list, A.ctr);
-    int64_t cnt = 1, edges = sdeg[1] - sdeg[0];
-    int rounds = 0, dense_rounds = 0;
-    int64_t tot_active = 0, tot_edges = 0;
-    Frontier *F = &A, *N = &B;
-    bool f_list = true;  // F->list holds the frontier (F->bits always does)
-    bool bu = false;     // the default policy's current direction
-    while (cnt > 0) {
-        rounds++;
-        tot_active += cnt;
-        tot_edges += edges;
-        bool dense;
-        if (threshold >= 0) {
-            dense = cnt + edges >= threshold;  // P9: >=
+    PG_LAUNCH_K(k_bfs_init, grid_for(g->n), s_int, g->n, nw, g->out_off, level, parent, V, A.bits,
+                B.bits, bits2, slots, base);
+    // the last round's region after a batch: its frontier's (count, out-degree)
+    // pairs and the state after its decision
+    unsigned long long hs[kBfsCtlEnd] = {};
+    int64_t launched_rounds = 0;
+    for (int L = first;; L = kBfsAhead) {
+        if (profile) {
+            PG_TRY(kt.begin());
+            PG_TRY(enqueue_batch(L));
+            PG_TRY(kt.end());
         } else {
-            // bottom-up once the frontier's edges exceed 1/alpha of the edges not yet
-            // reached; back to top-down when it holds fewer than n/beta vertices
-            const int64_t unexplored = std::max<int64_t>(g->m - tot_edges, 0);
-            bu = bu ? cnt >= g->n / kBfsBeta : edges > unexplored / kBfsAlpha;
-            dense = bu;
+            PG_CHECK_HIP(hipGraphLaunch(launched_rounds == 0 ? exec_first : exec_next,
+                                        stream()));
         }
-        // region of the spread counters and the list counter for this round
-        if ((rounds - 1) % kBfsRegions == 0 && rounds > 1)
-            PG_CHECK_HIP(hipMemsetAsync(slots, 0, 8 * kBfsRegion * kBfsRegions, stream()));
-        unsigned long long *rs = slots + (size_t)((rounds - 1) % kBfsRegions) * kBfsRegion;
-        // a bottom-up round writes every word of the next bitmap, a push sets bits
-        // in a cleared one.  A small frontier is pushed from its bitmap, a large one
-        // edge-balanced from a list.
-        const bool bitpush = !dense && edges < g->n / 64;
-        PG_TRY(kt.begin());
-        if (dense) {
-            dense_rounds++;
-            PG_LAUNCH_K(k_bfs_bottom_up<kBuChunks>, grid_for(nbody / (4 * kBuChunks), 2048),
-                        g->in_off, g->in_adj, first_in, g->out_off, F->bits, V, level, parent,
-                        (int32_t)rounds, nbody, N->bits, rs);
-        } else {
-            PG_CHECK_HIP(hipMemsetAsync(N->bits, 0, 4 * nw, stream()));
-            if (bitpush) {
-                PG_LAUNCH_K(k_bfs_push_bits, grid_for(nw, 2048), F->bits, (g->n + 31) / 32,
-                            g->out_off, g->out_adj, V, level, parent, N->bits, (int32_t)rounds,
-                            qbig, qhuge, rs + 130);
-                PG_LAUNCH_K(k_bfs_push_queue, 1024, qbig, qhuge, rs + 130, g->out_off, g->out_adj,
-                            V, level, parent, N->bits, (int32_t)rounds);
-            } else {
-                if (!f_list) PG_TRY(bits_to_list(g, F->bits, F->list, rs + 129));
-                BfsPushOp op{N->bits, V, level, parent, (int32_t)rounds};
-                PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));
-            }
-            PG_LAUNCH_K(k_bits_stats, grid_for((g->n + 31) / 32, 2048), N->bits, g->n,
-                        g->out_off, rs);
-        }
-        PG_TRY(kt.end());
-        PG_TRY(read_region(rs, &cnt, &edges));
-        f_list = false;  // only the source's list is built
-        std::swap(F, N);
+        launched_rounds += L;
+        const unsigned long long *last =
+            slots + (size_t)(launched_rounds % kBfsRegions) * kBfsRegion;
+        PG_CHECK_HIP(hipMemcpyAsync(hs, last, sizeof(hs), hipMemcpyDeviceToHost, stream()));
+        PG_CHECK_HIP(hipStreamSynchronize(stream()));
+        unsigned long long c = 0;
+        for (int i = 0; i < 64; i++) c += hs[2 * i];
+        if (hs[kBfsCtlDone] || c == 0) break;
+        PG_REQUIRE(launched_rounds <= 4 * g->n + 64, PG_ERR_STATE, "bfs: no convergence");
     }
+    // the next search's first batch: this one's rounds, lowered by at most one
+    g->bfs_rounds_hint = std::max<int>((int)hs[kBfsCtlRounds], g->bfs_rounds_hint - 1);
+    if (g->bfs_rounds_hint < 1) g->bfs_rounds_hint = 1;
     double ms = 0;
     PG_TRY(tm.stop(&ms));
     if (st) {
-        st->rounds = rounds;
-        st->dense_rounds = dense_rounds;
-        st->edges = tot_edges;
-        st->active = tot_active;
+        st->rounds = (int32_t)hs[kBfsCtlRounds];
+        st->dense_rounds = (int32_t)hs[kBfsCtlDense];
+        st->edges = (int64_t)hs[kBfsCtlEdges];
+        st->active = (int64_t)hs[kBfsCtlActive];
         st->ms = ms;
         PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
     }

@@ -251,6 +251,8 @@ struct pg_graph {
     // captured solves (hipGraph executables) and the buffer holding their result
     std::map<std::string, std::pair<hipGraphExec_t, float *>> graph_exec;
     // saved state of the last weighted PageRank forward (for the adjoint)
+    // rounds of the first captured batch of the next BFS (from the last search)
+    int bfs_rounds_hint = 0;
     bool wpr_valid = false;
     double wpr_damping = 0.85;
     // allocations owned by the graph
