@@ -814,10 +814,10 @@ __global__ void k_bfs_advance(unsigned long long *base, int64_t rounds) {
     if (threadIdx.x == 0 && blockIdx.x == 0) base[0] += (unsigned long long)rounds;
 }
 
-// the queued vertices of k_bfs_push_bits: the huge ones edge-balanced over the
-// whole grid (batches of kBlock vertices whose degree prefix every block forms in
-// LDS; edge k of a batch goes to thread k mod grid size), every big one to one
-// wave, four 64-edge slices in flight per step
+// the queued vertices of a top-down round (bfs_push_bits): the huge ones
+// edge-balanced over the whole grid (batches of kBlock vertices whose degree prefix
+// every block forms in LDS; edge k of a batch goes to thread k mod grid size),
+// every big one to one wave, four 64-edge slices in flight per step
 __global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
     BfsRound R, const int32_t *__restrict__ qbig, const int32_t *__restrict__ qhuge,
     const int32_t *__restrict__ out_off, const int32_t *__restrict__ out_adj, uint32_t *V,
