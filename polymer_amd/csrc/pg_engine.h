@@ -412,12 +412,17 @@ __host__ __device__ __forceinline__ unsigned long long to_fixed(float x) {
     return sh > -64 ? mant >> (-sh) : 0ull;
 }
 
-// the fixed-point term of tile edge i whose source value is x: x itself, or x times
-// the edge's weight (Op::kTileWeighted, weights streamed in tile order)
+// the weight of tile edge i (Op::kTileWeighted: weights streamed in tile order;
+// otherwise nothing is loaded) and the fixed-point term of an edge whose source
+// value is x: x itself, or x times the edge's weight
 template <class Op>
-__device__ __forceinline__ unsigned long long tile_term(const Op &op, float x, const float *wt,
-                                                        int64_t i) {
-    if constexpr (Op::kTileWeighted) return op.tile_fix(x * __builtin_nontemporal_load(wt + i));
+__device__ __forceinline__ float tile_weight(const Op &, const float *wt, int64_t i) {
+    if constexpr (Op::kTileWeighted) return __builtin_nontemporal_load(wt + i);
+    else return 1.f;
+}
+template <class Op>
+__device__ __forceinline__ unsigned long long tile_term(const Op &op, float x, float wv) {
+    if constexpr (Op::kTileWeighted) return op.tile_fix(x * wv);
     else return op.tile_fix(x);
 }
 
@@ -444,19 +449,21 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
         // the next: source ids from the packed words and group bases (an escaped
         // group's ids in a fix-up that is almost never taken, so no per-edge branch
         // separates the gathers); the U gathers, in flight together; the next
-        // step's packed words and bases (index clamped into the part), in flight
-        // while this step waits for its gathers; terms; LDS adds.  Measured on
-        // MI355X (S24 PageRank iteration): 613 us with one gather in flight per lane
-        // and no prefetch, 590 with the gathers together, 552 with the prefetch; a
-        // prefetch two steps ahead, or U = 3, 6, 8, no better.
+        // step's packed words, bases and weights (index clamped into the part), in
+        // flight while this step waits for its gathers; terms; LDS adds.  Measured
+        // on MI355X (S24 PageRank iteration): 613 us with one gather in flight per
+        // lane and no prefetch, 590 with the gathers together, 552 with the
+        // prefetch; a prefetch two steps ahead, or U = 3, 6, 8, no better.
         if (e + (U - 1) * kTileThreads < cnt) {
             unsigned w[U];
             int b[U];
+            float wv[U];
 #pragma unroll
             for (int k = 0; k < U; k++) {
                 const int64_t i = e0 + e + k * kTileThreads;
                 w[k] = __builtin_nontemporal_load(ta.packed + i);
                 b[k] = ta.gbase[i / kPackGroup];
+                wv[k] = tile_weight(op, ta.wt, i);
             }
             while (true) {
                 int s[U];
@@ -480,15 +487,17 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                 const int en = e + U * kTileThreads;
                 unsigned wn[U];
                 int bn[U];
+                float wvn[U];
 #pragma unroll
                 for (int k = 0; k < U; k++) {
                     const int64_t i = e0 + min(en + k * kTileThreads, cnt - 1);
                     wn[k] = __builtin_nontemporal_load(ta.packed + i);
                     bn[k] = ta.gbase[i / kPackGroup];
+                    wvn[k] = tile_weight(op, ta.wt, i);
                 }
                 unsigned long long q[U];
 #pragma unroll
-                for (int k = 0; k < U; k++) q[k] = tile_term(op, x[k], ta.wt, e0 + e + k * kTileThreads);
+                for (int k = 0; k < U; k++) q[k] = tile_term(op, x[k], wv[k]);
 #pragma unroll
                 for (int k = 0; k < U; k++) atomicAdd(&acc[w[k] & kDstMask], q[k]);
                 e = en;
@@ -497,6 +506,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                 for (int k = 0; k < U; k++) {
                     w[k] = wn[k];
                     b[k] = bn[k];
+                    wv[k] = wvn[k];
                 }
             }
         }
@@ -506,7 +516,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
             const int b = ta.gbase[i / kPackGroup];
             const int s = b >= 0 ? b + (int)(w >> kPackDstBits)
                                  : ta.esc[(int64_t)(-b - 1) * kPackGroup + (i % kPackGroup)];
-            atomicAdd(&acc[w & kDstMask], tile_term(op, op.tile_raw(s), ta.wt, i));
+            atomicAdd(&acc[w & kDstMask], tile_term(op, op.tile_raw(s), tile_weight(op, ta.wt, i)));
         }
     }
     {  // cold sources: values streamed from the pre-pass
@@ -515,20 +525,21 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
         const int cnt = (int)(part.e1 - part.cold_e);
         int e = threadIdx.x;
         for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
-            float x[U];
+            float x[U], wv[U];
             unsigned d[U];
 #pragma unroll
             for (int k = 0; k < U; k++) {
                 x[k] = __builtin_nontemporal_load(val + e + k * kTileThreads);
                 d[k] = __builtin_nontemporal_load(pk + e + k * kTileThreads) & kDstMask;
+                wv[k] = tile_weight(op, ta.wt, part.cold_e + e + k * kTileThreads);
             }
 #pragma unroll
-            for (int k = 0; k < U; k++)
-                atomicAdd(&acc[d[k]], tile_term(op, x[k], ta.wt, part.cold_e + e + k * kTileThreads));
+            for (int k = 0; k < U; k++) atomicAdd(&acc[d[k]], tile_term(op, x[k], wv[k]));
         }
         for (; e < cnt; e += kTileThreads)
             atomicAdd(&acc[__builtin_nontemporal_load(pk + e) & kDstMask],
-                      tile_term(op, __builtin_nontemporal_load(val + e), ta.wt, part.cold_e + e));
+                      tile_term(op, __builtin_nontemporal_load(val + e),
+                                tile_weight(op, ta.wt, part.cold_e + e)));
     }
     __syncthreads();
     if (part.slot >= 0) {
