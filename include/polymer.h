@@ -285,31 +285,32 @@ G wrap(pg_graph *h, bool weighted) {
 
 }  // namespace polymer_detail
 
-// readGraph (IO.h:186-272): a PBBS AdjacencyGraph / WeightedAdjacencyGraph
-// text file; isSymmetric marks an already symmetric file (-s).
+// readGraph (IO-numa.h:73-190, 543-635): a PBBS AdjacencyGraph /
+// WeightedAdjacencyGraph text file, or with binary the <fname>.config / .adj / .idx
+// triple; isSymmetric marks an already symmetric graph (-s).
 template <class vertex>
 graph<vertex> readGraph(const char *fname, bool isSymmetric, bool binary = false) {
     polymer_detail::ensure_init();
-    if (binary) {
-        std::fprintf(stderr, "polymer: binary graph files are not supported\n");
-        std::abort();
-    }
     pg_graph *h = nullptr;
-    polymer_detail::check(pg_graph_load_adj(fname, isSymmetric ? PG_GRAPH_SYMMETRIC : 0u, &h),
-                          "pg_graph_load_adj");
+    const uint32_t flags = isSymmetric ? PG_GRAPH_SYMMETRIC : 0u;
+    if (binary)
+        polymer_detail::check(pg_graph_load_bin(fname, flags, &h), "pg_graph_load_bin");
+    else
+        polymer_detail::check(pg_graph_load_adj(fname, flags, &h), "pg_graph_load_adj");
     return polymer_detail::wrap<graph<vertex>>(h, false);
 }
 
+// readWghGraph (IO.h:186-272, IO-numa.h:637-720); a binary graph has unit weights
 template <class vertex>
 wghGraph<vertex> readWghGraph(const char *fname, bool isSymmetric, bool binary = false) {
     polymer_detail::ensure_init();
-    if (binary) {
-        std::fprintf(stderr, "polymer: binary graph files are not supported\n");
-        std::abort();
-    }
     pg_graph *h = nullptr;
-    polymer_detail::check(pg_graph_load_adj(fname, isSymmetric ? PG_GRAPH_SYMMETRIC : 0u, &h),
-                          "pg_graph_load_adj");
+    const uint32_t flags = isSymmetric ? PG_GRAPH_SYMMETRIC : 0u;
+    if (binary)
+        polymer_detail::check(pg_graph_load_bin(fname, flags | PG_GRAPH_WEIGHT_I32, &h),
+                              "pg_graph_load_bin");
+    else
+        polymer_detail::check(pg_graph_load_adj(fname, flags, &h), "pg_graph_load_adj");
     return polymer_detail::wrap<wghGraph<vertex>>(h, true);
 }
 

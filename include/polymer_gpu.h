@@ -98,6 +98,12 @@ int pg_graph_rmat(int scale, int64_t m, double a, double b, double c, uint64_t s
                   uint32_t flags, uint64_t wseed, pg_graph **out);
 /* PBBS AdjacencyGraph / WeightedAdjacencyGraph text file. */
 int pg_graph_load_adj(const char *path, uint32_t flags, pg_graph **out);
+/* Binary graph <base>.config / .adj / .idx, replacing readGraphFromBinary and
+ * readWghGraphFromBinary (IO-numa.h:543-720): n as text, m 32-bit neighbour ids,
+ * n out-list offsets of 4 or 8 bytes (taken from the file's size; any other size
+ * is an error).  PG_GRAPH_WEIGHT_I32: unit weights, as the reference's weighted
+ * reader gives (IO-numa.h:683). */
+int pg_graph_load_bin(const char *base, uint32_t flags, pg_graph **out);
 int pg_graph_destroy(pg_graph *g);
 /* n, m (directed entries), max in-degree, has-weights (0 none, 1 i32, 2 f32) */
 int pg_graph_info(const pg_graph *g, int64_t *n, int64_t *m, int64_t *max_in_degree,

@@ -274,6 +274,105 @@ int or_write_wadj(const char *path, int64_t n, int64_t m, const int64_t *off,
     return 0;
 }
 
+/* The binary graph of readGraphFromBinary (IO-numa.h:543-635): <base>.config
+ * holds n as text, <base>.adj the m neighbour ids as 32-bit words, <base>.idx the
+ * n out-list offsets (no closing m), idx_bytes = 4 (IO-numa.h:579) or 8 (the
+ * 8-byte width readWghGraphFromBinary checks for, IO-numa.h:672). */
+static FILE *open_ext(const char *base, const char *ext, const char *mode) {
+    char path[4096];
+    if (snprintf(path, sizeof(path), "%s%s", base, ext) >= (int)sizeof(path)) return NULL;
+    return fopen(path, mode);
+}
+
+int or_write_bin(const char *base, int64_t n, int64_t m, const int64_t *off,
+                 const int32_t *adj, int idx_bytes) {
+    FILE *f = open_ext(base, ".config", "w");
+    if (!f) return -1;
+    fprintf(f, "%lld\n", (long long)n);
+    fclose(f);
+    f = open_ext(base, ".adj", "wb");
+    if (!f) return -1;
+    for (int64_t e = 0; e < m; e++) {
+        const uint32_t x = (uint32_t)adj[e];
+        fwrite(&x, 4, 1, f);
+    }
+    fclose(f);
+    f = open_ext(base, ".idx", "wb");
+    if (!f) return -1;
+    for (int64_t i = 0; i < n; i++) {
+        if (idx_bytes == 8) {
+            const int64_t x = off[i];
+            fwrite(&x, 8, 1, f);
+        } else {
+            const int32_t x = (int32_t)off[i];
+            fwrite(&x, 4, 1, f);
+        }
+    }
+    fclose(f);
+    return 0;
+}
+
+static int64_t file_size(FILE *f) {
+    if (fseek(f, 0, SEEK_END) != 0) return -1;
+    const long s = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    return (int64_t)s;
+}
+
+/* Returns 0, or -1 (missing file), -2 (bad config), -3 (idx size neither 4n nor
+ * 8n bytes: "File size wrong", where the reference aborts).  Out-degree of v is
+ * off[v+1] - off[v] with off[n] = m = size(.adj) / 4 (IO-numa.h:584-590). */
+int or_read_bin(const char *base, int64_t *n_out, int64_t *m_out, int64_t *off,
+                int32_t *adj) {
+    FILE *f = open_ext(base, ".config", "r");
+    if (!f) return -1;
+    long long n = 0;
+    const int ok = fscanf(f, "%lld", &n) == 1 && n > 0;
+    fclose(f);
+    if (!ok) return -2;
+    FILE *fa = open_ext(base, ".adj", "rb");
+    FILE *fi = open_ext(base, ".idx", "rb");
+    if (!fa || !fi) {
+        if (fa) fclose(fa);
+        if (fi) fclose(fi);
+        return -1;
+    }
+    const int64_t m = file_size(fa) / 4, isz = file_size(fi);
+    const int wide = isz == 8 * n;
+    if (!wide && isz != 4 * n) {
+        fclose(fa);
+        fclose(fi);
+        return -3;
+    }
+    *n_out = n;
+    *m_out = m;
+    int rc = 0;
+    if (off) {
+        for (int64_t i = 0; i < n && rc == 0; i++) {
+            if (wide) {
+                int64_t x;
+                if (fread(&x, 8, 1, fi) != 1) rc = -1;
+                off[i] = x;
+            } else {
+                int32_t x;
+                if (fread(&x, 4, 1, fi) != 1) rc = -1;
+                off[i] = x;
+            }
+        }
+        off[n] = m;
+    }
+    if (adj && rc == 0) {
+        for (int64_t e = 0; e < m && rc == 0; e++) {
+            uint32_t x;
+            if (fread(&x, 4, 1, fa) != 1) rc = -1;
+            adj[e] = (int32_t)x;
+        }
+    }
+    fclose(fa);
+    fclose(fi);
+    return rc;
+}
+
 /* Whitespace-separated tokens as in stringToWords (IO-numa.h:62-109). */
 static int next_tok(FILE *f, char *buf, int cap) {
     int c, len = 0;

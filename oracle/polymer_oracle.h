@@ -74,6 +74,10 @@ int or_write_wadj(const char *path, int64_t n, int64_t m, const int64_t *off,
  * skip_self_loops follows readGraphFromFileSkipRing (IO-numa.h:225-268). */
 int or_read_adj(const char *path, int skip_self_loops, int64_t *n, int64_t *m,
                 int64_t *off, int32_t *adj, int32_t *w);
+/* binary <base>.config / .adj / .idx (IO-numa.h:543-635); idx_bytes 4 or 8 */
+int or_write_bin(const char *base, int64_t n, int64_t m, const int64_t *off,
+                 const int32_t *adj, int idx_bytes);
+int or_read_bin(const char *base, int64_t *n, int64_t *m, int64_t *off, int32_t *adj);
 
 /* ---- the six applications ------------------------------------------ */
 int or_pagerank_push(int64_t n, const int64_t *off, const int32_t *adj,

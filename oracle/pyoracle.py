@@ -38,6 +38,9 @@ def lib():
             "or_fill_vertex_f32": (None, [u64, i64, ctypes.c_float, ctypes.c_float, F32P]),
             "or_write_adj": (ctypes.c_int, [ctypes.c_char_p, i64, i64, I64P, I32P]),
             "or_write_wadj": (ctypes.c_int, [ctypes.c_char_p, i64, i64, I64P, I32P, I32P]),
+            "or_write_bin": (ctypes.c_int, [ctypes.c_char_p, i64, i64, I64P, I32P, ctypes.c_int]),
+            "or_read_bin": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                           ctypes.c_void_p, ctypes.c_void_p]),
             "or_read_adj": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(i64),
                                            ctypes.POINTER(i64), ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_void_p]),
@@ -173,6 +176,30 @@ def read_adj(path, skip_self_loops=False):
     L.or_read_adj(path.encode(), int(skip_self_loops), ctypes.byref(n), ctypes.byref(m),
                   off.ctypes.data, adj.ctypes.data, w.ctypes.data)
     return Graph(n.value, off, adj[: m.value]), (w[: m.value] if r == 1 else None)
+
+
+def write_bin(base, g, idx_bytes=4):
+    """or_write_bin: the binary graph triple base.config / .adj / .idx."""
+    r = lib().or_write_bin(base.encode(), g.n, g.m, g.off, g.adj, int(idx_bytes))
+    if r != 0:
+        raise RuntimeError("or_write_bin failed: %d" % r)
+
+
+def read_bin(base):
+    """or_read_bin (readGraphFromBinary, IO-numa.h:543-635)."""
+    L = lib()
+    n = ctypes.c_int64()
+    m = ctypes.c_int64()
+    r = L.or_read_bin(base.encode(), ctypes.byref(n), ctypes.byref(m), None, None)
+    if r < 0:
+        raise RuntimeError("or_read_bin failed: %d" % r)
+    off = np.zeros(n.value + 1, np.int64)
+    adj = np.zeros(max(m.value, 1), np.int32)
+    r = L.or_read_bin(base.encode(), ctypes.byref(n), ctypes.byref(m), off.ctypes.data,
+                      adj.ctypes.data)
+    if r < 0:
+        raise RuntimeError("or_read_bin failed: %d" % r)
+    return Graph(n.value, off, adj[: m.value])
 
 
 def pagerank(g, iters=20, damping=0.85):

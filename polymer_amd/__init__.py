@@ -32,7 +32,8 @@ INF_DIST = 1073741823
 # every symbol include/polymer_gpu.h declares
 EXPORTS = [
     "pg_init", "pg_finalize", "pg_last_error", "pg_device_name",
-    "pg_graph_create", "pg_graph_rmat", "pg_graph_load_adj", "pg_graph_destroy",
+    "pg_graph_create", "pg_graph_rmat", "pg_graph_load_adj", "pg_graph_load_bin",
+    "pg_graph_destroy",
     "pg_graph_info", "pg_graph_download", "pg_graph_set_schedule", "pg_graph_set_tiles",
     "pg_graph_get_schedule", "pg_graph_tile_stats", "pg_graph_autotune",
     "pg_graph_bp_potentials", "pg_graph_view",
@@ -92,6 +93,7 @@ def load():
         "pg_graph_create": [i64, i64, P, P, P, ctypes.c_int, u32, PP],
         "pg_graph_rmat": [ctypes.c_int, i64, f64, f64, f64, u64, u32, u64, PP],
         "pg_graph_load_adj": [ctypes.c_char_p, u32, PP],
+        "pg_graph_load_bin": [ctypes.c_char_p, u32, PP],
         "pg_graph_destroy": [P],
         "pg_graph_info": [P, P, P, P, P],
         "pg_graph_download": [P, P, P, P],
@@ -216,6 +218,18 @@ class Graph:
         h = ctypes.c_void_p()
         _check(load().pg_graph_load_adj(path.encode(), flags, ctypes.byref(h)),
                "pg_graph_load_adj")
+        return cls(h)
+
+    @classmethod
+    def load_bin(cls, base, symmetric=False, weighted=False, skip_self_loops=False):
+        """The binary graph base.config / base.adj / base.idx (unit weights when
+        weighted, as the reference's binary reader gives)."""
+        flags = (PG_GRAPH_SYMMETRIC if symmetric else 0) | (
+            PG_GRAPH_SKIP_SELF_LOOPS if skip_self_loops else 0) | (
+            PG_GRAPH_WEIGHT_I32 if weighted else 0)
+        h = ctypes.c_void_p()
+        _check(load().pg_graph_load_bin(base.encode(), flags, ctypes.byref(h)),
+               "pg_graph_load_bin")
         return cls(h)
 
     @classmethod

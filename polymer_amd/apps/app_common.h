@@ -5,6 +5,7 @@
 //   -rmat <scale> <edges> <seed>   device rMat a/b/c = 0.57/0.19/0.19
 //   -file <path>                   PBBS AdjacencyGraph / WeightedAdjacencyGraph text
 // Modifiers: -s (symmetric: symmetrise the rMat graph / the file is symmetric),
+//            -b (the file is the binary <path>.config / .adj / .idx triple),
 //            -wseed <k> (rMat int32 weights U[1,255] from seed k).
 // Common options: -rounds <T>, -r <source>, -thr <threshold>, -o <output file>.
 // The output is the result array as raw little-endian binary.
@@ -31,6 +32,7 @@ struct AppArgs {
     unsigned long long seed = 1;
     std::string file;
     bool symmetric = false;
+    bool binary = false;
     long long wseed = -1;
     int rounds = 20;
     long source = 0;
@@ -60,6 +62,8 @@ inline AppArgs parse_args(int argc, char **argv) {
             a.file = argv[++i];
         } else if (k == "-s") {
             a.symmetric = true;
+        } else if (k == "-b") {
+            a.binary = true;
         } else if (k == "-wseed") {
             need(1);
             a.wseed = std::atoll(argv[++i]);

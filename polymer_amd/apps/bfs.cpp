@@ -34,7 +34,7 @@ int main(int argc, char **argv) {
     const AppArgs a = parse_args(argc, argv);
     graph<asymmetricVertex> G = a.file.empty()
                                     ? rmatGraph<asymmetricVertex>(a.scale, a.edges, a.seed, a.symmetric)
-                                    : readGraph<asymmetricVertex>(a.file.c_str(), a.symmetric);
+                                    : readGraph<asymmetricVertex>(a.file.c_str(), a.symmetric, a.binary);
     const intT n = (intT)G.n;
     if (a.source < 0 || a.source >= n) {
         std::fprintf(stderr, "source %ld out of range\n", a.source);

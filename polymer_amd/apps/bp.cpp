@@ -49,7 +49,7 @@ int main(int argc, char **argv) {
     }
     graph<asymmetricVertex> G = a.file.empty()
                                     ? rmatGraph<asymmetricVertex>(a.scale, a.edges, a.seed, a.symmetric)
-                                    : readGraph<asymmetricVertex>(a.file.c_str(), a.symmetric);
+                                    : readGraph<asymmetricVertex>(a.file.c_str(), a.symmetric, a.binary);
     const intT n = (intT)G.n;
     const long m = G.m;
     const std::vector<float> vp = read_input<float>(a.in1, (size_t)2 * n);

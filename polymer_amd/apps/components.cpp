@@ -46,7 +46,7 @@ int main(int argc, char **argv) {
     const AppArgs a = parse_args(argc, argv);
     graph<symmetricVertex> G = a.file.empty()
                                    ? rmatGraph<symmetricVertex>(a.scale, a.edges, a.seed, true)
-                                   : readGraph<symmetricVertex>(a.file.c_str(), true);
+                                   : readGraph<symmetricVertex>(a.file.c_str(), true, a.binary);
     const intT n = (intT)G.n;
     const long threshold = a.threshold < 0 ? G.m / 8 : a.threshold;
     intT sz = n;

@@ -54,7 +54,7 @@ int main(int argc, char **argv) {
     const AppArgs a = parse_args(argc, argv);
     graph<asymmetricVertex> G = a.file.empty()
                                     ? rmatGraph<asymmetricVertex>(a.scale, a.edges, a.seed, a.symmetric)
-                                    : readGraph<asymmetricVertex>(a.file.c_str(), a.symmetric);
+                                    : readGraph<asymmetricVertex>(a.file.c_str(), a.symmetric, a.binary);
     const intT n = (intT)G.n;
     const double damping = 0.85;
     intT sz = n;

@@ -43,7 +43,7 @@ int main(int argc, char **argv) {
         a.file.empty()
             ? rmatWghGraph<asymmetricWghVertex>(a.scale, a.edges, a.seed, a.symmetric,
                                                 (uint64_t)(a.wseed < 0 ? 1 : a.wseed))
-            : readWghGraph<asymmetricWghVertex>(a.file.c_str(), a.symmetric);
+            : readWghGraph<asymmetricWghVertex>(a.file.c_str(), a.symmetric, a.binary);
     const intT n = (intT)G.n;
     intT sz = n;
     double *p_curr = static_cast<double *>(mapDataArray(1, &sz, sizeof(double)));

@@ -1556,6 +1556,63 @@ extern "C" int pg_graph_load_adj(const char *path, uint32_t flags, pg_graph **ou
                             weighted ? PG_W_I32 : PG_W_NONE, flags, out);
 }
 
+// The binary graph of readGraphFromBinary / readWghGraphFromBinary
+// (IO-numa.h:543-635, 637-720): <base>.config holds n as text, <base>.adj the m
+// neighbour ids as 32-bit words (m = its size / 4), <base>.idx the n offsets of the
+// out-lists (no closing m).  The reference's two readers disagree on the width of
+// an offset (4 bytes at IO-numa.h:579, 8 at :672); here the width follows the
+// file's size (4n or 8n bytes) and any other size is "File size wrong", where the
+// reference aborts.  PG_GRAPH_WEIGHT_I32 gives every edge weight 1, as the
+// weighted reader does (IO-numa.h:683).
+extern "C" int pg_graph_load_bin(const char *base, uint32_t flags, pg_graph **out) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(base && out, PG_ERR_ARG, "null argument");
+    *out = nullptr;
+    const std::string b(base);
+    long long n = 0;
+    {
+        std::ifstream fc(b + ".config");
+        PG_REQUIRE(fc.is_open(), PG_ERR_ARG, "Unable to open file: %s.config", base);
+        PG_REQUIRE((fc >> n) && n > 0 && n < 0x7FFFFFFF, PG_ERR_ARG, "Bad config file: %s.config",
+                   base);
+    }
+    auto slurp = [](const std::string &path, std::vector<char> *buf) -> bool {
+        std::ifstream f(path, std::ios::in | std::ios::binary | std::ios::ate);
+        if (!f.is_open()) return false;
+        const std::streamsize size = f.tellg();
+        f.seekg(0);
+        buf->resize((size_t)size);
+        return size == 0 || (bool)f.read(buf->data(), size);
+    };
+    std::vector<char> abuf, ibuf;
+    PG_REQUIRE(slurp(b + ".adj", &abuf), PG_ERR_ARG, "Unable to read file: %s.adj", base);
+    PG_REQUIRE(slurp(b + ".idx", &ibuf), PG_ERR_ARG, "Unable to read file: %s.idx", base);
+    const int64_t m = (int64_t)(abuf.size() / 4);
+    const bool wide = ibuf.size() == 8 * (size_t)n;
+    PG_REQUIRE(wide || ibuf.size() == 4 * (size_t)n, PG_ERR_ARG,
+               "File size wrong: %s.idx has %zu bytes for n = %lld", base, ibuf.size(), n);
+    std::vector<int64_t> off((size_t)n + 1);
+    for (long long i = 0; i < n; i++) {
+        if (wide) {
+            int64_t x;
+            std::memcpy(&x, ibuf.data() + 8 * i, 8);
+            off[i] = x;
+        } else {
+            int32_t x;
+            std::memcpy(&x, ibuf.data() + 4 * i, 4);
+            off[i] = x;
+        }
+    }
+    off[n] = m;
+    std::vector<int32_t> adj((size_t)m);
+    if (m > 0) std::memcpy(adj.data(), abuf.data(), 4 * (size_t)m);
+    const bool weighted = (flags & PG_GRAPH_WEIGHT_I32) != 0;
+    std::vector<int32_t> w(weighted ? (size_t)m : 0, 1);
+    return create_from_host(n, m, off.data(), adj.data(), weighted ? w.data() : nullptr,
+                            weighted ? PG_W_I32 : PG_W_NONE, flags & ~(uint32_t)PG_GRAPH_WEIGHT_I32,
+                            out);
+}
+
 extern "C" int pg_graph_destroy(pg_graph *g) {
     if (!g) return PG_OK;
     delete_graph(g);

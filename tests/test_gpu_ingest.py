@@ -1,0 +1,118 @@
+"""GPU parity of graph ingest: the binary graph triple <base>.config / .adj / .idx
+(readGraphFromBinary / readWghGraphFromBinary, IO-numa.h:543-720) through
+pg_graph_load_bin, against the same graph as PBBS text through pg_graph_load_adj,
+the oracle's binary reader and the textbook traversals."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "polymer_amd",
+                   "apps", "bin")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def run(app, args, dtype, count, timeout=120):
+    exe = os.path.join(BIN, app)
+    if not os.path.exists(exe):
+        pytest.fail("driver %s is not built (run __graft_entry__.build())" % exe)
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "out.bin")
+        cmd = [exe] + [str(x) for x in args] + ["-o", out]
+        say("run:", " ".join(cmd[1:]))
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+        say(p.stdout.strip())
+        assert p.returncode == 0, p.stderr
+        r = np.fromfile(out, dtype=dtype)
+    assert r.size == count
+    return r
+
+
+def test_load_bin_matches_text(pa, orc, tmp_path):
+    g = orc.rmat(12, 1 << 16, seed=21)
+    txt = str(tmp_path / "g.txt")
+    orc.write_adj(txt, g)
+    gt = pa.Graph.load_adj(txt)
+    ot, at, _ = gt.download()
+    for ib in (4, 8):
+        base = str(tmp_path / ("g%d" % ib))
+        orc.write_bin(base, g, ib)
+        say("load_bin: n %d m %d, %d-byte offsets" % (g.n, g.m, ib))
+        gb = pa.Graph.load_bin(base)
+        ob, ab, wb = gb.download()
+        assert wb is None
+        assert np.array_equal(ob, ot) and np.array_equal(ab, at)
+        h = orc.read_bin(base)
+        assert np.array_equal(ob, h.off) and np.array_equal(ab, h.adj)
+        for src in (0, 5):
+            lev, _, _ = gb.bfs(src)
+            assert np.array_equal(lev, orc.bfs_levels(h, src)), (ib, src)
+        gb.close()
+    gt.close()
+
+
+def test_load_bin_weighted_unit_weights(pa, orc, tmp_path):
+    """The weighted binary reader gives every edge weight 1 (IO-numa.h:683), so
+    Bellman-Ford distances equal BFS levels (unreached: INF_DIST and -1)."""
+    g = orc.rmat(11, 1 << 15, seed=23)
+    base = str(tmp_path / "w")
+    orc.write_bin(base, g, 8)
+    gw = pa.Graph.load_bin(base, weighted=True)
+    _, _, w = gw.download()
+    assert w is not None and w.dtype == np.int32 and np.all(w == 1)
+    lev = orc.bfs_levels(g, 0)
+    dist, _ = gw.sssp(0)
+    assert np.array_equal(np.where(dist == pa.INF_DIST, -1, dist), lev)
+    gw.close()
+
+
+def test_load_bin_symmetric_flag(pa, orc, tmp_path):
+    gs = orc.rmat(11, 1 << 15, seed=25).symmetrized()
+    base = str(tmp_path / "s")
+    orc.write_bin(base, gs, 4)
+    txt = str(tmp_path / "s.txt")
+    orc.write_adj(txt, gs)
+    a = pa.Graph.load_bin(base, symmetric=True)
+    b = pa.Graph.load_adj(txt, symmetric=True)
+    la, _ = a.cc()
+    lb, _ = b.cc()
+    assert np.array_equal(la, lb) and np.array_equal(la, orc.cc_unionfind(gs))
+    a.close()
+    b.close()
+
+
+def test_load_bin_errors(pa, orc, tmp_path):
+    g = orc.rmat(8, 1 << 10, seed=3)
+    base = str(tmp_path / "g")
+    orc.write_bin(base, g, 4)
+    with open(base + ".idx", "ab") as f:
+        f.write(b"\0\0")
+    with pytest.raises(RuntimeError, match="File size wrong"):
+        pa.Graph.load_bin(base)
+    with pytest.raises(RuntimeError, match="Unable to open"):
+        pa.Graph.load_bin(str(tmp_path / "missing"))
+    # an id beyond n is rejected by the graph build, not read out of bounds
+    base2 = str(tmp_path / "bad_id")
+    orc.write_bin(base2, g, 4)
+    adj = np.fromfile(base2 + ".adj", np.uint32)
+    adj[3] = g.n + 7
+    adj.tofile(base2 + ".adj")
+    with pytest.raises(RuntimeError, match="out of range"):
+        pa.Graph.load_bin(base2)
+
+
+def test_dropin_binary_flag(orc, tmp_path):
+    """The drivers' -b (the reference's flag for a binary file): Components on a
+    symmetric binary graph gives the union-find labels."""
+    gs = orc.rmat(12, 1 << 16, seed=27).symmetrized()
+    base = str(tmp_path / "cc")
+    orc.write_bin(base, gs, 4)
+    lab = run("components", ["-file", base, "-b"], np.int32, gs.n)
+    assert np.array_equal(lab, orc.cc_unionfind(gs))

@@ -409,3 +409,41 @@ def test_pbbs_rejects_bad_length(orc, tmp_path):
         f.write("AdjacencyGraph\n3\n2\n0\n1\n2\n1\n")  # one edge missing
     with pytest.raises(RuntimeError):
         orc.read_adj(p)
+
+
+def test_binary_graph_layout_and_round_trip(orc, tmp_path):
+    """or_write_bin / or_read_bin (IO-numa.h:543-635): the file layout the
+    reference reads (n as text; m 32-bit ids; n offsets of 4 or 8 bytes, no
+    closing m) and the same graph as the PBBS text reader gives."""
+    g = orc.rmat(10, 1 << 13, seed=9)
+    p = str(tmp_path / "g.adj")
+    orc.write_adj(p, g)
+    ht, _ = orc.read_adj(p)
+    for ib in (4, 8):
+        base = str(tmp_path / ("g%d" % ib))
+        orc.write_bin(base, g, ib)
+        assert open(base + ".config").read().split() == [str(g.n)]
+        assert os.path.getsize(base + ".adj") == 4 * g.m
+        assert os.path.getsize(base + ".idx") == ib * g.n
+        idx = np.fromfile(base + ".idx", np.int32 if ib == 4 else np.int64)
+        assert np.array_equal(idx, g.off[:-1])
+        assert np.array_equal(np.fromfile(base + ".adj", np.uint32), g.adj.astype(np.uint32))
+        h = orc.read_bin(base)
+        assert h.n == g.n and h.m == g.m
+        assert np.array_equal(h.off, ht.off) and np.array_equal(h.adj, ht.adj)
+    # a graph whose last vertices have no out-edges: off[n] = m closes the last list
+    g2 = orc.from_edges(5, [0, 0, 2], [1, 2, 0])
+    base = str(tmp_path / "tail")
+    orc.write_bin(base, g2, 4)
+    h = orc.read_bin(base)
+    assert list(h.off) == [0, 2, 2, 3, 3, 3] and list(h.adj) == [1, 2, 0]
+
+
+def test_binary_graph_rejects_bad_idx_size(orc, tmp_path):
+    g = orc.rmat(8, 1 << 10, seed=3)
+    base = str(tmp_path / "g")
+    orc.write_bin(base, g, 4)
+    with open(base + ".idx", "ab") as f:
+        f.write(b"\0\0")  # neither 4n nor 8n bytes ("File size wrong")
+    with pytest.raises(RuntimeError):
+        orc.read_bin(base)
