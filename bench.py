@@ -201,6 +201,24 @@ def value_extra(pa, g, scale=24, m=1 << 28):
     return out
 
 
+def delta_extra(pa, g, iters=20):
+    """PageRankDelta (fp64, epsilon2 0.01, 20 rounds) on the scale-24 PageRank graph:
+    time, frontier edges per second (the edges the reference pushes), and the
+    algorithmic bandwidth of the reference's formulation: 20 bytes per frontier
+    edge (the 4-byte id and an 8-byte read-modify-write of the destination's sum)
+    plus 40 bytes per vertex and round (the filter reads and writes p and the sum
+    and writes the next contribution)."""
+    g.pagerank_delta(iters, 0.85, 0.01, out=False)
+    _, st = g.pagerank_delta(iters, 0.85, 0.01, out=False)
+    gbs = (20 * st.edges + 40 * g.n * iters) / (st.ms / 1e3) / 1e9
+    return {"pagerank_delta_scale24_ms": round(st.ms, 3),
+            "pagerank_delta_frontier_gteps": round(st.edges / (st.ms / 1e3) / 1e9, 2),
+            "pagerank_delta_frontier_mean": round(st.active / max(iters, 1)),
+            "pagerank_delta_pull_rounds": st.dense_rounds,
+            "pagerank_delta_algorithmic_gbs": round(gbs, 1),
+            "pagerank_delta_hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def profile_run(args):
     """Short run for rocprofv3: a plain streaming kernel of known size (torch's
     elementwise negation of a 1 GiB fp32 tensor into another: exactly 1 GiB
@@ -421,7 +439,8 @@ def main():
     if not args.no_extra:
         for name, fn in (("bfs", bfs_extra), ("bfs24", bfs24_extra), ("traversal", traversal_extra),
                          ("multisource", multisource_extra),
-                         ("value", lambda p: value_extra(p, g)), ("batched", batched_extra),
+                         ("value", lambda p: value_extra(p, g)),
+                         ("delta", lambda p: delta_extra(p, g)), ("batched", batched_extra),
                          ("adjoint", adjoint_extra), ("multigraph", multigraph_extra),
                          ("dropin", dropin_extra)):
             t1 = time.perf_counter()

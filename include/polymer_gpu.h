@@ -177,6 +177,13 @@ int pg_subset_size(pg_subset *s, int64_t *count, int64_t *out_edges);
  * dangling redistribution (P2); fp32 values, fp32 accumulation. rank: host n. */
 int pg_pagerank(pg_graph *g, int iters, double damping, float *rank, uint32_t flags,
                 pg_stats *st);
+/* numa-PageRankDelta.C:63-330: p = 0, Delta = 1/n, all active; each round the
+ * active sources' Delta/outdeg is summed into every vertex, which then updates
+ * p and stays active iff |Delta| > epsilon2 * p (the reference uses damping
+ * 0.85, epsilon2 0.01).  fp64 throughout.  p: host n doubles.  st->active and
+ * st->edges sum the frontier sizes and out-degrees over the rounds. */
+int pg_pagerank_delta(pg_graph *g, int iters, double damping, double epsilon2, double *p,
+                      uint32_t flags, pg_stats *st);
 /* numa-SPMV.C:168-193: x_{k+1}[d] = sum_{s->d} x_k[s] w(s,d) over all
  * vertices; fp32 weights (w_dtype F32, or I32 converted), fp32 arithmetic.
  * Each iteration carries one power-of-two exponent shared by all vertices,

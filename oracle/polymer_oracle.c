@@ -430,6 +430,53 @@ int or_read_adj(const char *path, int skip_self_loops, int64_t *n_out,
 }
 
 /* ------------------------------------------------------------------ */
+/* PageRankDelta (numa-PageRankDelta.C): p = 0, Delta = 1/n and every     */
+/* vertex active at the start (:280-284); each round the active sources */
+/* push Delta[s]/outdeg(s) into nghSum (PR_F :63-85, edgeMap            */
+/* DENSE_FORWARD at threshold 0, :223); every vertex is then filtered   */
+/* (:226-230): round 1 PR_Vertex_F_FirstRound (:87-102)                  */
+/*   Delta = d (p + nghSum) + (1-d)/n - p;  p += Delta;  Delta -= 1/n    */
+/* later rounds PR_Vertex_F (:104-117)                                   */
+/*   Delta = d nghSum;  p += Delta                                       */
+/* and stays active iff |Delta| > epsilon2 p; nghSum is reset (:119-127). */
+/* ------------------------------------------------------------------ */
+
+int or_pagerank_delta(int64_t n, const int64_t *off, const int32_t *adj, int iters,
+                      double damping, double epsilon2, double *p, int64_t *active) {
+    double *delta = (double *)malloc(sizeof(double) * (size_t)n);
+    double *ngh = (double *)calloc((size_t)n, sizeof(double));
+    uint8_t *F = (uint8_t *)malloc((size_t)n);
+    if (!delta || !ngh || !F) { free(delta); free(ngh); free(F); return -1; }
+    const double one_over_n = 1.0 / (double)n;
+    const double addc = (1.0 - damping) * one_over_n;
+    for (int64_t i = 0; i < n; i++) { p[i] = 0.0; delta[i] = one_over_n; F[i] = 1; }
+    for (int t = 0; t < iters; t++) {
+        int64_t cnt = 0;
+        for (int64_t s = 0; s < n; s++) {
+            if (!F[s]) continue;
+            cnt++;
+            const int64_t deg = off[s + 1] - off[s];
+            for (int64_t e = off[s]; e < off[s + 1]; e++) ngh[adj[e]] += delta[s] / (double)deg;
+        }
+        if (active) active[t] = cnt;
+        for (int64_t i = 0; i < n; i++) {
+            if (t == 0) {
+                delta[i] = damping * (p[i] + ngh[i]) + addc - p[i];
+                p[i] += delta[i];
+                delta[i] -= one_over_n;
+            } else {
+                delta[i] = ngh[i] * damping;
+                p[i] += delta[i];
+            }
+            F[i] = fabs(delta[i]) > epsilon2 * p[i];
+        }
+        for (int64_t i = 0; i < n; i++) ngh[i] = 0.0;
+    }
+    free(delta); free(ngh); free(F);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* PageRank  (numa-PageRank.C:64-148 functors, :273-335 loop,           */
 /*            :177-230 edgeMapDenseForwardOTHER)                        */
 /* ------------------------------------------------------------------ */

@@ -22,6 +22,14 @@
  *                           plus the (1,1) invariant; parity unpinned against
  *                           the reference itself (it has no inputs/outputs).
  *   or_ppr / or_wpr_*    -- dense numpy solves (closed form).
+ *   or_pagerank_delta    -- epsilon2 = 0 reduces to the power iteration (numpy),
+ *                           a huge epsilon2 freezes p after round 1, and a
+ *                           hand-derived 3-vertex known answer (3 rounds, the
+ *                           frontier sizes 3, 2, 2).  The boundary of the
+ *                           activity test (">" against the updated p) is pinned
+ *                           by reading numa-PageRankDelta.C:98,114 only.
+ *   or_read_bin          -- the file layout the reference reads and the PBBS
+ *                           text reader on the same graph.
  */
 #ifndef POLYMER_ORACLE_H
 #define POLYMER_ORACLE_H
@@ -83,6 +91,10 @@ int or_read_bin(const char *base, int64_t *n, int64_t *m, int64_t *off, int32_t 
 int or_pagerank_push(int64_t n, const int64_t *off, const int32_t *adj,
                      int iters, double damping, double *rank_out,
                      int64_t *edges_per_iter);
+/* numa-PageRankDelta.C:63-127, 205-330: p (n), active[t] = frontier size
+ * entering round t (iters entries; may be NULL). */
+int or_pagerank_delta(int64_t n, const int64_t *off, const int32_t *adj, int iters,
+                      double damping, double epsilon2, double *p, int64_t *active);
 int or_pagerank_pull_omp(int64_t n, const int64_t *off, const int64_t *toff,
                          const int32_t *tadj, int iters, double damping,
                          int threads, double *rank_out);

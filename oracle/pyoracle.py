@@ -45,6 +45,8 @@ def lib():
                                            ctypes.POINTER(i64), ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_void_p]),
             "or_pagerank_push": (ctypes.c_int, [i64, I64P, I32P, ctypes.c_int, f64, F64P, I64P]),
+            "or_pagerank_delta": (ctypes.c_int, [i64, I64P, I32P, ctypes.c_int, f64, f64, F64P,
+                                                 I64P]),
             "or_pagerank_pull_omp": (ctypes.c_int, [i64, I64P, I64P, I32P, ctypes.c_int, f64,
                                                     ctypes.c_int, F64P]),
             "or_bfs_pull": (ctypes.c_int, [i64, I64P, I32P, i32, I32P, I32P, I32P]),
@@ -207,6 +209,16 @@ def pagerank(g, iters=20, damping=0.85):
     ept = np.zeros(max(iters, 1), np.int64)
     lib().or_pagerank_push(g.n, g.off, g.adj, iters, damping, out, ept)
     return out, ept[:iters]
+
+
+def pagerank_delta(g, iters=10, damping=0.85, epsilon2=0.01):
+    """or_pagerank_delta: (p, frontier size entering each round)."""
+    p = np.zeros(max(g.n, 1), np.float64)
+    act = np.zeros(max(iters, 1), np.int64)
+    r = lib().or_pagerank_delta(g.n, g.off, g.adj, iters, damping, epsilon2, p, act)
+    if r != 0:
+        raise RuntimeError("or_pagerank_delta failed: %d" % r)
+    return p[: g.n], act[:iters]
 
 
 def pagerank_pull_omp(g, iters=20, damping=0.85, threads=0):

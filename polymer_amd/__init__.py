@@ -40,7 +40,7 @@ EXPORTS = [
     "pg_subset_create", "pg_subset_destroy", "pg_subset_clear", "pg_subset_fill",
     "pg_subset_from_dense", "pg_subset_from_sparse", "pg_subset_to_dense",
     "pg_subset_to_sparse", "pg_subset_size",
-    "pg_pagerank", "pg_spmv", "pg_bp", "pg_bfs", "pg_cc", "pg_sssp",
+    "pg_pagerank", "pg_pagerank_delta", "pg_spmv", "pg_bp", "pg_bfs", "pg_cc", "pg_sssp",
     "pg_msbfs", "pg_mssssp", "pg_ppr_batched", "pg_ppr",
     "pg_graph_multi", "pg_pagerank_multigraph",
     "pg_wpr_forward", "pg_wpr_adjoint",
@@ -113,6 +113,7 @@ def load():
         "pg_subset_to_sparse": [P, P, i64, P],
         "pg_subset_size": [P, P, P],
         "pg_pagerank": [P, ctypes.c_int, f64, P, u32, SP],
+        "pg_pagerank_delta": [P, ctypes.c_int, f64, f64, P, u32, SP],
         "pg_spmv": [P, ctypes.c_int, P, P, u32, SP],
         "pg_bp": [P, ctypes.c_int, P, P, P, P, u32, SP],
         "pg_bfs": [P, i64, i64, P, P, u32, SP],
@@ -313,6 +314,14 @@ class Graph:
         _check(load().pg_pagerank(self._h, iters, damping, _ptr(rank), _flags(profile),
                                   ctypes.byref(st)), "pg_pagerank")
         return rank, st
+
+    def pagerank_delta(self, iters=10, damping=0.85, epsilon2=0.01, out=True, profile=False):
+        """numa-PageRankDelta.C: fp64 ranks (host n) and stats."""
+        p = np.zeros(self.n, np.float64) if out else None
+        st = Stats()
+        _check(load().pg_pagerank_delta(self._h, iters, damping, epsilon2, _ptr(p),
+                                        _flags(profile), ctypes.byref(st)), "pg_pagerank_delta")
+        return p, st
 
     def spmv(self, x0, iters=20, out=True, profile=False):
         x0 = np.ascontiguousarray(x0, np.float32)

@@ -112,6 +112,127 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// ---- PageRankDelta (numa-PageRankDelta.C) --------------------------------------------
+// The reference pushes Delta[s]/outdeg(s) from the active sources into nghSum (PR_F
+// :63-85, edgeMap DENSE_FORWARD at threshold 0, :223), then filters every vertex:
+//   round 1 (PR_Vertex_F_FirstRound :87-102): Delta = d (p + nghSum) + (1-d)/n - p;
+//            p += Delta;  Delta -= 1/n
+//   later (PR_Vertex_F :104-117):            Delta = d nghSum;  p += Delta
+// keeping a vertex active iff |Delta| > epsilon2 p, and resets nghSum (:119-127).
+// Each source's contribution c[s] = Delta[s]/outdeg(s) is kept, zero for an
+// inactive source.  A round whose frontier is large is a pull of every vertex's
+// in-neighbours' c (the same sums as the push) with the filter as the pull's
+// finish: one launch, nghSum never stored.  A round with a small frontier pushes
+// the nonzero c with fp64 atomics into nghSum and then filters every vertex.  fp64
+// throughout, as the reference.  Statistics of the next frontier: (count << 32) |
+// out-degree sum (each below 2^31).
+__device__ __forceinline__ unsigned long long prd_filter(int v, double sum, int first, double *p,
+                                                         double *cn, const int32_t *out_off,
+                                                         double damp, double addc,
+                                                         double one_over_n, double eps2) {
+#pragma clang fp contract(off)
+    const double pv = p[v];
+    double delta, pn;
+    if (first) {
+        delta = damp * (pv + sum) + addc - pv;
+        pn = pv + delta;
+        delta -= one_over_n;
+    } else {
+        delta = sum * damp;
+        pn = pv + delta;
+    }
+    p[v] = pn;
+    const bool act = fabs(delta) > eps2 * pn;
+    const int od = out_off[v + 1] - out_off[v];
+    cn[v] = (act && od > 0) ? delta / (double)od : 0.0;
+    return act ? ((1ull << 32) | (unsigned long long)od) : 0ull;
+}
+
+struct PrDeltaOp {
+    using Acc = double;
+    using Ctx = int;
+    using Stat = unsigned long long;
+    static constexpr bool kEarlyExit = false;
+    static constexpr bool kStats = true;
+    const double *c;
+    double *cn;
+    double *p;
+    const int32_t *out_off;
+    double damp, addc, one_over_n, eps2;
+    int first;
+    __device__ bool active(int) const { return true; }
+    __device__ int ctx(int v) const { return v; }
+    __device__ double identity() const { return 0.0; }
+    __device__ double combine(double a, double b) const { return a + b; }
+    __device__ double edge(int s, int, const int &) const { return c[s]; }
+    __device__ bool found(double, const int &) const { return false; }
+    __device__ Stat finish(int v, double sum, const int &) const {
+        return prd_filter(v, sum, first, p, cn, out_off, damp, addc, one_over_n, eps2);
+    }
+    __device__ void skip(int v) const { cn[v] = 0.0; }
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
+    __device__ static void stat_commit(Stat *p, Stat v) {
+        atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
+    }
+};
+
+// a push round: the listed sources' contributions added into nghSum (the
+// hardware fp64 add, not a compare-and-swap loop)
+struct PrDeltaPushOp {
+    const double *c;
+    double *ngh;
+    __device__ bool src_active(int) const { return true; }
+    __device__ void push(int s, int d, int) const { unsafeAtomicAdd(&ngh[d], c[s]); }
+};
+
+// the filter after a push round (a later round): every vertex, nghSum read and
+// cleared for the next round
+__global__ __launch_bounds__(kBlock) void k_prd_filter(int64_t n, double *ngh, double *p,
+                                                       double *cn, const int32_t *out_off,
+                                                       double damp, double eps2,
+                                                       unsigned long long *stats) {
+    __shared__ unsigned long long ss[kBlock / 64];
+    unsigned long long st = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const double sum = ngh[v];
+        ngh[v] = 0.0;
+        st += prd_filter((int)v, sum, 0, p, cn, out_off, damp, 0.0, 0.0, eps2);
+    }
+    for (int o = 32; o >= 1; o >>= 1) st += __shfl_xor(st, o, 64);
+    if (lane_id() == 0) ss[threadIdx.x >> 6] = st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kBlock / 64; i++) t += ss[i];
+        if (t) atomicAdd(&stats[blockIdx.x & (kStatSlots - 1)], t);
+    }
+}
+
+// bitmap of the sources with a nonzero contribution (a wave writes its two words)
+__global__ void k_prd_bits(const double *c, int64_t n, uint32_t *bits) {
+    const int lane = lane_id();
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = base + lane;
+        const unsigned long long m = __ballot(v < n && c[v] != 0.0);
+        if (lane == 0) bits[base >> 5] = (uint32_t)m;
+        if (lane == 32) bits[(base >> 5) + 1] = (uint32_t)(m >> 32);
+    }
+}
+
+// p = 0 and every vertex active with Delta = 1/n (numa-PageRankDelta.C:280-284)
+__global__ void k_prd_init(int64_t n, const int32_t *out_off, double one_over_n, double *p,
+                           double *c, double *ngh) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int od = out_off[v + 1] - out_off[v];
+        p[v] = 0.0;
+        c[v] = od > 0 ? one_over_n / (double)od : 0.0;
+        ngh[v] = 0.0;
+    }
+}
+
 struct PrBuffers {
     float *a, *b, *c0, *c1;
     uint8_t *f0, *f1;
@@ -1361,6 +1482,93 @@ extern "C" int pg_pagerank(pg_graph *g, int iters, double damping, float *rank, 
     float *res = nullptr;
     PG_TRY(pagerank_run(g, iters, damping, flags, st, &res));
     return to_input_order_f32(g, res, rank, 1);
+}
+
+// rounds whose frontier has fewer out-edges than m / kPrdPushDiv push
+constexpr int64_t kPrdPushDiv = 8;
+
+extern "C" int pg_pagerank_delta(pg_graph *g, int iters, double damping, double epsilon2,
+                                 double *p_out, uint32_t flags, pg_stats *st) {
+    PG_TRY(check_graph(g));
+    PG_REQUIRE(iters >= 0, PG_ERR_ARG, "iters < 0");
+    PG_REQUIRE(damping >= 0.0 && damping <= 1.0, PG_ERR_ARG, "damping out of [0,1]");
+    PG_REQUIRE(epsilon2 >= 0.0, PG_ERR_ARG, "epsilon2 < 0");
+    const int64_t n = g->n;
+    double *p = nullptr, *c = nullptr, *cn = nullptr, *ngh = nullptr;
+    unsigned long long *stats = nullptr;
+    PG_TRY(workspace_t(g, "prd_p", (size_t)n, &p));
+    PG_TRY(workspace_t(g, "prd_c0", (size_t)n, &c));
+    PG_TRY(workspace_t(g, "prd_c1", (size_t)n, &cn));
+    PG_TRY(workspace_t(g, "prd_ngh", (size_t)n, &ngh));
+    PG_TRY(workspace_t(g, "prd_stats", ((size_t)iters + 1) * kStatSlots, &stats));
+    Frontier A, B;
+    PG_TRY(frontiers(g, &A, &B));
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    const double one_over_n = 1.0 / (double)n;
+    PG_LAUNCH_K(k_prd_init, grid_for(n), n, g->out_off, one_over_n, p, c, ngh);
+    PG_LAUNCH_K(k_zero_u64, (unsigned)(((iters + 1) * kStatSlots + kBlock - 1) / kBlock), stats,
+                (iters + 1) * kStatSlots);
+    PrDeltaOp op{};
+    op.out_off = g->out_off;
+    op.damp = damping;
+    op.addc = (1.0 - damping) * one_over_n;
+    op.one_over_n = one_over_n;
+    op.eps2 = epsilon2;
+    // round 1 runs on every vertex; round t+1 on the frontier round t left
+    int64_t cnt = n, edges = g->m, tot_active = 0, tot_edges = 0;
+    int dense_rounds = 0;
+    for (int t = 0; t < iters; t++) {
+        tot_active += cnt;
+        tot_edges += edges;
+        unsigned long long *st_t = &stats[(size_t)t * kStatSlots];
+        PG_TRY(kt.begin());
+        if (t == 0 || edges >= g->m / kPrdPushDiv) {
+            dense_rounds++;
+            op.c = c;
+            op.cn = cn;
+            op.p = p;
+            op.first = t == 0 ? 1 : 0;
+            PG_TRY(launch_pull(op, g->in_sched, st_t));
+        } else {
+            // the sources with a nonzero contribution, as an ordered list
+            PG_TRY(clear_frontier(g, A, false));
+            PG_LAUNCH_K(k_prd_bits, grid_for(n, 4096), c, n, A.bits);
+            PG_TRY(bits_to_list(g, A.bits, A.list, &A.ctr[0]));
+            int64_t lc = 0, le = 0;
+            PG_TRY(frontier_stats(g, A, &lc, &le));
+            PG_TRY(launch_push_global(PrDeltaPushOp{c, ngh}, g, g->out_off, g->out_adj, A.list, lc,
+                                      le));
+            PG_LAUNCH_K(k_prd_filter, grid_for(n, 4096), n, ngh, p, cn, g->out_off, damping,
+                        epsilon2, st_t);
+        }
+        PG_TRY(kt.end());
+        std::swap(c, cn);
+        if (t + 1 < iters) {  // the next frontier's size and out-degree sum
+            unsigned long long hs[kStatSlots];
+            PG_CHECK_HIP(hipMemcpyAsync(hs, st_t, sizeof(hs), hipMemcpyDeviceToHost, stream()));
+            PG_CHECK_HIP(hipStreamSynchronize(stream()));
+            unsigned long long sum = 0;
+            for (int k = 0; k < kStatSlots; k++) sum += hs[k];
+            cnt = (int64_t)(sum >> 32);
+            edges = (int64_t)(sum & 0xFFFFFFFFull);
+        }
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = iters;
+        st->dense_rounds = dense_rounds;
+        st->active = tot_active;
+        st->edges = tot_edges;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    // fp64 values travel as pairs of 32-bit words
+    return to_input_order_f32(g, reinterpret_cast<const float *>(p), reinterpret_cast<float *>(p_out),
+                              2);
 }
 
 __global__ void k_spmv_copy(const float *a, int64_t n, float *b) {

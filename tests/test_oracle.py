@@ -447,3 +447,55 @@ def test_binary_graph_rejects_bad_idx_size(orc, tmp_path):
         f.write(b"\0\0")  # neither 4n nor 8n bytes ("File size wrong")
     with pytest.raises(RuntimeError):
         orc.read_bin(base)
+
+
+def _power_iteration(g, iters, d):
+    """p_0 = 1/n; p_{t+1} = d * sum over in-edges of p_t[s] / outdeg(s) + (1-d)/n."""
+    n = g.n
+    deg = np.diff(g.off)
+    src = np.repeat(np.arange(n), deg)
+    p = np.full(n, 1.0 / n)
+    for _ in range(iters):
+        c = np.where(deg > 0, p / np.maximum(deg, 1), 0.0)
+        p = d * np.bincount(g.adj, weights=c[src], minlength=n) + (1.0 - d) / n
+    return p
+
+
+def test_pagerank_delta_without_threshold_is_the_power_iteration(orc):
+    """epsilon2 = 0 keeps every vertex with a nonzero delta active, so the delta
+    recurrence telescopes to p_t of the power iteration (round 1's -1/n turns p_1
+    into the first delta p_1 - p_0)."""
+    for seed, sym in ((31, False), (32, True)):
+        g = orc.rmat(10, 1 << 13, seed=seed)
+        if sym:
+            g = g.symmetrized()
+        for iters in (1, 2, 7):
+            p, act = orc.pagerank_delta(g, iters, 0.85, 0.0)
+            ref = _power_iteration(g, iters, 0.85)
+            assert np.abs(p - ref).sum() / ref.sum() < 1e-12, (seed, iters)
+            assert act[0] == g.n
+
+
+def test_pagerank_delta_large_threshold_freezes_after_round_one(orc):
+    g = orc.rmat(10, 1 << 13, seed=33)
+    p1, _ = orc.pagerank_delta(g, 1, 0.85, 1e9)
+    p5, act = orc.pagerank_delta(g, 5, 0.85, 1e9)
+    assert np.array_equal(p1, p5) and list(act) == [g.n, 0, 0, 0, 0]
+    ref = _power_iteration(g, 1, 0.85)
+    assert np.abs(p1 - ref).max() < 1e-15
+
+
+def test_pagerank_delta_known_answer(orc):
+    """Edges 0->1, 0->2, 1->2, 2->0, d = 0.85, epsilon2 = 0.01, derived by hand:
+    round 1 gives p = (1/3, d/6 + 0.05, d/2 + 0.05) with deltas (0, -d/6, d/6), so
+    1 and 2 stay active; round 2 moves d^2/6 from 2 to 0 and -d^2/6 from 1 to 2
+    (0 and 2 active, 1 not); round 3 moves -d^3/6 to 0 and d^3/12 to 1 and 2."""
+    g = orc.from_edges(3, [0, 0, 1, 2], [1, 2, 2, 0])
+    d = 0.85
+    p, act = orc.pagerank_delta(g, 3, d, 0.01)
+    want = [1 / 3 + d * d / 6 - d ** 3 / 6,
+            d / 6 + 0.05 + d ** 3 / 12,
+            d / 2 + 0.05 - d * d / 6 + d ** 3 / 12]
+    assert np.allclose(p, want, rtol=0, atol=1e-15), (p, want)
+    assert list(act) == [3, 2, 2]
+    assert abs(p.sum() - 1.0) < 1e-15
