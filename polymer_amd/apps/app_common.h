@@ -1,4 +1,4 @@
-// app_common.h -- command line and output helpers shared by the six driver
+// app_common.h -- command line and output helpers shared by the driver
 // programs written against polymer.h (the edgeMap/vertexMap interface).
 //
 // Graph selection (one of):

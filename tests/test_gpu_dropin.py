@@ -1,11 +1,12 @@
-"""GPU parity of the six driver programs written against include/polymer.h (the
+"""GPU parity of the driver programs written against include/polymer.h (the
 edgeMap / vertexMap interface: graph<vertex>, vertices, LocalFrontier, functors
 with update / updateAtomic / cond run as device code).
 
 Each driver runs as its own process on the GPU and writes its result array;
 the checks are the ones the C-ABI paths meet:
   * PageRank, SpMV (fp64 values, fp64 atomics): the reference binary's PR1 and
-    integer-weight SpMV goldens and the fp64 oracle;
+    integer-weight SpMV goldens and the fp64 oracle; PageRankDelta: the fp64
+    oracle;
   * BFS levels, CC labels, Bellman-Ford distances: bit-exact against the
     textbook BFS, union-find and Dijkstra; BFS parents checked for validity;
   * BP (fp64, atomic products): the fp64 oracle on the entries it keeps in range.
@@ -35,6 +36,9 @@ def rel_l1(a, b):
     return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
 
 
+LAST_STDOUT = [""]
+
+
 def run(app, args, dtype, count, timeout=120):
     exe = os.path.join(BIN, app)
     if not os.path.exists(exe):
@@ -45,6 +49,7 @@ def run(app, args, dtype, count, timeout=120):
         say("run:", " ".join(cmd[1:]))
         p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         say(p.stdout.strip())
+        LAST_STDOUT[0] = p.stdout
         assert p.returncode == 0, p.stderr
         r = np.fromfile(out, dtype=dtype)
     assert r.size == count
@@ -72,6 +77,24 @@ def test_dropin_pagerank(orc):
     assert rel_l1(r, z["rank"]) < 1e-8  # golden printed with 10 significant digits
     assert rel_l1(r, ref) < 1e-12
     assert np.array_equal(r == 0, ref == 0)
+
+
+@pytest.mark.parametrize("thr", [-1, 1 << 14])
+def test_dropin_pagerank_delta(orc, thr):
+    """PageRankDelta on the header: edgeMap DENSE_FORWARD + vertexFilter with the
+    reference's threshold 0 (every round dense), and with a threshold that sends
+    the late, small frontiers through the sparse push."""
+    g = orc.rmat(14, 1 << 18, seed=41)
+    args = ["-rmat", 14, 1 << 18, 41, "-rounds", 15] + (["-thr", thr] if thr > 0 else [])
+    r = run("pagerank_delta", args, np.float64, g.n)
+    sparse = int(LAST_STDOUT[0].split("sparse")[1].split()[0])
+    ref, act = orc.pagerank_delta(g, 15, 0.85, 0.01)
+    front = int(LAST_STDOUT[0].split("frontier")[1].split()[0])
+    say("pagerank_delta thr %d: rel L1 %.3e, frontier %d (oracle %d), sparse rounds %d" %
+        (thr, rel_l1(r, ref), front, int(act.sum()), sparse))
+    assert rel_l1(r, ref) < 1e-9
+    assert front == int(act.sum())
+    assert (sparse > 0) == (thr > 0)
 
 
 @pytest.mark.parametrize("thr", [-1, 0])
