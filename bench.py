@@ -108,6 +108,25 @@ def bfs24_extra(pa, scale=24, m=1 << 28, nsrc=8):
             "bfs_scale24_sources": used}
 
 
+def async_extra(pa, scale=24, m=1 << 28):
+    """The asynchronous traversal (numa-BFS-async-pipe.C) on the symmetrised
+    scale-24 rMat from the sources of bfs24_runs: time per source and reached
+    edges per second (its result is a spanning tree of the reachable set, so the
+    comparison with the breadth-first search is per reached edge)."""
+    gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True)
+    srcs = [s for s, _ in bfs24_runs(gs, 4)]
+    gs.bfs_async(srcs[0], out=False)
+    tot_ms, tot_edges = 0.0, 0
+    for s in srcs:
+        _, vis, st = gs.bfs_async(s, out=False)
+        _, _, stb = gs.bfs(s, out=False)
+        tot_ms += st.ms
+        tot_edges += stb.edges
+    gs.close()
+    return {"async_scale24_ms_per_source": round(tot_ms / len(srcs), 3),
+            "async_scale24_gteps_graph500": round(tot_edges / 2 / (tot_ms / 1e3) / 1e9, 2)}
+
+
 def traversal_extra(pa, scale=22, m=1 << 26):
     """Connected components on the symmetrised scale-22 graph; Bellman-Ford (int32
     weights U[1,255]) from vertex 0 (the max-degree vertex) and 8 seeded sources."""
@@ -437,7 +456,8 @@ def main():
     host_ms = (time.perf_counter() - t1) * 1e3
     extra = {"pagerank_step_with_host_copy_ms": round(host_ms, 3)}
     if not args.no_extra:
-        for name, fn in (("bfs", bfs_extra), ("bfs24", bfs24_extra), ("traversal", traversal_extra),
+        for name, fn in (("bfs", bfs_extra), ("bfs24", bfs24_extra), ("async", async_extra),
+                         ("traversal", traversal_extra),
                          ("multisource", multisource_extra),
                          ("value", lambda p: value_extra(p, g)),
                          ("delta", lambda p: delta_extra(p, g)), ("batched", batched_extra),

@@ -207,6 +207,13 @@ int pg_bp(pg_graph *g, int iters, const float *vpot, const float *epot, const fl
  * level/parent: host n (level -1 / parent -1 when unreached). */
 int pg_bfs(pg_graph *g, int64_t src, int64_t threshold, int32_t *level, int32_t *parent,
            uint32_t flags, pg_stats *st);
+/* numa-BFS-async-pipe.C:140-175 with edgeMapSparseAsyncPipe (polymer.h:1512-1751):
+ * one asynchronous traversal to quiescence from src; every reachable vertex is
+ * claimed once with a CAS of its parent from -1 by an already claimed in-neighbour
+ * (a spanning tree of the reachable set, not in general a breadth-first one).
+ * parent: host n (-1 unreached); visited: the number of reached vertices. */
+int pg_bfs_async(pg_graph *g, int64_t src, int32_t *parent, int64_t *visited, uint32_t flags,
+                 pg_stats *st);
 /* numa-Components.C:233-281: min-label propagation, labels = min input id
  * of the component (symmetric graph).  threshold < 0 -> m/8 (:226). */
 int pg_cc(pg_graph *g, int64_t threshold, int32_t *label, uint32_t flags, pg_stats *st);

@@ -40,7 +40,8 @@ EXPORTS = [
     "pg_subset_create", "pg_subset_destroy", "pg_subset_clear", "pg_subset_fill",
     "pg_subset_from_dense", "pg_subset_from_sparse", "pg_subset_to_dense",
     "pg_subset_to_sparse", "pg_subset_size",
-    "pg_pagerank", "pg_pagerank_delta", "pg_spmv", "pg_bp", "pg_bfs", "pg_cc", "pg_sssp",
+    "pg_pagerank", "pg_pagerank_delta", "pg_spmv", "pg_bp", "pg_bfs", "pg_bfs_async", "pg_cc",
+    "pg_sssp",
     "pg_msbfs", "pg_mssssp", "pg_ppr_batched", "pg_ppr",
     "pg_graph_multi", "pg_pagerank_multigraph",
     "pg_wpr_forward", "pg_wpr_adjoint",
@@ -117,6 +118,7 @@ def load():
         "pg_spmv": [P, ctypes.c_int, P, P, u32, SP],
         "pg_bp": [P, ctypes.c_int, P, P, P, P, u32, SP],
         "pg_bfs": [P, i64, i64, P, P, u32, SP],
+        "pg_bfs_async": [P, i64, P, P, u32, SP],
         "pg_cc": [P, i64, P, u32, SP],
         "pg_sssp": [P, i64, i64, P, u32, SP],
         "pg_msbfs": [P, P, ctypes.c_int, P, u32, SP],
@@ -350,6 +352,16 @@ class Graph:
         _check(load().pg_bfs(self._h, src, threshold, _ptr(level), _ptr(parent),
                              _flags(profile), ctypes.byref(st)), "pg_bfs")
         return level, parent, st
+
+    def bfs_async(self, src, out=True, profile=False):
+        """numa-BFS-async-pipe.C: (parents of a spanning tree of the reachable set,
+        reached count, stats)."""
+        parent = np.zeros(self.n, np.int32) if out else None
+        vis = ctypes.c_int64()
+        st = Stats()
+        _check(load().pg_bfs_async(self._h, src, _ptr(parent), ctypes.byref(vis), _flags(profile),
+                                   ctypes.byref(st)), "pg_bfs_async")
+        return parent, vis.value, st
 
     def cc(self, threshold=-1, out=True, profile=False):
         label = np.zeros(self.n, np.int32) if out else None

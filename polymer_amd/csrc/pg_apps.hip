@@ -1786,6 +1786,210 @@ extern "C" int pg_bp(pg_graph *g, int iters, const float *vpot, const float *epo
     return bp_run<double>(g, iters, pot, vp, p0, f0, f1, prod, flags, st);
 }
 
+// ---- asynchronous traversal (edgeMapSparseAsyncPipe, polymer.h:1512-1751;
+// numa-BFS-async-pipe.C:140-175) -------------------------------------------------------
+// The reference runs one edgeMap call to quiescence: workers take chunks of queued
+// vertices, claim each unvisited out-neighbour with a CAS of its parent from -1
+// (BFS_F), queue the claimed ones in chunks for the next worker, and stop when
+// every queue is empty and no chunk is in flight.  The parents form a tree of the
+// vertices reachable from the source (not, in general, a breadth-first one).
+//
+// Here one persistent grid works on a single queue of work items: a claimed vertex
+// of out-degree D is queued as ceil(D / kAsyncItemEdges) items (vertex, chunk), so
+// a hub's out-edges are spread over many waves instead of one wave's serial walk.
+// tail counts the queued items, processed those whose edges have all been claimed
+// or found visited, reached the claimed vertices.  A wave takes kAsyncChunk
+// consecutive slots and, until each of its lanes' slots is settled, expands
+// whichever of them are filled (so a wave never waits on work it holds); a slot is
+// settled empty once processed == tail (nothing is being expanded and nothing is
+// queued: the traversal is over) and it lies at or past the tail, or when it lies
+// past the queue's capacity (n + m / kAsyncItemEdges + 1 bounds every item
+// count).  A wave's wait is bounded: kAsyncPolls polls in a row without any item
+// processed anywhere (the count it reads unchanged), after which it sets the
+// error word and leaves, so a launch always ends.
+constexpr int kAsyncChunk = 64;
+constexpr unsigned kAsyncPolls = 1u << 20;
+constexpr int kAsyncItemEdges = 1024;
+constexpr int kAsyncLaneDeg = 32;  // an item's edges its lane expands alone; more: the wave
+constexpr unsigned long long kAsyncEmpty = ~0ull;
+// control words, each on its own 64-byte line
+constexpr int kAsyncHead = 0, kAsyncTail = 16, kAsyncProcessed = 32, kAsyncError = 48,
+              kAsyncReached = 56, kAsyncCtlWords = 64;
+
+__device__ __forceinline__ unsigned long long async_item(int v, int k) {
+    return ((unsigned long long)(unsigned)v << 32) | (unsigned)k;
+}
+
+// claim d for v (want: this lane has an edge); winners queue their items
+__device__ __forceinline__ void async_claim(int v, int d, int32_t *parent,
+                                            const int32_t *__restrict__ out_off,
+                                            unsigned long long *q, unsigned *ctl, bool want) {
+    bool won = false;
+    if (want && __hip_atomic_load(&parent[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == -1)
+        won = atomicCAS(&parent[d], -1, v) == -1;
+    const unsigned long long wmask = __ballot(won);
+    if (!wmask) return;
+    const int lane = lane_id();
+    int cnt = 0;
+    if (won) cnt = (out_off[d + 1] - out_off[d] + kAsyncItemEdges - 1) / kAsyncItemEdges;
+    int x = cnt;  // inclusive scan of the item counts over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    const int total = __shfl(x, 63, 64);
+    unsigned base = 0;
+    if (lane == 0) {
+        atomicAdd(ctl + kAsyncReached, (unsigned)__popcll(wmask));
+        if (total) base = atomicAdd(ctl + kAsyncTail, (unsigned)total);
+    }
+    base = __shfl(base, 0, 64);
+    for (int j = 0; j < cnt; j++)
+        __hip_atomic_store(&q[base + (unsigned)(x - cnt + j)], async_item(d, j), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfs_async(const int32_t *__restrict__ out_off,
+                                                      const int32_t *__restrict__ out_adj,
+                                                      int32_t *parent, unsigned long long *q,
+                                                      int64_t cap, unsigned *ctl) {
+    unsigned *head = ctl + kAsyncHead, *tail = ctl + kAsyncTail;
+    unsigned *processed = ctl + kAsyncProcessed, *error = ctl + kAsyncError;
+    const int lane = lane_id();
+    while (true) {
+        unsigned i0 = 0;
+        if (lane == 0) i0 = atomicAdd(head, (unsigned)kAsyncChunk);
+        i0 = __shfl(i0, 0, 64);
+        if ((int64_t)i0 >= cap) return;  // every slot of the chunk lies past the queue
+        const int64_t i = (int64_t)i0 + lane;
+        bool pending = i < cap;
+        bool over = false;
+        unsigned polls = 0, seen = ~0u;
+        while (__ballot(pending)) {
+            unsigned long long it = kAsyncEmpty;
+            if (pending)
+                it = __hip_atomic_load(&q[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const bool ready = it != kAsyncEmpty;
+            const unsigned long long rmask = __ballot(ready);
+            if (rmask) {
+                // expand the filled items: short ones by their own lane, longer ones
+                // by the whole wave, one at a time
+                int v = 0, beg = 0, end = 0;
+                if (ready) {
+                    v = (int)(it >> 32);
+                    const int k = (int)(it & 0xFFFFFFFFull);
+                    const int o0 = out_off[v], o1 = out_off[v + 1];
+                    beg = o0 + k * kAsyncItemEdges;
+                    end = min(o1, beg + kAsyncItemEdges);
+                }
+                const bool light = ready && end - beg <= kAsyncLaneDeg;
+                for (int k = 0; __ballot(light && beg + k < end); k++) {
+                    const bool act = light && beg + k < end;
+                    async_claim(v, act ? out_adj[beg + k] : 0, parent, out_off, q, ctl, act);
+                }
+                unsigned long long heavy = __ballot(ready && !light);
+                while (heavy) {
+                    const int l = __ffsll((long long)heavy) - 1;
+                    heavy &= heavy - 1;
+                    const int hv = __shfl(v, l, 64);
+                    const int hb = __shfl(beg, l, 64), he = __shfl(end, l, 64);
+                    for (int e0 = hb; e0 < he; e0 += 64) {
+                        const bool act = e0 + lane < he;
+                        async_claim(hv, act ? out_adj[e0 + lane] : 0, parent, out_off, q, ctl, act);
+                    }
+                }
+                // the items this wave queued are visible before it counts its own
+                // as processed
+                __threadfence();
+                if (lane == 0) atomicAdd(processed, (unsigned)__popcll(rmask));
+                if (ready) pending = false;
+                polls = 0;
+                continue;
+            }
+            // nothing filled: settled empty once the traversal is over
+            unsigned pr = 0, tl = 0;
+            if (lane == 0) {
+                pr = __hip_atomic_load(processed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                tl = __hip_atomic_load(tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            pr = __shfl(pr, 0, 64);
+            tl = __shfl(tl, 0, 64);
+            if (pr != seen) {  // progress elsewhere: the wait is not stuck
+                seen = pr;
+                polls = 0;
+            }
+            if (pr == tl) {
+                over = true;
+                if (pending && i >= (int64_t)tl) pending = false;
+            }
+            if (++polls > kAsyncPolls) {
+                if (lane == 0) atomicOr(error, 1u);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (over) return;  // the traversal has ended: no slot will be filled again
+    }
+}
+
+// the source claimed by itself, its items queued
+__global__ void k_async_seed(int32_t src, const int32_t *out_off, int32_t *parent,
+                             unsigned long long *q, unsigned *ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        parent[src] = src;
+        const int cnt = (out_off[src + 1] - out_off[src] + kAsyncItemEdges - 1) / kAsyncItemEdges;
+        for (int j = 0; j < cnt; j++) q[j] = async_item(src, j);
+        ctl[kAsyncTail] = (unsigned)cnt;
+        ctl[kAsyncReached] = 1u;
+    }
+}
+
+extern "C" int pg_bfs_async(pg_graph *g, int64_t src, int32_t *parent, int64_t *visited,
+                            uint32_t flags, pg_stats *st) {
+    PG_TRY(check_graph(g));
+    // queue capacity: every vertex's items, ceil(D / R) <= D / R + 1
+    const int64_t cap = g->n + g->m / kAsyncItemEdges + 1;
+    PG_REQUIRE(cap < 0x7FFFFFFF - (int64_t)kAsyncChunk * 65536, PG_ERR_UNSUPPORTED,
+               "graph too large for the async queue");
+    int32_t s_int;
+    PG_TRY(lookup_internal(g, src, &s_int));
+    int32_t *par = nullptr;
+    unsigned long long *q = nullptr;
+    unsigned *ctl = nullptr;
+    PG_TRY(workspace_t(g, "async_parent", (size_t)g->n, &par));
+    PG_TRY(workspace_t(g, "async_queue", (size_t)cap, &q));
+    PG_TRY(workspace_t(g, "async_ctl", (size_t)kAsyncCtlWords, &ctl));
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    PG_TRY(fill(par, -1, g->n));
+    PG_TRY(fill(q, kAsyncEmpty, cap));
+    PG_TRY(fill(ctl, 0u, kAsyncCtlWords));
+    PG_LAUNCH_K(k_async_seed, 1, s_int, g->out_off, par, q, ctl);
+    PG_TRY(kt.begin());
+    PG_LAUNCH_K(k_bfs_async, 2048, g->out_off, g->out_adj, par, q, cap, ctl);
+    PG_TRY(kt.end());
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    unsigned h[kAsyncCtlWords];
+    PG_CHECK_HIP(hipMemcpy(h, ctl, sizeof(h), hipMemcpyDeviceToHost));
+    PG_REQUIRE(h[kAsyncError] == 0, PG_ERR_STATE, "async traversal: a wave's wait ran out");
+    PG_REQUIRE(h[kAsyncProcessed] == h[kAsyncTail], PG_ERR_STATE,
+               "async traversal ended with %u of %u queued items processed", h[kAsyncProcessed],
+               h[kAsyncTail]);
+    if (visited) *visited = (int64_t)h[kAsyncReached];
+    if (st) {
+        st->rounds = 1;
+        st->dense_rounds = 0;
+        st->active = (int64_t)h[kAsyncReached];
+        st->edges = 0;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    return to_input_order_i32(g, par, parent, true);
+}
+
 extern "C" int pg_bfs(pg_graph *g, int64_t src, int64_t threshold, int32_t *level,
                       int32_t *parent, uint32_t flags, pg_stats *st) {
     PG_TRY(check_graph(g));
