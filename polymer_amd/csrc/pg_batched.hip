@@ -383,6 +383,15 @@ __global__ void k_ppr_init(int64_t n, const int32_t *out_off, int32_t src, float
 }
 
 // ---- multi-source BFS: 64-bit masks ---------------------------------------------------------
+// Bit c of F[v] / vis[v]: v is on source c's frontier / has been reached from source
+// c.  A round whose frontier has many out-edges pulls: every vertex not reached from
+// every source ORs the masks of its in-neighbours that are on some frontier (Fany,
+// n/8 bytes, screens the 8-byte gathers), stopping once all bits are set.  A round
+// with few out-edges pushes: the frontier vertices (listed from Fany) OR their masks
+// into their out-neighbours' next masks (returnless 64-bit ORs), then
+// k_msbfs_settle keeps the bits not yet visited.  Either way a new bit of v records
+// the round as v's level for that source.  Statistics of the next frontier:
+// (count << 32) | out-degree sum.
 struct MsBfsOp {
     using Acc = unsigned long long;
     using Ctx = unsigned long long;  // visited mask of the vertex
@@ -390,9 +399,11 @@ struct MsBfsOp {
     static constexpr bool kEarlyExit = true;
     static constexpr bool kStats = true;
     const unsigned long long *F;
+    const uint32_t *Fany;
     unsigned long long *Fn;
     unsigned long long *vis;
     int32_t *level;  // [k][n]
+    const int32_t *out_off;
     int64_t n;
     unsigned long long full;
     int32_t round;
@@ -400,21 +411,20 @@ struct MsBfsOp {
     __device__ unsigned long long ctx(int v) const { return vis[v]; }
     __device__ Acc identity() const { return 0ull; }
     __device__ Acc combine(Acc a, Acc b) const { return a | b; }
-    __device__ Acc edge(int s, int, const Ctx &) const { return F[s]; }
+    __device__ Acc edge(int s, int, const Ctx &) const { return bit_test(Fany, s) ? F[s] : 0ull; }
     __device__ bool found(Acc a, const Ctx &c) const { return (a | c) == full; }
     __device__ Stat finish(int v, Acc a, const Ctx &c) const {
         const unsigned long long nw = a & ~c & full;
         Fn[v] = nw;
-        if (nw) {
-            vis[v] = c | nw;
-            unsigned long long x = nw;
-            while (x) {
-                const int b = __ffsll((long long)x) - 1;
-                x &= x - 1;
-                level[(int64_t)b * n + v] = round;
-            }
+        if (!nw) return 0ull;
+        vis[v] = c | nw;
+        unsigned long long x = nw;
+        while (x) {
+            const int b = __ffsll((long long)x) - 1;
+            x &= x - 1;
+            level[(int64_t)b * n + v] = round;
         }
-        return nw ? 1ull : 0ull;
+        return (1ull << 32) | (unsigned long long)(out_off[v + 1] - out_off[v]);
     }
     __device__ void skip(int v) const { Fn[v] = 0ull; }
     __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
@@ -422,6 +432,51 @@ struct MsBfsOp {
         atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
     }
 };
+
+struct MsBfsPushOp {
+    const unsigned long long *F;
+    const unsigned long long *vis;
+    unsigned long long *Fn;
+    __device__ bool src_active(int) const { return true; }
+    __device__ void push(int s, int d, int) const {
+        const unsigned long long f = F[s] & ~vis[d];
+        if (f) atomicOr(&Fn[d], f);
+    }
+};
+
+// after a push round: the bits each vertex received that it had not visited
+__global__ __launch_bounds__(kBlock) void k_msbfs_settle(int64_t n, unsigned long long *Fn,
+                                                         unsigned long long *vis, int32_t *level,
+                                                         int32_t round, const int32_t *out_off,
+                                                         unsigned long long *stats) {
+    __shared__ unsigned long long ss[kBlock / 64];
+    unsigned long long st = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long r = Fn[v];
+        if (!r) continue;
+        const unsigned long long c = vis[v];
+        const unsigned long long nw = r & ~c;
+        Fn[v] = nw;
+        if (!nw) continue;
+        vis[v] = c | nw;
+        unsigned long long x = nw;
+        while (x) {
+            const int b = __ffsll((long long)x) - 1;
+            x &= x - 1;
+            level[(int64_t)b * n + v] = round;
+        }
+        st += (1ull << 32) | (unsigned long long)(out_off[v + 1] - out_off[v]);
+    }
+    for (int o = 32; o >= 1; o >>= 1) st += __shfl_xor(st, o, 64);
+    if (lane_id() == 0) ss[threadIdx.x >> 6] = st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kBlock / 64; i++) t += ss[i];
+        if (t) atomicAdd(&stats[blockIdx.x & (kStatSlots - 1)], t);
+    }
+}
 
 __global__ void k_msbfs_seed(const int32_t *src, int k, int64_t n, unsigned long long *F,
                              unsigned long long *vis, int32_t *level) {
@@ -752,20 +807,30 @@ extern "C" int pg_ppr_batched(pg_graph *g, const int64_t *src, int k, int iters,
     return rows_download(g, R, k, rank);
 }
 
+// rounds whose frontier has fewer out-edges than m / kMsPushDiv push (measured on
+// S22 with 64 sources: 4.32 ms at m/2 against 4.53 at m/4 to m/32)
+constexpr int64_t kMsPushDiv = 2;
+constexpr int kMsCtrWords = 4 + 2 * kStatSlots;
+
 extern "C" int pg_msbfs(pg_graph *g, const int64_t *src, int k, int32_t *level, uint32_t flags,
                         pg_stats *st) {
     PG_TRY(check_g(g));
     std::vector<int32_t> s;
     PG_TRY(map_sources(g, src, k, &s));
     const int64_t n = g->n;
-    unsigned long long *F0 = nullptr, *F1 = nullptr, *vis = nullptr, *cnt = nullptr;
-    int32_t *lev = nullptr, *dsrc = nullptr;
+    unsigned long long *F0 = nullptr, *F1 = nullptr, *vis = nullptr, *cnt = nullptr,
+                       *lctr = nullptr;
+    int32_t *lev = nullptr, *dsrc = nullptr, *list = nullptr;
+    uint32_t *Fany = nullptr;
     PG_TRY(workspace_t(g, "msbfs_f0", (size_t)n, &F0));
     PG_TRY(workspace_t(g, "msbfs_f1", (size_t)n, &F1));
     PG_TRY(workspace_t(g, "msbfs_vis", (size_t)n, &vis));
     PG_TRY(workspace_t(g, "msbfs_cnt", (size_t)kStatSlots, &cnt));
     PG_TRY(workspace_t(g, "msbfs_level", (size_t)n * k, &lev));
     PG_TRY(workspace_t(g, "msbfs_src", (size_t)k, &dsrc));
+    PG_TRY(workspace_t(g, "msbfs_any", (size_t)(n + 31) / 32 + 1, &Fany));
+    PG_TRY(workspace_t(g, "msbfs_list", (size_t)n + 1, &list));
+    PG_TRY(workspace_t(g, "msbfs_lctr", (size_t)kMsCtrWords, &lctr));
     PG_CHECK_HIP(hipMemcpyAsync(dsrc, s.data(), 4 * k, hipMemcpyHostToDevice, stream()));
     const unsigned long long full = (k == 64) ? ~0ull : ((1ull << k) - 1ull);
     KernelTimer kt;
@@ -777,29 +842,51 @@ extern "C" int pg_msbfs(pg_graph *g, const int64_t *src, int k, int32_t *level, 
     PG_TRY(fill(lev, -1, n * k));
     PG_LAUNCH_B(k_msbfs_seed, 1, dsrc, k, n, F0, vis, lev);
     unsigned long long *F = F0, *Fn = F1;
-    int rounds = 0;
-    int64_t active = k;
+    int rounds = 0, dense_rounds = 0;
+    int64_t active = k, fedges = 0, pushed = 0;  // the first frontier (the sources) pushes
     while (true) {
         rounds++;
         PG_TRY(fill(cnt, 0ull, kStatSlots));
-        MsBfsOp op{F, Fn, vis, lev, n, full, rounds};
         PG_TRY(kt.begin());
-        PG_TRY(launch_pull(op, g->in_sched, cnt));
+        PG_LAUNCH_B(k_mask_any_bits, grid_for((n + 31) / 32), F, n, Fany);
+        if (rounds > 1 && fedges >= g->m / kMsPushDiv) {
+            dense_rounds++;
+            MsBfsOp op{F, Fany, Fn, vis, lev, g->out_off, n, full, rounds};
+            PG_TRY(launch_pull(op, g->in_sched, cnt));
+        } else {
+            // the frontier as a list, its size and out-degree sum
+            PG_TRY(fill(lctr, 0ull, kMsCtrWords));
+            PG_TRY(bits_to_list(g, Fany, list, &lctr[0]));
+            PG_TRY(list_degree_sum(g, list, &lctr[0], g->out_off, lctr + 5, 2));
+            unsigned long long h[kMsCtrWords];
+            PG_CHECK_HIP(hipMemcpyAsync(h, lctr, sizeof(h), hipMemcpyDeviceToHost, stream()));
+            PG_CHECK_HIP(hipStreamSynchronize(stream()));
+            int64_t lc = (int64_t)h[0], le = 0;
+            for (int j = 0; j < kStatSlots; j++) le += (int64_t)h[5 + 2 * j];
+            pushed += le;
+            PG_TRY(fill(Fn, 0ull, n));
+            PG_TRY(launch_push_global(MsBfsPushOp{F, vis, Fn}, g, g->out_off, g->out_adj, list, lc,
+                                      le));
+            PG_LAUNCH_B(k_msbfs_settle, grid_for(n, 4096), n, Fn, vis, lev, rounds, g->out_off,
+                        cnt);
+        }
         PG_TRY(kt.end());
         unsigned long long c = 0;
         PG_TRY(sum_slots(cnt, &c));
         std::swap(F, Fn);
-        if (c == 0) break;
-        active += (int64_t)c;
+        const int64_t nc = (int64_t)(c >> 32);
+        fedges = (int64_t)(c & 0xFFFFFFFFull);
+        if (nc == 0) break;
+        active += nc;
         if (rounds > n + 2) break;
     }
     double ms = 0;
     PG_TRY(tm.stop(&ms));
     if (st) {
         st->rounds = rounds;
-        st->dense_rounds = rounds;
+        st->dense_rounds = dense_rounds;
         st->active = active;
-        st->edges = (int64_t)rounds * g->m;
+        st->edges = (int64_t)dense_rounds * g->m + pushed;
         st->ms = ms;
         PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
     }

@@ -74,11 +74,15 @@ def test_msbfs(pa, orc):
     L, st = d.msbfs(src)
     for c in range(64):
         assert np.array_equal(L[c], orc.bfs_levels(gs, int(src[c]))), c
-    # fewer sources, directed graph
+    say("msbfs rounds %d, pull rounds %d" % (st.rounds, st.dense_rounds))
+    assert 0 < st.dense_rounds < st.rounds  # small frontiers push, large ones pull
+    # fewer sources, directed graph, one source without out-edges
     dd = pa.Graph.rmat(14, 1 << 18, seed=41)
-    L, _ = dd.msbfs(src[:7])
+    iso = int(np.nonzero(np.diff(g.off) == 0)[0][0])
+    s7 = np.array(list(src[:6]) + [iso])
+    L, _ = dd.msbfs(s7)
     for c in range(7):
-        assert np.array_equal(L[c], orc.bfs_levels(g, int(src[c]))), c
+        assert np.array_equal(L[c], orc.bfs_levels(g, int(s7[c]))), c
 
 
 def test_mssssp(pa, orc):
