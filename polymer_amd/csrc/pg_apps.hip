@@ -529,9 +529,11 @@ struct BpOp {
     __device__ Acc combine(Acc a, Acc b) const {
         return Acc{a.p0 * b.p0, a.p1 * b.p1, a.any | b.any, 0};
     }
+    // the potential is loaded before the frontier test so that its (streamed,
+    // coalesced) load is not held behind the bitmap lookup
     __device__ Acc edge(int s, int e, const Ctx &q) const {
-        if (!bit_test(Fb, s)) return identity();
         const float4 w = pot[e];
+        if (!bit_test(Fb, s)) return identity();
         return Acc{(T)w.x * q.x + (T)w.y * q.y, (T)w.z * q.x + (T)w.w * q.y, 1, 0};
     }
     __device__ bool found(Acc, const Ctx &) const { return false; }
