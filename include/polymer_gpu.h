@@ -104,6 +104,12 @@ int pg_graph_load_adj(const char *path, uint32_t flags, pg_graph **out);
  * is an error).  PG_GRAPH_WEIGHT_I32: unit weights, as the reference's weighted
  * reader gives (IO-numa.h:683). */
 int pg_graph_load_bin(const char *base, uint32_t flags, pg_graph **out);
+/* The whole-graph dump of loadGraphFromBin / dumpGraphToBin (IO-numa.h:758-900):
+ * int64 total size, int32 n, int64 m, then per vertex int32 -1, id, out-degree,
+ * in-degree, out-neighbours, in-neighbours.  Loading checks the markers, ids,
+ * totals and recorded in-degrees; dumping writes the graph in input ids. */
+int pg_graph_load_dump(const char *path, uint32_t flags, pg_graph **out);
+int pg_graph_dump(pg_graph *g, const char *path);
 int pg_graph_destroy(pg_graph *g);
 /* n, m (directed entries), max in-degree, has-weights (0 none, 1 i32, 2 f32) */
 int pg_graph_info(const pg_graph *g, int64_t *n, int64_t *m, int64_t *max_in_degree,

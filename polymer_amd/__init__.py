@@ -33,7 +33,7 @@ INF_DIST = 1073741823
 EXPORTS = [
     "pg_init", "pg_finalize", "pg_last_error", "pg_device_name",
     "pg_graph_create", "pg_graph_rmat", "pg_graph_load_adj", "pg_graph_load_bin",
-    "pg_graph_destroy",
+    "pg_graph_load_dump", "pg_graph_dump", "pg_graph_destroy",
     "pg_graph_info", "pg_graph_download", "pg_graph_set_schedule", "pg_graph_set_tiles",
     "pg_graph_get_schedule", "pg_graph_tile_stats", "pg_graph_autotune",
     "pg_graph_bp_potentials", "pg_graph_view",
@@ -95,6 +95,8 @@ def load():
         "pg_graph_rmat": [ctypes.c_int, i64, f64, f64, f64, u64, u32, u64, PP],
         "pg_graph_load_adj": [ctypes.c_char_p, u32, PP],
         "pg_graph_load_bin": [ctypes.c_char_p, u32, PP],
+        "pg_graph_load_dump": [ctypes.c_char_p, u32, PP],
+        "pg_graph_dump": [P, ctypes.c_char_p],
         "pg_graph_destroy": [P],
         "pg_graph_info": [P, P, P, P, P],
         "pg_graph_download": [P, P, P, P],
@@ -234,6 +236,18 @@ class Graph:
         _check(load().pg_graph_load_bin(base.encode(), flags, ctypes.byref(h)),
                "pg_graph_load_bin")
         return cls(h)
+
+    @classmethod
+    def load_dump(cls, path, symmetric=False):
+        """The whole-graph dump of loadGraphFromBin (IO-numa.h:758-835)."""
+        h = ctypes.c_void_p()
+        _check(load().pg_graph_load_dump(path.encode(), PG_GRAPH_SYMMETRIC if symmetric else 0,
+                                         ctypes.byref(h)), "pg_graph_load_dump")
+        return cls(h)
+
+    def dump(self, path):
+        """Writes the graph as dumpGraphToBin does (IO-numa.h:837-900)."""
+        _check(load().pg_graph_dump(self._h, path.encode()), "pg_graph_dump")
 
     @classmethod
     def multi(cls, vbase, off, adj):

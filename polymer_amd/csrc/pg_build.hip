@@ -1613,6 +1613,111 @@ extern "C" int pg_graph_load_bin(const char *base, uint32_t flags, pg_graph **ou
                             out);
 }
 
+// The whole-graph dump of dumpGraphToBin / loadGraphFromBin (IO-numa.h:758-900,
+// read by numa-PageRank-bin.C): int64 total size, int32 n, int64 m (20 bytes, no
+// padding), then per vertex int32 -1, int32 its id, int32 out-degree, int32
+// in-degree, the out-neighbours and the in-neighbours.  The out-lists define the
+// graph; the markers, ids, the totals and every vertex's recorded in-degree are
+// checked against them (the reference aborts on a marker or id mismatch).
+extern "C" int pg_graph_load_dump(const char *path, uint32_t flags, pg_graph **out) {
+    PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(path && out, PG_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::ifstream f(path, std::ios::in | std::ios::binary | std::ios::ate);
+    PG_REQUIRE(f.is_open(), PG_ERR_ARG, "Unable to open file: %s", path);
+    const int64_t size = (int64_t)f.tellg();
+    f.seekg(0);
+    std::vector<char> buf((size_t)size);
+    PG_REQUIRE(size == 0 || (bool)f.read(buf.data(), size), PG_ERR_ARG, "Unable to read %s", path);
+    int64_t pos = 0;
+    auto get = [&](void *dst, int64_t bytes) -> bool {
+        if (pos + bytes > size) return false;
+        std::memcpy(dst, buf.data() + pos, (size_t)bytes);
+        pos += bytes;
+        return true;
+    };
+    int64_t total = 0, m = 0;
+    int32_t n = 0;
+    PG_REQUIRE(get(&total, 8) && get(&n, 4) && get(&m, 8), PG_ERR_ARG, "Bad dump file: %s", path);
+    PG_REQUIRE(total == size && n > 0 && m >= 0 && m < 0x7FFFFFFF, PG_ERR_ARG,
+               "Bad dump file %s: size %lld, recorded %lld, n %d, m %lld", path, (long long)size,
+               (long long)total, n, (long long)m);
+    std::vector<int64_t> off((size_t)n + 1, 0);
+    std::vector<int32_t> adj((size_t)m), indeg((size_t)n);
+    int64_t e = 0, in_total = 0;
+    for (int32_t i = 0; i < n; i++) {
+        int32_t hdr[4];
+        PG_REQUIRE(get(hdr, 16), PG_ERR_ARG, "Bad dump file %s: vertex %d truncated", path, i);
+        PG_REQUIRE(hdr[0] == -1 && hdr[1] == i, PG_ERR_ARG,
+                   "Bad dump file %s: vertex %d has marker %d and id %d", path, i, hdr[0], hdr[1]);
+        PG_REQUIRE(hdr[2] >= 0 && hdr[3] >= 0 && e + hdr[2] <= m, PG_ERR_ARG,
+                   "Bad dump file %s: vertex %d degrees %d / %d", path, i, hdr[2], hdr[3]);
+        PG_REQUIRE(get(adj.data() + e, 4 * (int64_t)hdr[2]), PG_ERR_ARG,
+                   "Bad dump file %s: vertex %d truncated", path, i);
+        e += hdr[2];
+        off[i + 1] = e;
+        indeg[i] = hdr[3];
+        in_total += hdr[3];
+        pos += 4 * (int64_t)hdr[3];  // the in-lists: the transpose, rebuilt on the device
+        PG_REQUIRE(pos <= size, PG_ERR_ARG, "Bad dump file %s: vertex %d truncated", path, i);
+    }
+    PG_REQUIRE(e == m && in_total == m && pos == size, PG_ERR_ARG,
+               "Bad dump file %s: %lld out-edges and %lld in-edges for m = %lld", path,
+               (long long)e, (long long)in_total, (long long)m);
+    std::vector<int32_t> count((size_t)n, 0);
+    for (int64_t k = 0; k < m; k++) {
+        PG_REQUIRE(adj[k] >= 0 && adj[k] < n, PG_ERR_ARG, "Bad dump file %s: neighbour %d out of range",
+                   path, adj[k]);
+        count[adj[k]]++;
+    }
+    for (int32_t i = 0; i < n; i++)
+        PG_REQUIRE(count[i] == indeg[i], PG_ERR_ARG,
+                   "Bad dump file %s: vertex %d records in-degree %d, its in-edges number %d", path,
+                   i, indeg[i], count[i]);
+    return create_from_host(n, m, off.data(), adj.data(), nullptr, PG_W_NONE, flags, out);
+}
+
+extern "C" int pg_graph_dump(pg_graph *g, const char *path) {
+    PG_REQUIRE(g && path, PG_ERR_ARG, "null argument");
+    const int64_t n = g->n, m = g->m;
+    std::vector<int64_t> off((size_t)n + 1);
+    std::vector<int32_t> adj((size_t)std::max<int64_t>(m, 1));
+    PG_TRY(pg_graph_download(g, off.data(), adj.data(), nullptr));
+    // in-lists by a counting sort of the out-edges on their destination
+    std::vector<int64_t> ioff((size_t)n + 1, 0);
+    for (int64_t k = 0; k < m; k++) ioff[(size_t)adj[k] + 1]++;
+    for (int64_t i = 0; i < n; i++) ioff[i + 1] += ioff[i];
+    std::vector<int32_t> iadj((size_t)std::max<int64_t>(m, 1));
+    {
+        std::vector<int64_t> fillp(ioff.begin(), ioff.end() - 1);
+        for (int64_t v = 0; v < n; v++)
+            for (int64_t k = off[v]; k < off[v + 1]; k++) iadj[(size_t)fillp[adj[k]]++] = (int32_t)v;
+    }
+    const int64_t total = 20 + 16 * n + 8 * m;
+    std::vector<char> buf((size_t)total);
+    char *p = buf.data();
+    auto put = [&](const void *src, size_t bytes) {
+        std::memcpy(p, src, bytes);
+        p += bytes;
+    };
+    const int32_t n32 = (int32_t)n;
+    put(&total, 8);
+    put(&n32, 4);
+    put(&m, 8);
+    for (int64_t i = 0; i < n; i++) {
+        const int32_t hdr[4] = {-1, (int32_t)i, (int32_t)(off[i + 1] - off[i]),
+                                (int32_t)(ioff[i + 1] - ioff[i])};
+        put(hdr, 16);
+        put(adj.data() + off[i], 4 * (size_t)(off[i + 1] - off[i]));
+        put(iadj.data() + ioff[i], 4 * (size_t)(ioff[i + 1] - ioff[i]));
+    }
+    std::ofstream o(path, std::ios::out | std::ios::binary | std::ios::trunc);
+    PG_REQUIRE(o.is_open(), PG_ERR_ARG, "Unable to write %s", path);
+    o.write(buf.data(), total);
+    PG_REQUIRE((bool)o, PG_ERR_ARG, "Unable to write %s", path);
+    return PG_OK;
+}
+
 extern "C" int pg_graph_destroy(pg_graph *g) {
     if (!g) return PG_OK;
     delete_graph(g);

@@ -116,3 +116,69 @@ def test_dropin_binary_flag(orc, tmp_path):
     orc.write_bin(base, gs, 4)
     lab = run("components", ["-file", base, "-b"], np.int32, gs.n)
     assert np.array_equal(lab, orc.cc_unionfind(gs))
+
+
+def _dump_bytes(g):
+    """The dumpGraphToBin layout (IO-numa.h:837-900) written field by field:
+    int64 total, int32 n, int64 m, then per vertex -1, id, out-degree, in-degree,
+    out-neighbours, in-neighbours (in-lists in ascending source order)."""
+    n, m = g.n, g.m
+    deg = np.diff(g.off)
+    src_of = np.repeat(np.arange(n, dtype=np.int32), deg)
+    order = np.lexsort((src_of, g.adj))
+    in_src = src_of[order]
+    ioff = np.zeros(n + 1, np.int64)
+    np.add.at(ioff, g.adj.astype(np.int64) + 1, 1)
+    ioff = np.cumsum(ioff)
+    total = 20 + 16 * n + 8 * m
+    parts = [np.array([total], "<i8").tobytes(), np.array([n], "<i4").tobytes(),
+             np.array([m], "<i8").tobytes()]
+    for v in range(n):
+        parts.append(np.array([-1, v, deg[v], ioff[v + 1] - ioff[v]], "<i4").tobytes())
+        parts.append(g.adj[g.off[v]:g.off[v + 1]].astype("<i4").tobytes())
+        parts.append(in_src[ioff[v]:ioff[v + 1]].astype("<i4").tobytes())
+    b = b"".join(parts)
+    assert len(b) == total
+    return b
+
+
+def test_load_dump_and_dump(pa, orc, tmp_path):
+    g = orc.rmat(11, 1 << 14, seed=29)
+    p = str(tmp_path / "g.bin")
+    want = _dump_bytes(g)
+    with open(p, "wb") as f:
+        f.write(want)
+    gd = pa.Graph.load_dump(p)
+    gc = pa.Graph.from_csr(g.n, g.off, g.adj)
+    od, ad, _ = gd.download()
+    oc, ac, _ = gc.download()
+    assert np.array_equal(od, oc) and np.array_equal(ad, ac)
+    lev, _, _ = gd.bfs(3)
+    assert np.array_equal(lev, orc.bfs_levels(g, 3))
+    p2 = str(tmp_path / "g2.bin")
+    gc.dump(p2)
+    with open(p2, "rb") as f:
+        got = f.read()
+    assert got == want  # the same bytes as the reference's layout
+    gd.close()
+    gc.close()
+
+
+def test_load_dump_rejects_corruption(pa, orc, tmp_path):
+    g = orc.rmat(8, 1 << 10, seed=31)
+    good = bytearray(_dump_bytes(g))
+    cases = {}
+    b = bytearray(good)
+    b[20:24] = np.array([-2], "<i4").tobytes()  # vertex 0's marker
+    cases["marker"] = b
+    b = bytearray(good)
+    b[32:36] = np.array([int(np.frombuffer(good[32:36], "<i4")[0]) + 1], "<i4").tobytes()
+    cases["in-degree"] = b  # vertex 0's recorded in-degree off by one (sizes then disagree)
+    cases["truncated"] = good[:-4]
+    for what, data in cases.items():
+        p = str(tmp_path / ("bad_" + what.replace("-", "")))
+        with open(p, "wb") as f:
+            f.write(bytes(data))
+        with pytest.raises(RuntimeError, match="Bad dump file"):
+            pa.Graph.load_dump(p)
+        say("dump corruption rejected:", what)
