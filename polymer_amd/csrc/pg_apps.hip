@@ -281,6 +281,8 @@ int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, fl
     // a built set is used
     const bool tiled = g->use_tiles && g->tiles.built && g->tiles.light_begin == g->in_sched.light_begin &&
                        g->tiles.light_end == g->in_sched.light_end;
+    // the next iteration's cold pre-pass runs beside this one's finishing pass
+    const bool ahead = tiled && g->tiles.cold_overlap && g->tiles.cold.nparts > 0;
     for (int t = 0; t < iters; t++) {
         op.F = F;
         op.Fn = Fn;
@@ -288,10 +290,13 @@ int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, fl
         op.contrib_n = cn;
         op.cur = cur;
         op.nxt = nxt;
+        PrOp next = op;  // its cold pre-pass reads the contributions written now
+        next.contrib = cn;
+        const bool pre = ahead && t + 1 < iters;
         if (kt) PG_TRY(kt->begin());
         if (tiled)  // the zero-in-degree vertices are settled after two iterations
             PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, &B.stats[(size_t)t * kStatSlots],
-                                     t >= 2));
+                                     t >= 2, pre ? &next : nullptr, ahead && t > 0));
         else
             PG_TRY(launch_pull(op, g->in_sched, &B.stats[(size_t)t * kStatSlots]));
         if (kt) PG_TRY(kt->end());
@@ -1630,15 +1635,20 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
     Timer tm;
     PG_TRY(tm.start());
     float *cur = xa, *nxt = xb;
+    // the next iteration's cold pre-pass (raw values, scaled later in the tiles) runs
+    // beside this one's finishing pass
+    const bool ahead = tiled && g->tiles.cold_overlap && g->tiles.cold.nparts > 0;
     for (int t = 0; t < iters; t++) {
         // iteration t scales its output by 2^-shift[t+1]; shift[t+1] comes from max(x_t)
         SpmvOp op{cur, w, nxt, shift + t + 1, tscale ? tscale + t : nullptr};
+        const SpmvOp next{nxt, w, cur, shift + t + 2, tscale ? tscale + t + 1 : nullptr};
         PG_TRY(kt.begin());
         if (tiled) {
             hipLaunchKernelGGL(k_spmv_tscale, dim3(1), dim3(64), 0, stream(),
                                t == 0 ? x0max : maxbits + t, wmax, tscale + t);
             // the zero-in-degree tail holds zeros in both buffers from iteration 2
-            PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, maxbits + t + 1, t >= 2));
+            PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, maxbits + t + 1, t >= 2,
+                                     ahead && t + 1 < iters ? &next : nullptr, ahead && t > 0));
         } else {
             PG_TRY(launch_pull(op, g->in_sched, maxbits + t + 1));
         }

@@ -28,6 +28,7 @@ namespace pg {
 
 static thread_local char g_err[1024] = "";
 static hipStream_t g_stream = nullptr;
+static hipStream_t g_side = nullptr;  // work overlapped with the library stream's
 static int g_device = -1;
 static bool g_init = false;
 
@@ -39,6 +40,7 @@ void set_error(const char *fmt, ...) {
 }
 
 hipStream_t stream() { return g_stream; }
+hipStream_t side_stream() { return g_side; }
 bool initialised() { return g_init; }
 
 // ---- temporary device buffers (graph build only) ------------------------------
@@ -828,6 +830,8 @@ void free_tiles(pg_graph *g) {
     if (t.d_cold_srcl) (void)hipFree(t.d_cold_srcl);
     if (t.d_cold_val) (void)hipFree(t.d_cold_val);
     if (t.d_wt) (void)hipFree(t.d_wt);
+    if (t.ev_tiles) (void)hipEventDestroy(t.ev_tiles);
+    if (t.ev_cold) (void)hipEventDestroy(t.ev_cold);
     t = TileSet{};
 }
 
@@ -1140,6 +1144,15 @@ int ensure_tiles(pg_graph *g) {
     const bool cold_split = mt > 0 && g->tile_cold >= 0 && g->tile_cold < g->n;
     if (cold_split) PG_TRY(build_cold(g, vbeg, off_h, parts));
     if (mt > 0) PG_TRY(pack_tiles(g, mt, cold_split ? (int32_t)g->tile_cold : (int32_t)g->n));
+    // the next iteration's cold pre-pass may overlap this one's finish pass when no
+    // vertex the finish pass writes is a cold source
+    t.cold_overlap = cold_split && t.cold.nparts > 0;
+    for (const auto &pt : parts)
+        if (pt.slot >= 0 && vbeg[(size_t)pt.tile + 1] > g->tile_cold) t.cold_overlap = false;
+    if (t.cold_overlap) {
+        PG_CHECK_HIP(hipEventCreateWithFlags(&t.ev_tiles, hipEventDisableTiming));
+        PG_CHECK_HIP(hipEventCreateWithFlags(&t.ev_cold, hipEventDisableTiming));
+    }
     if (!parts.empty())
         PG_CHECK_HIP(hipMemcpy(t.d_parts, parts.data(), sizeof(TilePart) * parts.size(),
                                hipMemcpyHostToDevice));
@@ -1455,6 +1468,7 @@ extern "C" int pg_init(int device) {
                device, count);
     PG_CHECK_HIP(hipSetDevice(device));
     PG_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    PG_CHECK_HIP(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
     g_device = device;
     g_init = true;
     return PG_OK;
@@ -1463,8 +1477,11 @@ extern "C" int pg_init(int device) {
 extern "C" int pg_finalize(void) {
     if (!g_init) return PG_OK;
     PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_side));
     PG_CHECK_HIP(hipStreamDestroy(g_stream));
+    PG_CHECK_HIP(hipStreamDestroy(g_side));
     g_stream = nullptr;
+    g_side = nullptr;
     g_init = false;
     g_device = -1;
     return PG_OK;

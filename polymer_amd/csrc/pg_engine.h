@@ -639,18 +639,32 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
     }
 }
 
-// Pull over the CSC with the light range taken from the tiles: the hot schedule
-// (multi-block and block-per-vertex segments, zero-degree tail), the tile parts,
-// then the vertices of the tiles with several parts.
+// Pull over the CSC with the light range taken from the tiles: the cold pre-pass,
+// the hot schedule (multi-block and block-per-vertex segments, zero-degree tail),
+// the tile parts, then the vertices of the tiles with several parts.
 // skip_tail: leave out the zero-in-degree vertices, for an Op whose results
 // there are already in place (PageRank from its third iteration: those vertices
 // left the frontier after the first, and both double buffers hold their zeros).
+// ahead: the next iteration's op.  When given and the tile set allows it (every
+// multi-part tile lies below the first cold source, so the finishing pass writes
+// no value the cold pre-pass reads), the next iteration's cold pre-pass is launched
+// on the side stream as soon as this iteration's tile parts are done, beside the
+// finishing pass; the next call, with cold_ready, waits for it instead of running
+// its own.  (The cold value array is read only by the tile parts, which are done.)
 template <class Op>
 inline int launch_pull_tiled(const Op &op, const PullSchedule &s, const TileSet &t,
-                             typename Op::Stat *stat_out = nullptr, bool skip_tail = false) {
-    if (t.cold.nparts > 0)
+                             typename Op::Stat *stat_out = nullptr, bool skip_tail = false,
+                             const Op *ahead = nullptr, bool cold_ready = false) {
+    if (cold_ready) {
+        hipError_t e = hipStreamWaitEvent(stream(), t.ev_cold, 0);
+        if (e != hipSuccess) {
+            set_error("tiled pull: wait for the cold pre-pass failed: %s", hipGetErrorString(e));
+            return PG_ERR_HIP;
+        }
+    } else if (t.cold.nparts > 0) {
         hipLaunchKernelGGL(k_cold_gather<Op>, dim3(t.cold.nparts), dim3(kTileThreads), 0, stream(),
                            op, t.cold);
+    }
     const int hot_blocks = skip_tail ? s.nblocks_hot_body : s.nblocks_hot;
     if (hot_blocks > 0)
         hipLaunchKernelGGL(k_pull<Op>, dim3(hot_blocks), dim3(kBlock), 0, stream(), op,
@@ -658,6 +672,19 @@ inline int launch_pull_tiled(const Op &op, const PullSchedule &s, const TileSet 
     if (t.args.nparts > 0)
         hipLaunchKernelGGL(k_pull_tiles<Op>, dim3(t.args.nparts), dim3(kTileThreads), 0, stream(),
                            op, t.args, stat_out);
+    if (ahead && t.cold_overlap && t.cold.nparts > 0) {
+        hipError_t e = hipEventRecord(t.ev_tiles, stream());
+        if (e == hipSuccess) e = hipStreamWaitEvent(side_stream(), t.ev_tiles, 0);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(k_cold_gather<Op>, dim3(t.cold.nparts), dim3(kTileThreads), 0,
+                               side_stream(), *ahead, t.cold);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(t.ev_cold, side_stream());
+        if (e != hipSuccess) {
+            set_error("tiled pull: cold pre-pass ahead failed: %s", hipGetErrorString(e));
+            return PG_ERR_HIP;
+        }
+    }
     if (t.args.nfin > 0)
         hipLaunchKernelGGL(k_tile_finish<Op>, dim3(t.args.nfin), dim3(kBlock), 0, stream(), op,
                            t.args, stat_out);

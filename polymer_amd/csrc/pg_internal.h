@@ -42,6 +42,7 @@ void set_error(const char *fmt, ...);
     } while (0)
 
 hipStream_t stream();
+hipStream_t side_stream();  // a second stream (default priority) for overlapped work
 bool initialised();
 
 // ---- pull schedule ----------------------------------------------------------
@@ -191,6 +192,10 @@ struct TileSet {
     uint16_t *d_cold_srcl = nullptr;
     float *d_cold_val = nullptr;
     int64_t m_cold = 0;
+    // every multi-part tile lies below the first cold source: the next iteration's
+    // cold pre-pass can run beside this iteration's finish pass (two events order it)
+    bool cold_overlap = false;
+    hipEvent_t ev_tiles = nullptr, ev_cold = nullptr;
 };
 
 struct PullSchedule {
