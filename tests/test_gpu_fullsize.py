@@ -1,9 +1,10 @@
-"""The benchmark's full sizes, checked through what does not depend on size:
-PageRank on the scale-24 rMat (2^28 edges) against the oracle's fp64 pull for a
-few iterations; BFS on the symmetrised scale-24 graph by the properties that
-define breadth-first levels and parents; CC labels on the symmetrised scale-22
-graph by the properties that define minimum-id labels; and the 64-source BFS on
-scale 22 against 64 single-source searches (two independent GPU paths)."""
+"""The benchmark's full sizes: PageRank and SpMV on the scale-24 rMat (2^28
+edges) against the oracle's fp64 iterations for a few steps; Bellman-Ford on the
+symmetrised scale-22 graph against the oracle's Dijkstra, bit-exact; and, through
+what does not depend on size, BFS on the symmetrised scale-24 graph by the
+properties that define breadth-first levels and parents, CC labels on scale 22 by
+the properties that define minimum-id labels, and the 64-source BFS on scale 22
+against 64 single-source searches (two independent GPU paths)."""
 import os
 import sys
 
@@ -100,4 +101,30 @@ def test_msbfs_scale22_matches_single_searches(pa):
         lv, _, _ = gs.bfs(int(src[c]))
         assert np.array_equal(L[c], lv), c
     say("  %d rounds, %d pull rounds" % (st.rounds, st.dense_rounds))
+    gs.close()
+
+
+def test_spmv_scale24_vs_fp64(pa, orc):
+    say("case: spmv scale 24, fp32 values, 2 iterations against the fp64 oracle")
+    g = pa.Graph.rmat(24, 1 << 28, seed=1, weights="f32", wseed=3)
+    x0 = np.random.default_rng(3).random(g.n).astype(np.float32)
+    y, _ = g.spmv(x0, 2)
+    off, adj, w = g.download()
+    ref = orc.spmv(orc.Graph(g.n, off, adj), w.astype(np.float64), x0.astype(np.float64), 2)
+    e = rel_l1(y, ref)
+    say("  rel L1 %.3e" % e)
+    assert e < 1e-6
+    g.close()
+
+
+def test_sssp_scale22_vs_dijkstra(pa, orc):
+    say("case: bellman-ford symmetrised scale 22, int32 weights, against Dijkstra")
+    gs = pa.Graph.rmat(22, 1 << 26, seed=1, symmetrize=True, weights="i32", wseed=7)
+    off, adj, w = gs.download()
+    hg = orc.Graph(gs.n, off, adj)
+    for src in (0, 1234567):
+        d, st = gs.sssp(src)
+        ref = orc.dijkstra(hg, w, src)
+        say("  source %d: %d rounds, %d reached" % (src, st.rounds, int((d < pa.INF_DIST).sum())))
+        assert np.array_equal(d, ref), src
     gs.close()
