@@ -346,7 +346,13 @@ inline void *mapDataArray(int numOfShards, intT *sizeArr, int sizeOfOneEle) {
     for (int i = 0; i < numOfShards; i++) total += (size_t)sizeArr[i];
     void *p = nullptr;
     polymer_detail::check_hip(hipMalloc(&p, (total ? total : 1) * (size_t)sizeOfOneEle), "hipMalloc");
-    polymer_detail::check_hip(hipMemset(p, 0, (total ? total : 1) * (size_t)sizeOfOneEle), "hipMemset");
+    // zeroed on the library stream, where every later use of the array is ordered (a
+    // null-stream memset is not ordered with that non-blocking stream and could land
+    // after the app's first writes)
+    polymer_detail::check_hip(hipMemsetAsync(p, 0, (total ? total : 1) * (size_t)sizeOfOneEle,
+                                             polymer_detail::lib_stream()),
+                              "hipMemsetAsync");
+    polymer_detail::check_hip(hipStreamSynchronize(polymer_detail::lib_stream()), "sync");
     return p;
 }
 

@@ -104,7 +104,8 @@ static int row_plan(pg_graph *g, RowPlan *P, const std::string &name = "rowplan"
         if (!items.empty())
             PG_CHECK_HIP(hipMemcpy(P->items, items.data(), sizeof(MultiItem) * items.size(),
                                    hipMemcpyHostToDevice));
-        PG_CHECK_HIP(hipMemset(P->tickets, 0, 4 * ((size_t)P->H + 1)));
+        // on the library stream, ahead of the kernels that use them
+        PG_CHECK_HIP(hipMemsetAsync(P->tickets, 0, 4 * ((size_t)P->H + 1), stream()));
     }
     return PG_OK;
 }

@@ -750,7 +750,8 @@ int build_schedule(pg_graph *g, bool in_direction) {
         PG_CHECK_HIP(hipMalloc(&s.d_partial, 32 * items.size() + 64));
         const int64_t nmv = bounds[1] - bounds[0];
         PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&s.d_tickets), sizeof(unsigned) * (nmv + 1)));
-        PG_CHECK_HIP(hipMemset(s.d_tickets, 0, sizeof(unsigned) * (nmv + 1)));
+        // on the library stream, ahead of the pulls that use them
+        PG_CHECK_HIP(hipMemsetAsync(s.d_tickets, 0, sizeof(unsigned) * (nmv + 1), g_stream));
     }
     a.multi = s.d_multi;
     a.partial = s.d_partial;
