@@ -620,8 +620,16 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
             op.skip(v);
         } else {
             const unsigned long long *p = ta.partial + f.slot0 + i;
-            unsigned long long q = 0ull;
-            for (int k = 0; k < f.nparts; k++) q += ld_stream_u64(p + (int64_t)k * nv);
+            unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull, q3 = 0ull;
+            int k = 0;
+            for (; k + 3 < f.nparts; k += 4) {  // four slots in flight
+                q0 += ld_stream_u64(p + (int64_t)k * nv);
+                q1 += ld_stream_u64(p + (int64_t)(k + 1) * nv);
+                q2 += ld_stream_u64(p + (int64_t)(k + 2) * nv);
+                q3 += ld_stream_u64(p + (int64_t)(k + 3) * nv);
+            }
+            for (; k < f.nparts; k++) q0 += ld_stream_u64(p + (int64_t)k * nv);
+            const unsigned long long q = (q0 + q1) + (q2 + q3);  // integers: any order
             st = op.finish(v, op.tile_sum(q), op.ctx(v));
         }
     }
