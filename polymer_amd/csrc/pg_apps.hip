@@ -535,7 +535,8 @@ struct BpOp {
         return Acc{a.p0 * b.p0, a.p1 * b.p1, a.any | b.any, 0};
     }
     // the potential is loaded before the frontier test so that its (streamed,
-    // coalesced) load is not held behind the bitmap lookup
+    // coalesced) load is not held behind the bitmap lookup (a non-temporal load
+    // of it measured 9% slower)
     __device__ Acc edge(int s, int e, const Ctx &q) const {
         const float4 w = pot[e];
         if (!bit_test(Fb, s)) return identity();
@@ -554,15 +555,19 @@ struct BpOp {
     }
 };
 
-// bits[w] = the 32 flags f[32w .. 32w+31] (zero past n)
+// bits[w] = the 32 flags f[32w .. 32w+31] (zero past n): a wave reads 64
+// consecutive flag bytes (one coalesced load) and its ballot is two words
 __global__ void k_flags_to_bits(const uint8_t *f, int64_t n, uint32_t *bits) {
     const int64_t nw = (n + 31) / 32;
-    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
-         w += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t x = 0;
-        const int64_t v0 = 32 * w;
-        for (int j = 0; j < 32 && v0 + j < n; j++) x |= (uint32_t)(f[v0 + j] != 0) << j;
-        bits[w] = x;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t v0 = wave * 64; v0 < n; v0 += nwaves * 64) {  // uniform per wave
+        const int64_t v = v0 + lane;
+        const unsigned long long b = __ballot(v < n && f[v] != 0);
+        const int64_t w = v0 / 32;
+        if (lane == 0) bits[w] = (uint32_t)b;
+        if (lane == 1 && w + 1 < nw) bits[w + 1] = (uint32_t)(b >> 32);
     }
 }
 
@@ -1737,7 +1742,7 @@ static int bp_run(pg_graph *g, int iters, const float4 *pot, const float2 *vp, c
     for (int t = 0; t < iters; t++) {
         BpOp<T> op{F, Fn, pot, vp, cur, nxt, g->out_off, Fb};
         PG_TRY(kt.begin());
-        PG_LAUNCH_K(k_flags_to_bits, grid_for((n + 31) / 32), F, n, Fb);
+        PG_LAUNCH_K(k_flags_to_bits, grid_for(n), F, n, Fb);
         PG_TRY(launch_pull(op, g->in_sched, &stats[(size_t)t * kStatSlots]));
         PG_TRY(kt.end());
         std::swap(cur, nxt);  // swap(vertD_curr, vertD_next), numa-BP.C:303-304
