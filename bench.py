@@ -267,6 +267,35 @@ def profile_run(args):
         (args.iters, st.ms, st.kernel_ms / max(st.kernel_launches, 1) * 1e3, st.edges))
 
 
+def profile_bp(args):
+    """Short run for rocprofv3: BP on the scale-24 rMat with the potentials of
+    value_extra, 10 iterations with fp64 state twice (warm-up, measured), then the
+    same with fp32 state; one k_flags_to_bits and one k_pull<BpOp> per iteration.
+    The calibration kernel of profile_run runs first."""
+    import polymer_amd as pa
+    import torch
+
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    log("calibration kernel (neg): 1073741824 bytes read, 1073741824 bytes written")
+    del x, y
+    pa.init(0)
+    g = pa.Graph.rmat(args.scale, args.edges, seed=1)
+    g.bp_potentials(9)
+    vpot = np.random.default_rng(8).uniform(0.1, 1.0, 2 * g.n).astype(np.float32)
+    prod0 = np.ones(2 * g.n, np.float32)
+    for fp32 in (False, True):
+        g.bp(vpot, prod0, 10, out=False, fp32=fp32)
+        _, st = g.bp(vpot, prod0, 10, out=False, fp32=fp32)
+        log("bp fp32=%d, 10 iterations: %.3f ms, %d edges, %.1f GB/s algorithmic" %
+            (fp32, st.ms, st.edges, (20 * st.edges + 28 * g.n * 10) / (st.ms / 1e3) / 1e9))
+    g.close()
+
+
 def profile_bfs(args):
     """Short run for rocprofv3: the calibration kernel of profile_run, then the
     searches of bfs24_extra on the symmetrised scale-24 rMat (the source selection,
@@ -378,6 +407,8 @@ def main():
                     help="short run to be wrapped by rocprofv3 (no JSON line)")
     ap.add_argument("--profile-bfs", action="store_true",
                     help="the same for the scale-24 BFS sources of the extras")
+    ap.add_argument("--profile-bp", action="store_true",
+                    help="the same for BP on the scale-24 rMat of the extras")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -399,6 +430,9 @@ def main():
         return
     if args.profile_bfs:
         profile_bfs(args)
+        return
+    if args.profile_bp:
+        profile_bp(args)
         return
 
     import polymer_amd as pa
