@@ -33,7 +33,7 @@ def pagerank_bytes(n, edges, iters):
     return 8.0 * edges + 20.0 * n * iters
 
 
-def cpu_baseline(pa, g, iters_sample=3):
+def cpu_baseline(pa, g, iters_sample=20):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
 
@@ -51,7 +51,8 @@ def cpu_baseline(pa, g, iters_sample=3):
         "cores": threads,
         "kind": "port",
         "sample": "%d of the 20 PageRank iterations (fp64 pull, OpenMP) on the same "
-                  "scale-24 graph, %.1f s" % (iters_sample, dt),
+                  "scale-24 graph, %.1f s on %d threads (%.0f thread-seconds)" %
+                  (iters_sample, dt, threads, dt * threads),
     }
 
 
