@@ -4,7 +4,8 @@ symmetrised scale-22 graph against the oracle's Dijkstra, bit-exact; and, throug
 what does not depend on size, BFS on the symmetrised scale-24 graph by the
 properties that define breadth-first levels and parents, CC labels on scale 22 by
 the properties that define minimum-id labels, and the 64-source BFS on scale 22
-against 64 single-source searches (two independent GPU paths)."""
+against 64 single-source searches (two independent GPU paths); BP on scale 22
+against the oracle's fp64 products where they stay in range."""
 import os
 import sys
 
@@ -128,3 +129,26 @@ def test_sssp_scale22_vs_dijkstra(pa, orc):
         say("  source %d: %d rounds, %d reached" % (src, st.rounds, int((d < pa.INF_DIST).sum())))
         assert np.array_equal(d, ref), src
     gs.close()
+
+
+def test_bp_scale22_vs_fp64(pa, orc):
+    """BP with fp64 state, 10 iterations on the scale-22 rMat, against the oracle's
+    fp64 products. Unnormalised products under- and overflow even in fp64 (a
+    vertex's product is raised to about its in-degree per iteration), so, as in the
+    small-graph test, the entries the oracle keeps inside [1e-30, 1e30] are compared
+    (about 65% of them at this size) and must agree to 1e-6 relative L1."""
+    say("case: bp scale 22, 10 iterations, fp64 state, against the fp64 oracle")
+    g = pa.Graph.rmat(22, 1 << 26, seed=1)
+    g.bp_potentials(9)
+    vpot = orc.vertex_f32(2 * g.n, 8, 0.1, 1.0)
+    prod0 = np.ones(2 * g.n, np.float32)
+    got, _ = g.bp(vpot, prod0, 10)
+    off, adj, _ = g.download()
+    hg = orc.Graph(g.n, off, adj)
+    ref = orc.bp(hg, vpot, orc.bp_potentials(hg, 9), prod0.astype(np.float64), 10)
+    ok = np.isfinite(ref) & (np.abs(ref) >= 1e-30) & (np.abs(ref) <= 1e30)
+    e = rel_l1(got[ok], ref[ok])
+    say("  in-range entries %d of %d, rel L1 %.3e" % (ok.sum(), ok.size, e))
+    assert ok.sum() > 0.5 * ok.size
+    assert e < 1e-6
+    g.close()
