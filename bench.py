@@ -56,10 +56,24 @@ def cpu_baseline(pa, g, iters_sample=20):
     }
 
 
+def traversal_sources(deg, k=8, seed=7):
+    """Sources of the scale-22 traversal workload: the vertex of maximum degree,
+    then k distinct seeded random vertices with at least one edge (an isolated
+    draw is skipped, as Graph500 does)."""
+    top = int(np.argmax(deg))
+    rng = np.random.default_rng(seed)
+    out = [top]
+    while len(out) < k + 1:
+        v = int(rng.integers(0, deg.size))
+        if deg[v] > 0 and v not in out:
+            out.append(v)
+    return out
+
+
 def bfs_extra(pa, scale=22, m=1 << 26, nsrc=8):
     gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True)
-    rng = np.random.default_rng(7)
-    srcs = [0] + [int(x) for x in rng.integers(0, gs.n, nsrc)]
+    off, _, _ = gs.download()
+    srcs = traversal_sources(np.diff(off), nsrc)
     gs.bfs(0, out=False)  # warm-up / workspace
     tot_ms, tot_edges, tot_bytes = 0.0, 0, 0
     for s in srcs:
@@ -130,13 +144,13 @@ def async_extra(pa, scale=24, m=1 << 28):
 
 def traversal_extra(pa, scale=22, m=1 << 26):
     """Connected components on the symmetrised scale-22 graph; Bellman-Ford (int32
-    weights U[1,255]) from vertex 0 (the max-degree vertex) and 8 seeded sources."""
+    weights U[1,255]) from the sources of traversal_sources."""
     gs = pa.Graph.rmat(scale, m, seed=1, symmetrize=True, weights="i32", wseed=7)
     gs.cc(out=False)
     _, stc = gs.cc(out=False)
-    rng = np.random.default_rng(7)
-    srcs = [0] + [int(x) for x in rng.integers(0, gs.n, 8)]
-    gs.sssp(0, out=False)
+    off, _, _ = gs.download()
+    srcs = traversal_sources(np.diff(off))
+    gs.sssp(srcs[0], out=False)
     tot_ms, tot_edges = 0.0, 0
     for s in srcs:
         _, st = gs.sssp(s, out=False)
@@ -377,9 +391,10 @@ def adjoint_extra(pa, orc_free=True, scale=22, m=1 << 26):
             "adjoint_over_forward": round(sta.ms / stf.ms, 3)}
 
 
-def multigraph_extra(pa, count=512, iters=20):
-    """512 independent rMat graphs, n = 2^s with s uniform in [8, 16], average degree
-    16, PageRank of every graph in one captured launch."""
+def multigraph_inputs(pa, count=512):
+    """The many-graph workload: count independent rMat graphs, n = 2^s with s
+    uniform in [8, 16] (log-uniform n in [2^8, 2^16]), average degree 16, seeds
+    1000 + i; returns vertex bases and the concatenated CSR in input ids."""
     rng = np.random.default_rng(5)
     offs, adjs, vb, base, ebase = [], [], [0], 0, 0
     for i in range(count):
@@ -392,8 +407,15 @@ def multigraph_extra(pa, count=512, iters=20):
         base += h.n
         ebase += h.m
         vb.append(base)
-    g = pa.Graph.multi(np.array(vb), np.concatenate(offs + [np.array([ebase])]),
-                       np.concatenate(adjs))
+    return (np.array(vb, np.int64), np.concatenate(offs + [np.array([ebase])]).astype(np.int64),
+            np.concatenate(adjs).astype(np.int32))
+
+
+def multigraph_extra(pa, count=512, iters=20):
+    """512 independent rMat graphs (multigraph_inputs), PageRank of every graph in
+    one captured launch."""
+    vb, off, adj = multigraph_inputs(pa, count)
+    g = pa.Graph.multi(vb, off, adj)
     g.autotune()
     g.pagerank_multigraph(iters, out=False)
     _, st = g.pagerank_multigraph(iters, out=False)

@@ -78,6 +78,10 @@ typedef struct pg_stats {
 
 #define PG_PROFILE_KERNELS 0x1u
 #define PG_BP_FP32 0x2u /* pg_bp: fp32 product state and messages (default fp64) */
+/* pg_pagerank / pg_spmv on tiles: run every iteration's cold pre-pass in order on
+ * the library stream instead of beside the previous iteration's finishing pass
+ * (same sums; a check of the overlapped schedule) */
+#define PG_SERIAL_COLD 0x4u
 
 /* ---- library ------------------------------------------------------------ */
 int pg_init(int device);
@@ -131,10 +135,12 @@ int pg_graph_set_tiles(pg_graph *g, int enable, int64_t max_degree, int64_t tile
                        int64_t cold_from);
 /* Sizes of the tile structures of the last PageRank-family solve (built on
  * first use): tiles, workgroup parts, tiled in-edges, cold in-edges served by
- * the pre-pass, 64-edge groups whose sources are stored unpacked.  Any pointer
- * may be NULL; all zero before the first tiled solve. */
+ * the pre-pass, 64-edge groups whose sources are stored unpacked, and whether
+ * the next iteration's cold pre-pass may run beside the finishing pass (1: every
+ * tile with several parts lies below the first cold source).  Any pointer may
+ * be NULL; all zero before the first tiled solve. */
 int pg_graph_tile_stats(const pg_graph *g, int64_t *tiles, int64_t *parts, int64_t *tiled_edges,
-                        int64_t *cold_edges, int64_t *escape_groups);
+                        int64_t *cold_edges, int64_t *escape_groups, int32_t *cold_overlap);
 /* The current pull schedule and tile setting (any pointer may be NULL). */
 int pg_graph_get_schedule(const pg_graph *g, int32_t *items_per_lane, int64_t *block_threshold,
                           int64_t *multi_threshold, int64_t *chunk, int32_t *tiles,

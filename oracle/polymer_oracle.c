@@ -866,6 +866,36 @@ int or_spmv(int64_t n, const int64_t *off, const int32_t *adj, const double *w,
     return 0;
 }
 
+/* or_spmv computed by pulling over the CSC with OpenMP threads: teid[p] is
+ * the CSR edge of in-edge p.  The in-lists of or_transpose are in ascending
+ * source order, which is the order in which or_spmv's push adds into each
+ * destination, so every sum is formed in the same order and the result is
+ * bit-identical to or_spmv (numa-SPMV.C:56-97, :168-193). */
+int or_spmv_pull_omp(int64_t n, const int64_t *toff, const int32_t *tadj, const int64_t *teid,
+                     const double *w, const double *x0, int iters, int threads,
+                     double *y_out) {
+    double *cur = (double *)malloc(sizeof(double) * (size_t)n);
+    double *nxt = (double *)malloc(sizeof(double) * (size_t)n);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    memcpy(cur, x0, sizeof(double) * (size_t)n);
+    for (int t = 0; t < iters; t++) {
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (int64_t d = 0; d < n; d++) {
+            double sum = 0.0;
+            for (int64_t p = toff[d]; p < toff[d + 1]; p++) sum += cur[tadj[p]] * w[teid[p]];
+            nxt[d] = sum;
+        }
+        double *tp = cur; cur = nxt; nxt = tp;
+    }
+    memcpy(y_out, cur, sizeof(double) * (size_t)n);
+    free(cur); free(nxt);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Belief propagation (numa-BP.C:89-140 functors, :179-230 push,        */
 /* :280-316 loop).  epot[e*4 + k*2 + l] = W_e[k][l] in CSR edge order.  */
@@ -919,6 +949,56 @@ static int bp_impl(int64_t n, const int64_t *off, const int32_t *adj,
         }
         if (state_f32)
             for (int64_t i = 0; i < 2 * n; i++) nxt[i] = (double)(float)nxt[i];
+        double *tp = cur; cur = nxt; nxt = tp;
+        uint8_t *tf = F; F = N; N = tf;
+    }
+    memcpy(prod_out, cur, sizeof(double) * 2 * (size_t)n);
+    free(cur); free(nxt); free(F); free(N);
+    return 0;
+}
+
+/* or_bp computed by pulling over the CSC with OpenMP threads (teid[p]: the
+ * CSR edge of in-edge p).  A destination multiplies the messages of its active
+ * in-neighbours in ascending source order, the order in which or_bp's push
+ * multiplies them, with the same expression for each message, so the result is
+ * bit-identical to or_bp; a destination reached by no active source keeps its
+ * product (1, 1) when active and the other buffer's content otherwise, and the
+ * next frontier is the set of reached destinations (numa-BP.C:280-316). */
+int or_bp_pull_omp(int64_t n, const int64_t *toff, const int32_t *tadj, const int64_t *teid,
+                   const float *vpot, const float *epot, const double *prod0, int iters,
+                   int threads, double *prod_out) {
+    double *cur = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *nxt = (double *)calloc(2 * (size_t)n, sizeof(double));
+    uint8_t *F = (uint8_t *)malloc((size_t)n);
+    uint8_t *N = (uint8_t *)malloc((size_t)n);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    memcpy(cur, prod0, sizeof(double) * 2 * (size_t)n);
+    for (int64_t i = 0; i < n; i++) F[i] = 1;
+    for (int t = 0; t < iters; t++) {
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (int64_t d = 0; d < n; d++) {
+            double p0 = F[d] ? 1.0 : nxt[2 * d], p1 = F[d] ? 1.0 : nxt[2 * d + 1];
+            uint8_t any = 0;
+            for (int64_t q = toff[d]; q < toff[d + 1]; q++) {
+                if (!F[tadj[q]]) continue;
+                const int64_t e = teid[q];
+                double b0 = 0.0, b1 = 0.0;
+                for (int l = 0; l < 2; l++)
+                    b0 += (double)vpot[2 * d + l] * (double)epot[4 * e + l] * cur[2 * d + l];
+                p0 *= b0;
+                for (int l = 0; l < 2; l++)
+                    b1 += (double)vpot[2 * d + l] * (double)epot[4 * e + 2 + l] * cur[2 * d + l];
+                p1 *= b1;
+                any = 1;
+            }
+            nxt[2 * d] = p0;
+            nxt[2 * d + 1] = p1;
+            N[d] = any;
+        }
         double *tp = cur; cur = nxt; nxt = tp;
         uint8_t *tf = F; F = N; N = tf;
     }

@@ -115,6 +115,12 @@ int or_dijkstra(int64_t n, const int64_t *off, const int32_t *adj,
                 const int32_t *w, int32_t src, int32_t *dist);
 int or_spmv(int64_t n, const int64_t *off, const int32_t *adj, const double *w,
             const double *x0, int iters, double *y_out);
+int or_spmv_pull_omp(int64_t n, const int64_t *toff, const int32_t *tadj, const int64_t *teid,
+                     const double *w, const double *x0, int iters, int threads,
+                     double *y_out);
+int or_bp_pull_omp(int64_t n, const int64_t *toff, const int32_t *tadj, const int64_t *teid,
+                   const float *vpot, const float *epot, const double *prod0, int iters,
+                   int threads, double *prod_out);
 int or_bp(int64_t n, const int64_t *off, const int32_t *adj,
           const float *vpot, const float *epot, const double *prod0, int iters,
           double *prod_out);

@@ -58,6 +58,10 @@ def lib():
             "or_dijkstra": (ctypes.c_int, [i64, I64P, I32P, I32P, i32, I32P]),
             "or_spmv": (ctypes.c_int, [i64, I64P, I32P, F64P, F64P, ctypes.c_int, F64P]),
             "or_bp": (ctypes.c_int, [i64, I64P, I32P, F32P, F32P, F64P, ctypes.c_int, F64P]),
+            "or_spmv_pull_omp": (ctypes.c_int, [i64, I64P, I32P, I64P, F64P, F64P, ctypes.c_int,
+                                                ctypes.c_int, F64P]),
+            "or_bp_pull_omp": (ctypes.c_int, [i64, I64P, I32P, I64P, F32P, F32P, F64P, ctypes.c_int,
+                                              ctypes.c_int, F64P]),
             "or_bp_f32": (ctypes.c_int, [i64, I64P, I32P, F32P, F32P, F32P, ctypes.c_int, F32P]),
             "or_bp_state_f32": (ctypes.c_int, [i64, I64P, I32P, F32P, F32P, F64P, ctypes.c_int,
                                                F64P]),
@@ -283,6 +287,25 @@ def spmv(g, w, x0, iters):
     lib().or_spmv(g.n, g.off, g.adj, np.ascontiguousarray(w, np.float64),
                   np.ascontiguousarray(x0, np.float64), iters, y)
     return y
+
+
+def spmv_pull_omp(g, w, x0, iters, threads=0):
+    """or_spmv pulled over the CSC with OpenMP threads (bit-identical to spmv)."""
+    toff, tadj, teid = g.transpose()
+    y = np.zeros(g.n, np.float64)
+    lib().or_spmv_pull_omp(g.n, toff, tadj, teid, np.ascontiguousarray(w, np.float64),
+                           np.ascontiguousarray(x0, np.float64), iters, threads, y)
+    return y
+
+
+def bp_pull_omp(g, vpot, epot, prod0, iters, threads=0):
+    """or_bp pulled over the CSC with OpenMP threads (bit-identical to bp)."""
+    toff, tadj, teid = g.transpose()
+    out = np.zeros(2 * g.n, np.float64)
+    lib().or_bp_pull_omp(g.n, toff, tadj, teid, np.ascontiguousarray(vpot, np.float32),
+                         np.ascontiguousarray(epot, np.float32),
+                         np.ascontiguousarray(prod0, np.float64).ravel(), iters, threads, out)
+    return out.reshape(g.n, 2)
 
 
 def bp(g, vpot, epot, prod0, iters):

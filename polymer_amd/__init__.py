@@ -27,6 +27,7 @@ PG_GRAPH_VALUE_F32 = 0x10
 PG_W_NONE, PG_W_I32, PG_W_F32 = 0, 1, 2
 PG_PROFILE_KERNELS = 0x1
 PG_BP_FP32 = 0x2
+PG_SERIAL_COLD = 0x4
 INF_DIST = 1073741823
 
 # every symbol include/polymer_gpu.h declares
@@ -103,7 +104,7 @@ def load():
         "pg_graph_set_schedule": [P, ctypes.c_int, i64, i64, i64],
         "pg_graph_set_tiles": [P, ctypes.c_int, i64, i64, i64],
         "pg_graph_get_schedule": [P, P, P, P, P, P, P, P, P],
-        "pg_graph_tile_stats": [P, P, P, P, P, P],
+        "pg_graph_tile_stats": [P, P, P, P, P, P, P],
         "pg_graph_autotune": [P, P, P],
         "pg_graph_bp_potentials": [P, u64, ctypes.c_float, ctypes.c_float],
         "pg_subset_create": [P, PP],
@@ -304,11 +305,11 @@ class Graph:
         return tuple(x.value for x in v)
 
     def tile_stats(self):
-        """dict: tiles, parts, tiled_edges, cold_edges, escape_groups"""
-        v = [ctypes.c_int64() for _ in range(5)]
+        """dict: tiles, parts, tiled_edges, cold_edges, escape_groups, cold_overlap"""
+        v = [ctypes.c_int64() for _ in range(5)] + [ctypes.c_int32()]
         _check(load().pg_graph_tile_stats(self._h, *[ctypes.byref(x) for x in v]),
                "pg_graph_tile_stats")
-        keys = ("tiles", "parts", "tiled_edges", "cold_edges", "escape_groups")
+        keys = ("tiles", "parts", "tiled_edges", "cold_edges", "escape_groups", "cold_overlap")
         return dict(zip(keys, (x.value for x in v)))
 
     def autotune(self):
@@ -324,10 +325,11 @@ class Graph:
         return Subset(self)
 
     # ---- applications -------------------------------------------------------------
-    def pagerank(self, iters=20, damping=0.85, out=True, profile=False):
+    def pagerank(self, iters=20, damping=0.85, out=True, profile=False, serial_cold=False):
         rank = np.zeros(self.n, np.float32) if out else None
         st = Stats()
-        _check(load().pg_pagerank(self._h, iters, damping, _ptr(rank), _flags(profile),
+        flags = _flags(profile) | (PG_SERIAL_COLD if serial_cold else 0)
+        _check(load().pg_pagerank(self._h, iters, damping, _ptr(rank), flags,
                                   ctypes.byref(st)), "pg_pagerank")
         return rank, st
 
@@ -339,11 +341,12 @@ class Graph:
                                         _flags(profile), ctypes.byref(st)), "pg_pagerank_delta")
         return p, st
 
-    def spmv(self, x0, iters=20, out=True, profile=False):
+    def spmv(self, x0, iters=20, out=True, profile=False, serial_cold=False):
         x0 = np.ascontiguousarray(x0, np.float32)
         y = np.zeros(self.n, np.float64) if out else None
         st = Stats()
-        _check(load().pg_spmv(self._h, iters, _ptr(x0), _ptr(y), _flags(profile),
+        flags = _flags(profile) | (PG_SERIAL_COLD if serial_cold else 0)
+        _check(load().pg_spmv(self._h, iters, _ptr(x0), _ptr(y), flags,
                               ctypes.byref(st)), "pg_spmv")
         return y, st
 

@@ -255,7 +255,7 @@ static int pr_buffers(pg_graph *g, int iters, PrBuffers *B) {
 // allocation once the workspace exists: safe inside a stream capture).  On return
 // *result points at the buffer that will hold p_curr.
 int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, float **result,
-                     unsigned long long **stats) {
+                     unsigned long long **stats, bool serial_cold) {
     PrBuffers B;
     PG_TRY(pr_buffers(g, iters, &B));
     const int64_t n = g->n;
@@ -282,7 +282,7 @@ int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, fl
     const bool tiled = g->use_tiles && g->tiles.built && g->tiles.light_begin == g->in_sched.light_begin &&
                        g->tiles.light_end == g->in_sched.light_end;
     // the next iteration's cold pre-pass runs beside this one's finishing pass
-    const bool ahead = tiled && g->tiles.cold_overlap && g->tiles.cold.nparts > 0;
+    const bool ahead = tiled && g->tiles.cold_overlap && g->tiles.cold.nparts > 0 && !serial_cold;
     for (int t = 0; t < iters; t++) {
         op.F = F;
         op.Fn = Fn;
@@ -318,7 +318,8 @@ int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stat
     Timer tm;
     PG_TRY(tm.start());
     unsigned long long *stats = nullptr;
-    PG_TRY(pagerank_enqueue(g, iters, damping, &kt, result, &stats));
+    PG_TRY(pagerank_enqueue(g, iters, damping, &kt, result, &stats,
+                            (flags & PG_SERIAL_COLD) != 0));
     double ms = 0;
     PG_TRY(tm.stop(&ms));
     PrBuffers B;
@@ -1642,7 +1643,8 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
     float *cur = xa, *nxt = xb;
     // the next iteration's cold pre-pass (raw values, scaled later in the tiles) runs
     // beside this one's finishing pass
-    const bool ahead = tiled && g->tiles.cold_overlap && g->tiles.cold.nparts > 0;
+    const bool ahead = tiled && g->tiles.cold_overlap && g->tiles.cold.nparts > 0 &&
+                       (flags & PG_SERIAL_COLD) == 0;
     for (int t = 0; t < iters; t++) {
         // iteration t scales its output by 2^-shift[t+1]; shift[t+1] comes from max(x_t)
         SpmvOp op{cur, w, nxt, shift + t + 1, tscale ? tscale + t : nullptr};

@@ -21,7 +21,7 @@ namespace pg {
 int pagerank_run(pg_graph *g, int iters, double damping, uint32_t flags, pg_stats *st,
                  float **result);
 int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, float **result,
-                     unsigned long long **stats);
+                     unsigned long long **stats, bool serial_cold = false);
 
 #define PG_LAUNCH_B(kern, grid, ...)                                                  \
     do {                                                                              \

@@ -1918,7 +1918,7 @@ extern "C" int pg_graph_set_tiles(pg_graph *g, int enable, int64_t max_degree, i
 
 extern "C" int pg_graph_tile_stats(const pg_graph *g, int64_t *tiles, int64_t *parts,
                                    int64_t *tiled_edges, int64_t *cold_edges,
-                                   int64_t *escape_groups) {
+                                   int64_t *escape_groups, int32_t *cold_overlap) {
     PG_REQUIRE(g, PG_ERR_ARG, "null graph");
     const TileSet &t = g->tiles;
     if (tiles) *tiles = t.built ? t.ntiles : 0;
@@ -1926,6 +1926,7 @@ extern "C" int pg_graph_tile_stats(const pg_graph *g, int64_t *tiles, int64_t *p
     if (tiled_edges) *tiled_edges = t.built ? t.m_tiled : 0;
     if (cold_edges) *cold_edges = t.built ? t.m_cold : 0;
     if (escape_groups) *escape_groups = t.built ? t.n_esc : 0;
+    if (cold_overlap) *cold_overlap = (t.built && t.cold_overlap && t.cold.nparts > 0) ? 1 : 0;
     return PG_OK;
 }
 

@@ -170,6 +170,38 @@ def test_pagerank_tiles(pa, orc):
     assert np.array_equal(r4, out[1])
 
 
+
+def test_cold_prepass_overlap(pa, orc):
+    """The next iteration's cold pre-pass runs on a side stream beside the finishing
+    pass when every multi-part tile lies below the first cold source.  Fixed-point
+    sums do not depend on order, so the overlapped schedule must give the same bits
+    as the serial one (PG_SERIAL_COLD); configurations with the overlap on and off
+    are both covered (tile_stats reports which)."""
+    say("case: cold pre-pass beside the finishing pass vs serial")
+    g = orc.rmat(16, 1 << 20, seed=8)
+    ref, _ = orc.pagerank(g, 20, 0.85)
+    d = pa.Graph.rmat(16, 1 << 20, seed=8, weights="f32", wseed=3)
+    x0 = np.random.default_rng(4).random(d.n).astype(np.float32)
+    seen = set()
+    # (edges per part, first cold source, tile in-degree limit): single-part tiles
+    # only (overlap on), the hub tile split with the cold sources above it (on),
+    # split tiles reaching past the first cold source (off)
+    for te, cold, maxdeg in ((1 << 30, 30000, 1 << 30), (262144, 30000, 1 << 30),
+                             (4096, 64, 1 << 30), (512, 300, 1 << 30), (4096, 1000, 2048)):
+        d.set_tiles(1, te, cold, max_degree=maxdeg)
+        r1, _ = d.pagerank(20, 0.85)
+        ts = d.tile_stats()
+        r2, _ = d.pagerank(20, 0.85, serial_cold=True)
+        y1, _ = d.spmv(x0, 8)
+        y2, _ = d.spmv(x0, 8, serial_cold=True)
+        say("tile edges %d cold %d max degree %d: parts %d cold edges %d overlap %d" %
+            (te, cold, maxdeg, ts["parts"], ts["cold_edges"], ts["cold_overlap"]))
+        assert np.array_equal(r1, r2), (te, cold)
+        assert np.array_equal(y1, y2), (te, cold)
+        assert rel_l1(r1, ref) < TOL
+        seen.add(ts["cold_overlap"])
+    assert seen == {0, 1}, seen
+
 def test_pagerank_tiles_escapes(pa, orc):
     # n = 2^20 > 2^19: 64-edge groups that straddle two tiles span more source ids
     # than a packed offset holds and keep their sources unpacked (escape rows)
@@ -277,6 +309,21 @@ def bp_compare(got, ref):
     err = rel_l1(got[ok], ref[ok])
     say("bp in-range entries %d of %d, relL1 %.3e" % (ok.sum(), ok.size, err))
     return ok, err
+
+
+def test_bp_hand_derived(pa):
+    """The 3-vertex BP answer worked out by hand in tests/bp_hand.py (asymmetric
+    W, distinct potentials and products, a vertex with two in-edges, one without):
+    exact in fp64 state and in fp32 state (every intermediate is a small integer)."""
+    import bp_hand as to
+    say("case: bp hand-derived 3-vertex answer")
+    d = pa.Graph.from_csr(3, to.BP_HAND_OFF, to.BP_HAND_ADJ)
+    for iters, want in to.BP_HAND_ANSWER.items():
+        got, _ = d.bp(to.BP_HAND_VPOT, to.BP_HAND_PROD0, iters, epot=to.BP_HAND_EPOT)
+        assert np.array_equal(got, want), (iters, got)
+        g32, _ = d.bp(to.BP_HAND_VPOT, to.BP_HAND_PROD0, iters, epot=to.BP_HAND_EPOT, fp32=True)
+        assert np.array_equal(g32, want), (iters, g32)
+    d.close()
 
 
 def test_bp(pa, orc):

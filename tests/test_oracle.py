@@ -328,6 +328,22 @@ def test_bp_reference_initial_state(orc):
     assert np.all(out[indeg == 0] == 1.0)
 
 
+from bp_hand import (BP_HAND_ADJ, BP_HAND_ANSWER, BP_HAND_EPOT, BP_HAND_OFF,  # noqa: E402
+                     BP_HAND_PROD0, BP_HAND_VPOT)
+
+
+def test_bp_hand_derived(orc):
+    g = orc.Graph(3, BP_HAND_OFF, BP_HAND_ADJ)
+    for iters, want in BP_HAND_ANSWER.items():
+        got = orc.bp(g, BP_HAND_VPOT, BP_HAND_EPOT, BP_HAND_PROD0, iters)
+        assert np.array_equal(got, want), (iters, got)
+        f32 = orc.bp_f32(g, BP_HAND_VPOT, BP_HAND_EPOT, BP_HAND_PROD0.astype(np.float32), iters)
+        assert np.array_equal(f32, want.astype(np.float32)), (iters, f32)
+    # a planted transposition of W fails the known answer
+    epot_t = BP_HAND_EPOT.reshape(3, 2, 2).transpose(0, 2, 1).ravel().copy()
+    assert not np.array_equal(orc.bp(g, BP_HAND_VPOT, epot_t, BP_HAND_PROD0, 1), BP_HAND_ANSWER[1])
+
+
 def test_bp_potential_range(orc, pr1):
     p = orc.bp_potentials(pr1, 9)
     assert p.min() >= 0.1 and p.max() < 1.0 and p.size == 4 * pr1.m
@@ -499,3 +515,27 @@ def test_pagerank_delta_known_answer(orc):
     assert np.allclose(p, want, rtol=0, atol=1e-15), (p, want)
     assert list(act) == [3, 2, 2]
     assert abs(p.sum() - 1.0) < 1e-15
+
+
+def test_pull_omp_variants_bit_identical(orc):
+    """The OpenMP pulls that check the full-size GPU runs form every sum and product
+    in the push's order: equal bits to or_spmv / or_bp on skewed graphs (duplicate-free
+    rMat with hubs, isolated vertices, a few frontier-shrinking BP rounds)."""
+    for scale, m, seed in ((8, 3000, 3), (12, 60000, 4)):
+        g = orc.rmat(scale, m, seed=seed)
+        w = orc.values_f32(g, 5).astype(np.float64)
+        x0 = orc.vertex_f32(g.n, 6, 0.0, 1.0).astype(np.float64)
+        for iters in (1, 4):
+            assert np.array_equal(orc.spmv_pull_omp(g, w, x0, iters, threads=4),
+                                  orc.spmv(g, w, x0, iters))
+        vpot = orc.vertex_f32(2 * g.n, 8, 0.1, 1.0)
+        epot = orc.bp_potentials(g, 9)
+        for prod0 in (np.ones(2 * g.n), orc.vertex_f32(2 * g.n, 2, 0.5, 2.0).astype(np.float64)):
+            for iters in (1, 2, 5):
+                a = orc.bp_pull_omp(g, vpot, epot, prod0, iters, threads=4)
+                b = orc.bp(g, vpot, epot, prod0, iters)
+                assert np.array_equal(a, b, equal_nan=True), (scale, iters)
+    gh = orc.Graph(3, BP_HAND_OFF, BP_HAND_ADJ)
+    for iters, want in BP_HAND_ANSWER.items():
+        assert np.array_equal(orc.bp_pull_omp(gh, BP_HAND_VPOT, BP_HAND_EPOT, BP_HAND_PROD0, iters),
+                              want)
