@@ -224,6 +224,19 @@ def value_extra(pa, g, scale=24, m=1 << 28):
     gbs = (12 * gw.m + 12 * gw.n) * 20 / (st.ms / 1e3) / 1e9
     out["spmv_scale24_algorithmic_gbs"] = round(gbs, 1)
     out["spmv_scale24_hbm_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+    # the adjoint of the same 20 iterations (loss g . y_20, g ~ N(0,1)): the forward
+    # keeps its iterates, the backward pulls over the CSR and accumulates the fp64
+    # weight gradient (28 bytes per edge and 12 per vertex per step)
+    gv = np.random.default_rng(4).standard_normal(gw.n).astype(np.float32)
+    gw.spmv(x0, 20, out=False, save=True)
+    _, stf = gw.spmv(x0, 20, out=False, save=True)
+    gw.spmv_adjoint(gv, out=False)
+    _, _, sta = gw.spmv_adjoint(gv, out=False)
+    out["spmv_scale24_forward_saved_ms"] = round(stf.ms, 3)
+    out["spmv_scale24_adjoint_ms"] = round(sta.ms, 3)
+    out["spmv_adjoint_over_forward"] = round(sta.ms / st.ms, 3)
+    out["spmv_adjoint_algorithmic_gbs"] = round(
+        (28 * gw.m + 12 * gw.n) * 20 / (sta.ms / 1e3) / 1e9, 1)
     gw.close()
     g.bp_potentials(9)
     vpot = np.random.default_rng(8).uniform(0.1, 1.0, 2 * g.n).astype(np.float32)

@@ -82,6 +82,8 @@ typedef struct pg_stats {
  * the library stream instead of beside the previous iteration's finishing pass
  * (same sums; a check of the overlapped schedule) */
 #define PG_SERIAL_COLD 0x4u
+/* pg_spmv: keep every iterate y_0..y_T on the device for pg_spmv_adjoint */
+#define PG_SPMV_SAVE 0x8u
 
 /* ---- library ------------------------------------------------------------ */
 int pg_init(int device);
@@ -275,6 +277,19 @@ int pg_wpr_forward(pg_graph *g, const float *wn, const float *tele, double dampi
  * grad_tele = (1-damping) lambda (host n).  lambda may be NULL. */
 int pg_wpr_adjoint(pg_graph *g, const float *gvec, int max_iters, double tol, float *lambda,
                    float *grad_w, float *grad_tele, uint32_t flags, pg_stats *st);
+
+/* ---- differentiation: SpMV ------------------------------------------------
+ * Adjoint of the last pg_spmv run with PG_SPMV_SAVE on g (numa-SPMV.C:56-97
+ * iterated T times, :168-193: y_{t+1}[d] = sum_{e=(s->d)} y_t[s] w[e]) for the
+ * loss L = sum_i gvec_i y_T[i]:
+ *   mu_T = gvec,  mu_t[s] = sum_{e=(s->d)} w[e] mu_{t+1}[d]   (a pull over the CSR,
+ *   i.e. the transposed graph), dL/dx0 = mu_0, dL/dw[e] = sum_{t<T} y_t[s] mu_{t+1}[d].
+ * fp32 iterates with one power-of-two exponent per iteration (as pg_spmv), the
+ * weight gradient accumulated in fp64.  gvec: host n; grad_x0: host n doubles;
+ * grad_w: host m doubles in CSR order of the input ids.  Either output may be
+ * NULL.  No counterpart in the reference, which has no derivatives. */
+int pg_spmv_adjoint(pg_graph *g, const float *gvec, double *grad_x0, double *grad_w,
+                    uint32_t flags, pg_stats *st);
 
 #ifdef __cplusplus
 }

@@ -866,6 +866,46 @@ int or_spmv(int64_t n, const int64_t *off, const int32_t *adj, const double *w,
     return 0;
 }
 
+/* Adjoint of or_spmv (numa-SPMV.C:56-97 iterated T times, :168-193) for the
+ * loss L = sum_i g_i y_T[i], y_{t+1}[d] = sum_{e=(s->d)} y_t[s] w[e]:
+ *   mu_T = g,   mu_t[s] = sum_{e=(s->d)} w[e] mu_{t+1}[d]   (the transposed map),
+ *   dL/dx0 = mu_0,   dL/dw[e] = sum_{t<T} y_t[s] mu_{t+1}[d]   (e = s->d).
+ * fp64 throughout; the forward iterates y_0..y_{T-1} are kept.  Not part of the
+ * reference (which has no derivatives): the chain rule applied to its loop. */
+int or_spmv_adjoint(int64_t n, const int64_t *off, const int32_t *adj, const double *w,
+                    const double *x0, const double *g, int iters, double *grad_x0,
+                    double *grad_w) {
+    const int64_t m = off[n];
+    double *ys = (double *)malloc(sizeof(double) * (size_t)n * (size_t)(iters + 1));
+    double *mu = (double *)malloc(sizeof(double) * (size_t)n);
+    double *mn = (double *)malloc(sizeof(double) * (size_t)n);
+    memcpy(ys, x0, sizeof(double) * (size_t)n);
+    for (int t = 0; t < iters; t++) {
+        const double *y = ys + (size_t)t * n;
+        double *yn = ys + (size_t)(t + 1) * n;
+        for (int64_t i = 0; i < n; i++) yn[i] = 0.0;
+        for (int64_t s = 0; s < n; s++)
+            for (int64_t e = off[s]; e < off[s + 1]; e++) yn[adj[e]] += y[s] * w[e];
+    }
+    for (int64_t e = 0; e < m; e++) grad_w[e] = 0.0;
+    memcpy(mu, g, sizeof(double) * (size_t)n);
+    for (int t = iters - 1; t >= 0; t--) {
+        const double *y = ys + (size_t)t * n;
+        for (int64_t s = 0; s < n; s++) {
+            double sum = 0.0;
+            for (int64_t e = off[s]; e < off[s + 1]; e++) {
+                sum += w[e] * mu[adj[e]];
+                grad_w[e] += y[s] * mu[adj[e]];
+            }
+            mn[s] = sum;
+        }
+        double *tp = mu; mu = mn; mn = tp;
+    }
+    memcpy(grad_x0, mu, sizeof(double) * (size_t)n);
+    free(ys); free(mu); free(mn);
+    return 0;
+}
+
 /* or_spmv computed by pulling over the CSC with OpenMP threads: teid[p] is
  * the CSR edge of in-edge p.  The in-lists of or_transpose are in ascending
  * source order, which is the order in which or_spmv's push adds into each

@@ -115,6 +115,9 @@ int or_dijkstra(int64_t n, const int64_t *off, const int32_t *adj,
                 const int32_t *w, int32_t src, int32_t *dist);
 int or_spmv(int64_t n, const int64_t *off, const int32_t *adj, const double *w,
             const double *x0, int iters, double *y_out);
+int or_spmv_adjoint(int64_t n, const int64_t *off, const int32_t *adj, const double *w,
+                    const double *x0, const double *g, int iters, double *grad_x0,
+                    double *grad_w);
 int or_spmv_pull_omp(int64_t n, const int64_t *toff, const int32_t *tadj, const int64_t *teid,
                      const double *w, const double *x0, int iters, int threads,
                      double *y_out);

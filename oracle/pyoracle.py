@@ -58,6 +58,8 @@ def lib():
             "or_dijkstra": (ctypes.c_int, [i64, I64P, I32P, I32P, i32, I32P]),
             "or_spmv": (ctypes.c_int, [i64, I64P, I32P, F64P, F64P, ctypes.c_int, F64P]),
             "or_bp": (ctypes.c_int, [i64, I64P, I32P, F32P, F32P, F64P, ctypes.c_int, F64P]),
+            "or_spmv_adjoint": (ctypes.c_int, [i64, I64P, I32P, F64P, F64P, F64P, ctypes.c_int,
+                                               F64P, F64P]),
             "or_spmv_pull_omp": (ctypes.c_int, [i64, I64P, I32P, I64P, F64P, F64P, ctypes.c_int,
                                                 ctypes.c_int, F64P]),
             "or_bp_pull_omp": (ctypes.c_int, [i64, I64P, I32P, I64P, F32P, F32P, F64P, ctypes.c_int,
@@ -287,6 +289,17 @@ def spmv(g, w, x0, iters):
     lib().or_spmv(g.n, g.off, g.adj, np.ascontiguousarray(w, np.float64),
                   np.ascontiguousarray(x0, np.float64), iters, y)
     return y
+
+
+def spmv_adjoint(g, w, x0, gvec, iters):
+    """Gradients of sum_i gvec_i y_T[i] for y_T = spmv(g, w, x0, iters):
+    (dL/dx0 per vertex, dL/dw per CSR edge)."""
+    gx = np.zeros(g.n, np.float64)
+    gw = np.zeros(max(g.m, 1), np.float64)
+    lib().or_spmv_adjoint(g.n, g.off, g.adj, np.ascontiguousarray(w, np.float64),
+                          np.ascontiguousarray(x0, np.float64),
+                          np.ascontiguousarray(gvec, np.float64), iters, gx, gw)
+    return gx, gw[: g.m]
 
 
 def spmv_pull_omp(g, w, x0, iters, threads=0):

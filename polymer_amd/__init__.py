@@ -28,6 +28,7 @@ PG_W_NONE, PG_W_I32, PG_W_F32 = 0, 1, 2
 PG_PROFILE_KERNELS = 0x1
 PG_BP_FP32 = 0x2
 PG_SERIAL_COLD = 0x4
+PG_SPMV_SAVE = 0x8
 INF_DIST = 1073741823
 
 # every symbol include/polymer_gpu.h declares
@@ -45,7 +46,7 @@ EXPORTS = [
     "pg_sssp",
     "pg_msbfs", "pg_mssssp", "pg_ppr_batched", "pg_ppr",
     "pg_graph_multi", "pg_pagerank_multigraph",
-    "pg_wpr_forward", "pg_wpr_adjoint",
+    "pg_wpr_forward", "pg_wpr_adjoint", "pg_spmv_adjoint",
 ]
 
 
@@ -119,6 +120,7 @@ def load():
         "pg_pagerank": [P, ctypes.c_int, f64, P, u32, SP],
         "pg_pagerank_delta": [P, ctypes.c_int, f64, f64, P, u32, SP],
         "pg_spmv": [P, ctypes.c_int, P, P, u32, SP],
+        "pg_spmv_adjoint": [P, P, P, P, u32, SP],
         "pg_bp": [P, ctypes.c_int, P, P, P, P, u32, SP],
         "pg_bfs": [P, i64, i64, P, P, u32, SP],
         "pg_bfs_async": [P, i64, P, P, u32, SP],
@@ -341,14 +343,28 @@ class Graph:
                                         _flags(profile), ctypes.byref(st)), "pg_pagerank_delta")
         return p, st
 
-    def spmv(self, x0, iters=20, out=True, profile=False, serial_cold=False):
+    def spmv(self, x0, iters=20, out=True, profile=False, serial_cold=False, save=False):
+        """numa-SPMV.C: fp64 y_T (host n) and stats; save keeps the iterates for
+        spmv_adjoint."""
         x0 = np.ascontiguousarray(x0, np.float32)
         y = np.zeros(self.n, np.float64) if out else None
         st = Stats()
-        flags = _flags(profile) | (PG_SERIAL_COLD if serial_cold else 0)
+        flags = (_flags(profile) | (PG_SERIAL_COLD if serial_cold else 0) |
+                 (PG_SPMV_SAVE if save else 0))
         _check(load().pg_spmv(self._h, iters, _ptr(x0), _ptr(y), flags,
                               ctypes.byref(st)), "pg_spmv")
         return y, st
+
+    def spmv_adjoint(self, gvec, out=True, profile=False):
+        """Gradients of sum_i gvec_i y_T[i] after spmv(..., save=True): (dL/dx0 host n
+        fp64, dL/dw host m fp64 in input CSR order, stats)."""
+        gvec = np.ascontiguousarray(gvec, np.float32)
+        gx = np.zeros(self.n, np.float64) if out else None
+        gw = np.zeros(self.m, np.float64) if out else None
+        st = Stats()
+        _check(load().pg_spmv_adjoint(self._h, _ptr(gvec), _ptr(gx), _ptr(gw), _flags(profile),
+                                      ctypes.byref(st)), "pg_spmv_adjoint")
+        return gx, gw, st
 
     def bp(self, vpot, prod0, iters=10, epot=None, out=True, profile=False, fp32=False):
         vpot = np.ascontiguousarray(vpot, np.float32).ravel()

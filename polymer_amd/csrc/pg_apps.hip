@@ -1601,8 +1601,16 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
     unsigned *maxbits = nullptr;
     int32_t *shift = nullptr;
     long long *total = nullptr;
-    PG_TRY(workspace_t(g, "spmv_xa", (size_t)n, &xa));
-    PG_TRY(workspace_t(g, "spmv_xb", (size_t)n, &xb));
+    const bool save = (flags & PG_SPMV_SAVE) != 0;
+    g->spmv_saved_iters = 0;  // the kept iterates (if any) are overwritten below
+    float *saved = nullptr;   // save: y_t (scaled) at saved + t n, t = 0..iters
+    if (save) {
+        PG_TRY(workspace_t(g, "spmv_saved", (size_t)n * (size_t)(iters + 1), &saved));
+        xa = saved;
+    } else {
+        PG_TRY(workspace_t(g, "spmv_xa", (size_t)n, &xa));
+        PG_TRY(workspace_t(g, "spmv_xb", (size_t)n, &xb));
+    }
     PG_TRY(workspace_t(g, "spmv_max", (size_t)iters + 2, &maxbits));
     PG_TRY(workspace_t(g, "spmv_shift", (size_t)iters + 2, &shift));
     PG_TRY(workspace_t(g, "spmv_total", 2, &total));
@@ -1646,15 +1654,21 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
     const bool ahead = tiled && g->tiles.cold_overlap && g->tiles.cold.nparts > 0 &&
                        (flags & PG_SERIAL_COLD) == 0;
     for (int t = 0; t < iters; t++) {
+        if (save) {  // every iterate in its own slot
+            cur = saved + (size_t)t * n;
+            nxt = saved + (size_t)(t + 1) * n;
+        }
         // iteration t scales its output by 2^-shift[t+1]; shift[t+1] comes from max(x_t)
         SpmvOp op{cur, w, nxt, shift + t + 1, tscale ? tscale + t : nullptr};
-        const SpmvOp next{nxt, w, cur, shift + t + 2, tscale ? tscale + t + 1 : nullptr};
+        const SpmvOp next{nxt, w, save ? nxt + n : cur, shift + t + 2,
+                          tscale ? tscale + t + 1 : nullptr};
         PG_TRY(kt.begin());
         if (tiled) {
             hipLaunchKernelGGL(k_spmv_tscale, dim3(1), dim3(64), 0, stream(),
                                t == 0 ? x0max : maxbits + t, wmax, tscale + t);
             // the zero-in-degree tail holds zeros in both buffers from iteration 2
-            PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, maxbits + t + 1, t >= 2,
+            // (a fresh slot of the kept iterates does not: it is written every time)
+            PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, maxbits + t + 1, t >= 2 && !save,
                                      ahead && t + 1 < iters ? &next : nullptr, ahead && t > 0));
         } else {
             PG_TRY(launch_pull(op, g->in_sched, maxbits + t + 1));
@@ -1667,6 +1681,7 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
     }
     double ms = 0;
     PG_TRY(tm.stop(&ms));
+    if (save) g->spmv_saved_iters = iters > 0 ? iters : 0;
     if (st) {
         st->rounds = iters;
         st->dense_rounds = iters;
@@ -1686,6 +1701,168 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
     PG_TRY(workspace_t(g, "spmv_out", (size_t)n, &tmp));
     PG_LAUNCH_K(k_to_f64_scaled, grid_for(n), cur, g->perm, n, total, tmp);
     PG_CHECK_HIP(hipMemcpyAsync(y, tmp, 8 * n, hipMemcpyDeviceToHost, stream()));
+    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    return PG_OK;
+}
+
+// ---- SpMV adjoint -----------------------------------------------------------------------
+// Backward step t of L = g . y_T (pull over the CSR: the transposed graph):
+//     mu_t[s] = sum_{e=(s->d)} w[e] mu_{t+1}[d]     (stored scaled by 2^-shift)
+//     grad[e] (+)= y_t[s] mu_{t+1}[d] 2^(E_t + F_{t+1})  (fp64; the first step stores)
+// with y_t = ys (stored as pg_spmv keeps it, true value ys 2^E_t) and mu_{t+1} stored
+// scaled by 2^-F_{t+1}; *fac = 2^(E_t + F_{t+1}).
+struct SpmvAdjOp {
+    using Acc = float;
+    using Ctx = double;  // y_t[s] * 2^(E_t + F_{t+1})
+    using Stat = unsigned;  // bits of max |mu_t| as stored
+    static constexpr bool kEarlyExit = false;
+    static constexpr bool kStats = true;
+    const float *mu;   // mu_{t+1}, stored
+    const float *w;    // CSR order
+    const float *ys;   // y_t, stored
+    float *mun;        // mu_t, stored
+    double *grad;      // CSR order
+    const double *fac;
+    const int32_t *shift;
+    int first;
+    __device__ bool active(int) const { return true; }
+    __device__ double ctx(int s) const { return (double)ys[s] * *fac; }
+    __device__ float identity() const { return 0.f; }
+    __device__ float combine(float a, float b) const { return a + b; }
+    __device__ float edge(int d, int e, const double &yf) const {
+        const float md = mu[d];
+        const double term = yf * (double)md;
+        grad[e] = first ? term : grad[e] + term;
+        return w[e] * md;
+    }
+    __device__ bool found(float, const double &) const { return false; }
+    __device__ Stat finish(int s, float sum, const double &) const {
+        const float r = ldexpf(sum, -*shift);
+        mun[s] = r;
+        return __float_as_uint(fabsf(r));
+    }
+    __device__ void skip(int) const {}
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a > b ? a : b; }
+    __device__ static void stat_commit(Stat *p, Stat v) { atomicMax(p, v); }
+};
+
+// Exponent bookkeeping of backward step t (one thread): shiftA[t] from the stored
+// max of mu_{t+1}; F[t] = F[t+1] + shiftA[t]; fac[t] = 2^(E_t + F[t+1]) where
+// E_t = shiftF[0] + ... + shiftF[t] (the forward's applied shifts).
+__global__ void k_spmv_adj_step(const unsigned *maxbits_next, const int32_t *shiftF, int t,
+                                int32_t *shiftA, long long *F, double *fac) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const float mx = __uint_as_float(*maxbits_next);
+        int s = 0;
+        if (mx > 0.f && isfinite(mx)) s = ilogbf(mx) - 64;
+        shiftA[t] = s;
+        long long E = 0;
+        for (int k = 0; k <= t; k++) E += shiftF[k];
+        F[t] = F[t + 1] + s;
+        fac[t] = ldexp(1.0, (int)(E + F[t + 1]));
+    }
+}
+
+// mu_T = g scaled: shiftA[T] from max |g|, F[T] = shiftA[T]
+__global__ void k_spmv_adj_init(const unsigned *maxbits, int T, int32_t *shiftA, long long *F) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const float mx = __uint_as_float(*maxbits);
+        int s = 0;
+        if (mx > 0.f && isfinite(mx)) s = ilogbf(mx) - 64;
+        shiftA[T] = s;
+        F[T] = s;
+    }
+}
+
+__global__ void k_scatter_f64(const double *in, const int32_t *idx, int64_t m, double *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[idx[i]] = in[i];
+}
+
+extern "C" int pg_spmv_adjoint(pg_graph *g, const float *gvec, double *grad_x0, double *grad_w,
+                               uint32_t flags, pg_stats *st) {
+    PG_TRY(check_graph(g));
+    PG_REQUIRE(gvec, PG_ERR_ARG, "null gvec");
+    PG_REQUIRE(g->spmv_saved_iters > 0, PG_ERR_STATE,
+               "pg_spmv has not been run with PG_SPMV_SAVE on this graph");
+    const int T = g->spmv_saved_iters;
+    const int64_t n = g->n, m = g->m;
+    PG_TRY(ensure_out_schedule(g));
+    // fp32 weights in internal CSR order
+    const float *w = nullptr;
+    if (g->w_dtype == PG_W_F32) {
+        w = g->out_w_f32;
+    } else {
+        PG_REQUIRE(g->w_dtype == PG_W_I32, PG_ERR_STATE, "SpMV needs a weighted graph");
+        float *wf = nullptr;
+        const bool have = g->ws.find("spmv_wf_out") != g->ws.end();
+        PG_TRY(workspace_t(g, "spmv_wf_out", (size_t)m, &wf));
+        if (!have) PG_LAUNCH_K(k_i32_to_f32, grid_for(m), g->out_w_i32, m, wf);
+        w = wf;
+    }
+    float *saved = nullptr, *ma = nullptr, *mb = nullptr;
+    int32_t *shiftF = nullptr, *shiftA = nullptr;
+    unsigned *maxA = nullptr;
+    long long *F = nullptr;
+    double *fac = nullptr, *grad = nullptr;
+    PG_TRY(workspace_t(g, "spmv_saved", (size_t)n * (size_t)(T + 1), &saved));
+    PG_TRY(workspace_t(g, "spmv_shift", (size_t)T + 2, &shiftF));
+    PG_TRY(workspace_t(g, "spmv_adj_a", (size_t)n, &ma));
+    PG_TRY(workspace_t(g, "spmv_adj_b", (size_t)n, &mb));
+    PG_TRY(workspace_t(g, "spmv_adj_shift", (size_t)T + 2, &shiftA));
+    PG_TRY(workspace_t(g, "spmv_adj_max", (size_t)T + 2, &maxA));
+    PG_TRY(workspace_t(g, "spmv_adj_F", (size_t)T + 2, &F));
+    PG_TRY(workspace_t(g, "spmv_adj_fac", (size_t)T + 1, &fac));
+    PG_TRY(workspace_t(g, "spmv_adj_grad", (size_t)m, &grad));
+    PG_TRY(from_input_order_f32(g, gvec, ma, 1));
+    KernelTimer kt;
+    kt.on = (flags & PG_PROFILE_KERNELS) != 0;
+    Timer tm;
+    PG_TRY(tm.start());
+    PG_TRY(fill(maxA, 0u, T + 2));
+    // max |g| (slot T+1) sets mu_T's scale; slot T then holds the max of mu_T as
+    // stored, from which the first backward step takes its shift
+    PG_LAUNCH_K(k_absmax_bits, grid_for(n, 2048), ma, n, maxA + T + 1);
+    hipLaunchKernelGGL(k_spmv_adj_init, dim3(1), dim3(64), 0, stream(), maxA + T + 1, T, shiftA,
+                       F);
+    PG_CHECK_HIP(hipGetLastError());
+    PG_LAUNCH_K(k_ldexp_inplace, grid_for(n), ma, n, shiftA + T);
+    PG_LAUNCH_K(k_absmax_bits, grid_for(n, 2048), ma, n, maxA + T);
+    float *cur = ma, *nxt = mb;  // cur = mu_{t+1}
+    for (int t = T - 1; t >= 0; t--) {
+        hipLaunchKernelGGL(k_spmv_adj_step, dim3(1), dim3(64), 0, stream(), maxA + t + 1, shiftF,
+                           t, shiftA, F, fac);
+        PG_CHECK_HIP(hipGetLastError());
+        SpmvAdjOp op{cur, w, saved + (size_t)t * n, nxt, grad, fac + t, shiftA + t,
+                     t == T - 1 ? 1 : 0};
+        PG_TRY(kt.begin());
+        PG_TRY(launch_pull(op, g->out_sched, maxA + t));
+        PG_TRY(kt.end());
+        std::swap(cur, nxt);
+    }
+    double ms = 0;
+    PG_TRY(tm.stop(&ms));
+    if (st) {
+        st->rounds = T;
+        st->dense_rounds = T;
+        st->edges = (int64_t)T * m;
+        st->active = (int64_t)T * n;
+        st->ms = ms;
+        PG_TRY(kt.total(&st->kernel_ms, &st->kernel_launches));
+    }
+    if (grad_x0) {  // mu_0 = stored * 2^F[0]
+        double *tmp = nullptr;
+        PG_TRY(workspace_t(g, "spmv_out", (size_t)n, &tmp));
+        PG_LAUNCH_K(k_to_f64_scaled, grid_for(n), cur, g->perm, n, F, tmp);
+        PG_CHECK_HIP(hipMemcpyAsync(grad_x0, tmp, 8 * n, hipMemcpyDeviceToHost, stream()));
+    }
+    if (grad_w) {
+        double *tmp = nullptr;
+        PG_TRY(workspace_t(g, "spmv_adj_gw_in", (size_t)m, &tmp));
+        PG_LAUNCH_K(k_scatter_f64, grid_for(m), grad, g->out_eid, m, tmp);
+        PG_CHECK_HIP(hipMemcpyAsync(grad_w, tmp, 8 * m, hipMemcpyDeviceToHost, stream()));
+    }
     PG_CHECK_HIP(hipStreamSynchronize(stream()));
     return PG_OK;
 }

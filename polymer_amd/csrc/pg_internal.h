@@ -259,6 +259,9 @@ struct pg_graph {
     // rounds of the first captured batch of the next BFS (from the last search)
     int bfs_rounds_hint = 0;
     bool wpr_valid = false;
+    // iterations of the last pg_spmv run with PG_SPMV_SAVE (its iterates and exponent
+    // shifts are kept for pg_spmv_adjoint), 0 = none kept
+    int spmv_saved_iters = 0;
     double wpr_damping = 0.85;
     // allocations owned by the graph
     std::vector<void *> owned;

@@ -203,6 +203,48 @@ def test_wpr_adjoint_dense_and_fd(pa, orc):
         assert abs(gt[i] - fd) <= 1e-4 * max(abs(fd), 1e-3), (i, gt[i], fd)
 
 
+
+def test_spmv_adjoint_vs_oracle(pa, orc):
+    """pg_spmv_adjoint (numa-SPMV.C's iteration differentiated: a pull over the CSR
+    per step, the weight gradient accumulated per edge) against the oracle's fp64
+    adjoint, which tests/test_oracle.py pins by the dense-matrix gradient and
+    central differences: fp32 weights on the tiles and on the degree-binned pull,
+    integer weights, and a second loss on the same kept iterates."""
+    say("case: spmv adjoint")
+    g = orc.rmat(13, 1 << 17, seed=41)
+    rng = np.random.default_rng(3)
+    for wkind in ("f32", "i32"):
+        if wkind == "f32":
+            w = orc.values_f32(g, 5)
+            d = pa.Graph.from_csr(g.n, g.off, g.adj, w=w)
+        else:
+            w = orc.weights_i32(g, 6)
+            d = pa.Graph.from_csr(g.n, g.off, g.adj, w=w)
+        x0 = rng.random(g.n).astype(np.float32)
+        for tiles in (1, 0):
+            d.set_tiles(tiles)
+            for T in (1, 6):
+                y_plain, _ = d.spmv(x0, T)
+                y, _ = d.spmv(x0, T, save=True)
+                assert np.array_equal(y, y_plain)  # keeping the iterates changes nothing
+                for k in range(2):
+                    gv = rng.standard_normal(g.n).astype(np.float32)
+                    gx, gw, st = d.spmv_adjoint(gv)
+                    rx, rw = orc.spmv_adjoint(g, w.astype(np.float64), x0.astype(np.float64),
+                                              gv.astype(np.float64), T)
+                    ex, ew = rel_l1(gx, rx), rel_l1(gw, rw)
+                    say("%s weights, tiles %d, T %d: dL/dx0 relL1 %.3e, dL/dw relL1 %.3e" %
+                        (wkind, tiles, T, ex, ew))
+                    assert ex < TOL and ew < TOL
+                    assert st.rounds == T
+        d.close()
+    # no kept iterates: refused
+    d = pa.Graph.from_csr(g.n, g.off, g.adj, w=orc.values_f32(g, 5))
+    d.spmv(np.ones(g.n, np.float32), 3)
+    with pytest.raises(pa.PolymerError):
+        d.spmv_adjoint(np.ones(g.n, np.float32))
+    d.close()
+
 def test_wpr_torch_autograd(pa, orc):
     """A torch.autograd.Function over the C ABI (harness only): gradients flow
     through the column normalisation of raw weights done in torch."""

@@ -330,3 +330,24 @@ def test_wpr_adjoint_fd_scale20(pa, orc):
         tol = 1e-4 * abs(f) + 1e-6 * (scale_w if kind == "w" else scale_v)
         say("  %s[%d]: adjoint %.9e, central difference %.9e" % (kind, i, got, f))
         assert abs(got - f) <= tol, (kind, i, got, f)
+
+
+def test_spmv_adjoint_scale22_20_iterations(pa, orc):
+    """The SpMV adjoint over 20 iterations on the scale-22 rMat (fp32 values U(0,1],
+    x0 U[0,1), g ~ N(0,1)) against the oracle's fp64 adjoint."""
+    say("case: spmv adjoint scale 22, 20 iterations, against the fp64 oracle")
+    g = pa.Graph.rmat(22, 1 << 26, seed=1, weights="f32", wseed=3)
+    rng = np.random.default_rng(3)
+    x0 = rng.random(g.n).astype(np.float32)
+    gv = rng.standard_normal(g.n).astype(np.float32)
+    _, stf = g.spmv(x0, 20, save=True)
+    gx, gw, sta = g.spmv_adjoint(gv)
+    say("  forward %.3f ms, adjoint %.3f ms" % (stf.ms, sta.ms))
+    off, adj, w = g.download()
+    g.close()
+    rx, rw = orc.spmv_adjoint(orc.Graph(g.n, off, adj), w.astype(np.float64),
+                              x0.astype(np.float64), gv.astype(np.float64), 20)
+    ex, ew = rel_l1(gx, rx), rel_l1(gw, rw)
+    say("  dL/dx0 rel L1 %.3e, dL/dw rel L1 %.3e (max |dL/dw| %.3e)" %
+        (ex, ew, float(np.abs(rw).max())))
+    assert ex < 1e-6 and ew < 1e-6

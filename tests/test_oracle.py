@@ -539,3 +539,43 @@ def test_pull_omp_variants_bit_identical(orc):
     for iters, want in BP_HAND_ANSWER.items():
         assert np.array_equal(orc.bp_pull_omp(gh, BP_HAND_VPOT, BP_HAND_EPOT, BP_HAND_PROD0, iters),
                               want)
+
+
+def test_spmv_adjoint_dense_and_fd(orc):
+    """or_spmv_adjoint against the dense-matrix gradient (numpy: y_{t+1} = A y_t,
+    mu_t = A^T mu_{t+1}, dL/dA[d, s] = sum_t mu_{t+1}[d] y_t[s]) and fp64 central
+    differences of or_spmv's loss (a planted sign or index error fails both)."""
+    g = orc.rmat(9, 4000, seed=31)
+    src = np.repeat(np.arange(g.n), np.diff(g.off))
+    rng = np.random.default_rng(2)
+    w = rng.uniform(0.05, 1.0, g.m) / 4.0
+    x0 = rng.random(g.n)
+    gv = rng.standard_normal(g.n)
+    T = 5
+    A = np.zeros((g.n, g.n))
+    A[g.adj, src] = w
+    ys = [x0]
+    for _ in range(T):
+        ys.append(A @ ys[-1])
+    mus = [None] * (T + 1)
+    mus[T] = gv
+    for t in range(T - 1, -1, -1):
+        mus[t] = A.T @ mus[t + 1]
+    gw_dense = sum(mus[t + 1][g.adj] * ys[t][src] for t in range(T))
+    gx, gw = orc.spmv_adjoint(g, w, x0, gv, T)
+    assert rel_l1(gx, mus[0]) < 1e-13
+    assert rel_l1(gw, gw_dense) < 1e-13
+    assert abs(gv @ orc.spmv(g, w, x0, T) - gv @ ys[T]) < 1e-10 * abs(gv @ ys[T])
+    h = 1e-6
+    for e in (0, 17, g.m // 2, g.m - 1):
+        wp, wm = w.copy(), w.copy()
+        wp[e] += h
+        wm[e] -= h
+        fd = (gv @ orc.spmv(g, wp, x0, T) - gv @ orc.spmv(g, wm, x0, T)) / (2 * h)
+        assert abs(gw[e] - fd) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(gw).max()), (e, gw[e], fd)
+    for i in (1, 100, g.n - 1):
+        xp, xm = x0.copy(), x0.copy()
+        xp[i] += h
+        xm[i] -= h
+        fd = (gv @ orc.spmv(g, w, xp, T) - gv @ orc.spmv(g, w, xm, T)) / (2 * h)
+        assert abs(gx[i] - fd) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(gx).max()), (i, gx[i], fd)
