@@ -32,6 +32,7 @@ static hipStream_t g_side = nullptr;  // work overlapped with the library stream
 static int g_device = -1;
 static bool g_init = false;
 
+
 void set_error(const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -212,19 +213,134 @@ static int select_flagged(const uint64_t *in, const uint8_t *flags, uint64_t *ou
     return PG_OK;
 }
 
-// Scans inside traversal rounds keep their temporary storage in a graph workspace:
-// a hipFree per call would wait for the device every round.
-int exclusive_scan_u32(pg_graph *g, const uint32_t *in, uint32_t *out, int64_t n) {
-    size_t tb = 0, tmax = 0;
-    PG_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, g_stream));
-    // sized for the longest scan of this graph (n + 1 items) on the first call
-    PG_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmax, in, out,
-                                                  std::max<int64_t>(n, g->n + 1), g_stream));
-    tb = std::max(tb, tmax);
-    void *tmp = nullptr;
-    PG_TRY(workspace(g, "scan_tmp", tb, &tmp));
-    PG_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, g_stream));
+// ---- exclusive scans of 32-bit counts (frontier compaction, push degree scans) ----------
+// Wave scans by shuffles, 16 waves of a 1024-thread workgroup joined in LDS.  A scan of
+// n items is one workgroup when n fits a few tiles, otherwise three launches: tile sums,
+// one workgroup over the tile sums, tiles rescanned with their offsets (2 reads and one
+// write of the items; no temporary storage beyond one word per 4096 items).
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+
+// exclusive scan of one value per thread over the workgroup; *total = the sum.  sm holds
+// kScanThreads / 64 + 1 words; the call ends with a barrier, so it can be repeated.
+__device__ __forceinline__ uint32_t workgroup_exclusive_scan(uint32_t v, uint32_t *sm,
+                                                             uint32_t *total) {
+    constexpr int W = kScanThreads / 64;
+    const uint32_t inc = wave_inclusive_scan(v);
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    if (lane == 63) sm[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        const uint32_t x = lane < W ? sm[lane] : 0u;
+        const uint32_t xi = wave_inclusive_scan(x);
+        if (lane < W) sm[lane] = xi - x;
+        if (lane == W - 1) sm[W] = xi;
+    }
+    __syncthreads();
+    const uint32_t r = sm[w] + inc - v;
+    *total = sm[W];
+    __syncthreads();
+    return r;
+}
+
+// out[i] = in[0] + ... + in[i-1] for i < n, by one workgroup walking tiles with a carry
+// (in and out may be the same array); out[n] = the total when with_total
+__global__ __launch_bounds__(kScanThreads) void k_scan_one_group(const uint32_t *in, uint32_t *out,
+                                                                 int64_t n, int with_total) {
+    __shared__ uint32_t sm[kScanThreads / 64 + 1];
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < n; base += kScanTile) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+        uint32_t v[kScanItems], s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            v[k] = i0 + k < n ? in[i0 + k] : 0u;
+            s += v[k];
+        }
+        uint32_t tot = 0;
+        uint32_t run = carry + workgroup_exclusive_scan(s, sm, &tot);
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            if (i0 + k < n) out[i0 + k] = run;
+            run += v[k];
+        }
+        carry += tot;
+    }
+    if (with_total && threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(const uint32_t *in, int64_t n,
+                                                                 uint32_t *sums) {
+    __shared__ uint32_t sm[kScanThreads / 64 + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) s += i0 + k < n ? in[i0 + k] : 0u;
+    uint32_t tot = 0;
+    (void)workgroup_exclusive_scan(s, sm, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint32_t *in, int64_t n,
+                                                             const uint32_t *offs, uint32_t *out) {
+    __shared__ uint32_t sm[kScanThreads / 64 + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems], s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = i0 + k < n ? in[i0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot = 0;
+    uint32_t run = offs[blockIdx.x] + workgroup_exclusive_scan(s, sm, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (i0 + k < n) out[i0 + k] = run;
+        run += v[k];
+    }
+}
+
+// out[i] = in[0] + ... + in[i-1], i < n, on the library stream; tile sums (one word per
+// 4096 items) in the caller's buffer tmp (scan_tmp_words(n) words), or in a graph
+// workspace when tmp is null
+int64_t scan_tmp_words(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+int exclusive_scan_u32_on(hipStream_t st, const uint32_t *in, uint32_t *out, int64_t n,
+                          uint32_t *tmp) {
+    if (n <= 0) return PG_OK;
+    if (n <= 4 * (int64_t)kScanTile) {
+        hipLaunchKernelGGL(k_scan_one_group, dim3(1), dim3(kScanThreads), 0, st, in, out, n, 0);
+    } else {
+        const int64_t nt = (n + kScanTile - 1) / kScanTile;
+        PG_REQUIRE(nt < 0x7FFFFFFF, PG_ERR_UNSUPPORTED, "scan: too many items");
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nt), dim3(kScanThreads), 0, st, in, n,
+                           tmp);
+        hipLaunchKernelGGL(k_scan_one_group, dim3(1), dim3(kScanThreads), 0, st, tmp, tmp, nt, 0);
+        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(kScanThreads), 0, st, in, n, tmp,
+                           out);
+    }
+    PG_CHECK_HIP(hipGetLastError());
     return PG_OK;
+}
+
+// Scans inside traversal rounds keep their tile sums in a graph workspace sized for the
+// longest scan of the graph (n + 1 items): no allocation after the first call.
+int exclusive_scan_u32(pg_graph *g, const uint32_t *in, uint32_t *out, int64_t n) {
+    uint32_t *tmp = nullptr;
+    PG_TRY(workspace_t(g, "scan_tmp", (size_t)scan_tmp_words(std::max<int64_t>(n, g->n + 1)),
+                       &tmp));
+    return exclusive_scan_u32_on(g_stream, in, out, n, tmp);
 }
 
 static int bits_for(int64_t n) {
@@ -571,7 +687,7 @@ int bits_to_list(pg_graph *g, const uint32_t *bits, int32_t *list, unsigned long
     PG_TRY(workspace_t(g, "compact_bc", (size_t)nb + 1, &bc));
     PG_TRY(workspace_t(g, "compact_bo", (size_t)nb + 1, &bo));
     PG_LAUNCH(k_word_counts, (unsigned)nb, bits, nwords, bc);
-    PG_TRY(exclusive_scan_u32(g, bc, bo, nb));
+    hipLaunchKernelGGL(k_scan_one_group, dim3(1), dim3(kScanThreads), 0, g_stream, bc, bo, nb, 0);
     PG_LAUNCH(k_word_write, (unsigned)nb, bits, nwords, g->n, bo, list, d_count, nb);
     return PG_OK;
 }
