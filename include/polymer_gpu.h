@@ -183,6 +183,16 @@ int pg_subset_from_sparse(pg_subset *s, const int32_t *ids, int64_t k); /* host 
 int pg_subset_to_dense(pg_subset *s, uint8_t *flags);
 /* toSparse = packIndex (polymer.h:552-567): ascending ids, host. */
 int pg_subset_to_sparse(pg_subset *s, int32_t *ids, int64_t cap, int64_t *k);
+/* No counterpart in the reference: pg_subset_to_sparse for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Writes the ascending input ids of s to d_ids (device, at
+ * least n entries) and their number to *d_count (device int64), asynchronously
+ * on stream (a hipStream_t; NULL = the library stream).  Device memory beyond
+ * the caller's comes from one static array per device (about 4 MiB, set up by
+ * pg_init): n above 2^25 vertices is refused with PG_ERR_UNSUPPORTED.  Every call
+ * on a device shares that array, so calls that overlap on one device race; the
+ * subset must not be modified while the call runs. */
+int pg_subset_to_sparse_noalloc(pg_subset *s, int32_t *d_ids, int64_t *d_count, void *stream);
 /* vertexCounter (polymer.h:2312-2338): |F| and sum of out-degrees. */
 int pg_subset_size(pg_subset *s, int64_t *count, int64_t *out_edges);
 

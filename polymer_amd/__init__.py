@@ -41,7 +41,7 @@ EXPORTS = [
     "pg_graph_bp_potentials", "pg_graph_view",
     "pg_subset_create", "pg_subset_destroy", "pg_subset_clear", "pg_subset_fill",
     "pg_subset_from_dense", "pg_subset_from_sparse", "pg_subset_to_dense",
-    "pg_subset_to_sparse", "pg_subset_size",
+    "pg_subset_to_sparse", "pg_subset_to_sparse_noalloc", "pg_subset_size",
     "pg_pagerank", "pg_pagerank_delta", "pg_spmv", "pg_bp", "pg_bfs", "pg_bfs_async", "pg_cc",
     "pg_sssp",
     "pg_msbfs", "pg_mssssp", "pg_ppr_batched", "pg_ppr",
@@ -116,6 +116,7 @@ def load():
         "pg_subset_from_sparse": [P, P, i64],
         "pg_subset_to_dense": [P, P],
         "pg_subset_to_sparse": [P, P, i64, P],
+        "pg_subset_to_sparse_noalloc": [P, P, P, P],
         "pg_subset_size": [P, P, P],
         "pg_pagerank": [P, ctypes.c_int, f64, P, u32, SP],
         "pg_pagerank_delta": [P, ctypes.c_int, f64, f64, P, u32, SP],

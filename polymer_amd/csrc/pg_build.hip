@@ -32,6 +32,18 @@ static hipStream_t g_side = nullptr;  // work overlapped with the library stream
 static int g_device = -1;
 static bool g_init = false;
 
+// Device memory of pg_subset_to_sparse_noalloc: a file-scope array, so the entry point
+// allocates nothing.  Its host-side name is not a device address: the address is taken
+// with hipGetSymbolAddress at pg_init, once per device (each device holds its own copy),
+// and only that pointer is used.  Layout: the input-order bitmap (kNoAllocMaxN / 32 + 1
+// words), then per-block counts and their offsets (kNoAllocBlocks words each).  About
+// 4.03 MiB per device, held until the process ends.
+constexpr int64_t kNoAllocMaxN = (int64_t)1 << 25;
+constexpr int64_t kNoAllocWords = kNoAllocMaxN / 32 + 1;
+constexpr int64_t kNoAllocBlocks = (kNoAllocWords + 255) / 256 + 1;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords + 2 * kNoAllocBlocks];
+constexpr int kMaxDevices = 64;
+static uint32_t *g_scratch[kMaxDevices] = {};
 
 void set_error(const char *fmt, ...) {
     va_list ap;
@@ -1584,6 +1596,11 @@ extern "C" int pg_init(int device) {
     PG_REQUIRE(device >= 0 && device < count, PG_ERR_ARG, "device %d not present (%d devices)",
                device, count);
     PG_CHECK_HIP(hipSetDevice(device));
+    if (device < kMaxDevices && !g_scratch[device]) {
+        void *p = nullptr;
+        PG_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[device] = static_cast<uint32_t *>(p);
+    }
     PG_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     PG_CHECK_HIP(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
     g_device = device;
@@ -2161,6 +2178,48 @@ extern "C" int pg_subset_to_dense(pg_subset *s, uint8_t *flags) {
     PG_CHECK_HIP(hipMemcpyAsync(hb.data(), ib, 4 * nw, hipMemcpyDeviceToHost, g_stream));
     PG_CHECK_HIP(hipStreamSynchronize(g_stream));
     for (int64_t u = 0; u < g->n; u++) flags[u] = (hb[u >> 5] >> (u & 31)) & 1u;
+    return PG_OK;
+}
+
+// toSparse of a subset into caller-owned device memory, allocating nothing: the
+// kernels of pg_subset_to_sparse (input-order bitmap, per-block popcounts, one-group
+// scan, ordered write) on the file-scope scratch array.  Asynchronous on the given
+// stream (null: the library stream).
+extern "C" int pg_subset_to_sparse_noalloc(pg_subset *s, int32_t *d_ids, int64_t *d_count,
+                                           void *stream) {
+    PG_REQUIRE(g_init, PG_ERR_STATE, "pg_init has not been called");
+    PG_REQUIRE(s && d_ids && d_count, PG_ERR_ARG, "null argument");
+    pg_graph *g = s->g;
+    PG_REQUIRE(g->n <= kNoAllocMaxN, PG_ERR_UNSUPPORTED,
+               "pg_subset_to_sparse_noalloc: n = %lld exceeds its maximum of %lld vertices",
+               (long long)g->n, (long long)kNoAllocMaxN);
+    PG_REQUIRE(g_device >= 0 && g_device < kMaxDevices && g_scratch[g_device], PG_ERR_STATE,
+               "no scratch address for device %d", g_device);
+    uint32_t *ib = g_scratch[g_device];
+    uint32_t *bc = ib + kNoAllocWords;
+    uint32_t *bo = bc + kNoAllocBlocks;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g_stream;
+    const int64_t n = g->n, nw = (n + 31) / 32;
+    const int64_t nb = (nw + kBlock - 1) / kBlock;
+    // the bitmap keeps the last call's contents: cleared with a kernel (no memset node
+    // in a captured graph)
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(grid_for(nw + 1)), dim3(kBlock), 0, st, ib, 0u,
+                       nw + 1);
+    hipLaunchKernelGGL(k_bits_to_input_bits, dim3(grid_for(n)), dim3(kBlock), 0, st,
+                       (const uint32_t *)s->bits, (const int32_t *)g->perm, n, ib);
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_word_counts, dim3((unsigned)nb), dim3(kBlock), 0, st,
+                           (const uint32_t *)ib, nw, bc);
+        hipLaunchKernelGGL(k_scan_one_group, dim3(1), dim3(kScanThreads), 0, st,
+                           (const uint32_t *)bc, bo, nb, 0);
+        hipLaunchKernelGGL(k_word_write, dim3((unsigned)nb), dim3(kBlock), 0, st,
+                           (const uint32_t *)ib, nw, n, (const uint32_t *)bo, d_ids,
+                           reinterpret_cast<unsigned long long *>(d_count), nb);
+    } else {
+        hipLaunchKernelGGL(k_fill<int64_t>, dim3(1), dim3(kBlock), 0, st, d_count, (int64_t)0,
+                           (int64_t)1);
+    }
+    PG_CHECK_HIP(hipGetLastError());
     return PG_OK;
 }
 
