@@ -1,57 +1,179 @@
 /*
  * polymer.h -- the edgeMap / vertexMap programming interface of Polymer on one
  * MI355X (gfx950).  Header-only C++ compiled by hipcc over the device graphs of
- * libpolymer_gpu.so: an app keeps the reference's types and calls
+ * libpolymer_gpu.so.  An application keeps the reference's types and calls
  *
- *     graph<vertex>, wghGraph<vertex>           graph.h:11-164
- *     LocalFrontier, vertices (vertexSubset)    polymer.h:527-881
+ *     graph<vertex>, wghGraph<vertex>, vertex structs     graph.h:11-164
+ *     LocalFrontier, vertices (vertexSubset)              polymer.h:521-881
  *     edgeMap, edgeMapDense, edgeMapDenseForward, edgeMapDenseReduce,
- *     edgeMapSparseV3, edgeMapDenseBP           polymer.h:924-1306, 1928-2003, 2241-2309
+ *     edgeMapDenseBP, edgeMapSparseV3, edgeMapSparseAsync(Pipe), ...
+ *                                                         polymer.h:924-2309
  *     vertexMap, vertexFilter, vertexCounter,
- *     clearLocalFrontier, switchFrontier        polymer.h:2224-2441
- *     CAS, writeMin, writeAdd                   utils.h:244-310
- *     mapDataArray                              polymer.h:499-519
+ *     clearLocalFrontier, switchFrontier                  polymer.h:2224-2441
+ *     CAS, writeMin, writeAdd, fetchAndAdd                utils.h:244-310
+ *     mapDataArray, partitionByDegree, graphFilter, graphHasher,
+ *     Subworker_Partitioner, Custom_barrier, Default_Hash_F,
+ *     Default_worker_arg, dummyPartitioner                polymer.h:66-519, 883-919;
+ *                                                         custom-barrier.h:20-37
+ *     parallel_for, parallel_main, intT, intE             parallel.h
  *
- * and its functors (update / updateAtomic / cond, the weighted (s, d, w) and
- * BP (s, d, j) forms, the initFunc / reduceFunc / combineFunc trio) run as
- * device code.  Functor classes written between POLYMER_FUNCTORS_BEGIN and
- * POLYMER_FUNCTORS_END (clang's force_cuda_host_device region) need no HIP
- * annotations: the reference's functor code compiles for the GPU as it is.
+ * so the reference's programs compile with this header in place of its own.
+ * Their functors (update / updateAtomic / cond, the weighted (s, d, w) and BP
+ * (s, d, j) forms, the initFunc / reduceFunc / combineFunc trio) run as device
+ * code: either compile with hipcc --hipstdpar (unannotated functions called from
+ * kernels are compiled for the GPU), or write the functor classes between
+ * POLYMER_FUNCTORS_BEGIN and POLYMER_FUNCTORS_END.
  *
- * What differs from the NUMA engine, by design:
- *   - one GPU replaces the NUMA nodes: every call is made once (the node /
- *     sub-worker arguments are accepted; only node 0, sub-worker 0 launches);
- *     a call returns when its kernels have been queued on the library stream
- *     (hipStream of the graph), and counts the host reads are synchronised;
- *   - value arrays from mapDataArray and frontier arrays from
- *     newFrontierArray are device memory: initialise them with vertexMap or
- *     copy with toDevice / toHost, never through host loops;
- *   - vertex ids are the ids of the input graph (the library's relabelling
- *     stays inside libpolymer_gpu.so).
- * The dense pull keeps the reference's per-destination order: one thread
- * scans a destination's in-neighbours in id order, calls updateAtomic and
- * stops when cond turns false (edgeMapDense polymer.h:924-968).
+ * One GPU replaces the NUMA machine:
+ *   - numa_num_configured_nodes() and numa_num_configured_cpus() report one node
+ *     with one core, so an SPMD program runs one worker thread and every
+ *     collective call is made once (its barriers count one thread);
+ *     graphFilter / graphFilter2Direction of the one node's range return the
+ *     whole graph, which stays in HBM;
+ *   - memory from mapDataArray and numa_alloc_local is visible to the host and to
+ *     the GPU at one address (managed memory), so the programs' host loops over
+ *     value and frontier arrays work unchanged; on a GPU without page migration
+ *     the kernels then reach it over the host link.  Compiled with
+ *     POLYMER_DEVICE_RESIDENT defined, those arrays and graph.V live in HBM
+ *     instead (initialise them with vertexMap or toDevice; host loops over them
+ *     must go);
+ *   - every call returns when its kernels are done (the reference's calls are
+ *     synchronous), on the library stream of libpolymer_gpu.so;
+ *   - vertex ids are the ids of the input graph.
+ * The edge maps run on the engine's schedules: the dense pull binned by
+ * in-degree (a thread, 16 lanes, a wavefront or a workgroup per destination,
+ * stopping when cond turns false), the pushes edge-balanced over the active
+ * sources' out-edges (a frontier's hub is spread over many workgroups), the
+ * sparse push appending its successes with one atomic per wavefront.
  */
 #ifndef POLYMER_HIP_H
 #define POLYMER_HIP_H
 
 #include <hip/hip_runtime.h>
+#include <limits.h>
+#include <pthread.h>
+#include <sys/mman.h>
+#include <sys/time.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <string>
 #include <type_traits>
 #include <vector>
 
+#if __has_include(<numa.h>)
+#include <numa.h>
+#define POLYMER_HAVE_LIBNUMA 1
+#endif
+
 #include "polymer_gpu.h"
 
+/* ---- parallel.h (its sequential branch: one host thread per worker) ------- */
+#ifndef parallel_for
+#define cilk_spawn
+#define cilk_sync
+#define parallel_main main
+#define parallel_for for
+#define parallel_for_1 for
+#define parallel_for_256 for
+#define cilk_for for
+#endif
 typedef int intT;
 typedef unsigned int uintT;
 typedef int intE;
 typedef unsigned int uintE;
+#ifndef INT_T_MAX
+#define INT_T_MAX INT_MAX
+#define PRIintT "d"
+#define PRIuintT "u"
+#define PRIintE "d"
+#define PRIuintE "u"
+#endif
+#ifndef PAGESIZE
+#define PAGESIZE (4096)
+#endif
+#ifndef MIN
+#define MIN(x, y) ((x > y) ? (y) : (x))
+#endif
 
 #define PG_HD __host__ __device__
+
+/* ---- gettime.h: the apps' phase timer ("<phase> : <seconds>") -------------- */
+#ifndef _BENCH_GETTIME_INCLUDED
+#define _BENCH_GETTIME_INCLUDED
+struct timer {
+    double totalTime = 0.0, lastTime = 0.0, totalWeight = 0.0;
+    bool on = false;
+    double getTime() {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+            .count();
+    }
+    void start() {
+        on = true;
+        lastTime = getTime();
+    }
+    double stop() {
+        on = false;
+        const double d = getTime() - lastTime;
+        totalTime += d;
+        return d;
+    }
+    double stop(double weight) {
+        on = false;
+        totalWeight += weight;
+        const double d = getTime() - lastTime;
+        totalTime += weight * d;
+        return d;
+    }
+    double total() { return on ? totalTime + getTime() - lastTime : totalTime; }
+    double next() {
+        if (!on) return 0.0;
+        const double t = getTime(), d = t - lastTime;
+        totalTime += d;
+        lastTime = t;
+        return d;
+    }
+    void reportT(double time) { std::cout << std::setprecision(3) << time << std::endl; }
+    void reportTime(double time) { reportT(time); }
+    void reportStop(double weight, std::string str) {
+        std::cout << str << " :" << weight << ": ";
+        reportTime(stop(weight));
+    }
+    void reportTotal() {
+        reportTime(totalWeight > 0.0 ? total() / totalWeight : total());
+        totalTime = 0.0;
+        totalWeight = 0.0;
+    }
+    void reportTotal(std::string str) {
+        std::cout << str << " : ";
+        reportTotal();
+    }
+    void reportNext() { reportTime(next()); }
+    void reportNext(std::string str) {
+        std::cout << str << " : ";
+        reportNext();
+    }
+};
+static timer _tm;
+#define timeStatement(_A, _string) \
+    _tm.start();                   \
+    _A;                            \
+    _tm.reportNext(_string);
+#define startTime() _tm.start();
+#define stopTime(_weight, _str) _tm.reportStop(_weight, _str);
+#define reportTime(_str) _tm.reportTotal(_str);
+#define nextTime(_string) _tm.reportNext(_string);
+#define nextTimeN() _tm.reportT(_tm.next());
+#endif
 
 namespace polymer_detail {
 
@@ -71,7 +193,12 @@ inline void check_hip(hipError_t e, const char *what) {
     }
 }
 
-// the stream of the library (set by the first graph; every call is ordered on it)
+[[noreturn]] inline void fail(const char *msg) {
+    std::fprintf(stderr, "polymer: %s\n", msg);
+    std::abort();
+}
+
+// the library stream: every call of this header is ordered on it
 inline hipStream_t &lib_stream() {
     static hipStream_t s = nullptr;
     return s;
@@ -85,10 +212,14 @@ inline void ensure_init() {
     }
 }
 
-inline unsigned blocks_for(int64_t n, int threads = 256) {
+inline void sync() { check_hip(hipStreamSynchronize(lib_stream()), "hipStreamSynchronize"); }
+
+inline void launched(const char *what) { check_hip(hipGetLastError(), what); }
+
+inline unsigned blocks_for(int64_t n, int threads = 256, int64_t cap = 65536) {
     int64_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;
-    if (b > 65536) b = 65536;
+    if (b > cap) b = cap;
     return (unsigned)b;
 }
 
@@ -99,7 +230,84 @@ T *dev_alloc(size_t count) {
     return static_cast<T *>(p);
 }
 
+// Named device buffers kept for the life of the process, grown on demand (an
+// allocation only when a call needs more than any call before it).
+inline void *workspace(const char *name, size_t bytes) {
+    static std::map<std::string, std::pair<void *, size_t>> ws;
+    auto &e = ws[name];
+    if (e.second < bytes) {
+        if (e.first) check_hip(hipFree(e.first), "hipFree");
+        e.first = nullptr;
+        check_hip(hipMalloc(&e.first, bytes + 64), "hipMalloc");
+        e.second = bytes;
+    }
+    return e.first;
+}
+template <class T>
+T *workspace_t(const char *name, size_t count) {
+    return static_cast<T *>(workspace(name, (count ? count : 1) * sizeof(T)));
+}
+
+// Memory the application's host loops and the kernels both address: managed
+// memory (one address on the host and the GPU), zero-filled like the reference's
+// anonymous mappings.  With POLYMER_DEVICE_RESIDENT: device memory.
+inline void *shared_alloc(size_t bytes) {
+    ensure_init();
+    void *p = nullptr;
+    const size_t b = bytes ? bytes : 1;
+#ifdef POLYMER_DEVICE_RESIDENT
+    check_hip(hipMalloc(&p, b), "hipMalloc");
+    check_hip(hipMemsetAsync(p, 0, b, lib_stream()), "hipMemsetAsync");
+    sync();
+#else
+    check_hip(hipMallocManaged(&p, b, hipMemAttachGlobal), "hipMallocManaged");
+    std::memset(p, 0, b);
+#endif
+    return p;
+}
+
+inline void shared_free(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+// A device-readable view of count elements at p: p itself when the GPU can address
+// it (device, managed or pinned memory), else a staged copy (a list the program
+// built in malloc'd memory, e.g. numa-BFS.C:380).
+template <class T>
+const T *device_readable(const T *p, size_t count, const char *stage) {
+    if (!p || count == 0) return p;
+    hipPointerAttribute_t a{};
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e == hipSuccess && a.type != hipMemoryTypeUnregistered &&
+        (a.type != hipMemoryTypeHost || a.devicePointer != nullptr))
+        return p;
+    (void)hipGetLastError();  // clear the lookup's error state
+    T *d = workspace_t<T>(stage, count);
+    check_hip(hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, lib_stream()),
+              "hipMemcpyAsync");
+    return d;
+}
+
 }  // namespace polymer_detail
+
+/* ---- libnuma: one node with one core --------------------------------------- */
+namespace polymer_detail {
+inline int numa_nodes() { return 1; }
+inline int numa_cpus() { return 1; }
+}  // namespace polymer_detail
+#define numa_num_configured_nodes() (polymer_detail::numa_nodes())
+#define numa_num_configured_cpus() (polymer_detail::numa_cpus())
+#define numa_alloc_local(sz) (polymer_detail::shared_alloc(sz))
+#define numa_alloc_onnode(sz, node) (polymer_detail::shared_alloc(sz))
+#define numa_alloc_interleaved(sz) (polymer_detail::shared_alloc(sz))
+#define numa_free(p, sz) (polymer_detail::shared_free(p))
+#ifndef POLYMER_HAVE_LIBNUMA
+struct bitmask;
+inline bitmask *numa_parse_nodestring(const char *) { return nullptr; }
+inline void numa_bind(bitmask *) {}
+inline void numa_set_interleave_mask(bitmask *) {}
+static bitmask *numa_all_nodes_ptr = nullptr;
+#endif
 
 /* ---- copies between host and the device arrays of an app ----------------- */
 template <class T>
@@ -107,14 +315,14 @@ void toDevice(T *dev, const T *host, size_t count) {
     polymer_detail::check_hip(hipMemcpyAsync(dev, host, count * sizeof(T), hipMemcpyHostToDevice,
                                              polymer_detail::lib_stream()),
                               "hipMemcpyAsync");
-    polymer_detail::check_hip(hipStreamSynchronize(polymer_detail::lib_stream()), "sync");
+    polymer_detail::sync();
 }
 template <class T>
 void toHost(T *host, const T *dev, size_t count) {
     polymer_detail::check_hip(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost,
                                              polymer_detail::lib_stream()),
                               "hipMemcpyAsync");
-    polymer_detail::check_hip(hipStreamSynchronize(polymer_detail::lib_stream()), "sync");
+    polymer_detail::sync();
 }
 
 /* ---- atomics (utils.h:244-310) --------------------------------------------- */
@@ -159,15 +367,145 @@ PG_HD inline bool writeMin(ET *a, ET b) {
 #endif
 }
 
+// On device memory (POLYMER_DEVICE_RESIDENT) floating-point additions use the
+// hardware's atomic add; memory the host also maps takes the compare-and-swap form.
 template <class ET>
 PG_HD inline void writeAdd(ET *a, ET b) {
 #if __HIP_DEVICE_COMPILE__
+#ifdef POLYMER_DEVICE_RESIDENT
+    if constexpr (std::is_same<ET, double>::value || std::is_same<ET, float>::value)
+        unsafeAtomicAdd(a, b);
+    else
+        atomicAdd(a, b);
+#else
     atomicAdd(a, b);
+#endif
 #else
     ET c = *a;
     while (!CAS(a, c, (ET)(c + b))) c = *(volatile ET *)a;
 #endif
 }
+
+template <class ET>
+PG_HD inline ET fetchAndAdd(ET *a, ET b) {
+#if __HIP_DEVICE_COMPILE__
+    return atomicAdd(a, b);
+#else
+    return __atomic_fetch_add(a, b, __ATOMIC_SEQ_CST);
+#endif
+}
+
+/* ---- custom-barrier.h: a counting barrier over two shared words ------------ */
+#ifndef CUSTOM_BARRIER
+#define CUSTOM_BARRIER
+// Custom_barrier (custom-barrier.h:20-37): count threads meet; the last one to
+// arrive resets the counter and flips the toggle word the others wait on.
+struct Custom_barrier {
+    volatile int *counter = nullptr;
+    volatile int *toggleVar = nullptr;
+    int count = 0;
+    Custom_barrier() {}
+    Custom_barrier(volatile int *_counter, volatile int *_toggle, int _count)
+        : counter(_counter), toggleVar(_toggle), count(_count) {}
+    inline void wait() {
+        if (!counter || !toggleVar || count <= 1) return;  // one worker: nothing to meet
+        const int before = __atomic_load_n(toggleVar, __ATOMIC_ACQUIRE);
+        if (__atomic_add_fetch(counter, 1, __ATOMIC_ACQ_REL) >= count) {
+            __atomic_store_n(counter, 0, __ATOMIC_RELAXED);
+            __atomic_store_n(toggleVar, before ^ 1, __ATOMIC_RELEASE);
+        } else {
+            while (__atomic_load_n(toggleVar, __ATOMIC_ACQUIRE) == before) __builtin_ia32_pause();
+        }
+    }
+};
+#endif
+
+/* ---- SPMD roles and worker arguments (polymer.h:66-129, 883-919) ----------- */
+inline long long roundUp(double x) {
+    long long ones = (long long)x;
+    return (x - (double)ones > 0) ? ones + 1 : ones;
+}
+
+struct Subworker_Partitioner {
+    int tid = 0;
+    int subTid = 0;
+    int numOfSub = 1;
+    intT dense_start = 0;
+    intT dense_end = 0;
+    pthread_barrier_t *global_barr = nullptr;
+    pthread_barrier_t *local_barr = nullptr;
+    pthread_barrier_t *leader_barr = nullptr;
+    Custom_barrier local_custom;
+    Custom_barrier subMaster_custom;
+    explicit Subworker_Partitioner(int nSub) : numOfSub(nSub) {}
+    inline bool isMaster() const { return tid + subTid == 0; }
+    inline bool isSubMaster() const { return subTid == 0; }
+    inline intT getStartPos(intT m) const { return subTid * (m / numOfSub); }
+    inline intT getEndPos(intT m) const {
+        return subTid == numOfSub - 1 ? m : (subTid + 1) * (m / numOfSub);
+    }
+    inline void localWait() { local_custom.wait(); }
+    inline void globalWait() {
+        local_custom.wait();
+        if (isSubMaster()) subMaster_custom.wait();
+        local_custom.wait();
+    }
+};
+
+// the shard-interleaving relabel of the apps (identity for one shard)
+struct Default_Hash_F {
+    int shardNum;
+    intT vertPerShard;
+    intT n;
+    Default_Hash_F(intT _n, int _shardNum) : shardNum(_shardNum), vertPerShard(_n / _shardNum), n(_n) {}
+    inline intT hashFunc(intT index) {
+        if (index >= shardNum * vertPerShard) return index;
+        return (index % shardNum) * vertPerShard + index / shardNum;
+    }
+    inline intT hashBackFunc(intT index) {
+        if (index >= shardNum * vertPerShard) return index;
+        return index / vertPerShard + (index % vertPerShard) * shardNum;
+    }
+};
+
+struct Default_worker_arg {
+    void *GA;
+    int maxIter;
+    int tid;
+    int numOfNode;
+    intT rangeLow;
+    intT rangeHi;
+};
+
+struct LocalFrontier;
+struct Default_subworker_arg {
+    void *GA;
+    void *Global_G;
+    int maxIter;
+    int tid;
+    int subTid;
+    intT startPos;
+    intT endPos;
+    intT rangeLow;
+    intT rangeHi;
+    pthread_barrier_t *global_barr;
+    pthread_barrier_t *node_barr;
+    pthread_barrier_t *master_barr;
+    LocalFrontier *localFrontier;
+    volatile int *local_custom_counter;
+    volatile int *local_custom_toggle;
+};
+
+struct nonNegF {
+    bool operator()(intT a) { return a >= 0; }
+};
+
+// edgeMap options (polymer.h:917-919)
+enum options { DENSE, DENSE_PARALLEL, DENSE_FORWARD };
+
+inline Subworker_Partitioner dummyPartitioner(1);
+
+inline Subworker_Partitioner &polymerDummyPartitioner() { return dummyPartitioner; }
 
 /* ---- vertices of the device graph (graph.h:11-164) -------------------------- */
 struct symmetricVertex {
@@ -237,11 +575,44 @@ __global__ void k_fill_vertices(pg_view v, vertex *V) {
     }
 }
 
+// Destinations of a graph binned by in-degree for the dense pull (built on first
+// use, dropped with the graph): bin g is ids[start[g], start[g] + count[g]), g = 0
+// a thread per destination (in-degree 1-16), 1 sixteen lanes (17-256), 2 a
+// wavefront (257-4096), 3 a workgroup (more); heavier bins come first in ids.
+struct Bins {
+    intT *ids = nullptr;
+    int64_t start[4] = {0};
+    int64_t count[4] = {0};
+    int64_t nonzero = 0;  // destinations with in-edges
+};
+inline std::map<const void *, Bins> &bins_cache() {
+    static std::map<const void *, Bins> cache;
+    return cache;
+}
+
+// Owner of one library graph and the vertex array over it; graph<vertex> copies
+// share it (a program's filtered "local" graph of the one node is the graph itself)
+struct GraphOwner {
+    pg_graph *handle = nullptr;
+    void *V = nullptr;
+    const void *key = nullptr;  // the graph's in-offsets, key of its bins
+    ~GraphOwner() {
+        auto it = bins_cache().find(key);
+        if (it != bins_cache().end()) {
+            if (it->second.ids) (void)hipFree(it->second.ids);
+            bins_cache().erase(it);
+        }
+        if (V) (void)hipFree(V);
+        if (handle) pg_graph_destroy(handle);
+    }
+};
+
 }  // namespace polymer_detail
 
 /* ---- graphs ------------------------------------------------------------------ */
-// graph<vertex> (graph.h:61-87): V is a device array of n vertices; the
-// arrays behind it belong to the library graph handle.
+// graph<vertex> (graph.h:61-87).  V holds n vertices whose neighbour pointers
+// address the device adjacency of the library graph; V itself is host-visible
+// (device-resident under POLYMER_DEVICE_RESIDENT).
 template <class vertex>
 struct graph {
     vertex *V = nullptr;
@@ -249,9 +620,10 @@ struct graph {
     long m = 0;
     pg_graph *handle = nullptr;
     pg_view view{};
+    std::shared_ptr<polymer_detail::GraphOwner> own;
+    // this copy lets go of the graph; the last copy to do so frees it
     void del() {
-        if (V) (void)hipFree(V);
-        if (handle) pg_graph_destroy(handle);
+        own.reset();
         V = nullptr;
         handle = nullptr;
     }
@@ -263,23 +635,37 @@ struct wghGraph : graph<vertex> {};
 namespace polymer_detail {
 
 template <class G>
-G wrap(pg_graph *h, bool weighted) {
-    G g;
+struct is_weighted : std::false_type {};
+template <class V>
+struct is_weighted<wghGraph<V>> : std::true_type {};
+
+inline const Bins &bins_of(const pg_view &v);
+
+template <class G>
+void wrap_into(G &g, pg_graph *h, bool weighted) {
     g.handle = h;
     check(pg_graph_view(h, &g.view), "pg_graph_view");
     lib_stream() = static_cast<hipStream_t>(g.view.stream);
-    if (weighted && g.view.w_dtype != PG_W_I32) {
-        std::fprintf(stderr, "polymer: weighted graph without int32 weights\n");
-        std::abort();
-    }
+    if (weighted && g.view.w_dtype != PG_W_I32) fail("weighted graph without int32 weights");
     g.n = (long)g.view.n;
     g.m = (long)g.view.m;
     using vertex = typename std::remove_pointer<decltype(g.V)>::type;
-    g.V = dev_alloc<vertex>((size_t)g.n);
+    g.own = std::make_shared<GraphOwner>();
+    g.own->handle = h;
+    g.own->key = g.view.in_off;
+    g.V = static_cast<vertex *>(shared_alloc(sizeof(vertex) * (size_t)g.n));
+    g.own->V = g.V;
     hipLaunchKernelGGL(k_fill_vertices<vertex>, dim3(blocks_for(g.n)), dim3(256), 0, lib_stream(),
                        g.view, g.V);
-    check_hip(hipGetLastError(), "k_fill_vertices");
-    check_hip(hipStreamSynchronize(lib_stream()), "sync");
+    launched("k_fill_vertices");
+    sync();
+    (void)bins_of(g.view);  // the dense pull's in-degree bins, built with the graph
+}
+
+template <class G>
+G wrap(pg_graph *h, bool weighted) {
+    G g;
+    wrap_into(g, h, weighted);
     return g;
 }
 
@@ -293,6 +679,19 @@ graph<vertex> readGraph(const char *fname, bool isSymmetric, bool binary = false
     polymer_detail::ensure_init();
     pg_graph *h = nullptr;
     const uint32_t flags = isSymmetric ? PG_GRAPH_SYMMETRIC : 0u;
+    if (binary)
+        polymer_detail::check(pg_graph_load_bin(fname, flags, &h), "pg_graph_load_bin");
+    else
+        polymer_detail::check(pg_graph_load_adj(fname, flags, &h), "pg_graph_load_adj");
+    return polymer_detail::wrap<graph<vertex>>(h, false);
+}
+
+// readGraphSkipRing (IO-numa.h:192-315): the same without self-loops
+template <class vertex>
+graph<vertex> readGraphSkipRing(const char *fname, bool isSymmetric, bool binary = false) {
+    polymer_detail::ensure_init();
+    pg_graph *h = nullptr;
+    const uint32_t flags = (isSymmetric ? PG_GRAPH_SYMMETRIC : 0u) | PG_GRAPH_SKIP_SELF_LOOPS;
     if (binary)
         polymer_detail::check(pg_graph_load_bin(fname, flags, &h), "pg_graph_load_bin");
     else
@@ -338,72 +737,861 @@ wghGraph<vertex> rmatWghGraph(int scale, long m, uint64_t seed, bool symmetrize,
     return polymer_detail::wrap<wghGraph<vertex>>(h, true);
 }
 
+/* ---- NUMA partitioning of the graph (polymer.h:131-497) --------------------- */
+namespace polymer_detail {
+
+// degrees of the graph in input ids (in- or out-), on the host
+template <class G>
+std::vector<int32_t> host_degrees(const G &GA, bool out) {
+    std::vector<int32_t> off((size_t)GA.n + 1);
+    toHost(off.data(), out ? GA.view.out_off : GA.view.in_off, off.size());
+    std::vector<int32_t> deg((size_t)GA.n);
+    for (long i = 0; i < GA.n; i++) deg[i] = off[i + 1] - off[i];
+    return deg;
+}
+
+// cut [lo, hi) into k consecutive ranges of about equal degree sum, each a whole
+// number of pages of eleSize-byte elements except the last
+inline void split_by_degree(const std::vector<int32_t> &deg, intT lo, intT hi, int k,
+                            intT *sizeArr, int eleSize) {
+    if (k <= 1) {
+        sizeArr[0] = hi - lo;
+        return;
+    }
+    long long total = 0;
+    for (intT i = lo; i < hi; i++) total += deg[i];
+    const intT page = std::max<intT>(1, PAGESIZE / std::max(1, eleSize));
+    intT start = lo;
+    long long acc = 0;
+    int shard = 0;
+    for (intT i = lo; i < hi && shard < k - 1; i++) {
+        acc += deg[i];
+        if (acc * k >= total * (shard + 1)) {
+            intT end = std::min<intT>(hi, lo + ((i + 1 - lo + page - 1) / page) * page);
+            if (end < start) end = start;
+            sizeArr[shard++] = end - start;
+            start = end;
+            i = end - 1;
+        }
+    }
+    while (shard < k - 1) sizeArr[shard++] = 0;
+    sizeArr[k - 1] = hi - start;
+}
+
+}  // namespace polymer_detail
+
+// partitionByDegree (polymer.h:131-192): node ranges of equal in- (or out-)degree
+template <class G>
+void partitionByDegree(const G &GA, int numOfShards, intT *sizeArr, intT sizeOfOneEle,
+                       bool useOutDegree = false) {
+    if (numOfShards <= 1) {
+        sizeArr[0] = (intT)GA.n;
+        return;
+    }
+    polymer_detail::split_by_degree(polymer_detail::host_degrees(GA, useOutDegree), 0, (intT)GA.n,
+                                    numOfShards, sizeArr, sizeOfOneEle);
+}
+
+// subPartitionByDegree (polymer.h:194-282): the sub-worker ranges of one node
+template <class G>
+void subPartitionByDegree(const G &GA, int numOfShards, intT *sizeArr, int sizeOfOneEle,
+                          bool useOutDegree = false, bool useFakeDegree = false) {
+    (void)useFakeDegree;
+    if (numOfShards <= 1) {
+        sizeArr[0] = (intT)GA.n;
+        return;
+    }
+    polymer_detail::split_by_degree(polymer_detail::host_degrees(GA, useOutDegree), 0, (intT)GA.n,
+                                    numOfShards, sizeArr, sizeOfOneEle);
+}
+
+template <class G>
+void subPartitionByDegree(const G &GA, int numOfShards, intT *sizeArr, int sizeOfOneEle,
+                          intT subStart, intT subEnd, bool useOutDegree = false,
+                          bool useFakeDegree = false) {
+    (void)useFakeDegree;
+    if (numOfShards <= 1) {
+        sizeArr[0] = subEnd - subStart;
+        return;
+    }
+    polymer_detail::split_by_degree(polymer_detail::host_degrees(GA, useOutDegree), subStart,
+                                    subEnd, numOfShards, sizeArr, sizeOfOneEle);
+}
+
+namespace polymer_detail {
+
+// relabel the graph by hash.hashFunc (a permutation of [0, n)); nothing to do
+// for the identity, which is what one shard gives
+template <class G, class Hash_F>
+void relabel(G &GA, Hash_F &hash) {
+    const long n = GA.n;
+    std::vector<int32_t> h((size_t)n);
+    bool identity = true;
+    for (long i = 0; i < n; i++) {
+        h[i] = (int32_t)hash.hashFunc((intT)i);
+        identity = identity && h[i] == i;
+    }
+    if (identity) return;
+    std::vector<int64_t> off((size_t)n + 1);
+    std::vector<int32_t> adj((size_t)GA.m);
+    std::vector<int32_t> w;
+    const bool weighted = GA.view.w_dtype == PG_W_I32;
+    if (weighted) w.resize((size_t)GA.m);
+    check(pg_graph_download(GA.handle, off.data(), adj.data(), weighted ? w.data() : nullptr),
+          "pg_graph_download");
+    std::vector<int64_t> noff((size_t)n + 1, 0);
+    for (long u = 0; u < n; u++) noff[(size_t)h[u] + 1] = off[u + 1] - off[u];
+    for (long v = 0; v < n; v++) noff[v + 1] += noff[v];
+    std::vector<int32_t> nadj((size_t)GA.m), nw(weighted ? (size_t)GA.m : 0);
+    for (long u = 0; u < n; u++) {
+        int64_t p = noff[(size_t)h[u]];
+        for (int64_t e = off[u]; e < off[u + 1]; e++, p++) {
+            nadj[(size_t)p] = h[(size_t)adj[(size_t)e]];
+            if (weighted) nw[(size_t)p] = w[(size_t)e];
+        }
+    }
+    pg_graph *nh = nullptr;
+    check(pg_graph_create(n, GA.m, noff.data(), nadj.data(), weighted ? nw.data() : nullptr,
+                          weighted ? PG_W_I32 : PG_W_NONE, GA.view.symmetric ? PG_GRAPH_SYMMETRIC : 0u,
+                          &nh),
+          "pg_graph_create");
+    GA.del();
+    wrap_into(GA, nh, weighted);
+}
+
+}  // namespace polymer_detail
+
+// graphHasher / graphInEdgeHasher / graphAllEdgeHasher (polymer.h:284-344): vertex
+// ids relabelled by hash.hashFunc, every edge list with them
+template <class G, class Hash_F>
+void graphHasher(G &GA, Hash_F hash) {
+    polymer_detail::relabel(GA, hash);
+}
+template <class G, class Hash_F>
+void graphInEdgeHasher(G &GA, Hash_F hash) {
+    polymer_detail::relabel(GA, hash);
+}
+template <class G, class Hash_F>
+void graphAllEdgeHasher(G &GA, Hash_F hash) {
+    polymer_detail::relabel(GA, hash);
+}
+
+// graphFilter / graphFilter2Direction (polymer.h:346-497): the edges of one node's
+// range.  One GPU is one node: its range is the whole graph, which is returned.
+template <class G>
+G graphFilter(G &GA, intT rangeLow, intT rangeHi, bool useOutEdge = true) {
+    (void)useOutEdge;
+    if (rangeLow != 0 || rangeHi != (intT)GA.n)
+        polymer_detail::fail("graphFilter: a range other than the whole graph (the GPU is one node)");
+    return GA;
+}
+template <class G>
+G graphFilter2Direction(G &GA, intT rangeLow, intT rangeHi) {
+    return graphFilter(GA, rangeLow, rangeHi);
+}
+
 /* ---- value arrays (polymer.h:499-519) ------------------------------------------ */
-// Device memory for sum(sizeArr[0..numOfShards)) elements of sizeOfOneEle bytes.
+// sum(sizeArr[0..numOfShards)) elements of sizeOfOneEle bytes, zeroed, visible to
+// the host and the GPU (device memory under POLYMER_DEVICE_RESIDENT)
 inline void *mapDataArray(int numOfShards, intT *sizeArr, int sizeOfOneEle) {
-    polymer_detail::ensure_init();
     size_t total = 0;
     for (int i = 0; i < numOfShards; i++) total += (size_t)sizeArr[i];
-    void *p = nullptr;
-    polymer_detail::check_hip(hipMalloc(&p, (total ? total : 1) * (size_t)sizeOfOneEle), "hipMalloc");
-    // zeroed on the library stream, where every later use of the array is ordered (a
-    // null-stream memset is not ordered with that non-blocking stream and could land
-    // after the app's first writes)
-    polymer_detail::check_hip(hipMemsetAsync(p, 0, (total ? total : 1) * (size_t)sizeOfOneEle,
-                                             polymer_detail::lib_stream()),
+    return polymer_detail::shared_alloc(total * (size_t)sizeOfOneEle);
+}
+
+inline void freeDataArray(void *p) { polymer_detail::shared_free(p); }
+
+// zeroed device memory for n elements of T (HBM whatever the mode)
+template <class T>
+T *mapDeviceArray(size_t n) {
+    polymer_detail::ensure_init();
+    T *p = polymer_detail::dev_alloc<T>(n);
+    polymer_detail::check_hip(hipMemsetAsync(p, 0, (n ? n : 1) * sizeof(T), polymer_detail::lib_stream()),
                               "hipMemsetAsync");
-    polymer_detail::check_hip(hipStreamSynchronize(polymer_detail::lib_stream()), "sync");
+    polymer_detail::sync();
     return p;
 }
 
-inline void freeDataArray(void *p) {
-    if (p) (void)hipFree(p);
-}
-
-// A device bool array of n entries, all false (the frontier storage the
-// reference takes from numa_alloc_local).
+// A bool array of n entries, all false: the frontier storage the reference takes
+// from numa_alloc_local.
 inline bool *newFrontierArray(intT n) {
-    intT sz = n;
-    return static_cast<bool *>(mapDataArray(1, &sz, (int)sizeof(bool)));
+    return static_cast<bool *>(polymer_detail::shared_alloc((size_t)n));
 }
 
-/* ---- frontiers (polymer.h:527-639, 642-881) ------------------------------------ */
+/* ---- frontier kernels ---------------------------------------------------------- */
 namespace polymer_detail {
 
-// ordered compaction of b[0, n) into s (ascending ids + offset); count -> *m
-__global__ void k_count_flags(const bool *b, int64_t n, int64_t per_block, unsigned *counts) {
-    __shared__ unsigned part[256];
-    const int64_t base = (int64_t)blockIdx.x * per_block;
-    unsigned c = 0;
-    for (int64_t i = base + threadIdx.x; i < base + per_block && i < n; i += blockDim.x) c += b[i] ? 1u : 0u;
-    part[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-        __syncthreads();
+constexpr int kT = 256;          // threads of most kernels
+constexpr int kChunk = 16 * kT;  // bytes of a frontier per compaction block
+
+__device__ __forceinline__ int lane() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int l = lane();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (l >= o) v += y;
     }
-    if (threadIdx.x == 0) counts[blockIdx.x] = part[0];
+    return v;
 }
 
-__global__ void k_write_flags(const bool *b, int64_t n, int64_t per_block, const unsigned *starts,
-                              intT offset, intT *s) {
-    __shared__ unsigned scan[256];
-    const int64_t base = (int64_t)blockIdx.x * per_block;
-    unsigned out = starts[blockIdx.x];
-    for (int64_t c0 = base; c0 < base + per_block && c0 < n; c0 += blockDim.x) {
-        const int64_t i = c0 + threadIdx.x;
-        const bool f = i < n && i < base + per_block && b[i];
-        scan[threadIdx.x] = f ? 1u : 0u;
-        __syncthreads();
-        for (int o = 1; o < 256; o <<= 1) {  // inclusive scan
-            const unsigned add = (int)threadIdx.x >= o ? scan[threadIdx.x - o] : 0u;
-            __syncthreads();
-            scan[threadIdx.x] += add;
-            __syncthreads();
+// exclusive scan over a 256-thread block; *tot = the sum; ends with a barrier
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sm, uint32_t *tot) {
+    const uint32_t inc = wave_incl_scan(v);
+    const int w = threadIdx.x >> 6, l = lane();
+    if (l == 63) sm[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < kT / 64; i++) {
+            const uint32_t x = sm[i];
+            sm[i] = run;
+            run += x;
         }
-        if (f) s[out + scan[threadIdx.x] - 1] = (intT)i + offset;
-        out += scan[255];
+        sm[kT / 64] = run;
+    }
+    __syncthreads();
+    const uint32_t r = sm[w] + inc - v;
+    *tot = sm[kT / 64];
+    __syncthreads();
+    return r;
+}
+
+// count of set flags per kChunk bytes
+__global__ __launch_bounds__(kT) void k_flag_counts(const bool *b, int64_t n, uint32_t *cnt) {
+    __shared__ uint32_t sm[kT / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * kChunk;
+    uint32_t c = 0;
+#pragma unroll 4
+    for (int k = 0; k < kChunk / kT; k++) {
+        const int64_t i = base + k * kT + threadIdx.x;
+        c += (i < n && b[i]) ? 1u : 0u;
+    }
+    uint32_t tot = 0;
+    (void)block_excl_scan(c, sm, &tot);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+// out[i] = in[0] + ... + in[i-1] for i < n, out[n] = total; one workgroup (n small)
+__global__ __launch_bounds__(kT) void k_scan_small(const uint32_t *in, uint32_t *out, int64_t n) {
+    __shared__ uint32_t sm[kT / 64 + 1];
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < n; base += kT) {
+        const int64_t i = base + threadIdx.x;
+        const uint32_t v = i < n ? in[i] : 0u;
+        uint32_t tot = 0;
+        const uint32_t x = block_excl_scan(v, sm, &tot);
+        if (i < n) out[i] = carry + x;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+// ordered compaction: the ids (+ offset) of the set flags, block by block
+__global__ __launch_bounds__(kT) void k_flag_write(const bool *b, int64_t n, const uint32_t *starts,
+                                                   intT offset, intT *s) {
+    __shared__ uint32_t sm[kT / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * kChunk;
+    uint32_t out = starts[blockIdx.x];
+    for (int k = 0; k < kChunk / kT; k++) {
+        const int64_t i = base + k * kT + threadIdx.x;
+        const bool f = i < n && b[i];
+        uint32_t tot = 0;
+        const uint32_t x = block_excl_scan(f ? 1u : 0u, sm, &tot);
+        if (f) s[out + x] = (intT)i + offset;
+        out += tot;
+    }
+}
+
+// the frontier's flags into an ordered list; returns the count (host)
+inline intT compact_flags(const bool *b, int64_t n, intT offset, intT *s) {
+    const int64_t nb = (n + kChunk - 1) / kChunk;
+    if (nb == 0) return 0;
+    uint32_t *cnt = workspace_t<uint32_t>("compact_cnt", (size_t)nb + 1);
+    uint32_t *st = workspace_t<uint32_t>("compact_start", (size_t)nb + 1);
+    hipLaunchKernelGGL(k_flag_counts, dim3((unsigned)nb), dim3(kT), 0, lib_stream(), b, n, cnt);
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kT), 0, lib_stream(), (const uint32_t *)cnt, st, nb);
+    hipLaunchKernelGGL(k_flag_write, dim3((unsigned)nb), dim3(kT), 0, lib_stream(), b, n,
+                       (const uint32_t *)st, offset, s);
+    launched("compaction");
+    uint32_t total = 0;
+    toHost(&total, st + nb, 1);
+    return (intT)total;
+}
+
+// scan over the out-degrees of list[0, count): deg scan in tiles of 4096
+constexpr int kScanTile = 16 * kT;
+
+__global__ __launch_bounds__(kT) void k_deg_tile_sums(const intT *list, int64_t count,
+                                                      const int32_t *off, uint32_t *sums) {
+    __shared__ uint32_t sm[kT / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t c = 0;
+    for (int k = 0; k < kScanTile / kT; k++) {
+        const int64_t i = base + k * kT + threadIdx.x;
+        if (i < count) {
+            const intT v = list[i];
+            c += (uint32_t)(off[v + 1] - off[v]);
+        }
+    }
+    uint32_t tot = 0;
+    (void)block_excl_scan(c, sm, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kT) void k_deg_tiles(const intT *list, int64_t count, const int32_t *off,
+                                                  const uint32_t *tile_start, uint32_t *scan) {
+    __shared__ uint32_t sm[kT / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t run = tile_start[blockIdx.x];
+    for (int k = 0; k < kScanTile / kT; k++) {
+        const int64_t i = base + k * kT + threadIdx.x;
+        uint32_t d = 0;
+        if (i < count) {
+            const intT v = list[i];
+            d = (uint32_t)(off[v + 1] - off[v]);
+        }
+        uint32_t tot = 0;
+        const uint32_t x = block_excl_scan(d, sm, &tot);
+        if (i < count) scan[i] = run + x;
+        run += tot;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) scan[count] = run;
+}
+
+// scan[i] = sum of outdeg(list[0..i)), scan[count] = total; returns the total (host)
+inline uint32_t degree_scan(const intT *list, int64_t count, const int32_t *off, uint32_t *scan) {
+    const int64_t nt = (count + kScanTile - 1) / kScanTile;
+    uint32_t *sums = workspace_t<uint32_t>("deg_sums", (size_t)nt + 1);
+    uint32_t *starts = workspace_t<uint32_t>("deg_starts", (size_t)nt + 1);
+    hipLaunchKernelGGL(k_deg_tile_sums, dim3((unsigned)nt), dim3(kT), 0, lib_stream(), list, count,
+                       off, sums);
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kT), 0, lib_stream(), (const uint32_t *)sums,
+                       starts, nt);
+    hipLaunchKernelGGL(k_deg_tiles, dim3((unsigned)nt), dim3(kT), 0, lib_stream(), list, count, off,
+                       (const uint32_t *)starts, scan);
+    launched("degree scan");
+    uint32_t total = 0;
+    toHost(&total, scan + count, 1);
+    return total;
+}
+
+template <bool W, class F>
+__device__ __forceinline__ bool call_update_atomic(F &f, intT s, intT d, const intE *w, int64_t e) {
+    if constexpr (W)
+        return f.updateAtomic(s, d, w[e]);
+    else
+        return f.updateAtomic(s, d);
+}
+
+// wave-aggregated append of val to list (capacity cap) when pred; the slot counter
+// is ctr[0]; appends beyond cap are counted in ctr[0] but not stored
+__device__ __forceinline__ void wave_append(intT *list, size_t cap, unsigned long long *ctr,
+                                            bool pred, intT val) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int l = lane();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (l == leader) base = atomicAdd(&ctr[0], (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) {
+        const unsigned long long pos = base + __popcll(mask & ((1ull << l) - 1ull));
+        if (pos < cap) list[pos] = val;
+    }
+}
+
+// Edge-balanced push over list[0, count) with degree scan `scan` (total edges):
+// block b takes the frontier's edges [b*epb, (b+1)*epb).  Mode 0 sets the next
+// flags (edgeMapDenseForward), 1 appends to the next list and adds the
+// destinations' out-degrees to ctr[1] (edgeMapSparseV3), 2 is mode 0 with the
+// edge's index in its source's list passed to updateAtomic (edgeMapDenseBP).
+constexpr int kWin = 1024;
+
+template <bool W, int Mode, class F>
+__global__ __launch_bounds__(kT) void k_push_edges(F f, pg_view v, const intT *list,
+                                                   const uint32_t *scan, int64_t count,
+                                                   uint32_t total, uint32_t epb, bool *next,
+                                                   intT *out_list, size_t cap,
+                                                   unsigned long long *ctr) {
+    __shared__ uint32_t s_scan[kWin + 1];
+    __shared__ int32_t s_beg[kWin];
+    __shared__ int64_t s_j0, s_j1;
+    const uint32_t E0 = (uint32_t)blockIdx.x * epb;
+    const uint32_t E1 = min(total, E0 + epb);
+    if (threadIdx.x == 0) {
+        int64_t lo = 0, hi = count + 1;  // last j with scan[j] <= E0
+        while (lo + 1 < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (scan[mid] <= E0) lo = mid; else hi = mid;
+        }
+        s_j0 = lo;
+        lo = 0;
+        hi = count + 1;  // last j with scan[j] <= E1 - 1
+        while (lo + 1 < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (scan[mid] <= E1 - 1) lo = mid; else hi = mid;
+        }
+        s_j1 = lo;
+    }
+    __syncthreads();
+    const int64_t j0 = s_j0, nwin = s_j1 - s_j0 + 1;
+    const bool staged = nwin <= kWin;
+    if (staged) {
+        for (int64_t i = threadIdx.x; i < nwin; i += kT) {
+            s_scan[i] = scan[j0 + i];
+            s_beg[i] = v.out_off[list[j0 + i]];
+        }
+    }
+    __syncthreads();
+    const intE *w = W ? static_cast<const intE *>(v.out_w) : nullptr;
+    unsigned long long outdeg = 0;
+    for (uint32_t k = E0 + threadIdx.x; k < E1; k += kT) {
+        int64_t j;
+        int32_t beg;
+        uint32_t sc;
+        if (staged) {
+            int lo = 0, hi = (int)nwin - 1;  // last i with s_scan[i] <= k
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_scan[mid] <= k) lo = mid; else hi = mid - 1;
+            }
+            j = j0 + lo;
+            beg = s_beg[lo];
+            sc = s_scan[lo];
+        } else {
+            int64_t lo = j0, hi = s_j1;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) >> 1;
+                if (scan[mid] <= k) lo = mid; else hi = mid - 1;
+            }
+            j = lo;
+            sc = scan[lo];
+            beg = v.out_off[list[lo]];
+        }
+        const intT src = list[j];
+        const int64_t e = (int64_t)beg + (k - sc);
+        const intT ngh = v.out_adj[e];
+        bool act;
+        if constexpr (Mode == 2)
+            act = f.cond(ngh) && f.updateAtomic(src, ngh, (intT)(k - sc));
+        else
+            act = f.cond(ngh) && call_update_atomic<W>(f, src, ngh, w, e);
+        if constexpr (Mode == 1) {
+            wave_append(out_list, cap, ctr, act, ngh);
+            if (act) outdeg += (unsigned long long)(v.out_off[ngh + 1] - v.out_off[ngh]);
+        } else {
+            if (act) next[ngh] = true;
+        }
+    }
+    if constexpr (Mode == 1) {
+        for (int o = 32; o >= 1; o >>= 1) outdeg += __shfl_xor(outdeg, o, 64);
+        if (lane() == 0 && outdeg) atomicAdd(&ctr[1 + (blockIdx.x & 31)], outdeg);
+    }
+}
+
+template <bool W, int Mode, class F>
+inline void push_list(F &f, const pg_view &v, const intT *list, int64_t count, bool *next,
+                      intT *out_list, size_t cap, unsigned long long *ctr) {
+    if (count <= 0) return;
+    uint32_t *scan = workspace_t<uint32_t>("push_scan", (size_t)count + 1);
+    const uint32_t total = degree_scan(list, count, v.out_off, scan);
+    if (total == 0) return;
+    // about 2048 edges per workgroup, fewer for small frontiers (at least 2048
+    // workgroups while each keeps a full wave of edges)
+    int64_t epb = std::min<int64_t>(2048, std::max<int64_t>(64, ((int64_t)total + 2047) / 2048));
+    const int64_t nb = ((int64_t)total + epb - 1) / epb;
+    hipLaunchKernelGGL((k_push_edges<W, Mode, F>), dim3((unsigned)nb), dim3(kT), 0, lib_stream(), f,
+                       v, list, (const uint32_t *)scan, count, total, (uint32_t)epb, next, out_list,
+                       cap, ctr);
+    launched("edge push");
+}
+
+// Dense pull over the in-degree bins of the graph (Bins).  A destination with
+// cond(d) scans its in-neighbours; an active one calling updateAtomic(s, d) with
+// success marks d in next; the scan stops once cond(d) is false (checked after
+// every neighbour for a thread, after every group of G edges for a group).
+__device__ __forceinline__ int bin_of(int deg) {
+    return deg == 0 ? 4 : deg <= 16 ? 0 : deg <= 256 ? 1 : deg <= 4096 ? 2 : 3;
+}
+
+// per-bin counts: reduced in the block, four atomics per block
+__global__ __launch_bounds__(kT) void k_bin_count(const int32_t *in_off, int64_t n,
+                                                  unsigned long long *cnt) {
+    __shared__ unsigned long long sm[4];
+    if (threadIdx.x < 4) sm[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long c[4] = {0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = bin_of(in_off[i + 1] - in_off[i]);
+        if (b < 4) c[b]++;
+    }
+    for (int b = 0; b < 4; b++)
+        if (c[b]) atomicAdd(&sm[b], c[b]);
+    __syncthreads();
+    if (threadIdx.x < 4 && sm[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], sm[threadIdx.x]);
+}
+
+// scatter into the bins with one atomic per wavefront and bin
+__global__ __launch_bounds__(kT) void k_bin_fill(const int32_t *in_off, int64_t n,
+                                                 unsigned long long *pos, intT *ids) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t rounds = (n + stride - 1) / stride;  // the same for every lane
+    for (int64_t r = 0; r < rounds; r++) {
+        const int64_t i = first + r * stride;
+        const int b = i < n ? bin_of(in_off[i + 1] - in_off[i]) : 4;
+        for (int k = 0; k < 4; k++) {
+            const unsigned long long mask = __ballot(b == k);
+            if (!mask) continue;
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane() == leader) base = atomicAdd(&pos[k], (unsigned long long)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (b == k) ids[base + __popcll(mask & ((1ull << lane()) - 1ull))] = (intT)i;
+        }
+    }
+}
+
+inline const Bins &bins_of(const pg_view &v) {
+    auto it = bins_cache().find(v.in_off);
+    if (it != bins_cache().end()) return it->second;
+    Bins b;
+    unsigned long long *cnt = workspace_t<unsigned long long>("bin_cnt", 8);
+    check_hip(hipMemsetAsync(cnt, 0, 8 * sizeof(unsigned long long), lib_stream()), "hipMemsetAsync");
+    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(v.n, kT, 4096)), dim3(kT), 0, lib_stream(),
+                       v.in_off, (int64_t)v.n, cnt);
+    unsigned long long h[8];
+    toHost(h, cnt, 8);
+    // heaviest bins first, so the workgroup-per-destination bin starts first
+    int64_t s = 0;
+    unsigned long long pos[8] = {0};
+    for (int g = 3; g >= 0; g--) {
+        b.start[g] = s;
+        b.count[g] = (int64_t)h[g];
+        pos[g] = (unsigned long long)s;
+        s += (int64_t)h[g];
+    }
+    b.nonzero = s;
+    b.ids = dev_alloc<intT>((size_t)s + 1);
+    toDevice(cnt, pos, 8);
+    hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(v.n, kT, 4096)), dim3(kT), 0, lib_stream(),
+                       v.in_off, (int64_t)v.n, cnt, b.ids);
+    launched("k_bin_fill");
+    sync();
+    return bins_cache().emplace(v.in_off, b).first->second;
+}
+
+template <int G>
+__device__ __forceinline__ bool group_any(bool p) {
+    const unsigned long long bal = __ballot(p);
+    if constexpr (G >= 64) {
+        return bal != 0ull;
+    } else {
+        const int base = lane() & ~(G - 1);
+        return ((bal >> base) & ((1ull << G) - 1ull)) != 0ull;
+    }
+}
+
+// the lanes of this lane's group of G whose predicate holds, as a mask in the group
+template <int G>
+__device__ __forceinline__ unsigned long long group_ballot(bool p) {
+    const unsigned long long bal = __ballot(p);
+    if constexpr (G >= 64) return bal;
+    else return (bal >> (lane() & ~(G - 1))) & ((1ull << G) - 1ull);
+}
+
+// G lanes per destination (G = 1, 16 or 64) over ids[0, count).  Within a group's
+// chunk of G in-neighbours the first active one (in list order) is applied
+// alone; if cond(d) still holds the chunk's other active ones are applied
+// together, and the scan stops as soon as cond(d) turns false.  A functor whose
+// first success ends the scan (the BFS claim) thus makes one update per
+// destination, on the in-neighbour the reference's sequential scan would take
+// first within the chunk; one that keeps cond true (a minimum, a sum) applies
+// every active in-neighbour.
+template <bool W, int G, class F>
+__global__ __launch_bounds__(kT) void k_pull_group(F f, pg_view v, const bool *cur, bool *next,
+                                                   const intT *ids, int64_t count) {
+    const int64_t slot = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
+    if (slot >= count) return;  // uniform within the group
+    const int l = lane() & (G - 1);
+    const intT d = ids[slot];
+    if (!group_any<G>(f.cond(d))) return;
+    const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
+    const int beg = v.in_off[d], end = v.in_off[d + 1];
+    bool hit = false;
+    if constexpr (G == 1) {
+        for (int e = beg; e < end; e++) {
+            const intT s = v.in_adj[e];
+            if (cur[s] && call_update_atomic<W>(f, s, d, w, e)) hit = true;
+            if (!f.cond(d)) break;
+        }
+        if (hit) next[d] = true;
+    } else {
+        for (int base = beg; base < end; base += G) {
+            const int e = base + l;
+            intT s = 0;
+            bool a = false;
+            if (e < end) {
+                s = v.in_adj[e];
+                a = cur[s];
+            }
+            const unsigned long long am = group_ballot<G>(a);
+            if (am) {
+                const int first = __ffsll((long long)am) - 1;
+                if (l == first && call_update_atomic<W>(f, s, d, w, e)) hit = true;
+                if (!group_any<G>(f.cond(d))) break;
+                if (a && l != first && call_update_atomic<W>(f, s, d, w, e)) hit = true;
+            }
+            if (!group_any<G>(f.cond(d))) break;
+        }
+        if (group_any<G>(hit) && l == 0) next[d] = true;
+    }
+}
+
+// a 256-thread workgroup per destination over ids[0, count)
+template <bool W, class F>
+__global__ __launch_bounds__(kT) void k_pull_block(F f, pg_view v, const bool *cur, bool *next,
+                                                   const intT *ids) {
+    __shared__ int s_stop, s_hit;
+    const intT d = ids[blockIdx.x];
+    if (threadIdx.x == 0) {
+        s_stop = f.cond(d) ? 0 : 1;
+        s_hit = 0;
+    }
+    __syncthreads();
+    if (s_stop) return;  // uniform over the block
+    const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
+    const int beg = v.in_off[d], end = v.in_off[d + 1];
+    for (int base = beg; base < end; base += kT) {
+        const int e = base + (int)threadIdx.x;
+        if (e < end) {
+            const intT s = v.in_adj[e];
+            if (cur[s] && call_update_atomic<W>(f, s, d, w, e)) s_hit = 1;
+        }
         __syncthreads();
+        if (threadIdx.x == 0 && !f.cond(d)) s_stop = 1;
+        __syncthreads();
+        if (s_stop) break;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_hit) next[d] = true;
+}
+
+template <bool W, class F>
+inline void pull_dense(F &f, const pg_view &v, const bool *cur, bool *next) {
+    const Bins &b = bins_of(v);
+    const int64_t nblk = b.count[3], nwave = b.count[2], n16 = b.count[1], n1 = b.count[0];
+    // heavy first: they set the tail of the launch sequence
+    if (nblk > 0)
+        hipLaunchKernelGGL((k_pull_block<W, F>), dim3((unsigned)nblk), dim3(kT), 0, lib_stream(), f, v,
+                           cur, next, (const intT *)(b.ids + b.start[3]));
+    if (nwave > 0)
+        hipLaunchKernelGGL((k_pull_group<W, 64, F>), dim3(blocks_for(nwave * 64, kT, 1ll << 31)),
+                           dim3(kT), 0, lib_stream(), f, v, cur, next,
+                           (const intT *)(b.ids + b.start[2]), nwave);
+    if (n16 > 0)
+        hipLaunchKernelGGL((k_pull_group<W, 16, F>), dim3(blocks_for(n16 * 16, kT, 1ll << 31)), dim3(kT),
+                           0, lib_stream(), f, v, cur, next, (const intT *)(b.ids + b.start[1]), n16);
+    if (n1 > 0)
+        hipLaunchKernelGGL((k_pull_group<W, 1, F>), dim3(blocks_for(n1, kT, 1ll << 31)), dim3(kT), 0,
+                           lib_stream(), f, v, cur, next, (const intT *)(b.ids + b.start[0]), n1);
+    launched("dense pull");
+}
+
+// edgeMapDenseReduce (polymer.h:1097-1168) over the in-degree bins: the G lanes of
+// a destination split its in-list, each reducing its part into its own 16-byte
+// scratch (initFunc, then reduceFunc per active in-neighbour, an activation marks d
+// in next), stopping once cond(d) is false; the lanes that saw an edge then fold
+// their scratches into d with combineFunc one after another, in lane order.  The
+// reference folds one scratch per destination; folding partial scratches gives the
+// same result for combine functions that accumulate into d (its PageRank and SpMV
+// add with writeAdd, its Components lowers with writeMin).
+template <bool W, class F>
+__device__ __forceinline__ bool reduce_edge(F &f, void *data, intT s, const intE *w, int64_t e) {
+    if constexpr (W)
+        return f.reduceFunc(data, s, w[e]);
+    else
+        return f.reduceFunc(data, s);
+}
+
+template <bool W, int G, class F>
+__global__ __launch_bounds__(kT) void k_reduce_group(F f, pg_view v, const bool *cur, bool *next,
+                                                     const intT *ids, int64_t count) {
+    const int64_t slot = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
+    if (slot >= count) return;  // uniform within the group
+    const int l = lane() & (G - 1);
+    const intT d = ids[slot];
+    if (!group_any<G>(f.cond(d))) return;
+    const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
+    const int beg = v.in_off[d], end = v.in_off[d + 1];
+    double data[2];
+    f.initFunc((void *)data, d);
+    bool hit = false, saw = false;
+    if constexpr (G == 1) {
+        for (int e = beg; e < end; e++) {
+            saw = true;
+            const intT s = v.in_adj[e];
+            if (cur[s] && reduce_edge<W>(f, (void *)data, s, w, e)) hit = true;
+            if (!f.cond(d)) break;
+        }
+    } else {
+        // four chunks of G edges per cond test: their neighbour ids and flags are
+        // loaded together, so four gathers per lane are in flight
+        constexpr int U = 4;
+        for (int base = beg; base < end; base += U * G) {
+            intT s[U];
+            bool a[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int e = base + k * G + l;
+                s[k] = e < end ? v.in_adj[e] : 0;
+                saw = saw || e < end;
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++) a[k] = base + k * G + l < end && cur[s[k]];
+#pragma unroll
+            for (int k = 0; k < U; k++)
+                if (a[k] && reduce_edge<W>(f, (void *)data, s[k], w, base + k * G + l)) hit = true;
+            if (!group_any<G>(f.cond(d))) break;
+        }
+    }
+    if (group_any<G>(hit) && l == 0) next[d] = true;
+    if constexpr (G == 1) {
+        if (saw) f.combineFunc((void *)data, d);
+    } else {
+        for (int r = 0; r < G; r++)
+            if (l == r && saw) f.combineFunc((void *)data, d);
+    }
+}
+
+// a 256-thread workgroup per destination; the scratches fold wave after wave
+template <bool W, class F>
+__global__ __launch_bounds__(kT) void k_reduce_block(F f, pg_view v, const bool *cur, bool *next,
+                                                     const intT *ids) {
+    __shared__ int s_stop, s_hit;
+    const intT d = ids[blockIdx.x];
+    if (threadIdx.x == 0) {
+        s_stop = f.cond(d) ? 0 : 1;
+        s_hit = 0;
+    }
+    __syncthreads();
+    if (s_stop) return;  // uniform over the block
+    const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
+    const int beg = v.in_off[d], end = v.in_off[d + 1];
+    double data[2];
+    f.initFunc((void *)data, d);
+    bool saw = false;
+    for (int base = beg; base < end; base += kT) {
+        const int e = base + (int)threadIdx.x;
+        if (e < end) {
+            saw = true;
+            const intT s = v.in_adj[e];
+            if (cur[s] && reduce_edge<W>(f, (void *)data, s, w, e)) s_hit = 1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && !f.cond(d)) s_stop = 1;
+        __syncthreads();
+        if (s_stop) break;
+    }
+    const int wv = threadIdx.x >> 6, l = lane();
+    for (int k = 0; k < kT / 64; k++) {
+        if (wv == k)
+            for (int r = 0; r < 64; r++)
+                if (l == r && saw) f.combineFunc((void *)data, d);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && s_hit) next[d] = true;
+}
+
+template <bool W, class F>
+inline void reduce_dense(F &f, const pg_view &v, const bool *cur, bool *next) {
+    const Bins &b = bins_of(v);
+    const int64_t nblk = b.count[3], nwave = b.count[2], n16 = b.count[1], n1 = b.count[0];
+    if (nblk > 0)
+        hipLaunchKernelGGL((k_reduce_block<W, F>), dim3((unsigned)nblk), dim3(kT), 0, lib_stream(), f,
+                           v, cur, next, (const intT *)(b.ids + b.start[3]));
+    if (nwave > 0)
+        hipLaunchKernelGGL((k_reduce_group<W, 64, F>), dim3(blocks_for(nwave * 64, kT, 1ll << 31)),
+                           dim3(kT), 0, lib_stream(), f, v, cur, next,
+                           (const intT *)(b.ids + b.start[2]), nwave);
+    if (n16 > 0)
+        hipLaunchKernelGGL((k_reduce_group<W, 16, F>), dim3(blocks_for(n16 * 16, kT, 1ll << 31)),
+                           dim3(kT), 0, lib_stream(), f, v, cur, next,
+                           (const intT *)(b.ids + b.start[1]), n16);
+    if (n1 > 0)
+        hipLaunchKernelGGL((k_reduce_group<W, 1, F>), dim3(blocks_for(n1, kT, 1ll << 31)), dim3(kT), 0,
+                           lib_stream(), f, v, cur, next, (const intT *)(b.ids + b.start[0]), n1);
+    launched("dense reduce");
+}
+
+template <class F>
+__global__ void k_vertex_map_dense(F f, const bool *b, int64_t n, intT offset) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (b[i]) f((intT)i + offset);
+}
+
+template <class F>
+__global__ void k_vertex_map_sparse(F f, const intT *s, int64_t m) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        f(s[i]);
+}
+
+template <class F>
+__global__ __launch_bounds__(kT) void k_vertex_filter(F f, const bool *b, int64_t lo, int64_t hi,
+                                                      intT offset, bool *dst,
+                                                      unsigned long long *count) {
+    __shared__ unsigned long long sc[kT / 64];
+    unsigned long long c = 0;
+    for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        bool r = false;
+        if (b[i]) r = f((intT)i + offset);
+        dst[i] = r;
+        c += r ? 1ull : 0ull;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane() == 0) sc[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kT / 64; w++) c += sc[w];
+        if (c) atomicAdd(count, c);
+    }
+}
+
+// |F| and the out-degree sum of a flag frontier: reduced per block, two atomics
+// per block (a grid of at most 2048 blocks)
+__global__ __launch_bounds__(kT) void k_counter(const bool *b, int64_t n, intT offset,
+                                                const int32_t *out_off, unsigned long long *out) {
+    __shared__ unsigned long long sc[kT / 64], se[kT / 64];
+    unsigned long long c = 0, e = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (b[i]) {
+            const int64_t v = i + offset;
+            c++;
+            e += (unsigned long long)(out_off[v + 1] - out_off[v]);
+        }
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o, 64);
+        e += __shfl_xor(e, o, 64);
+    }
+    if (lane() == 0) {
+        sc[threadIdx.x >> 6] = c;
+        se[threadIdx.x >> 6] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kT / 64; w++) {
+            c += sc[w];
+            e += se[w];
+        }
+        if (c) atomicAdd(&out[0], c);
+        if (e) atomicAdd(&out[1], e);
     }
 }
 
@@ -413,115 +1601,111 @@ __global__ void k_set_list(bool *b, const intT *s, int64_t m, intT offset) {
         b[s[i] - offset] = true;
 }
 
-template <class vertex>
-__global__ void k_counter(const bool *b, int64_t n, intT offset, const vertex *V,
-                          unsigned long long *out) {
-    unsigned long long c = 0, e = 0;
+__global__ void k_fill_bytes(bool *b, int64_t n, bool v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        if (b[i]) {
-            c++;
-            e += (unsigned long long)V[i + offset].getOutDegree();
-        }
-    for (int o = 32; o > 0; o >>= 1) {
-        c += __shfl_xor(c, o, 64);
-        e += __shfl_xor(e, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0 && (c || e)) {
-        atomicAdd(&out[0], c);
-        atomicAdd(&out[1], e);
-    }
+        b[i] = v;
 }
 
 }  // namespace polymer_detail
 
+/* ---- frontiers (polymer.h:521-881) ------------------------------------------ */
+struct AsyncChunk {
+    intT accessCounter;
+    intT m;
+    intT *s;
+};
+
 struct LocalFrontier {
     intT n;
     intT m;
+    intT head = 0;
+    intT tail = 0;
+    intT insertTail = 0;
+    intT emptySignal = 0;
     intT outEdgesCount;
     intT startID;
     intT endID;
-    bool *b;          // device, endID - startID entries
-    intT *s;          // device list (valid when !isDense), capacity sCap
-    size_t sCap;
+    bool *b;
+    intT *s;
+    intT sparseCounter = 0;
+    intT **sparseChunks = nullptr;
+    intT *chunkSizes = nullptr;
+    intT *tmp = nullptr;
+    AsyncChunk **localQueue = nullptr;
     bool isDense;
-    unsigned long long *counters;  // device scratch: [0] count, [1] edges
+    size_t sCap = 0;        // capacity of s when the header allocated it
+    bool sOwned = false;    // s was allocated here
+    unsigned long long *counters = nullptr;  // device: [0] count, [1..32] out-degree sums
 
     LocalFrontier(bool *_b, intT start, intT end)
         : n(end - start), m(0), outEdgesCount(0), startID(start), endID(end), b(_b), s(nullptr),
-          sCap(0), isDense(true), counters(nullptr) {
-        counters = polymer_detail::dev_alloc<unsigned long long>(4);
+          isDense(true) {
+        polymer_detail::ensure_init();
+        counters = polymer_detail::dev_alloc<unsigned long long>(40);
     }
     ~LocalFrontier() {
-        if (s) (void)hipFree(s);
+        if (s && sOwned) polymer_detail::shared_free(s);
         if (counters) (void)hipFree(counters);
     }
     LocalFrontier(const LocalFrontier &) = delete;
     LocalFrontier &operator=(const LocalFrontier &) = delete;
 
     bool inRange(intT index) const { return startID <= index && index < endID; }
-    // device-side bit access (b is device memory)
-    PG_HD void setBit(intT index, bool val) { b[index - startID] = val; }
-    PG_HD bool getBit(intT index) const { return b[index - startID]; }
+    PG_HD inline void setBit(intT index, bool val) { b[index - startID] = val; }
+    PG_HD inline bool getBit(intT index) const { return b[index - startID]; }
 
+    // a list of at least cap entries owned by the frontier (kept across calls)
     void reserveSparse(size_t cap) {
-        if (cap <= sCap) return;
-        if (s) (void)hipFree(s);
-        s = polymer_detail::dev_alloc<intT>(cap);
+        if (s && sOwned && cap <= sCap) return;
+        if (s && sOwned) polymer_detail::shared_free(s);
+        s = static_cast<intT *>(polymer_detail::shared_alloc(sizeof(intT) * (cap ? cap : 1)));
         sCap = cap;
+        sOwned = true;
     }
 
-    // packIndex (polymer.h:552-567): ascending ids of the set bits
+    // packIndex (polymer.h:552-567): ascending ids of the set flags
     void toSparse() {
         if (isDense) {
-            using namespace polymer_detail;
             reserveSparse((size_t)n + 1);
-            const int64_t per_block = 8192;
-            const int64_t nb = (n + per_block - 1) / per_block;
-            std::vector<unsigned> cnt((size_t)(nb > 0 ? nb : 1));
-            unsigned *d = dev_alloc<unsigned>(cnt.size() + 1);
-            if (nb > 0) {
-                hipLaunchKernelGGL(k_count_flags, dim3((unsigned)nb), dim3(256), 0, lib_stream(), b,
-                                   (int64_t)n, per_block, d);
-                toHost(cnt.data(), d, (size_t)nb);
-            }
-            unsigned tot = 0;
-            for (auto &c : cnt) {
-                const unsigned x = c;
-                c = tot;
-                tot += x;
-            }
-            if (nb > 0) {
-                toDevice(d, cnt.data(), (size_t)nb);
-                hipLaunchKernelGGL(k_write_flags, dim3((unsigned)nb), dim3(256), 0, lib_stream(), b,
-                                   (int64_t)n, per_block, d, startID, s);
-                check_hip(hipGetLastError(), "k_write_flags");
-            }
-            check_hip(hipStreamSynchronize(lib_stream()), "sync");
-            (void)hipFree(d);
-            m = (intT)tot;
+            m = polymer_detail::compact_flags(b, n, startID, s);
         }
         isDense = false;
+    }
+    void toSparseAsync(int nextID, LocalFrontier *next) {
+        (void)nextID;
+        (void)next;
+        toSparse();
     }
 
     // toDense (polymer.h:594-601)
     void toDense() {
         if (!isDense) {
             using namespace polymer_detail;
-            check_hip(hipMemsetAsync(b, 0, (size_t)n, lib_stream()), "hipMemsetAsync");
-            if (m > 0)
-                hipLaunchKernelGGL(k_set_list, dim3(blocks_for(m)), dim3(256), 0, lib_stream(), b, s,
+            hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for(n)), dim3(256), 0, lib_stream(), b,
+                               (int64_t)n, false);
+            if (m > 0) {
+                const intT *sl = device_readable(s, (size_t)m, "todense_list");
+                hipLaunchKernelGGL(k_set_list, dim3(blocks_for(m)), dim3(256), 0, lib_stream(), b, sl,
                                    (int64_t)m, startID);
-            check_hip(hipGetLastError(), "k_set_list");
+            }
+            launched("toDense");
+            sync();
         }
         isDense = true;
     }
+    void toDenseWithMerge(int numOfSub) {
+        (void)numOfSub;
+        toDense();
+    }
 
-    void setSparse(intT _m, intT *_s) {  // takes a device list
-        if (s) (void)hipFree(s);
+    // takes a list the caller allocated (host-visible, device or malloc'd memory)
+    void setSparse(intT _m, intT *_s) {
+        if (s && sOwned) polymer_detail::shared_free(s);
         m = _m;
         s = _s;
         sCap = (size_t)_m;
+        sOwned = false;
         isDense = false;
     }
 
@@ -542,14 +1726,19 @@ struct vertices {
     int numOfNodes;
     intT numOfVertices;
     std::vector<intT> numOfVertexOnNode, offsets, numOfNonZero;
+    std::vector<bool *> d;
     std::vector<LocalFrontier *> frontiers, nextFrontiers;
     bool isDense;
     bool firstSparse;
+    AsyncChunk **asyncQueue = nullptr;
+    intT asyncEndSignal = 0;
+    intT readerTail = 0;
+    intT insertTail = 0;
 
     explicit vertices(int _numOfNodes)
         : n(0), m(-1), numOfNodes(_numOfNodes), numOfVertices(0),
           numOfVertexOnNode(_numOfNodes, 0), offsets(_numOfNodes + 1, 0),
-          numOfNonZero(_numOfNodes, 0), frontiers(_numOfNodes, nullptr),
+          numOfNonZero(_numOfNodes, 0), d(_numOfNodes, nullptr), frontiers(_numOfNodes, nullptr),
           nextFrontiers(_numOfNodes, nullptr), isDense(true), firstSparse(false) {}
 
     void registerFrontier(int nodeNum, LocalFrontier *frontier) {
@@ -557,35 +1746,14 @@ struct vertices {
         numOfVertexOnNode[nodeNum] = frontier->n;
         isDense = frontier->isDense;
     }
-    void calculateOffsets() {
-        numOfVertices = 0;
-        for (int i = 0; i < numOfNodes; i++) numOfVertices += numOfVertexOnNode[i];
-        offsets[0] = 0;
-        for (int i = 1; i <= numOfNodes; i++) offsets[i] = offsets[i - 1] + numOfVertexOnNode[i - 1];
-        n = numOfVertices;
-    }
-    intT getSize(int nodeNum) const { return numOfVertexOnNode[nodeNum]; }
-    intT getOffset(int nodeNum) const { return offsets[nodeNum]; }
-    intT getSparseSize(int nodeNum) const { return numOfNonZero[nodeNum]; }
-    void calculateNumOfNonZero(int nodeNum) { numOfNonZero[nodeNum] = frontiers[nodeNum]->m; }
-    intT numNonzeros() {
-        if (m < 0) {
-            intT sum = 0;
-            for (int i = 0; i < numOfNodes; i++) sum += frontiers[i]->m;
-            m = sum;
-        }
-        return m;
-    }
-    bool isEmpty() { return numNonzeros() == 0; }
-    long long getEdgeStat() const {
-        long long sum = 0;
-        for (int i = 0; i < numOfNodes; i++) sum += frontiers[i]->outEdgesCount;
-        return sum;
-    }
     void toDense() {
         if (isDense) return;
         isDense = true;
         for (auto *f : frontiers) f->toDense();
+    }
+    void toDenseWithMerge(int numOfSub) {
+        (void)numOfSub;
+        toDense();
     }
     void toSparse() {
         if (!isDense) {
@@ -595,6 +1763,52 @@ struct vertices {
         isDense = false;
         firstSparse = true;
         for (auto *f : frontiers) f->toSparse();
+    }
+    void toSparseAsync() { toSparse(); }
+    long long getEdgeStat() const {
+        long long sum = 0;
+        for (int i = 0; i < numOfNodes; i++) sum += frontiers[i]->outEdgesCount;
+        return sum;
+    }
+    void calculateOffsets() {
+        numOfVertices = 0;
+        for (int i = 0; i < numOfNodes; i++) numOfVertices += numOfVertexOnNode[i];
+        offsets[0] = 0;
+        for (int i = 1; i <= numOfNodes; i++) offsets[i] = offsets[i - 1] + numOfVertexOnNode[i - 1];
+        n = numOfVertices;
+    }
+    intT getSize(int nodeNum) const { return numOfVertexOnNode[nodeNum]; }
+    intT getSparseSize(int nodeNum) const { return numOfNonZero[nodeNum]; }
+    void calculateNumOfNonZero(int nodeNum) { numOfNonZero[nodeNum] = frontiers[nodeNum]->m; }
+    intT numNonzeros() {
+        if (m < 0) {
+            intT sum = 0;
+            for (int i = 0; i < numOfNodes; i++) sum += numOfNonZero[i];
+            m = sum;
+        }
+        return m;
+    }
+    bool isEmpty() { return numNonzeros() == 0; }
+    int getNodeNumOfIndex(intT index) const {
+        int r = 0;
+        while (r < numOfNodes && offsets[r] <= index) r++;
+        return r - 1;
+    }
+    int getNodeNumOfSparseIndex(intT index) const {
+        int r = 0;
+        intT acc = 0;
+        while (r < numOfNodes && acc <= index) acc += numOfNonZero[r++];
+        return r - 1;
+    }
+    intT getOffset(int nodeNum) const { return offsets[nodeNum]; }
+    // host access to one flag of the (host-visible) frontier arrays
+    void setBit(intT index, bool bit) {
+        const int i = std::max(0, getNodeNumOfIndex(index));
+        frontiers[i]->b[index - offsets[i]] = bit;
+    }
+    bool getBit(intT index) const {
+        const int i = std::max(0, getNodeNumOfIndex(index));
+        return frontiers[i]->b[index - offsets[i]];
     }
     bool *getArr(int nodeNum) { return frontiers[nodeNum]->b; }
     bool *getNextArr(int nodeNum) { return nextFrontiers[nodeNum] ? nextFrontiers[nodeNum]->b : nullptr; }
@@ -607,281 +1821,205 @@ struct vertices {
         m = -1;
         return old;
     }
+    bool eq(vertices &) { return false; }
+    void print() {}
     void del() {}
 };
 
-// Subworker_Partitioner (polymer.h:75-104): the SPMD roles of the NUMA engine.
-// On the GPU there is one caller; the object keeps the interface.
-struct Subworker_Partitioner {
-    int tid = 0, subTid = 0, numOfSub = 1;
-    intT dense_start = 0, dense_end = 0;
-    void *global_barr = nullptr, *local_barr = nullptr;
-    explicit Subworker_Partitioner(int nSub) : numOfSub(nSub) {}
-    bool isMaster() const { return tid + subTid == 0; }
-    bool isSubMaster() const { return subTid == 0; }
-    void globalWait() {}
-    void localWait() {}
-    intT getStartPos(intT m) const { return subTid * (m / numOfSub); }
-    intT getEndPos(intT m) const { return subTid == numOfSub - 1 ? m : (subTid + 1) * (m / numOfSub); }
-};
-
-inline Subworker_Partitioner &polymerDummyPartitioner() {
-    static Subworker_Partitioner p(1);
-    return p;
-}
-
-// edgeMap options (polymer.h:917-919)
-enum options { DENSE, DENSE_PARALLEL, DENSE_FORWARD };
-
-/* ---- edge kernels ---------------------------------------------------------------- */
+/* ---- edge maps ---------------------------------------------------------------------- */
 namespace polymer_detail {
 
-template <class G>
-struct is_weighted : std::false_type {};
-template <class V>
-struct is_weighted<wghGraph<V>> : std::true_type {};
+// the frontier's flags as the kernels read them (the calls' node is 0: one GPU)
+inline const bool *cur_flags(vertices *frontier) { return frontier->getArr(0); }
 
-template <bool W, class F>
-__device__ __forceinline__ bool call_update_atomic(F &f, intT s, intT d, const intE *w, int64_t e) {
-    if constexpr (W)
-        return f.updateAtomic(s, d, w[e]);
-    else
-        return f.updateAtomic(s, d);
+// the active sources of a dense frontier as an ordered list (kept in a workspace)
+inline intT list_active(vertices *frontier, intT **list) {
+    LocalFrontier *f = frontier->getFrontier(0);
+    intT *l = workspace_t<intT>("active_list", (size_t)f->n + 1);
+    *list = l;
+    return compact_flags(f->b, f->n, f->startID, l);
 }
 
-// Push over sources [0, n) (dense: bit set) or list[0, count) (sparse).  A block
-// takes 256 sources, scans their out-degrees in LDS and spreads its edges over
-// its threads.  Mode 0: set next bits (edgeMapDenseForward); 1: append to the
-// next list (edgeMapSparseV3); 2: set next bits, functor gets the edge's
-// index in the source's list (edgeMapDenseBP).
-template <bool W, int Mode, class F>
-__global__ __launch_bounds__(256) void k_push(F f, pg_view v, const bool *cur, const intT *list,
-                                              int64_t count, bool *next, intT *out_list,
-                                              unsigned long long *counters) {
-    __shared__ int64_t start[257];
-    __shared__ intT srcs[256];
-    const int64_t base = (int64_t)blockIdx.x * 256;
-    const int t = threadIdx.x;
-    intT s = -1;
-    int64_t deg = 0;
-    if (base + t < count) {
-        s = list ? list[base + t] : (intT)(base + t);
-        if (list || cur[s]) deg = v.out_off[s + 1] - v.out_off[s];
-        else s = -1;
-    }
-    srcs[t] = s;
-    start[t + 1] = deg;
-    if (t == 0) start[0] = 0;
-    __syncthreads();
-    if (t == 0)
-        for (int i = 1; i <= 256; i++) start[i] += start[i - 1];
-    __syncthreads();
-    const int64_t total = start[256];
-    const intE *w = W ? static_cast<const intE *>(v.out_w) : nullptr;
-    for (int64_t k = t; k < total; k += 256) {
-        int lo = 0, hi = 255;  // last i with start[i] <= k
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (start[mid] <= k) lo = mid; else hi = mid - 1;
-        }
-        const intT src = srcs[lo];
-        const int64_t j = k - start[lo];
-        const int64_t e = v.out_off[src] + j;
-        const intT ngh = v.out_adj[e];
-        bool act;
-        if constexpr (Mode == 2)
-            act = f.cond(ngh) && f.updateAtomic(src, ngh, (intT)j);
-        else
-            act = f.cond(ngh) && call_update_atomic<W>(f, src, ngh, w, e);
-        if (act) {
-            if constexpr (Mode == 1) {
-                const unsigned long long pos = atomicAdd(&counters[0], 1ull);
-                out_list[pos] = ngh;
-                atomicAdd(&counters[1], (unsigned long long)(v.out_off[ngh + 1] - v.out_off[ngh]));
-            } else {
-                next[ngh] = true;
-            }
-        }
-    }
+template <class G, class F, int Mode>
+void dense_push(G &GA, vertices *frontier, F &f, LocalFrontier *next) {
+    constexpr bool W = is_weighted<G>::value && Mode != 2;
+    intT *list = nullptr;
+    const intT count = list_active(frontier, &list);
+    push_list<W, Mode>(f, GA.view, list, count, next->b, nullptr, 0, next->counters);
+    sync();
 }
-
-// Dense pull in the reference's order: one thread per destination d with
-// cond(d) scans its in-neighbours, calls updateAtomic for the active ones, marks
-// d in next on success and stops when cond(d) turns false.
-template <bool W, class F>
-__global__ void k_pull(F f, pg_view v, const bool *cur, bool *next) {
-    const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const intT d = (intT)i;
-        if (!f.cond(d)) continue;
-        for (int64_t e = v.in_off[d]; e < v.in_off[d + 1]; e++) {
-            const intT ngh = v.in_adj[e];
-            if (cur[ngh] && call_update_atomic<W>(f, ngh, d, w, e)) next[d] = true;
-            if (!f.cond(d)) break;
-        }
-    }
-}
-
-// edgeMapDenseReduce (polymer.h:1097-1168): initFunc / reduceFunc per active
-// in-neighbour / combineFunc when the destination has in-edges.
-template <bool W, class F>
-__global__ void k_pull_reduce(F f, pg_view v, const bool *cur, bool *next) {
-    const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const intT d = (intT)i;
-        if (!f.cond(d)) continue;
-        double data[2];
-        f.initFunc((void *)data, d);
-        const int64_t b = v.in_off[d], e1 = v.in_off[d + 1];
-        for (int64_t e = b; e < e1; e++) {
-            const intT ngh = v.in_adj[e];
-            bool r = false;
-            if (cur[ngh]) {
-                if constexpr (W) r = f.reduceFunc((void *)data, ngh, w[e]);
-                else r = f.reduceFunc((void *)data, ngh);
-            }
-            if (r) next[d] = true;
-            if (!f.cond(d)) break;
-        }
-        if (e1 > b) f.combineFunc((void *)data, d);
-    }
-}
-
-template <class F>
-__global__ void k_vertex_map_dense(F f, const bool *b, int64_t n, intT offset) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        if (b[i]) f((intT)i + offset);
-}
-
-template <class F>
-__global__ void k_vertex_map_sparse(F f, const intT *s, int64_t m) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-         i += (int64_t)gridDim.x * blockDim.x)
-        f(s[i]);
-}
-
-template <class F>
-__global__ void k_vertex_filter(F f, const bool *b, int64_t n, intT offset, bool *dst,
-                                unsigned long long *count) {
-    unsigned long long c = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        bool r = false;
-        if (b[i]) r = f((intT)i + offset);
-        dst[i] = r;
-        c += r ? 1ull : 0ull;
-    }
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
-}
-
-inline void launched(const char *what) { check_hip(hipGetLastError(), what); }
-
-// one 256-thread block per 256 sources
-inline unsigned push_blocks(int64_t count) { return (unsigned)(count > 0 ? (count + 255) / 256 : 1); }
 
 }  // namespace polymer_detail
 
-/* ---- edge maps ---------------------------------------------------------------------- */
 // edgeMapDenseForward (polymer.h:970-1032): for every active s and out-edge
 // (s, ngh): if cond(ngh) && updateAtomic(s, ngh) then ngh joins next.
 template <class F, class G>
-bool *edgeMapDenseForward(G &GA, vertices *frontier, F f, LocalFrontier *next, bool part = false,
+bool *edgeMapDenseForward(G GA, vertices *frontier, F f, LocalFrontier *next, bool part = false,
                           intT start = 0, intT end = 0) {
-    using namespace polymer_detail;
     (void)part; (void)start; (void)end;
-    constexpr bool W = is_weighted<G>::value;
-    hipLaunchKernelGGL((k_push<W, 0, F>), dim3(push_blocks(GA.n)), dim3(256), 0, lib_stream(), f,
-                       GA.view, (const bool *)frontier->getArr(0), (const intT *)nullptr,
-                       (int64_t)GA.n, next->b, (intT *)nullptr, next->counters);
-    launched("edgeMapDenseForward");
+    polymer_detail::dense_push<G, F, 0>(GA, frontier, f, next);
+    return nullptr;
+}
+
+// edgeMapDenseForwardDynamic (polymer.h:1036-1095): the same push
+template <class F, class G>
+bool *edgeMapDenseForwardDynamic(G GA, vertices *frontier, F f, LocalFrontier *next,
+                                 Subworker_Partitioner &subworker = dummyPartitioner) {
+    (void)subworker;
+    polymer_detail::dense_push<G, F, 0>(GA, frontier, f, next);
     return nullptr;
 }
 
 // edgeMapDenseBP (polymer.h:1263-1306): the forward push with the edge's index
 // j in the source's out-list passed to updateAtomic(s, ngh, j).
 template <class F, class G>
-bool *edgeMapDenseBP(G &GA, vertices *frontier, F f, LocalFrontier *next, bool part = false,
+bool *edgeMapDenseBP(G GA, vertices *frontier, F f, LocalFrontier *next, bool part = false,
                      intT start = 0, intT end = 0) {
-    using namespace polymer_detail;
     (void)part; (void)start; (void)end;
-    hipLaunchKernelGGL((k_push<false, 2, F>), dim3(push_blocks(GA.n)), dim3(256), 0, lib_stream(), f,
-                       GA.view, (const bool *)frontier->getArr(0), (const intT *)nullptr,
-                       (int64_t)GA.n, next->b, (intT *)nullptr, next->counters);
-    launched("edgeMapDenseBP");
+    polymer_detail::dense_push<G, F, 2>(GA, frontier, f, next);
     return nullptr;
 }
 
 // edgeMapDense / edgeMapDenseDynamic (polymer.h:924-968, 1170-1261)
 template <class F, class G>
-bool *edgeMapDense(G &GA, vertices *frontier, F f, LocalFrontier *next, bool parallel = 0,
-                   Subworker_Partitioner &subworker = polymerDummyPartitioner()) {
-    using namespace polymer_detail;
+bool *edgeMapDense(G GA, vertices *frontier, F f, LocalFrontier *next, bool parallel = 0,
+                   Subworker_Partitioner &subworker = dummyPartitioner) {
     (void)parallel; (void)subworker;
-    constexpr bool W = is_weighted<G>::value;
+    constexpr bool W = polymer_detail::is_weighted<G>::value;
     frontier->nextFrontiers[0] = next;
-    hipLaunchKernelGGL((k_pull<W, F>), dim3(blocks_for(GA.n)), dim3(256), 0, lib_stream(), f,
-                       GA.view, (const bool *)frontier->getArr(0), next->b);
-    launched("edgeMapDense");
+    polymer_detail::pull_dense<W>(f, GA.view, polymer_detail::cur_flags(frontier), next->b);
+    polymer_detail::sync();
     return nullptr;
+}
+
+template <class F, class G>
+bool *edgeMapDenseDynamic(G GA, vertices *frontier, F f, LocalFrontier *next,
+                          Subworker_Partitioner &subworker = dummyPartitioner) {
+    return edgeMapDense(GA, frontier, f, next, 0, subworker);
 }
 
 // edgeMapDenseReduce (polymer.h:1097-1168)
 template <class F, class G>
-bool *edgeMapDenseReduce(G &GA, vertices *frontier, F f, LocalFrontier *next, bool parallel = 0,
-                         Subworker_Partitioner &subworker = polymerDummyPartitioner()) {
+bool *edgeMapDenseReduce(G GA, vertices *frontier, F f, LocalFrontier *next, bool parallel = 0,
+                         Subworker_Partitioner &subworker = dummyPartitioner) {
     using namespace polymer_detail;
     (void)parallel; (void)subworker;
     constexpr bool W = is_weighted<G>::value;
     frontier->nextFrontiers[0] = next;
-    hipLaunchKernelGGL((k_pull_reduce<W, F>), dim3(blocks_for(GA.n)), dim3(256), 0, lib_stream(),
-                       f, GA.view, (const bool *)frontier->getArr(0), next->b);
-    launched("edgeMapDenseReduce");
+    reduce_dense<W>(f, GA.view, cur_flags(frontier), next->b);
+    sync();
     return nullptr;
 }
 
 // edgeMapSparseV3 (polymer.h:1928-2003): the active list pushes; each success
-// appends ngh to next's list (duplicates possible, as in the reference) and
-// adds outdeg(ngh) to next->outEdgesCount.
+// appends ngh to next's list (duplicates possible, as in the reference) and adds
+// outdeg(ngh) to next->outEdgesCount.  The list holds the frontier's out-degree
+// sum (getEdgeStat); a stale count that would overflow it is an error.
 template <class F, class G>
-void edgeMapSparseV3(G &GA, vertices *frontier, F f, LocalFrontier *next, bool part = false,
-                     Subworker_Partitioner &subworker = polymerDummyPartitioner()) {
+void edgeMapSparseV3(G GA, vertices *frontier, F f, LocalFrontier *next, bool part = false,
+                     Subworker_Partitioner &subworker = dummyPartitioner) {
     using namespace polymer_detail;
     (void)part; (void)subworker;
     constexpr bool W = is_weighted<G>::value;
     LocalFrontier *cur = frontier->getFrontier(0);
     const long long bufferLen = frontier->getEdgeStat();
     next->reserveSparse((size_t)(bufferLen > 0 ? bufferLen : 1));
-    check_hip(hipMemsetAsync(next->counters, 0, 2 * sizeof(unsigned long long), lib_stream()),
+    check_hip(hipMemsetAsync(next->counters, 0, 40 * sizeof(unsigned long long), lib_stream()),
               "hipMemsetAsync");
     const int64_t count = cur->m;
     if (count > 0) {
-        hipLaunchKernelGGL((k_push<W, 1, F>), dim3(push_blocks(count)), dim3(256), 0,
-                           lib_stream(), f, GA.view, (const bool *)nullptr, (const intT *)cur->s, count,
-                           (bool *)nullptr, next->s, next->counters);
-        launched("edgeMapSparseV3");
+        const intT *list = device_readable(cur->s, (size_t)count, "sparse_in");
+        push_list<W, 1>(f, GA.view, list, count, nullptr, next->s, next->sCap, next->counters);
     }
-    unsigned long long c[2];
-    toHost(c, next->counters, 2);
+    unsigned long long c[40];
+    toHost(c, next->counters, 40);
+    unsigned long long e = 0;
+    for (int i = 1; i < 40; i++) e += c[i];
+    if (c[0] > next->sCap) {
+        std::fprintf(stderr,
+                     "polymer: edgeMapSparseV3: %llu successes exceed the list of %zu entries sized "
+                     "from getEdgeStat()\n",
+                     c[0], next->sCap);
+        std::abort();
+    }
     next->m = (intT)c[0];
-    next->outEdgesCount = (intT)c[1];
+    next->outEdgesCount = (intT)e;
     next->isDense = false;
+}
+
+template <class F, class G>
+void edgeMapSparseV2(G GA, vertices *frontier, F f, LocalFrontier *next, bool part = false,
+                     Subworker_Partitioner &subworker = dummyPartitioner) {
+    edgeMapSparseV3(GA, frontier, f, next, part, subworker);
+}
+template <class F, class G>
+void edgeMapSparseV4(G GA, vertices *frontier, F f, LocalFrontier *next, bool firstTime = false,
+                     Subworker_Partitioner &subworker = dummyPartitioner) {
+    (void)firstTime;
+    edgeMapSparseV3(GA, frontier, f, next, false, subworker);
+}
+template <class F, class G>
+void edgeMapSparseV5(G GA, vertices *frontier, F f, LocalFrontier *next,
+                     Subworker_Partitioner &subworker = dummyPartitioner) {
+    edgeMapSparseV3(GA, frontier, f, next, false, subworker);
+}
+
+// edgeMapSparseAsync / edgeMapSparseAsyncPipe (polymer.h:1358-1751): every vertex
+// the push activates is expanded within the same call, until no activation is
+// left (the reference passes discoveries between nodes' queues and ends with an
+// end-game chunk).  Here the discoveries are pushed again, list after list, on the
+// GPU; the call ends with next empty.  For functors whose updates are monotone
+// (BFS claims, Bellman-Ford's writeMin) the end state is the reference's; which
+// claimer wins a CAS is not fixed in either.
+template <class F, class G>
+void edgeMapSparseAsyncPipe(G GA, vertices *frontier, F f, LocalFrontier *next,
+                            Subworker_Partitioner &subworker = dummyPartitioner) {
+    using namespace polymer_detail;
+    (void)subworker;
+    constexpr bool W = is_weighted<G>::value;
+    LocalFrontier *cur = frontier->getFrontier(0);
+    if (cur->isDense) cur->toSparse();
+    const size_t cap = (size_t)std::max(GA.m, GA.n) + 1;
+    intT *qa = workspace_t<intT>("async_a", cap);
+    intT *qb = workspace_t<intT>("async_b", cap);
+    int64_t count = cur->m;
+    if (count > 0)
+        check_hip(hipMemcpyAsync(qa, device_readable(cur->s, (size_t)count, "async_in"),
+                                 sizeof(intT) * (size_t)count, hipMemcpyDefault, lib_stream()),
+                  "hipMemcpyAsync");
+    unsigned long long *ctr = workspace_t<unsigned long long>("async_ctr", 40);
+    while (count > 0) {
+        check_hip(hipMemsetAsync(ctr, 0, 40 * sizeof(unsigned long long), lib_stream()), "hipMemsetAsync");
+        push_list<W, 1>(f, GA.view, qa, count, nullptr, qb, cap, ctr);
+        unsigned long long c = 0;
+        toHost(&c, ctr, 1);
+        if (c > cap) fail("edgeMapSparseAsyncPipe: more activations in one pass than edges");
+        count = (int64_t)c;
+        std::swap(qa, qb);
+    }
+    next->m = 0;
+    next->outEdgesCount = 0;
+    next->isDense = false;
+}
+
+template <class F, class G>
+void edgeMapSparseAsync(G GA, vertices *frontier, F f, LocalFrontier *next,
+                        Subworker_Partitioner &subworker = dummyPartitioner) {
+    edgeMapSparseAsyncPipe(GA, frontier, f, next, subworker);
 }
 
 void clearLocalFrontier(LocalFrontier *next, int nodeNum, int subNum, int totalSub);
 
 // edgeMap (polymer.h:2241-2309): dense iff |F| + sum of out-degrees of F >= threshold
 template <class F, class G>
-void edgeMap(G &GA, vertices *V, F f, LocalFrontier *next, intT threshold = -1, char option = DENSE,
+void edgeMap(G GA, vertices *V, F f, LocalFrontier *next, intT threshold = -1, char option = DENSE,
              bool remDups = false, bool part = false,
-             Subworker_Partitioner &subworker = polymerDummyPartitioner()) {
+             Subworker_Partitioner &subworker = dummyPartitioner) {
     (void)remDups;
     const long long m = (long long)V->numNonzeros() + V->getEdgeStat();
     if (m >= threshold) {
         V->toDense();
+        next->sparseCounter = 0;
         clearLocalFrontier(next, 0, 0, 1);
         if (option == DENSE_FORWARD)
             edgeMapDenseForward(GA, V, f, next, part, subworker.dense_start, subworker.dense_end);
@@ -904,10 +2042,9 @@ void vertexCounter(G &GA, LocalFrontier *frontier, int nodeNum, int subNum, int 
     if (!frontier->isDense || nodeNum != 0 || subNum != 0) return;
     check_hip(hipMemsetAsync(frontier->counters, 0, 2 * sizeof(unsigned long long), lib_stream()),
               "hipMemsetAsync");
-    using vertex = typename std::remove_pointer<decltype(GA.V)>::type;
-    hipLaunchKernelGGL(k_counter<vertex>, dim3(blocks_for(frontier->n)), dim3(256), 0, lib_stream(),
+    hipLaunchKernelGGL(k_counter, dim3(blocks_for(frontier->n, 256, 2048)), dim3(256), 0, lib_stream(),
                        (const bool *)frontier->b, (int64_t)frontier->n, frontier->startID,
-                       (const vertex *)GA.V, frontier->counters);
+                       GA.view.out_off, frontier->counters);
     launched("vertexCounter");
     unsigned long long c[2];
     toHost(c, frontier->counters, 2);
@@ -926,30 +2063,55 @@ void vertexMap(vertices *V, F add, int nodeNum, int subNum = 0, int totalSub = 1
         hipLaunchKernelGGL(k_vertex_map_dense<F>, dim3(blocks_for(fr->n)), dim3(256), 0, lib_stream(),
                            add, (const bool *)fr->b, (int64_t)fr->n, V->getOffset(nodeNum));
     } else if (fr->m > 0) {
+        const intT *sl = device_readable(fr->s, (size_t)fr->m, "vmap_list");
         hipLaunchKernelGGL(k_vertex_map_sparse<F>, dim3(blocks_for(fr->m)), dim3(256), 0, lib_stream(),
-                           add, (const intT *)fr->s, (int64_t)fr->m);
+                           add, sl, (int64_t)fr->m);
     }
     launched("vertexMap");
+    sync();
 }
 
 // clearLocalFrontier (polymer.h:2383-2397)
 inline void clearLocalFrontier(LocalFrontier *next, int nodeNum, int subNum, int totalSub) {
+    using namespace polymer_detail;
     (void)nodeNum; (void)totalSub;
     if (subNum != 0) return;
-    polymer_detail::check_hip(hipMemsetAsync(next->b, 0, (size_t)next->n, polymer_detail::lib_stream()),
-                              "hipMemsetAsync");
+    hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for(next->n)), dim3(256), 0, lib_stream(), next->b,
+                       (int64_t)next->n, false);
+    launched("clearLocalFrontier");
+    sync();
 }
 
 // Every vertex of the frontier's range active (the apps' "all true" start,
 // e.g. numa-PageRank.C:427-429); m and outEdgesCount come from vertexCounter.
 inline void fillLocalFrontier(LocalFrontier *f) {
-    polymer_detail::check_hip(hipMemsetAsync(f->b, 1, (size_t)f->n, polymer_detail::lib_stream()),
-                              "hipMemsetAsync");
+    using namespace polymer_detail;
+    hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for(f->n)), dim3(256), 0, lib_stream(), f->b,
+                       (int64_t)f->n, true);
+    launched("fillLocalFrontier");
+    sync();
     f->isDense = true;
     f->m = f->n;
 }
 
-// vertexFilter (polymer.h:2411-2441): result bit i = active(i) && filter(i); m counted
+// vertexFilter (polymer.h:2399-2409): result[i] = active(i) && filter(i) over the
+// node's range (a host-visible or device bool array)
+template <class F>
+void vertexFilter(vertices *V, F filter, int nodeNum, bool *result) {
+    using namespace polymer_detail;
+    V->toDense();
+    LocalFrontier *fr = V->getFrontier(nodeNum);
+    unsigned long long *cnt = workspace_t<unsigned long long>("filter_cnt", 1);
+    check_hip(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), lib_stream()), "hipMemsetAsync");
+    hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n, 256, 2048)), dim3(256), 0, lib_stream(), filter,
+                       (const bool *)fr->b, (int64_t)0, (int64_t)fr->n, V->getOffset(nodeNum), result,
+                       cnt);
+    launched("vertexFilter");
+    sync();
+}
+
+// vertexFilter (polymer.h:2411-2441): the same into a frontier; the count of
+// passing vertices is added to result->m
 template <class F>
 void vertexFilter(vertices *V, F filter, int nodeNum, int subNum, int totalSub, LocalFrontier *result) {
     using namespace polymer_detail;
@@ -959,13 +2121,13 @@ void vertexFilter(vertices *V, F filter, int nodeNum, int subNum, int totalSub, 
     LocalFrontier *fr = V->getFrontier(nodeNum);
     check_hip(hipMemsetAsync(result->counters, 0, sizeof(unsigned long long), lib_stream()),
               "hipMemsetAsync");
-    hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n)), dim3(256), 0, lib_stream(), filter,
-                       (const bool *)fr->b, (int64_t)fr->n, V->getOffset(nodeNum), result->b,
+    hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n, 256, 2048)), dim3(256), 0, lib_stream(), filter,
+                       (const bool *)fr->b, (int64_t)0, (int64_t)fr->n, V->getOffset(nodeNum), result->b,
                        result->counters);
     launched("vertexFilter");
     unsigned long long c = 0;
     toHost(&c, result->counters, 1);
-    result->m = (intT)c;
+    result->m += (intT)c;
     result->isDense = true;
 }
 
@@ -977,14 +2139,13 @@ inline void switchFrontier(int nodeNum, vertices *V, LocalFrontier *&next) {
     next->clearFrontier();
 }
 
-// Waits for the queued work (the reference's calls are synchronous).
-inline void polymerSync() {
-    polymer_detail::check_hip(hipStreamSynchronize(polymer_detail::lib_stream()), "sync");
-}
+// Waits for the queued work (every call above already does).
+inline void polymerSync() { polymer_detail::sync(); }
 
 // Functor classes written between these two lines compile for the host and the
-// device without annotations (the app's host code, main() included, stays
-// outside: a host+device main() would be compiled for the GPU as well).
+// device without annotations when the program is not compiled with --hipstdpar
+// (the program's host code, main() included, stays outside: a host+device main()
+// would be compiled for the GPU as well).
 #define POLYMER_FUNCTORS_BEGIN _Pragma("clang force_cuda_host_device begin")
 #define POLYMER_FUNCTORS_END _Pragma("clang force_cuda_host_device end")
 

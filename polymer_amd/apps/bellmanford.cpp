@@ -70,6 +70,7 @@ int main(int argc, char **argv) {
     vertices *Frontier = new vertices(1);
     Frontier->registerFrontier(0, current);
     Frontier->calculateOffsets();
+    Frontier->calculateNumOfNonZero(0);  // the count isEmpty / edgeMap read
 
     intT round = 0;
     while (!Frontier->isEmpty()) {
@@ -81,6 +82,7 @@ int main(int argc, char **argv) {
         edgeMap(G, Frontier, BF_F{SP, Visited}, next, (intT)threshold, DENSE_FORWARD);
         if (next->isDense) vertexCounter(G, next, 0, 0, 1);
         switchFrontier(0, Frontier, next);
+        Frontier->calculateNumOfNonZero(0);
         vertexMap(Frontier, BF_Vertex_F{Visited}, 0);
     }
     polymerSync();

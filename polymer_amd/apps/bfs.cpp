@@ -60,6 +60,7 @@ int main(int argc, char **argv) {
     vertices *Frontier = new vertices(1);
     Frontier->registerFrontier(0, current);
     Frontier->calculateOffsets();
+    Frontier->calculateNumOfNonZero(0);  // the count isEmpty / edgeMap read
 
     intT round = 0, dense_rounds = 0;
     long long reached = 1;
@@ -73,6 +74,7 @@ int main(int argc, char **argv) {
             vertexCounter(G, next, 0, 0, 1);
         }
         switchFrontier(0, Frontier, next);
+        Frontier->calculateNumOfNonZero(0);
         vertexMap(Frontier, Level_F{Levels, round}, 0);
         reached += Frontier->numNonzeros();
     }

@@ -59,6 +59,7 @@ int main(int argc, char **argv) {
     Frontier->calculateOffsets();
     fillLocalFrontier(current);
     vertexCounter(G, current, 0, 0, 1);
+    Frontier->calculateNumOfNonZero(0);  // the count isEmpty / edgeMap read
     vertexMap(Frontier, Id_F{IDs}, 0);
 
     intT round = 0;
@@ -68,6 +69,7 @@ int main(int argc, char **argv) {
         edgeMap(G, Frontier, CC_F{IDs, prevIDs}, next, (intT)threshold, DENSE);
         if (next->isDense) vertexCounter(G, next, 0, 0, 1);
         switchFrontier(0, Frontier, next);
+        Frontier->calculateNumOfNonZero(0);
     }
     polymerSync();
     std::vector<intT> out((size_t)n);

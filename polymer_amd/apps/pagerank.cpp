@@ -1,5 +1,6 @@
 // PageRank driver on polymer.h (the iteration of numa-PageRank.C:273-335):
-// fixed rounds; the active sources push p_curr[s]/outdeg(s) into p_next; the
+// fixed rounds; the active sources' p_curr[s]/outdeg(s) are summed into p_next
+// (pulled per destination with the functor's reduce trio); the
 // active vertices then take p_next = d * p_next + (1-d)/n and reset p_curr; the
 // next frontier is the set of out-neighbours of the active vertices.  fp64
 // values and atomics, as the reference.  Output: p_curr, n doubles.
@@ -18,6 +19,16 @@ struct PR_F {
     }
     inline bool updateAtomic(intT s, intT d) {
         writeAdd(&p_next[d], p_curr[s] / V[s].getOutDegree());
+        return true;
+    }
+    // the pull form (edgeMapDenseReduce): a private sum per destination, folded once
+    inline void initFunc(void *data, intT d) { *(double *)data = 0.0; }
+    inline bool reduceFunc(void *data, intT s) {
+        *(double *)data += p_curr[s] / (double)V[s].getOutDegree();
+        return true;
+    }
+    inline bool combineFunc(void *data, intT d) {
+        writeAdd(&p_next[d], *(double *)data);
         return true;
     }
     inline bool cond(intT d) { return true; }
@@ -74,7 +85,9 @@ int main(int argc, char **argv) {
         vertexCounter(G, Frontier->getFrontier(0), 0, 0, 1);
         edges += Frontier->getEdgeStat();
         clearLocalFrontier(next, 0, 0, 1);
-        edgeMapDenseForward(G, Frontier, PR_F<asymmetricVertex>(p_curr, p_next, G.V), next);
+        // the active sources' contributions pulled per destination (the push of
+        // numa-PageRank.C:297 with its reduce trio, :98-122)
+        edgeMapDenseReduce(G, Frontier, PR_F<asymmetricVertex>(p_curr, p_next, G.V), next);
         vertexMap(Frontier, PR_Vertex_F{damping, (1.0 - damping) / (double)n, p_next}, 0);
         vertexMap(Frontier, PR_Vertex_Reset{p_curr}, 0);
         std::swap(p_curr, p_next);

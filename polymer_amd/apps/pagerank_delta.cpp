@@ -95,6 +95,7 @@ int main(int argc, char **argv) {
     const double t_start = wall_ms();
     for (int t = 0; t < a.rounds; t++) {
         vertexCounter(G, Frontier->getFrontier(0), 0, 0, 1);
+        Frontier->calculateNumOfNonZero(0);
         edges += Frontier->getEdgeStat();
         active += Frontier->numNonzeros();
         edgeMap(G, Frontier, DeltaPush_F<asymmetricVertex>{G.V, Delta, nghSum}, dummy,
