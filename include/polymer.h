@@ -1616,6 +1616,17 @@ struct AsyncChunk {
     intT *s;
 };
 
+// a chunk of blockSize queue entries (polymer.h:1350-1356); the asynchronous edge
+// maps here seed from the frontier's flags, so chunks only carry the program's own
+// bookkeeping
+inline AsyncChunk *newChunk(intT blockSize) {
+    AsyncChunk *c = static_cast<AsyncChunk *>(std::malloc(sizeof(AsyncChunk)));
+    c->s = static_cast<intT *>(std::malloc(sizeof(intT) * (size_t)(blockSize > 0 ? blockSize : 1)));
+    c->m = 0;
+    c->accessCounter = 0;
+    return c;
+}
+
 struct LocalFrontier {
     intT n;
     intT m;

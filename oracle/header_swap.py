@@ -7,7 +7,7 @@ programs' functors compile for the GPU unannotated) and linked with
 libpolymer_gpu.so.  The copies never enter the repository; binaries go to
 oracle/_ref/swap/ (git-ignored).
 
-    python oracle/header_swap.py            # all six programs, rerouted
+    python oracle/header_swap.py            # the six programs (and the asynchronous BFS)
     python oracle/header_swap.py --check    # compile only, as tests/test_header_swap.py
 
 Reroutes: numa-PageRank.C, numa-BFS.C and numa-BP.C define their own edge loops
@@ -16,7 +16,8 @@ that walk the graph's neighbour lists on the host; those programs compile
 unchanged, and to run on the GPU their call sites are pointed at the header's
 edgeMapDenseForward, edgeMap and edgeMapDenseBP.  numa-BP.C also takes the array
 its functor reads (vertex potentials) from plain malloc, which the GPU cannot
-address: it is taken from numa_alloc_local instead.
+address: it is taken from numa_alloc_local instead.  numa-BFS-async-pipe.C
+hard-codes eight NUMA nodes: its commented-out libnuma query is restored.
 """
 import argparse
 import os
@@ -34,7 +35,7 @@ OUT = os.path.join(HERE, "_ref", "swap")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 APPS = ["numa-SPMV", "numa-Components", "numa-BellmanFord", "numa-PageRank", "numa-BFS",
-        "numa-BP"]
+        "numa-BP", "numa-BFS-async-pipe"]
 HELPERS = ["gettime.h", "parallel.h", "custom-barrier.h"]
 OURS = ["polymer.h", "polymer-wgh.h", "polymer_gpu.h"]
 
@@ -48,6 +49,10 @@ REROUTES = {
                 # malloc (numa-BP.C:557), which the GPU cannot address
                 (r"\(VertexInfo \*\)malloc\(sizeof\(VertexInfo\) \* GA\.n\)",
                  "(VertexInfo *)numa_alloc_local(sizeof(VertexInfo) * GA.n)")],
+    # it hard-codes eight NUMA nodes (numa-BFS-async-pipe.C:340, with the call that
+    # asks libnuma commented out beside it): the call is restored
+    "numa-BFS-async-pipe": [(r"numOfNode = 8;//numa_num_configured_nodes\(\);",
+                             "numOfNode = numa_num_configured_nodes();")],
 }
 
 FLAGS = ["--offload-arch=gfx950", "-O2", "-std=c++17", "--hipstdpar", "-w", "-x", "hip"]

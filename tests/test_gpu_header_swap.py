@@ -6,6 +6,7 @@ graph and checked against the reference binaries' golden output and the oracle:
   numa-SPMV      10 iterations, -result      values vs the reference binary's (golden)
   numa-BFS       -result -s, four sources    vertices visited and rounds (golden)
   numa-Components -s -result                 labels vs union-find
+  numa-BFS-async-pipe -result -s             vertices with a parent (golden reach)
   numa-BellmanFord                           finishes; frontier totals vs Dijkstra's reach
   numa-BP        10 iterations               runs to its timing line
 
@@ -111,6 +112,20 @@ def test_swap_bfs_vs_reference_golden(files):
         say("  source %d: visited %d rounds %d (reference %d, %d)" %
             (r["src"], visited, rounds, r["vertices_visited"], r["finished_in"]))
         assert visited == r["vertices_visited"] and rounds == r["finished_in"]
+        assert counted == r["vertices_visited"]
+
+
+def test_swap_bfs_async_pipe_vs_reference_golden(files):
+    """numa-BFS-async-pipe.C (edgeMapSparseAsyncPipe, polymer.h:1512-1751): one
+    asynchronous traversal; with -result it counts the vertices that got a parent,
+    which must be the reachable set the reference's numa-BFS reports."""
+    say("case: numa-BFS-async-pipe built on polymer.h, symmetric PR1")
+    gold = json.load(open(os.path.join(GOLD, "sym_bfs.json")))
+    for r in gold["runs"]:
+        out = run([exe("numa-BFS-async-pipe"), files["sym"], str(r["src"]), "-result", "-s"])
+        counted = int(re.search(r"Vert visited: (\d+)", out).group(1))
+        say("  source %d: %d vertices with a parent (reference %d)" %
+            (r["src"], counted, r["vertices_visited"]))
         assert counted == r["vertices_visited"]
 
 

@@ -1,5 +1,5 @@
 """The header swap, compile-only (CPU): each of the reference's six numa-*.C
-programs, copied from the reference tree into a scratch directory beside
+programs (and numa-BFS-async-pipe.C), copied from the reference tree into a scratch directory beside
 include/polymer.h (oracle/header_swap.py), compiles with hipcc --hipstdpar and no
 change to its text; numa-PageRank.C, numa-BFS.C and numa-BP.C also compile with
 their host-side edge loops pointed at the header's edge maps (the form that runs
