@@ -37,7 +37,9 @@ def cpu_baseline(pa, g, iters_sample=20):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # the host cores this process may use, at most the 16 a one-GPU box shares out
+    # (OMP_NUM_THREADS there)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     off, adj, _ = g.download()
     hg = orc.Graph(g.n, off, adj)
     hg.transpose()
@@ -376,18 +378,25 @@ def batched_extra(pa, scale=22, m=1 << 26, k=64, iters=20):
     src[0] = 0
     g.ppr_batched(src, iters, out=False)  # workspace / warm-up
     _, stb = g.ppr_batched(src, iters, out=False)
-    g.ppr(int(src[0]), iters, out=False)
-    plain_ms = 0.0
-    for s in src:
-        _, st = g.ppr(int(s), iters, out=False)
-        plain_ms += st.ms
-    return {"ppr64_scale22_batched_ms": round(stb.ms, 3),
-            "ppr64_scale22_plain_total_ms": round(plain_ms, 3),
-            "ppr64_batched_speedup": round(plain_ms / stb.ms, 2)}
+    out = {"ppr64_scale22_batched_ms": round(stb.ms, 3)}
+    # the plain runs on the PageRank tiles (the fastest single-vector pull), then on
+    # the degree-binned pull alone for comparison with earlier measurements
+    for label, tiles in (("", 1), ("_untiled", 0)):
+        g.set_tiles(tiles)
+        g.ppr(int(src[0]), iters, out=False)
+        plain_ms = 0.0
+        for s in src:
+            _, st = g.ppr(int(s), iters, out=False)
+            plain_ms += st.ms
+        out["ppr64_scale22_plain%s_total_ms" % label] = round(plain_ms, 3)
+        out["ppr64_batched_speedup%s" % label] = round(plain_ms / stb.ms, 2)
+    g.set_tiles(1)
+    return out
 
 
 def adjoint_extra(pa, orc_free=True, scale=22, m=1 << 26):
-    """Weighted PageRank forward solve vs adjoint (fixed point + gradient pass)."""
+    """Weighted PageRank forward solve vs adjoint (fixed point + gradient pass), both
+    iterated to the same relative tolerance."""
     g = pa.Graph.rmat(scale, m, seed=2, weights="f32", wseed=4)
     off, adj, w = g.download()
     src = np.repeat(np.arange(g.n), np.diff(off))
@@ -395,12 +404,17 @@ def adjoint_extra(pa, orc_free=True, scale=22, m=1 << 26):
     wn = (w / tot[src]).astype(np.float32)
     tele = np.full(g.n, 1.0 / g.n, np.float32)
     gv = np.random.default_rng(0).standard_normal(g.n).astype(np.float32)
-    g.wpr_forward(wn, tele, 0.85, 200, 1e-7, out=False)
-    _, stf = g.wpr_forward(wn, tele, 0.85, 200, 1e-7, out=False)
-    g.wpr_adjoint(gv, 200, 1e-6 * float(np.abs(gv).sum()), out=False)
-    _, _, _, sta = g.wpr_adjoint(gv, 200, 1e-6 * float(np.abs(gv).sum()), out=False)
+    # the same relative tolerance for both fixed points: the L1 change of an
+    # iteration below 1e-7 of the iterate's scale (|x|_1 = 1; |lambda| scales with |g|_1)
+    rel = 1e-7
+    tol_f, tol_a = rel * 1.0, rel * float(np.abs(gv).sum())
+    g.wpr_forward(wn, tele, 0.85, 500, tol_f, out=False)
+    _, stf = g.wpr_forward(wn, tele, 0.85, 500, tol_f, out=False)
+    g.wpr_adjoint(gv, 500, tol_a, out=False)
+    _, _, _, sta = g.wpr_adjoint(gv, 500, tol_a, out=False)
     return {"wpr_scale22_forward_ms": round(stf.ms, 3), "wpr_forward_iters": stf.rounds,
             "wpr_scale22_adjoint_ms": round(sta.ms, 3), "wpr_adjoint_iters": sta.rounds,
+            "wpr_relative_tolerance": rel, "wpr_forward_tol": tol_f, "wpr_adjoint_tol": tol_a,
             "adjoint_over_forward": round(sta.ms / stf.ms, 3)}
 
 

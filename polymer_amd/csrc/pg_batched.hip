@@ -120,11 +120,25 @@ struct PprRowOp {
     float damp, tele;
     bool write_r;  // ranks are stored by the last iteration only
     bool steady;   // iteration >= 2: a vertex without in-edges no longer changes
+    // Rows that are zero in every column (the first iterations: a column is nonzero
+    // only within t hops of its source): nz marks the nonzero rows of C and the
+    // gathers of the others are not issued (skip); track has the iteration mark the
+    // nonzero rows of Cn for the next one.  Adding the skipped zeros changes no bit.
+    const uint32_t *nz;
+    uint32_t *nz_next;
+    bool skip, track;
     // A vertex with no in-edge has r = (1-d)[v is a source] from iteration 1 on, so
     // once both contribution buffers hold that value (iterations 1 and 2) it is
     // skipped until the last iteration stores its rank.
     __device__ bool vertex_live(int, bool has_in) const { return has_in || !steady || write_r; }
+    __device__ bool row_nonzero(int s) const { return !skip || ((nz[s >> 5] >> (s & 31)) & 1u); }
 };
+
+// iterations whose gathers test the nonzero-row marks: from 64 one-hot sources on
+// the scale-22 graph the first iteration gathers 64 rows, and by the third the
+// tests cost more than the gathers they skip (measured on MI355X: 1.33, 1.86 and
+// 2.28 ms for iterations 0-2 with tests, 1.89 ms without)
+constexpr int kPprSkipIters = 2;
 
 // in-degree classes of the batched PPR iteration (measured on the scale-22 graph:
 // 39.5 ms for 20 iterations at these, 46 ms at 4096 / 2048 / 256, 66 ms with a
@@ -140,12 +154,24 @@ __device__ __forceinline__ void ppr_row_finish(const PprRowOp &op, int64_t v, in
     r.y = op.damp * acc.y + (sc[1] == (int)v ? op.tele : 0.f);
     r.z = op.damp * acc.z + (sc[2] == (int)v ? op.tele : 0.f);
     r.w = op.damp * acc.w + (sc[3] == (int)v ? op.tele : 0.f);
-    if (op.write_r) reinterpret_cast<float4 *>(op.R)[v * (kRow / 4) + l] = r;
+    // streamed stores: the rows written are not read before the next iteration, and
+    // the cache is better spent on the hot rows being gathered
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    if (op.write_r) {
+        const f4v rv = {r.x, r.y, r.z, r.w};
+        __builtin_nontemporal_store(rv, reinterpret_cast<f4v *>(op.R) + v * (kRow / 4) + l);
+    }
     const int od = op.out_off[v + 1] - op.out_off[v];
-    if (od > 0) {
+    if (od > 0 && !op.write_r) {  // the last iteration's contributions are not read
         const float f = (float)od;
-        reinterpret_cast<float4 *>(op.Cn)[v * (kRow / 4) + l] =
-            make_float4(r.x / f, r.y / f, r.z / f, r.w / f);
+        const f4v cv = {r.x / f, r.y / f, r.z / f, r.w / f};
+        __builtin_nontemporal_store(cv, reinterpret_cast<f4v *>(op.Cn) + v * (kRow / 4) + l);
+    }
+    if (op.track) {  // called by the vertex's 16 lanes together
+        const bool any = r.x != 0.f || r.y != 0.f || r.z != 0.f || r.w != 0.f;
+        const unsigned long long b = __ballot(any);
+        const bool row = ((b >> (threadIdx.x & 48)) & 0xFFFFull) != 0ull;
+        if (l == 0 && row && od > 0) atomicOr(&op.nz_next[v >> 5], 1u << (v & 31));
     }
 }
 
@@ -158,17 +184,17 @@ __device__ __forceinline__ void ppr_row_finish(const PprRowOp &op, int64_t v, in
 // the vertices (consecutive groups take consecutive vertices, whose in-degrees are
 // alike in the in-degree order, so a wavefront's groups finish together).  The
 // in-degree prefix [0, H) goes to k_ppr_rows_wave.
-__global__ __launch_bounds__(kBlock) void k_ppr_rows4(PprRowOp op, const int32_t *__restrict__ in_off,
-                                                      const int32_t *__restrict__ in_adj, int32_t H,
-                                                      int64_t n) {
+__device__ __forceinline__ void ppr_rows4_body(const PprRowOp &op,
+                                               const int32_t *__restrict__ in_off,
+                                               const int32_t *__restrict__ in_adj, int32_t H,
+                                               int64_t n, int64_t bid, int64_t nblocks) {
     const int l = threadIdx.x & 15;
-    const int64_t ngroups = (int64_t)gridDim.x * (kBlock / 16);
+    const int64_t ngroups = nblocks * (kBlock / 16);
     const float4 *__restrict__ C4 = reinterpret_cast<const float4 *>(op.C);
     int sc[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) sc[j] = op.src[4 * l + j];
-    for (int64_t v = H + (int64_t)blockIdx.x * (kBlock / 16) + (threadIdx.x >> 4); v < n;
-         v += ngroups) {
+    for (int64_t v = H + bid * (kBlock / 16) + (threadIdx.x >> 4); v < n; v += ngroups) {
         const int beg = in_off[v], end = in_off[v + 1];
         if (!op.vertex_live((int)v, end > beg)) continue;
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -179,7 +205,10 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows4(PprRowOp op, const int32_t
             for (int k = 0; k < 8; k++) s[k] = ld_stream(in_adj + e + k);
             float4 a[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+            for (int k = 0; k < 8; k++) {
+                a[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (op.row_nonzero(s[k])) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+            }
             acc.x += ((a[0].x + a[1].x) + (a[2].x + a[3].x)) + ((a[4].x + a[5].x) + (a[6].x + a[7].x));
             acc.y += ((a[0].y + a[1].y) + (a[2].y + a[3].y)) + ((a[4].y + a[5].y) + (a[6].y + a[7].y));
             acc.z += ((a[0].z + a[1].z) + (a[2].z + a[3].z)) + ((a[4].z + a[5].z) + (a[6].z + a[7].z));
@@ -191,14 +220,19 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows4(PprRowOp op, const int32_t
             for (int k = 0; k < 4; k++) s[k] = ld_stream(in_adj + e + k);
             float4 a[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+            for (int k = 0; k < 4; k++) {
+                a[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (op.row_nonzero(s[k])) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+            }
             acc.x += (a[0].x + a[1].x) + (a[2].x + a[3].x);
             acc.y += (a[0].y + a[1].y) + (a[2].y + a[3].y);
             acc.z += (a[0].z + a[1].z) + (a[2].z + a[3].z);
             acc.w += (a[0].w + a[1].w) + (a[2].w + a[3].w);
         }
         for (; e < end; e++) {
-            const float4 a = C4[(int64_t)ld_stream(in_adj + e) * (kRow / 4) + l];
+            const int s = ld_stream(in_adj + e);
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (op.row_nonzero(s)) a = C4[(int64_t)s * (kRow / 4) + l];
             acc.x += a.x;
             acc.y += a.y;
             acc.z += a.z;
@@ -214,11 +248,11 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows4(PprRowOp op, const int32_t
 // four 16-lane groups take every fourth edge (four gathers in flight each) and their
 // sums meet by two cross-group shuffles; a multi-chunk vertex is finished by its
 // last chunk to arrive, which adds the chunks' partial rows in chunk order.
-__global__ __launch_bounds__(kBlock) void k_ppr_rows_wave(PprRowOp op,
-                                                          const int32_t *__restrict__ in_off,
-                                                          const int32_t *__restrict__ in_adj,
-                                                          RowPlan P, int64_t nmedium) {
-    const int64_t w = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64;
+__device__ __forceinline__ void ppr_rows_wave_body(const PprRowOp &op,
+                                                   const int32_t *__restrict__ in_off,
+                                                   const int32_t *__restrict__ in_adj,
+                                                   const RowPlan &P, int64_t nmedium, int64_t bid) {
+    const int64_t w = (bid * kBlock + threadIdx.x) / 64;
     if (w >= P.nheavy + nmedium) return;  // uniform per wavefront
     const int lane = threadIdx.x & 63, q = lane >> 4, l = lane & 15;
     const bool heavy = w < P.nheavy;
@@ -243,14 +277,19 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows_wave(PprRowOp op,
         for (int k = 0; k < 4; k++) s[k] = ld_stream(in_adj + e + 4 * k);
         float4 a[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+        for (int k = 0; k < 4; k++) {
+            a[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (op.row_nonzero(s[k])) a[k] = C4[(int64_t)s[k] * (kRow / 4) + l];
+        }
         acc.x += (a[0].x + a[1].x) + (a[2].x + a[3].x);
         acc.y += (a[0].y + a[1].y) + (a[2].y + a[3].y);
         acc.z += (a[0].z + a[1].z) + (a[2].z + a[3].z);
         acc.w += (a[0].w + a[1].w) + (a[2].w + a[3].w);
     }
     for (; e < end; e += 4) {
-        const float4 a = C4[(int64_t)ld_stream(in_adj + e) * (kRow / 4) + l];
+        const int s = ld_stream(in_adj + e);
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (op.row_nonzero(s)) a = C4[(int64_t)s * (kRow / 4) + l];
         acc.x += a.x;
         acc.y += a.y;
         acc.z += a.z;
@@ -312,12 +351,34 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows_wave(PprRowOp op,
     if (q == 0) ppr_row_finish(op, v, l, tot, sc);
 }
 
-__global__ void k_ppr_rows_init(int64_t n, const int32_t *out_off, const int32_t *src, float *C) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * kRow;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int v = (int)(i / kRow), c = (int)(i % kRow);
+// One launch per batched PPR iteration: the first nwb blocks take the heavy chunks
+// and medium vertices (a wavefront per item, ppr_rows_wave_body), the rest stride
+// over the light vertices (ppr_rows4_body); the light blocks fill the machine while
+// the last heavy items finish, which two launches in a row left idle.
+__global__ __launch_bounds__(kBlock) void k_ppr_rows(PprRowOp op, const int32_t *__restrict__ in_off,
+                                                     const int32_t *__restrict__ in_adj, RowPlan P,
+                                                     int64_t nmedium, int64_t nwb, int32_t H,
+                                                     int64_t n, int64_t nb4) {
+    if ((int64_t)blockIdx.x < nwb)
+        ppr_rows_wave_body(op, in_off, in_adj, P, nmedium, blockIdx.x);
+    else
+        ppr_rows4_body(op, in_off, in_adj, H, n, (int64_t)blockIdx.x - nwb, nb4);
+}
+
+// r_0 = e_src: the contribution rows of the sources (every column of such a row;
+// 1/outdeg where the column's source is the vertex) and their nonzero-row marks.
+// The other rows of C are zero in r_0 and are never read: the first iteration
+// gathers marked rows only, and the second writes every row before any is read.
+__global__ void k_ppr_init_sources(int64_t n, const int32_t *out_off, const int32_t *src, float *C,
+                                   uint32_t *nz) {
+    for (int i = threadIdx.x; i < kRow * kRow; i += blockDim.x) {
+        const int row = i / kRow, c = i % kRow;  // row: the row of column row's source
+        const int v = src[row];
+        if (v < 0 || v >= n) continue;
         const int od = out_off[v + 1] - out_off[v];
-        C[i] = (src[c] == v && od > 0) ? 1.f / (float)od : 0.f;  // r_0 = e_src
+        if (od == 0) continue;
+        C[(int64_t)v * kRow + c] = (src[c] == v) ? 1.f / (float)od : 0.f;
+        if (c == 0) atomicOr(&nz[v >> 5], 1u << (v & 31));
     }
 }
 
@@ -373,6 +434,14 @@ struct PprOp {
         return 0ull;
     }
     __device__ void skip(int) const {}
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
+    __device__ static void stat_commit(Stat *, Stat) {}
+    // Tiled pull (the PageRank tiles): contributions in fixed point with 62 fraction
+    // bits; a personalized rank vector sums to at most 1, so no sum reaches 4.
+    static constexpr bool kTileWeighted = false;
+    __device__ float tile_raw(int s) const { return contrib[s]; }
+    __device__ unsigned long long tile_fix(float x) const { return to_fixed<62>(x); }
+    __device__ float tile_sum(unsigned long long q) const { return (float)((double)q * 0x1p-62); }
 };
 
 __global__ void k_ppr_init(int64_t n, const int32_t *out_off, int32_t src, float *C) {
@@ -724,12 +793,26 @@ extern "C" int pg_ppr(pg_graph *g, int64_t src, int iters, double damping, float
         hipLaunchKernelGGL(k_fill<float>, dim3(1), dim3(1), 0, stream(), R + s[0], 1.f, (int64_t)1);
         PG_CHECK_HIP(hipGetLastError());
     }
+    // the PageRank tiles when the graph uses them (the zero-in-degree tail is settled
+    // after two iterations, but the last one writes every rank)
+    if (g->use_tiles) PG_TRY(ensure_tiles(g));
+    const bool tiled = g->use_tiles && g->tiles.built &&
+                       g->tiles.light_begin == g->in_sched.light_begin &&
+                       g->tiles.light_end == g->in_sched.light_end;
+    const bool ahead = tiled && g->tiles.cold_overlap && g->tiles.cold.nparts > 0;
     float *c = c0, *cn = c1;
     for (int t = 0; t < iters; t++) {
         PprOp op{c,     cn,   R, g->out_off, s[0], (float)damping, (float)(1.0 - damping),
                  t == iters - 1, t >= 2, g->in_off};
+        PprOp next = op;  // its cold pre-pass reads the contributions written now
+        next.contrib = cn;
         PG_TRY(kt.begin());
-        PG_TRY(launch_pull(op, g->in_sched));
+        if (tiled)
+            PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles,
+                                     (PprOp::Stat *)nullptr, t >= 2 && t != iters - 1,
+                                     ahead && t + 1 < iters ? &next : nullptr, ahead && t > 0));
+        else
+            PG_TRY(launch_pull(op, g->in_sched));
         PG_TRY(kt.end());
         std::swap(c, cn);
     }
@@ -773,22 +856,32 @@ extern "C" int pg_ppr_batched(pg_graph *g, const int64_t *src, int k, int iters,
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
     Timer tm;
     PG_TRY(tm.start());
-    PG_LAUNCH_B(k_ppr_rows_init, grid_for(n * kRow), n, g->out_off, dsrc, C0);
     if (iters == 0) PG_TRY(fill(R, 0.f, n * kRow));
+    // nonzero-row marks of the contribution buffers for the first iterations; r_0's
+    // rows are those of the sources
+    const int64_t nzw = (n + 31) / 32 + 1;
+    uint32_t *nz0 = nullptr, *nz1 = nullptr;
+    PG_TRY(workspace_t(g, "pprb_nz0", (size_t)nzw, &nz0));
+    PG_TRY(workspace_t(g, "pprb_nz1", (size_t)nzw, &nz1));
+    PG_TRY(fill(nz0, 0u, nzw));
+    PG_LAUNCH_B(k_ppr_init_sources, 1, n, g->out_off, dsrc, C0, nz0);
     float *C = C0, *Cn = C1;
+    uint32_t *nz = nz0, *nzn = nz1;
     for (int t = 0; t < iters; t++) {
-        PprRowOp op{C, Cn, R, g->out_off, dsrc, (float)damping, (float)(1.0 - damping),
-                    t == iters - 1, t >= 2};
+        const bool skip = t < kPprSkipIters, track = t + 1 < kPprSkipIters;
+        if (track) PG_TRY(fill(nzn, 0u, nzw));
+        PprRowOp op{C,     Cn,   R,  g->out_off, dsrc, (float)damping, (float)(1.0 - damping),
+                    t == iters - 1, t >= 2, nz, nzn, skip, track};
         PG_TRY(kt.begin());
         const int64_t nwave = P.nheavy + (M - P.H);
-        if (nwave > 0)
-            PG_LAUNCH_B(k_ppr_rows_wave, (nwave * 64 + kBlock - 1) / kBlock, op, g->in_off,
-                        g->in_adj, P, M - P.H);
-        if (n > M)
-            PG_LAUNCH_B(k_ppr_rows4, grid_for((n - M) * 16, 4096), op, g->in_off, g->in_adj,
-                        (int32_t)M, n);
+        const int64_t nwb = (nwave * 64 + kBlock - 1) / kBlock;
+        const int64_t nb4 = n > M ? (int64_t)grid_for((n - M) * 16, 4096) : 0;
+        if (nwb + nb4 > 0)
+            PG_LAUNCH_B(k_ppr_rows, (unsigned)(nwb + nb4), op, g->in_off, g->in_adj, P, M - P.H, nwb,
+                        (int32_t)M, n, nb4);
         PG_TRY(kt.end());
         std::swap(C, Cn);
+        std::swap(nz, nzn);
     }
     double ms = 0;
     PG_TRY(tm.stop(&ms));

@@ -34,13 +34,20 @@ def onehot(n, s):
 
 # ---- personalized PageRank ---------------------------------------------------------------
 def test_ppr_plain(pa, orc):
+    """The single-vector form on the PageRank tiles (with and without the cold
+    pre-pass, split tiles) and on the degree-binned pull; 1-3 iterations cover the
+    settling of the zero-in-degree tail."""
     say("case: ppr plain")
     g = orc.rmat(14, 1 << 18, seed=31)
     d = pa.Graph.rmat(14, 1 << 18, seed=31)
-    for s in (0, 77, 9999):
-        r, _ = d.ppr(s, 20, 0.85)
-        ref = orc.ppr(g, onehot(g.n, s), 20, 0.85)
-        assert rel_l1(r, ref) < TOL, s
+    for tiles, te, cold in ((1, 0, 0), (1, 4096, 2000), (1, 1 << 30, 6000), (0, 0, 0)):
+        d.set_tiles(tiles, te, cold if cold else 0)
+        for s in (0, 77, 9999):
+            for iters in (1, 2, 3, 20):
+                r, _ = d.ppr(s, iters, 0.85)
+                ref = orc.ppr(g, onehot(g.n, s), iters, 0.85)
+                assert rel_l1(r, ref) < TOL, (tiles, te, cold, s, iters)
+    d.set_tiles(1)
 
 
 @pytest.mark.parametrize("k", [64, 5])
