@@ -510,13 +510,23 @@ struct Vec2 {
 // T = double: the product state and the messages are fp64 (default); T = float:
 // fp32 throughout (PG_BP_FP32).  Potentials are fp32 inputs either way
 // (numa-BP.C:59-73 stores them as float).
-template <class T>
+//
+// The frontier of the reference's loop depends on the graph alone: F_0 = V and
+// F_{t+1} = the out-neighbours of F_t (every vertex the push reaches is set,
+// numa-BP.C:122-126, whatever the values).  So F_{t+1} is a subset of F_t, and a
+// vertex v leaves it for good at a fixed iteration D[v] (v is in F_t iff t < D[v]).
+// kDeath: the per-edge source test reads D of the edge's source from a byte per
+// CSC edge (de, 255 = never leaves), streamed beside the potentials, instead of
+// looking the source up in a bitmap of F_t (a dependent random load per edge).
+// The source id itself is not needed: the message uses the destination's state.
+template <class T, bool kDeath>
 struct BpOp {
     using Acc = BpAcc<T>;
     using Ctx = Vec2<T>;  // q_l = phi_d[l] * P_cur[d][l]
     using Stat = unsigned long long;
     static constexpr bool kEarlyExit = false;
     static constexpr bool kStats = true;  // sum of out-degrees of the next frontier
+    static constexpr bool kEdgeToken = kDeath;
     const uint8_t *F;
     uint8_t *Fn;
     const float4 *pot;   // CSC order, W_e[0][0], W_e[0][1], W_e[1][0], W_e[1][1]
@@ -524,7 +534,9 @@ struct BpOp {
     const Vec2<T> *cur;
     Vec2<T> *nxt;
     const int32_t *out_off;
-    const uint32_t *Fb;  // F as a bitmap: the per-edge source test reads n/8 bytes, not n
+    const uint32_t *Fb;  // !kDeath: F as a bitmap (n/8 bytes)
+    const uint8_t *de;   // kDeath: D of the source of each CSC edge
+    int t;               // kDeath: the iteration
     __device__ bool active(int v) const { return F[v] != 0; }
     __device__ Ctx ctx(int v) const {
         const float2 phi = vpot[v];
@@ -535,19 +547,38 @@ struct BpOp {
     __device__ Acc combine(Acc a, Acc b) const {
         return Acc{a.p0 * b.p0, a.p1 * b.p1, a.any | b.any, 0};
     }
+    __device__ int edge_token(int e) const {
+        return (int)__builtin_nontemporal_load(de + e);
+    }
+    // tok: D of the source (kDeath) or the source id
+    __device__ bool src_active(int tok) const {
+        if constexpr (kDeath) return tok == 255 || tok > t;
+        else return bit_test(Fb, tok);
+    }
+    // b_e[k] = sum_l W_e[k][l] q_l (numa-BP.C:118-121)
+    __device__ void message(float4 w, const Ctx &q, T &b0, T &b1) const {
+        b0 = (T)w.x * q.x + (T)w.y * q.y;
+        b1 = (T)w.z * q.x + (T)w.w * q.y;
+    }
     // the potential is loaded before the frontier test so that its (streamed,
-    // coalesced) load is not held behind the bitmap lookup (a non-temporal load
-    // of it measured 9% slower)
-    __device__ Acc edge(int s, int e, const Ctx &q) const {
+    // coalesced) load is not held behind the test's load (a non-temporal load of
+    // it measured 9% slower)
+    __device__ Acc edge(int tok, int e, const Ctx &q) const {
         const float4 w = pot[e];
-        if (!bit_test(Fb, s)) return identity();
-        return Acc{(T)w.x * q.x + (T)w.y * q.y, (T)w.z * q.x + (T)w.w * q.y, 1, 0};
+        if (!src_active(tok)) return identity();
+        Acc a{T(1), T(1), 1, 0};
+        message(w, q, a.p0, a.p1);
+        return a;
     }
     __device__ bool found(Acc, const Ctx &) const { return false; }
-    __device__ Stat finish(int v, Acc a, const Ctx &) const {
+    // v's product and next-frontier flag; the statistic is its out-degree if set
+    __device__ Stat finish_deg(int v, Acc a, int outdeg) const {
         nxt[v] = Vec2<T>{a.p0, a.p1};
         Fn[v] = a.any ? 1 : 0;
-        return a.any ? (Stat)(out_off[v + 1] - out_off[v]) : 0ull;
+        return a.any ? (Stat)outdeg : 0ull;
+    }
+    __device__ Stat finish(int v, Acc a, const Ctx &) const {
+        return finish_deg(v, a, a.any ? out_off[v + 1] - out_off[v] : 0);
     }
     __device__ void skip(int v) const { Fn[v] = 0; }
     __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
@@ -1898,6 +1929,404 @@ __global__ void k_bp_out(const Vec2<T> *a, const int32_t *perm, int64_t n, doubl
     }
 }
 
+// ---- edge-balanced pull of the light destinations -------------------------------
+// The destinations of the lane-group classes up to kBpLightMaxG lanes (in-degree
+// at most 16 * items per lane: positions [lb, be) of the CSC, all with at least one
+// in-edge) are cut into chunks of about
+// kBpChunkEdges consecutive in-edges, aligned to vertices: chunk c holds the
+// vertices whose first in-edge lies in [off[lb] + c*K, off[lb] + (c+1)*K).  One
+// wave walks a chunk 64 edges at a time, one edge per lane (the potentials are read
+// as whole 1 KiB rows), and forms each destination's product with a segmented scan
+// across the lanes: a bitmap over the CSC edges marks each light vertex's first
+// in-edge, so a lane finds its destination with a popcount and its segment's start
+// with a leading-zero count.  A destination whose edges run past a step carries its
+// partial product into the next step.  The heavier destinations run on the
+// degree-binned blocks of the same launch.
+constexpr int kBpChunkEdges = 2048;
+constexpr int kBpSteps = 2;     // 64-edge steps whose loads are issued together
+constexpr int kBpLightMaxG = 16;  // the light range: the lane-group classes up to this one
+
+struct BpLight {
+    const int32_t *chunk_v;               // nchunks + 1 vertex positions
+    const unsigned long long *heads;      // bit e set: e is a light vertex's first in-edge
+    int nchunks;
+};
+
+// v moved across the lanes of a wave by a DPP control; the lanes it does not
+// feed (outside the row mask, or without a source lane) get 1
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROWS, 0xF, false);
+    const int hi =
+        __builtin_amdgcn_update_dpp(0x3FF00000, (int)(unsigned)(u >> 32), CTRL, ROWS, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0x3F800000, __builtin_bit_cast(int, v), CTRL, ROWS, 0xF, false));
+}
+
+template <class T>
+__device__ __forceinline__ T readlane_t(T v, int l) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    }
+}
+
+// Inclusive segmented product of (b0, b1) over the 64 lanes, a segment starting at
+// lane ss (the last head at or below this lane, -1: none): Hillis-Steele steps
+// within rows of 16 lanes (row_shr 1, 2, 4, 8), then the last lane of row 0 into
+// row 1 and of row 2 into row 3 (row_bcast 15), then lane 31 into rows 2 and 3
+// (row_bcast 31); a lane takes a value only from its own segment.
+template <class T>
+__device__ __forceinline__ void seg_scan_mul(T &b0, T &b1, int lane, int ss) {
+#define PG_SEG_STAGE(CTRL, ROWS, COND)                                      \
+    {                                                                       \
+        const T u0 = dpp_move<CTRL, ROWS>(b0), u1 = dpp_move<CTRL, ROWS>(b1); \
+        if (COND) {                                                         \
+            b0 *= u0;                                                       \
+            b1 *= u1;                                                       \
+        }                                                                   \
+    }
+    PG_SEG_STAGE(0x111, 0xF, (lane & 15) >= 1 && lane - 1 >= ss)
+    PG_SEG_STAGE(0x112, 0xF, (lane & 15) >= 2 && lane - 2 >= ss)
+    PG_SEG_STAGE(0x114, 0xF, (lane & 15) >= 4 && lane - 4 >= ss)
+    PG_SEG_STAGE(0x118, 0xF, (lane & 15) >= 8 && lane - 8 >= ss)
+    PG_SEG_STAGE(0x142, 0xA, ((lane >> 4) & 1) && ss < (lane & ~15))
+    PG_SEG_STAGE(0x143, 0xC, lane >= 32 && ss < 32)
+#undef PG_SEG_STAGE
+}
+
+// the vertex protocol of k_pull for one destination, with its frontier flag and
+// out-degree already loaded: skipped when it is not in the frontier, else
+// finished with its product
+template <class T, bool D>
+__device__ __forceinline__ unsigned long long bp_close(const BpOp<T, D> &op, int v, T p0, T p1,
+                                                       bool any, int in_frontier, int outdeg) {
+    if (!in_frontier) {
+        op.skip(v);
+        return 0ull;
+    }
+    return op.finish_deg(v, BpAcc<T>{p0, p1, any ? 1 : 0, 0}, outdeg);
+}
+
+template <class T, bool D>
+__device__ unsigned long long bp_light_chunk(const BpOp<T, D> &op, const int32_t *__restrict__ off,
+                                             const int32_t *__restrict__ adj,
+                                             const unsigned long long *__restrict__ heads, int vs,
+                                             int ve) {
+    unsigned long long st = 0;
+    if (vs >= ve) return st;
+    const int lane = lane_id();
+    const int E0 = off[vs], E1 = off[ve];
+    const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= lane
+    int vprev = vs - 1;      // destination of the edge before the next step's first edge
+    T c0 = T(1), c1 = T(1);  // the open destination's product so far (uniform)
+    bool cany = false, open = false;
+    int cfl = 0, cod = 0;  // the open destination's frontier flag and out-degree
+    // the head words of the next batch, loaded one batch ahead
+    unsigned long long hn[kBpSteps];
+#pragma unroll
+    for (int u = 0; u < kBpSteps; u++) hn[u] = heads[min((E0 & ~63) + 64 * u, E1 - 1) >> 6];
+    for (int base = E0 & ~63; base < E1; base += 64 * kBpSteps) {
+        int tok[kBpSteps], own[kBpSteps], vpb[kBpSteps], fl[kBpSteps], od[kBpSteps];
+        float4 w[kBpSteps];
+        float2 phi[kBpSteps];
+        Vec2<T> pc[kBpSteps];
+        unsigned long long hw[kBpSteps];
+        int vp = vprev;
+        // every load is issued unconditionally, from an index clamped into the
+        // chunk (a lane outside it loads an edge or vertex of the chunk and its
+        // values are not used), so that no load waits behind a branch
+#pragma unroll
+        for (int u = 0; u < kBpSteps; u++) {
+            const int bu = base + 64 * u;
+            const int ec = min(max(bu + lane, E0), E1 - 1);
+            unsigned long long h = hn[u];
+            if (bu >= E1) h = 0ull;
+            if (bu < E0) h &= ~0ull << (E0 - bu);  // the previous chunk's vertices
+            hw[u] = h;
+            vpb[u] = vp;
+            own[u] = vp + __popcll(h & le);
+            vp += __popcll(h);
+            const int oc = min(max(own[u], vs), ve - 1);
+            tok[u] = edge_src(op, adj, ec);
+            w[u] = op.pot[ec];
+            phi[u] = op.vpot[oc];
+            pc[u] = op.cur[oc];
+            fl[u] = op.F[oc];
+            od[u] = op.out_off[oc + 1] - op.out_off[oc];
+        }
+#pragma unroll
+        for (int u = 0; u < kBpSteps; u++)
+            hn[u] = heads[min(base + 64 * (kBpSteps + u), E1 - 1) >> 6];
+        vprev = vp;
+#pragma unroll
+        for (int u = 0; u < kBpSteps; u++) {
+            const int bu = base + 64 * u;
+            if (bu >= E1) break;
+            const unsigned long long h = hw[u];
+            if (open && (h & 1ull)) {  // the carried destination ended at bu - 1
+                if (lane == 0) st += bp_close(op, vpb[u], c0, c1, cany, cfl, cod);
+                open = false;
+                c0 = T(1);
+                c1 = T(1);
+                cany = false;
+            }
+            const int e = bu + lane;
+            const bool valid = e >= E0 && e < E1;
+            T b0 = T(1), b1 = T(1);
+            const bool act = valid && op.src_active(tok[u]);
+            if (act) {
+                const Vec2<T> q{(T)phi[u].x * pc[u].x, (T)phi[u].y * pc[u].y};
+                op.message(w[u], q, b0, b1);
+            }
+            // inclusive segmented product over the lanes; a segment starts at a head
+            const unsigned long long hl = h & le;
+            const int ss = hl ? 63 - __builtin_clzll(hl) : -1;
+            seg_scan_mul(b0, b1, lane, ss);
+            const unsigned long long A = __ballot(act);
+            bool any = (A & le & (ss >= 0 ? ~((1ull << ss) - 1ull) : ~0ull)) != 0ull;
+            if (ss < 0) {  // the lanes before the step's first head continue the carry
+                b0 = c0 * b0;
+                b1 = c1 * b1;
+                any = any || cany;
+            }
+            const bool tail = valid && (e + 1 == E1 || (lane < 63 && ((h >> (lane + 1)) & 1ull)));
+            if (tail) st += bp_close(op, own[u], b0, b1, any, fl[u], od[u]);
+            // lane 63's destination may run on into the next step: carry it (the
+            // carry taken in has been used by the lanes before the first head, or
+            // is in lane 63's product when the step has no head)
+            open = bu + 64 < E1;
+            if (open) {
+                c0 = readlane_t(b0, 63);
+                c1 = readlane_t(b1, 63);
+                cany = (__ballot(any) >> 63) != 0ull;
+                cfl = __builtin_amdgcn_readlane(fl[u], 63);
+                cod = __builtin_amdgcn_readlane(od[u], 63);
+            }
+        }
+    }
+    return st;
+}
+
+// blocks [0, hot_blocks): the degree-binned pull of the heavy destinations and the
+// vertices without in-edges; the rest: one light chunk per wave
+template <class T, bool D>
+__global__ __launch_bounds__(kBlock) void k_bp_pull(BpOp<T, D> op, PullArgs pa, int hot_blocks,
+                                                   BpLight L, unsigned long long *stat_out) {
+    unsigned long long st = 0;
+    if ((int)blockIdx.x < hot_blocks) {
+        st = pull_dispatch(op, pa, (int)blockIdx.x);
+    } else {
+        const int c = __builtin_amdgcn_readfirstlane(((int)blockIdx.x - hot_blocks) * (kBlock / 64) +
+                                                     (int)(threadIdx.x >> 6));
+        if (c < L.nchunks)
+            st = bp_light_chunk(op, pa.off, pa.adj, L.heads, L.chunk_v[c], L.chunk_v[c + 1]);
+    }
+    commit_block_stat<BpOp<T, D>>(st, stat_out);
+}
+
+// chunk c's first vertex: the first position v in [lb, be] with off[v] >= off[lb] + c*K
+__global__ void k_bp_chunks(const int32_t *off, int lb, int be, int nch, int32_t *chunk_v) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > nch) return;
+    if (c == nch) {
+        chunk_v[c] = be;
+        return;
+    }
+    const long long target = (long long)off[lb] + (long long)c * kBpChunkEdges;
+    int lo = lb, hi = be;  // off[hi] >= target
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if ((long long)off[mid] >= target) hi = mid;
+        else lo = mid + 1;
+    }
+    chunk_v[c] = lo;
+}
+
+__global__ void k_bp_mark_heads(const int32_t *off, int lb, int be, unsigned long long *heads) {
+    for (int64_t v = lb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < be;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int e = off[v];
+        atomicOr(&heads[e >> 6], 1ull << (e & 63));
+    }
+}
+
+// The light range and its chunks, built once per schedule.  Returns the blocks of
+// the degree-binned part in *hot (its segments are copied into *pa) and the
+// light-range description in *L.
+static int bp_light_setup(pg_graph *g, PullArgs *pa, int *hot, BpLight *L) {
+    const PullSchedule &s = g->in_sched;
+    const PullArgs &a = s.args;
+    const int64_t n = g->n, be = s.body_end;
+    int64_t lb = be;
+    int i = 0;
+    while (i < a.nseg && !(a.seg[i].g >= 1 && a.seg[i].g <= kBpLightMaxG)) i++;
+    if (i < a.nseg) lb = std::min<int64_t>(a.seg[i].v_begin, be);
+    *pa = a;
+    pa->nseg = i;
+    int64_t blocks = i < a.nseg ? a.seg[i].block_begin : s.nblocks;
+    if (i == a.nseg) blocks = s.nblocks;  // no light class: the schedule unchanged
+    if (i < a.nseg && be < n) {           // the vertices without in-edges, G = 1
+        if (pa->nseg >= kMaxSegs) {
+            set_error("bp: too many segments");
+            return PG_ERR_STATE;
+        }
+        pa->seg[pa->nseg++] = PullSeg{(int32_t)be, (int32_t)n, 1, (int32_t)blocks};
+        blocks += (n - be + kBlock - 1) / kBlock;
+    }
+    *hot = (int)blocks;
+    if (i == a.nseg) {
+        *L = BpLight{nullptr, nullptr, 0};
+        return PG_OK;
+    }
+    if (!g->bp_light_built || g->bp_light_lb != lb || g->bp_light_be != be) {
+        int32_t e2[2];
+        PG_CHECK_HIP(hipMemcpyAsync(&e2[0], g->in_off + lb, 4, hipMemcpyDeviceToHost, stream()));
+        PG_CHECK_HIP(hipMemcpyAsync(&e2[1], g->in_off + be, 4, hipMemcpyDeviceToHost, stream()));
+        PG_CHECK_HIP(hipStreamSynchronize(stream()));
+        const int64_t nch = ((int64_t)e2[1] - e2[0] + kBpChunkEdges - 1) / kBpChunkEdges;
+        int32_t *cv = nullptr;
+        unsigned long long *heads = nullptr;
+        const int64_t nw = g->m / 64 + 2;
+        PG_TRY(workspace_t(g, "bp_chunks", (size_t)nch + 1, &cv));
+        PG_TRY(workspace_t(g, "bp_heads", (size_t)nw, &heads));
+        PG_LAUNCH_K(k_zero_u64, grid_for(nw), heads, nw);
+        if (be > lb)
+            PG_LAUNCH_K(k_bp_mark_heads, grid_for(be - lb), g->in_off, (int)lb, (int)be, heads);
+        PG_LAUNCH_K(k_bp_chunks, (unsigned)((nch + 1 + kBlock - 1) / kBlock), g->in_off, (int)lb,
+                    (int)be, (int)nch, cv);
+        g->bp_light_lb = lb;
+        g->bp_light_be = be;
+        g->bp_light_nch = (int)nch;
+        g->bp_light_built = true;
+    }
+    int32_t *cv = nullptr;
+    unsigned long long *heads = nullptr;
+    PG_TRY(workspace_t(g, "bp_chunks", (size_t)g->bp_light_nch + 1, &cv));
+    PG_TRY(workspace_t(g, "bp_heads", (size_t)(g->m / 64 + 2), &heads));
+    *L = BpLight{cv, heads, g->bp_light_nch};
+    const int64_t tot = blocks + ((int64_t)g->bp_light_nch + kBlock / 64 - 1) / (kBlock / 64);
+    if (tot > 0x7FFFFFFF) {
+        set_error("bp: grid too large");
+        return PG_ERR_UNSUPPORTED;
+    }
+    return PG_OK;
+}
+
+// ---- the iteration at which each vertex leaves the frontier -----------------------
+// One pull per iteration of the frontier alone (F_{t+1}[v] = some in-neighbour in
+// F_t, stopping at the first), until no vertex leaves: those left never do (255).
+// Built once per graph.  A graph whose frontier still shrinks after 254 steps
+// keeps the bitmap test (bp_death_state = -1).
+struct BpReachOp {
+    using Acc = int;
+    using Ctx = int;
+    using Stat = unsigned long long;
+    static constexpr bool kEarlyExit = true;
+    static constexpr bool kStats = true;  // vertices that left
+    const uint8_t *F;
+    uint8_t *Fn;
+    uint8_t *D;
+    const uint32_t *Fb;
+    int t1;  // the iteration whose frontier is formed
+    __device__ bool active(int v) const { return F[v] != 0; }
+    __device__ Ctx ctx(int) const { return 0; }
+    __device__ Acc identity() const { return 0; }
+    __device__ Acc combine(Acc a, Acc b) const { return a | b; }
+    __device__ Acc edge(int s, int, const Ctx &) const { return bit_test(Fb, s) ? 1 : 0; }
+    __device__ bool found(Acc a, const Ctx &) const { return a != 0; }
+    __device__ Stat finish(int v, Acc a, const Ctx &) const {
+        Fn[v] = a ? 1 : 0;
+        if (a) return 0ull;
+        D[v] = (uint8_t)t1;
+        return 1ull;
+    }
+    __device__ void skip(int v) const { Fn[v] = 0; }
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
+    __device__ static void stat_commit(Stat *p, Stat v) {
+        atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
+    }
+};
+
+__global__ void k_gather_u8(const uint8_t *src, const int32_t *idx, int64_t m, uint8_t *out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = src[idx[i]];
+}
+
+static int bp_death_times(pg_graph *g) {
+    if (g->bp_death_state != 0) return PG_OK;
+    const int64_t n = g->n, m = g->m;
+    uint8_t *D = nullptr, *fa = nullptr, *fb = nullptr, *de = nullptr;
+    uint32_t *bits = nullptr;
+    unsigned long long *cnt = nullptr;
+    PG_TRY(workspace_t(g, "bp_dv", (size_t)n, &D));
+    PG_TRY(workspace_t(g, "bp_de", (size_t)m, &de));
+    PG_TRY(workspace_t(g, "bp_df0", (size_t)n, &fa));
+    PG_TRY(workspace_t(g, "bp_df1", (size_t)n, &fb));
+    PG_TRY(workspace_t(g, "bp_fbits", (size_t)(n + 31) / 32 + 1, &bits));
+    PG_TRY(workspace_t(g, "bp_dcnt", (size_t)kStatSlots, &cnt));
+    PG_LAUNCH_K(k_fill_u8, grid_for(n), D, (uint8_t)255, n);
+    PG_LAUNCH_K(k_fill_u8, grid_for(n), fa, (uint8_t)1, n);
+    int state = 0;
+    for (int t = 0; t < 255 && state == 0; t++) {
+        PG_LAUNCH_K(k_zero_u64, 1, cnt, kStatSlots);
+        PG_LAUNCH_K(k_flags_to_bits, grid_for(n), fa, n, bits);
+        PG_TRY(launch_pull(BpReachOp{fa, fb, D, bits, t + 1}, g->in_sched, cnt));
+        unsigned long long h[kStatSlots];
+        PG_CHECK_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, stream()));
+        PG_CHECK_HIP(hipStreamSynchronize(stream()));
+        unsigned long long left = 0;
+        for (int k = 0; k < kStatSlots; k++) left += h[k];
+        if (left == 0) state = 1;            // fixed point: the rest stay (255)
+        else if (t + 1 == 255) state = -1;   // a 255th iteration would be needed
+        std::swap(fa, fb);
+    }
+    if (state == 1) PG_LAUNCH_K(k_gather_u8, grid_for(m), D, g->in_adj, m, de);
+    g->bp_death_state = state;
+    return PG_OK;
+}
+
+template <class T, bool Death>
+static int bp_iterations(pg_graph *g, int iters, const float4 *pot, const float2 *vp,
+                         Vec2<T> *&cur, Vec2<T> *&nxt, uint8_t *&F, uint8_t *&Fn,
+                         unsigned long long *stats, const PullArgs &pa, int hot, const BpLight &L,
+                         KernelTimer &kt) {
+    const int64_t n = g->n;
+    uint32_t *Fb = nullptr;
+    if (!Death) PG_TRY(workspace_t(g, "bp_fbits", (size_t)(n + 31) / 32 + 1, &Fb));
+    const uint8_t *de = nullptr;
+    if (Death) {
+        uint8_t *p = nullptr;
+        PG_TRY(workspace_t(g, "bp_de", (size_t)g->m, &p));
+        de = p;
+    }
+    const int nb = hot + (L.nchunks + kBlock / 64 - 1) / (kBlock / 64);
+    for (int t = 0; t < iters; t++) {
+        BpOp<T, Death> op{F, Fn, pot, vp, cur, nxt, g->out_off, Fb, de, t};
+        PG_TRY(kt.begin());
+        if (!Death) PG_LAUNCH_K(k_flags_to_bits, grid_for(n), F, n, Fb);
+        if (nb > 0) {
+            hipLaunchKernelGGL((k_bp_pull<T, Death>), dim3(nb), dim3(kBlock), 0, stream(), op, pa, hot,
+                               L, &stats[(size_t)t * kStatSlots]);
+            PG_CHECK_HIP(hipGetLastError());
+        }
+        PG_TRY(kt.end());
+        std::swap(cur, nxt);  // swap(vertD_curr, vertD_next), numa-BP.C:303-304
+        std::swap(F, Fn);     // switchFrontier
+    }
+    return PG_OK;
+}
+
 template <class T>
 static int bp_run(pg_graph *g, int iters, const float4 *pot, const float2 *vp, const float *p0,
                   uint8_t *f0, uint8_t *f1, double *prod, uint32_t flags, pg_stats *st) {
@@ -1906,6 +2335,10 @@ static int bp_run(pg_graph *g, int iters, const float4 *pot, const float2 *vp, c
     PG_TRY(workspace_t(g, sizeof(T) == 8 ? "bp_pa64" : "bp_pa32", (size_t)n, &pa));
     PG_TRY(workspace_t(g, sizeof(T) == 8 ? "bp_pb64" : "bp_pb32", (size_t)n, &pb));
     PG_LAUNCH_K(k_bp_state_init<T>, grid_for(n), p0, n, pa, pb);
+    PullArgs hpa{};
+    int hot = 0;
+    BpLight L{};
+    PG_TRY(bp_light_setup(g, &hpa, &hot, &L));
     KernelTimer kt;
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
     Timer tm;
@@ -1916,17 +2349,10 @@ static int bp_run(pg_graph *g, int iters, const float4 *pot, const float2 *vp, c
     PG_TRY(workspace_t(g, "bp_stats", ((size_t)iters + 1) * kStatSlots, &stats));
     PG_LAUNCH_K(k_zero_u64, (unsigned)(((iters + 1) * kStatSlots + kBlock - 1) / kBlock), stats,
                 (iters + 1) * kStatSlots);
-    uint32_t *Fb = nullptr;
-    PG_TRY(workspace_t(g, "bp_fbits", (size_t)(n + 31) / 32 + 1, &Fb));
-    for (int t = 0; t < iters; t++) {
-        BpOp<T> op{F, Fn, pot, vp, cur, nxt, g->out_off, Fb};
-        PG_TRY(kt.begin());
-        PG_LAUNCH_K(k_flags_to_bits, grid_for(n), F, n, Fb);
-        PG_TRY(launch_pull(op, g->in_sched, &stats[(size_t)t * kStatSlots]));
-        PG_TRY(kt.end());
-        std::swap(cur, nxt);  // swap(vertD_curr, vertD_next), numa-BP.C:303-304
-        std::swap(F, Fn);     // switchFrontier
-    }
+    if (g->bp_death_state == 1)
+        PG_TRY((bp_iterations<T, true>(g, iters, pot, vp, cur, nxt, F, Fn, stats, hpa, hot, L, kt)));
+    else
+        PG_TRY((bp_iterations<T, false>(g, iters, pot, vp, cur, nxt, F, Fn, stats, hpa, hot, L, kt)));
     double ms = 0;
     PG_TRY(tm.stop(&ms));
     if (st) {
@@ -1956,6 +2382,7 @@ extern "C" int pg_bp(pg_graph *g, int iters, const float *vpot, const float *epo
     PG_REQUIRE(iters >= 0 && vpot && prod0, PG_ERR_ARG, "bad argument");
     PG_REQUIRE(epot || g->bp_pot, PG_ERR_STATE,
                "no edge potentials: pass epot or call pg_graph_bp_potentials");
+    PG_TRY(bp_death_times(g));  // once per graph
     const int64_t n = g->n, m = g->m;
     float4 *pot = nullptr;
     if (epot) {

@@ -799,6 +799,7 @@ int build_schedule(pg_graph *g, bool in_direction) {
     drop_captures(g);
     PG_CHECK_HIP(hipStreamSynchronize(g_stream));
     free_schedule(s);
+    if (in_direction) g->bp_light_built = false;
     const int32_t *off = in_direction ? g->in_off : g->out_off;
     const int32_t *adj = in_direction ? g->in_adj : g->out_adj;
     const int32_t *order = in_direction ? nullptr : g->out_order;

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "pg_internal.h"
 
@@ -160,6 +161,20 @@ __device__ __forceinline__ T load_partial(const T *p) {
 // evicting the gathered per-vertex values from L1/L2.
 __device__ __forceinline__ int ld_stream(const int32_t *p) { return __builtin_nontemporal_load(p); }
 
+// An Op may replace the neighbour id with a per-edge token of its own: with
+// `static constexpr bool kEdgeToken = true` and `int edge_token(int e)`, the
+// engine passes edge_token(e) to edge() in place of adj[e] (and never reads adj).
+template <class Op, class = void>
+struct has_edge_token : std::false_type {};
+template <class Op>
+struct has_edge_token<Op, std::enable_if_t<Op::kEdgeToken>> : std::true_type {};
+
+template <class Op>
+__device__ __forceinline__ int edge_src(const Op &op, const int32_t *adj, int e) {
+    if constexpr (has_edge_token<Op>::value) return op.edge_token(e);
+    else return ld_stream(adj + e);
+}
+
 // acc combined with op.edge over e, e+S, e+2S, ... < end; eight loads in flight
 template <int S, class Op>
 __device__ __forceinline__ typename Op::Acc edge_range(const Op &op, const int32_t *__restrict__ adj,
@@ -169,7 +184,7 @@ __device__ __forceinline__ typename Op::Acc edge_range(const Op &op, const int32
     for (; e + 7 * S < end; e += 8 * S) {
         int s[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) s[k] = ld_stream(adj + e + k * S);
+        for (int k = 0; k < 8; k++) s[k] = edge_src(op, adj, e + k * S);
         Acc a[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) a[k] = op.edge(s[k], e + k * S, c);
@@ -180,14 +195,14 @@ __device__ __forceinline__ typename Op::Acc edge_range(const Op &op, const int32
         acc = op.combine(acc, op.combine(a[0], a[1]));
     }
     if (e + 3 * S < end) {
-        const int s0 = ld_stream(adj + e), s1 = ld_stream(adj + e + S),
-                  s2 = ld_stream(adj + e + 2 * S), s3 = ld_stream(adj + e + 3 * S);
+        const int s0 = edge_src(op, adj, e), s1 = edge_src(op, adj, e + S),
+                  s2 = edge_src(op, adj, e + 2 * S), s3 = edge_src(op, adj, e + 3 * S);
         const Acc a0 = op.edge(s0, e, c), a1 = op.edge(s1, e + S, c), a2 = op.edge(s2, e + 2 * S, c),
                   a3 = op.edge(s3, e + 3 * S, c);
         acc = op.combine(acc, op.combine(op.combine(a0, a1), op.combine(a2, a3)));
         e += 4 * S;
     }
-    for (; e < end; e += S) acc = op.combine(acc, op.edge(ld_stream(adj + e), e, c));
+    for (; e < end; e += S) acc = op.combine(acc, op.edge(edge_src(op, adj, e), e, c));
     return acc;
 }
 
@@ -220,7 +235,7 @@ __device__ __forceinline__ typename Op::Stat pull_group(const Op &op, const Pull
 #pragma unroll
             for (int k = 0; k < U; k++) {
                 const int e = base + k * G + lane;
-                s[k] = e < end ? ld_stream(pa.adj + e) : -1;
+                s[k] = e < end ? edge_src(op, pa.adj, e) : -1;
             }
 #pragma unroll
             for (int k = 0; k < U; k++)
@@ -270,7 +285,7 @@ __device__ __forceinline__ typename Op::Stat pull_block(const Op &op, const Pull
         __syncthreads();
         for (int base = beg; base < end; base += kBlock) {
             const int e = base + (int)threadIdx.x;
-            if (e < end) acc = op.combine(acc, op.edge(pa.adj[e], e, c));
+            if (e < end) acc = op.combine(acc, op.edge(edge_src(op, pa.adj, e), e, c));
             if (group_any<64>(op.found(acc, c)) && lane_id() == 0) sflag = 1;
             __syncthreads();
             const int f = sflag;
@@ -325,11 +340,10 @@ __device__ __forceinline__ typename Op::Stat pull_multi(const Op &op, const Pull
     return st;
 }
 
+// the work of block b of a k_pull launch (uniform over the block)
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_pull(Op op, PullArgs pa,
-                                                 typename Op::Stat *stat_out) {
+__device__ __forceinline__ typename Op::Stat pull_dispatch(const Op &op, const PullArgs &pa, int b) {
     using Stat = typename Op::Stat;
-    const int b = blockIdx.x;
     int si = 0;
     while (si + 1 < pa.nseg && b >= pa.seg[si + 1].block_begin) ++si;
     const PullSeg seg = pa.seg[si];
@@ -346,6 +360,13 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, PullArgs pa,
         case 2: st = pull_group<2>(op, pa, seg, rb); break;
         default: st = pull_group<1>(op, pa, seg, rb); break;
     }
+    return st;
+}
+
+// combines every thread's statistic over the block and commits it once
+template <class Op>
+__device__ __forceinline__ void commit_block_stat(typename Op::Stat st, typename Op::Stat *stat_out) {
+    using Stat = typename Op::Stat;
     if constexpr (Op::kStats) {
         __shared__ Stat ssm[kBlock / 64];
 #pragma unroll
@@ -359,6 +380,12 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, PullArgs pa,
             if (tot != Stat(0)) Op::stat_commit(stat_out, tot);
         }
     }
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_pull(Op op, PullArgs pa,
+                                                 typename Op::Stat *stat_out) {
+    commit_block_stat<Op>(pull_dispatch(op, pa, (int)blockIdx.x), stat_out);
 }
 
 // body_only: skip the vertices without edges (for an Op whose finish() and skip()

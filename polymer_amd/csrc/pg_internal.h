@@ -262,6 +262,15 @@ struct pg_graph {
     // iterations of the last pg_spmv run with PG_SPMV_SAVE (its iterates and exponent
     // shifts are kept for pg_spmv_adjoint), 0 = none kept
     int spmv_saved_iters = 0;
+    // BP: the iteration at which each vertex leaves the frontier, as a byte per CSC
+    // edge (workspace bp_de): 0 not built, 1 built, -1 not representable (the
+    // frontier shrinks for more than 254 iterations)
+    int bp_death_state = 0;
+    // BP: the light range [lb, be) of the CSC and its edge chunks (workspaces
+    // bp_chunks, bp_heads), valid while the in-edge schedule is unchanged
+    bool bp_light_built = false;
+    int64_t bp_light_lb = 0, bp_light_be = 0;
+    int bp_light_nch = 0;
     double wpr_damping = 0.85;
     // allocations owned by the graph
     std::vector<void *> owned;

@@ -378,6 +378,64 @@ def test_bp(pa, orc):
     assert st.edges == e
 
 
+def bp_mixed_graph(orc, seed=3):
+    """Destinations of every pull class at once: a hub with 70000 in-neighbours
+    (several blocks), one with 20000 (one block), a random core of light vertices
+    with in-degrees from 1 to ~2000, and 40 chains of 12 vertices whose heads have
+    no in-edge, each chain vertex also pointing into the core, so sources leave the
+    frontier at iterations 1..12 (and the rest never)."""
+    rng = np.random.default_rng(seed)
+    n_core, nchain, clen = 90000, 40, 12
+    n = n_core + nchain * clen
+    src, dst = [], []
+    src.append(rng.choice(n_core, 70000, replace=False)); dst.append(np.zeros(70000, np.int64))
+    src.append(rng.choice(n_core, 20000, replace=False)); dst.append(np.ones(20000, np.int64))
+    deg = np.minimum(rng.zipf(1.7, n_core - 2), 2000)
+    d = np.repeat(np.arange(2, n_core), deg)
+    src.append(rng.integers(0, n_core, d.size)); dst.append(d)
+    for c in range(nchain):
+        ids = n_core + c * clen + np.arange(clen)
+        src.append(ids[:-1]); dst.append(ids[1:])
+        src.append(ids); dst.append(rng.integers(0, n_core, clen))
+    src, dst = np.concatenate(src), np.concatenate(dst)
+    keep = src != dst
+    return orc.from_edges(n, src[keep], dst[keep])
+
+
+def test_bp_pull_classes_and_frontier_exits(pa, orc):
+    """BP over destinations of every pull class (multi-block, block, the
+    edge-balanced light chunks) with sources that leave the frontier at iterations
+    1..12, under the default schedule and under one whose light class reaches
+    in-degree 2048 (destinations longer than a chunk, chunks with no vertex
+    starting in them); the frontier edge totals follow F_{t+1} = N_out(F_t)."""
+    say("case: bp mixed pull classes and frontier exits")
+    g = bp_mixed_graph(orc)
+    vpot = orc.vertex_f32(2 * g.n, 11, 0.1, 1.0)
+    epot = orc.bp_potentials(g, 12)
+    prod0 = np.ones(2 * g.n, np.float32)
+    d = pa.Graph.from_csr(g.n, g.off, g.adj)
+    outdeg = np.diff(g.off)
+    src_of = np.repeat(np.arange(g.n), outdeg)
+    for sched in ((0, 0, 0, 0), (64, 16384, 65536, 16384)):
+        if sched[0]:
+            d.set_schedule(*sched)
+        for iters in (1, 5, 13, 14):
+            ref = orc.bp(g, vpot, epot, prod0.astype(np.float64), iters)
+            got, st = d.bp(vpot, prod0, iters, epot=epot)
+            ok, err = bp_compare(got, ref)
+            say("  schedule %s, %d iterations: %d in range, rel L1 %.3e" % (sched, iters, ok.sum(), err))
+            assert ok.sum() > 0.3 * ok.size
+            assert err < TOL
+            F, e = np.ones(g.n, bool), 0
+            for _ in range(iters):
+                e += int(outdeg[F].sum())
+                nxt = np.zeros(g.n, bool)
+                nxt[g.adj[F[src_of]]] = True
+                F = nxt
+            assert st.edges == e
+    d.close()
+
+
 # ---- BFS --------------------------------------------------------------------------------
 def check_parents(toff, tadj, level, parent, src):
     reached = level >= 0
