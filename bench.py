@@ -274,12 +274,10 @@ def delta_extra(pa, g, iters=20):
             "pagerank_delta_hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
-def profile_run(args):
-    """Short run for rocprofv3: a plain streaming kernel of known size (torch's
-    elementwise negation of a 1 GiB fp32 tensor into another: exactly 1 GiB
-    read and 1 GiB written in one dispatch, the counter calibration), then one
-    warm-up and one measured PageRank of --iters iterations."""
-    import polymer_amd as pa
+def calibrate():
+    """The counter calibration kernel of every profile run: torch's elementwise
+    negation of a 1 GiB fp32 tensor into another (exactly 1 GiB read and 1 GiB
+    written in one dispatch)."""
     import torch
 
     x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
@@ -290,6 +288,16 @@ def profile_run(args):
     torch.cuda.synchronize()
     log("calibration kernel (neg): 1073741824 bytes read, 1073741824 bytes written")
     del x, y
+
+
+def profile_run(args):
+    """Short run for rocprofv3: a plain streaming kernel of known size (torch's
+    elementwise negation of a 1 GiB fp32 tensor into another: exactly 1 GiB
+    read and 1 GiB written in one dispatch, the counter calibration), then one
+    warm-up and one measured PageRank of --iters iterations."""
+    import polymer_amd as pa
+
+    calibrate()
     pa.init(0)
     g = pa.Graph.rmat(args.scale, args.edges, seed=1)
     sched = [int(x) for x in args.schedule.split(",")]
@@ -310,16 +318,8 @@ def profile_bp(args):
     once-per-graph preparation: the frontier's exit iteration per edge and the light
     chunks).  The calibration kernel of profile_run runs first."""
     import polymer_amd as pa
-    import torch
 
-    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    x.fill_(1.0)
-    torch.cuda.synchronize()
-    torch.neg(x, out=y)
-    torch.cuda.synchronize()
-    log("calibration kernel (neg): 1073741824 bytes read, 1073741824 bytes written")
-    del x, y
+    calibrate()
     pa.init(0)
     g = pa.Graph.rmat(args.scale, args.edges, seed=1)
     g.bp_potentials(9)
@@ -339,22 +339,73 @@ def profile_bfs(args):
     whose isolated draws are searches too), then the eight sources once more: the
     last eight searches of the run are the measured ones."""
     import polymer_amd as pa
-    import torch
 
-    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    x.fill_(1.0)
-    torch.cuda.synchronize()
-    torch.neg(x, out=y)
-    torch.cuda.synchronize()
-    log("calibration kernel (neg): 1073741824 bytes read, 1073741824 bytes written")
-    del x, y
+    calibrate()
     pa.init(0)
     gs = pa.Graph.rmat(args.scale, args.edges, seed=1, symmetrize=True)
     for s, _ in bfs24_runs(gs):
         _, _, st = gs.bfs(s, out=False)
         log("bfs source %d: %.3f ms, %d rounds (%d bottom-up), %d edges, %d reached" %
             (s, st.ms, st.rounds, st.dense_rounds, st.edges, st.active))
+
+
+def profile_extra(args):
+    """Short runs for rocprofv3 of the workloads in the JSON line's extras: the
+    calibration kernel, then each solve once to warm up and once measured.
+      spmv    SpMV on the scale-24 rMat, 20 iterations, fp32 values (value_extra)
+      delta   PageRankDelta on the scale-24 rMat, 20 rounds (delta_extra)
+      bfs22   BFS on the symmetrised scale-22 rMat from the nine traversal sources
+              (bfs_extra)
+      ppr     batched PPR, 64 sources, scale 22, 20 iterations, then two of the
+              plain runs on the tiles (batched_extra)
+      async   the asynchronous traversal from the first two scale-24 BFS sources
+              (async_extra)
+    (The drop-in drivers are profiled directly: rocprofv3 ... --
+    polymer_amd/apps/bin/pagerank -rmat 24 268435456 1 -rounds 20.)"""
+    import polymer_amd as pa
+
+    calibrate()
+    name = args.profile_extra
+    pa.init(0)
+    if name == "spmv":
+        gw = pa.Graph.rmat(24, 1 << 28, seed=1, weights="f32", wseed=3)
+        x0 = np.random.default_rng(3).random(gw.n).astype(np.float32)
+        for r in range(2):
+            _, st = gw.spmv(x0, 20, out=False)
+            log("spmv run %d: %.3f ms" % (r, st.ms))
+    elif name == "delta":
+        g = pa.Graph.rmat(24, 1 << 28, seed=1)
+        for r in range(2):
+            _, st = g.pagerank_delta(20, 0.85, 0.01, out=False)
+            log("pagerank_delta run %d: %.3f ms, %d frontier edges, %d pull rounds" %
+                (r, st.ms, st.edges, st.dense_rounds))
+    elif name == "bfs22":
+        gs = pa.Graph.rmat(22, 1 << 26, seed=1, symmetrize=True)
+        off, _, _ = gs.download()
+        srcs = traversal_sources(np.diff(off))
+        gs.bfs(0, out=False)
+        for s in srcs:
+            _, _, st = gs.bfs(s, out=False)
+            log("bfs source %d: %.3f ms, %d rounds (%d bottom-up), %d edges, %d reached" %
+                (s, st.ms, st.rounds, st.dense_rounds, st.edges, st.active))
+    elif name == "ppr":
+        g = pa.Graph.rmat(22, 1 << 26, seed=1)
+        src = np.random.default_rng(11).choice(g.n, 64, replace=False)
+        src[0] = 0
+        for r in range(2):
+            _, st = g.ppr_batched(src, 20, out=False)
+            log("ppr batched run %d: %.3f ms" % (r, st.ms))
+        for s in src[:2]:
+            _, st = g.ppr(int(s), 20, out=False)
+            log("ppr plain source %d: %.3f ms" % (s, st.ms))
+    elif name == "async":
+        gs = pa.Graph.rmat(24, 1 << 28, seed=1, symmetrize=True)
+        srcs = [s for s, _ in bfs24_runs(gs, 2)]
+        for s in srcs:
+            _, vis, st = gs.bfs_async(s, out=False)
+            log("async source %d: %.3f ms, %d reached" % (s, st.ms, vis))
+    else:
+        raise SystemExit("unknown --profile-extra %r" % name)
 
 
 def measured_traffic(schedule):
@@ -466,6 +517,8 @@ def main():
                     help="the same for the scale-24 BFS sources of the extras")
     ap.add_argument("--profile-bp", action="store_true",
                     help="the same for BP on the scale-24 rMat of the extras")
+    ap.add_argument("--profile-extra", default=None,
+                    help="the same for one extra workload: spmv, delta, bfs22, ppr, async")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -490,6 +543,9 @@ def main():
         return
     if args.profile_bp:
         profile_bp(args)
+        return
+    if args.profile_extra:
+        profile_extra(args)
         return
 
     import polymer_amd as pa

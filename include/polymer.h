@@ -1303,6 +1303,28 @@ __device__ __forceinline__ unsigned long long group_ballot(bool p) {
     else return (bal >> (lane() & ~(G - 1))) & ((1ull << G) - 1ull);
 }
 
+// The current frontier as a bitmap (n/8 bytes, held in L2) for the dense pulls'
+// per-edge source test: the frontier's flag bytes (n bytes) made that test a third
+// random line fetch per in-edge beside the functor's own gathers.
+struct FrontierBits {
+    const uint32_t *w;
+    __device__ bool operator[](intT s) const { return (w[s >> 5] >> (s & 31)) & 1u; }
+};
+
+// w[k] = the flags b[32k .. 32k+31]: a wave's ballot over 64 flags is two words
+__global__ void k_flag_bits(const bool *b, int64_t n, uint32_t *w) {
+    const int64_t nw = (n + 31) / 32;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t v0 = wave * 64; v0 < n; v0 += nwaves * 64) {
+        const int64_t v = v0 + lane();
+        const unsigned long long bal = __ballot(v < n && b[v]);
+        const int64_t k = v0 / 32;
+        if (lane() == 0) w[k] = (uint32_t)bal;
+        if (lane() == 1 && k + 1 < nw) w[k + 1] = (uint32_t)(bal >> 32);
+    }
+}
+
 // G lanes per destination (G = 1, 16 or 64) over ids[0, count).  Within a group's
 // chunk of G in-neighbours the first active one (in list order) is applied
 // alone; if cond(d) still holds the chunk's other active ones are applied
@@ -1312,7 +1334,7 @@ __device__ __forceinline__ unsigned long long group_ballot(bool p) {
 // first within the chunk; one that keeps cond true (a minimum, a sum) applies
 // every active in-neighbour.
 template <bool W, int G, class F>
-__global__ __launch_bounds__(kT) void k_pull_group(F f, pg_view v, const bool *cur, bool *next,
+__global__ __launch_bounds__(kT) void k_pull_group(F f, pg_view v, FrontierBits cur, bool *next,
                                                    const intT *ids, int64_t count) {
     const int64_t slot = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
     if (slot >= count) return;  // uniform within the group
@@ -1353,7 +1375,7 @@ __global__ __launch_bounds__(kT) void k_pull_group(F f, pg_view v, const bool *c
 
 // a 256-thread workgroup per destination over ids[0, count)
 template <bool W, class F>
-__global__ __launch_bounds__(kT) void k_pull_block(F f, pg_view v, const bool *cur, bool *next,
+__global__ __launch_bounds__(kT) void k_pull_block(F f, pg_view v, FrontierBits cur, bool *next,
                                                    const intT *ids) {
     __shared__ int s_stop, s_hit;
     const intT d = ids[blockIdx.x];
@@ -1381,7 +1403,7 @@ __global__ __launch_bounds__(kT) void k_pull_block(F f, pg_view v, const bool *c
 }
 
 template <bool W, class F>
-inline void pull_dense(F &f, const pg_view &v, const bool *cur, bool *next) {
+inline void pull_dense(F &f, const pg_view &v, FrontierBits cur, bool *next) {
     const Bins &b = bins_of(v);
     const int64_t nblk = b.count[3], nwave = b.count[2], n16 = b.count[1], n1 = b.count[0];
     // heavy first: they set the tail of the launch sequence
@@ -1418,7 +1440,7 @@ __device__ __forceinline__ bool reduce_edge(F &f, void *data, intT s, const intE
 }
 
 template <bool W, int G, class F>
-__global__ __launch_bounds__(kT) void k_reduce_group(F f, pg_view v, const bool *cur, bool *next,
+__global__ __launch_bounds__(kT) void k_reduce_group(F f, pg_view v, FrontierBits cur, bool *next,
                                                      const intT *ids, int64_t count) {
     const int64_t slot = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
     if (slot >= count) return;  // uniform within the group
@@ -1469,7 +1491,7 @@ __global__ __launch_bounds__(kT) void k_reduce_group(F f, pg_view v, const bool 
 
 // a 256-thread workgroup per destination; the scratches fold wave after wave
 template <bool W, class F>
-__global__ __launch_bounds__(kT) void k_reduce_block(F f, pg_view v, const bool *cur, bool *next,
+__global__ __launch_bounds__(kT) void k_reduce_block(F f, pg_view v, FrontierBits cur, bool *next,
                                                      const intT *ids) {
     __shared__ int s_stop, s_hit;
     const intT d = ids[blockIdx.x];
@@ -1507,7 +1529,7 @@ __global__ __launch_bounds__(kT) void k_reduce_block(F f, pg_view v, const bool 
 }
 
 template <bool W, class F>
-inline void reduce_dense(F &f, const pg_view &v, const bool *cur, bool *next) {
+inline void reduce_dense(F &f, const pg_view &v, FrontierBits cur, bool *next) {
     const Bins &b = bins_of(v);
     const int64_t nblk = b.count[3], nwave = b.count[2], n16 = b.count[1], n1 = b.count[0];
     if (nblk > 0)
@@ -1843,6 +1865,16 @@ namespace polymer_detail {
 // the frontier's flags as the kernels read them (the calls' node is 0: one GPU)
 inline const bool *cur_flags(vertices *frontier) { return frontier->getArr(0); }
 
+// the frontier's flags as a bitmap (a workspace rebuilt on each dense call)
+inline FrontierBits cur_bits(vertices *frontier, int64_t n) {
+    const int64_t nw = (n + 31) / 32;
+    uint32_t *w = workspace_t<uint32_t>("frontier_bits", (size_t)nw + 1);
+    hipLaunchKernelGGL(k_flag_bits, dim3(blocks_for(nw * 32, kT)), dim3(kT), 0, lib_stream(),
+                       cur_flags(frontier), n, w);
+    launched("frontier bitmap");
+    return FrontierBits{w};
+}
+
 // the active sources of a dense frontier as an ordered list (kept in a workspace)
 inline intT list_active(vertices *frontier, intT **list) {
     LocalFrontier *f = frontier->getFrontier(0);
@@ -1898,7 +1930,7 @@ bool *edgeMapDense(G GA, vertices *frontier, F f, LocalFrontier *next, bool para
     (void)parallel; (void)subworker;
     constexpr bool W = polymer_detail::is_weighted<G>::value;
     frontier->nextFrontiers[0] = next;
-    polymer_detail::pull_dense<W>(f, GA.view, polymer_detail::cur_flags(frontier), next->b);
+    polymer_detail::pull_dense<W>(f, GA.view, polymer_detail::cur_bits(frontier, GA.n), next->b);
     polymer_detail::sync();
     return nullptr;
 }
@@ -1917,7 +1949,7 @@ bool *edgeMapDenseReduce(G GA, vertices *frontier, F f, LocalFrontier *next, boo
     (void)parallel; (void)subworker;
     constexpr bool W = is_weighted<G>::value;
     frontier->nextFrontiers[0] = next;
-    reduce_dense<W>(f, GA.view, cur_flags(frontier), next->b);
+    reduce_dense<W>(f, GA.view, cur_bits(frontier, GA.n), next->b);
     sync();
     return nullptr;
 }

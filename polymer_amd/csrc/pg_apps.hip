@@ -2421,7 +2421,8 @@ extern "C" int pg_bp(pg_graph *g, int iters, const float *vpot, const float *epo
 // of out-degree D is queued as ceil(D / kAsyncItemEdges) items (vertex, chunk), so
 // a hub's out-edges are spread over many waves instead of one wave's serial walk.
 // tail counts the queued items, processed those whose edges have all been claimed
-// or found visited, reached the claimed vertices.  A wave takes kAsyncChunk
+// or found visited (the vertices reached are counted from the visited bitmap
+// afterwards).  A wave takes kAsyncChunk
 // consecutive slots and, until each of its lanes' slots is settled, expands
 // whichever of them are filled (so a wave never waits on work it holds); a slot is
 // settled empty once processed == tail (nothing is being expanded and nothing is
@@ -2429,10 +2430,12 @@ extern "C" int pg_bp(pg_graph *g, int iters, const float *vpot, const float *epo
 // past the queue's capacity (n + m / kAsyncItemEdges + 1 bounds every item
 // count).  A wave's wait is bounded: kAsyncPolls polls in a row without any item
 // processed anywhere (the count it reads unchanged), after which it sets the
-// error word and leaves, so a launch always ends.
+// error word and leaves, so a launch always ends.  (Waves that instead took only
+// items already queued, by a compare-and-swap on head, all polled the same three
+// control words: the traffic on those lines starved the working waves.)
 constexpr int kAsyncChunk = 64;
 constexpr unsigned kAsyncPolls = 1u << 20;
-constexpr int kAsyncItemEdges = 1024;
+constexpr int kAsyncItemEdges = 256;
 constexpr int kAsyncLaneDeg = 32;  // an item's edges its lane expands alone; more: the wave
 constexpr unsigned long long kAsyncEmpty = ~0ull;
 // control words, each on its own 64-byte line
@@ -2444,28 +2447,39 @@ __device__ __forceinline__ unsigned long long async_item(int v, int k) {
 }
 
 // claim d for v (want: this lane has an edge); winners queue their items
-__device__ __forceinline__ void async_claim(int v, int d, int32_t *parent,
+// The claim of d by v (BFS_F's CAS of parent[d] from -1, numa-BFS-async-pipe.C:
+// 60-79): the visited bitmap (n/8 bytes, held in L2) is screened with an L2 load
+// and claimed with an OR whose old word picks the one winner, which then stores
+// parent[d]; parent is written only by winners and never read during the traversal.
+// The winners' items are queued at once (holding them back to queue them in
+// batches starved the other waves of work: 3x slower); the vertices reached are
+// counted from the bitmap afterwards.
+__device__ __forceinline__ void async_claim(int v, int d, int32_t *parent, uint32_t *visited,
                                             const int32_t *__restrict__ out_off,
                                             unsigned long long *q, unsigned *ctl, bool want) {
     bool won = false;
-    if (want && __hip_atomic_load(&parent[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == -1)
-        won = atomicCAS(&parent[d], -1, v) == -1;
-    const unsigned long long wmask = __ballot(won);
-    if (!wmask) return;
+    if (want) {
+        const uint32_t bit = 1u << (d & 31);
+        uint32_t *word = &visited[d >> 5];
+        if (!(__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
+            won = (atomicOr(word, bit) & bit) == 0u;
+    }
+    if (!__ballot(won)) return;
     const int lane = lane_id();
     int cnt = 0;
-    if (won) cnt = (out_off[d + 1] - out_off[d] + kAsyncItemEdges - 1) / kAsyncItemEdges;
+    if (won) {
+        parent[d] = v;
+        cnt = (out_off[d + 1] - out_off[d] + kAsyncItemEdges - 1) / kAsyncItemEdges;
+    }
     int x = cnt;  // inclusive scan of the item counts over the wave
     for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
     const int total = __shfl(x, 63, 64);
+    if (!total) return;
     unsigned base = 0;
-    if (lane == 0) {
-        atomicAdd(ctl + kAsyncReached, (unsigned)__popcll(wmask));
-        if (total) base = atomicAdd(ctl + kAsyncTail, (unsigned)total);
-    }
+    if (lane == 0) base = atomicAdd(ctl + kAsyncTail, (unsigned)total);
     base = __shfl(base, 0, 64);
     for (int j = 0; j < cnt; j++)
         __hip_atomic_store(&q[base + (unsigned)(x - cnt + j)], async_item(d, j), __ATOMIC_RELAXED,
@@ -2474,8 +2488,9 @@ __device__ __forceinline__ void async_claim(int v, int d, int32_t *parent,
 
 __global__ __launch_bounds__(kBlock) void k_bfs_async(const int32_t *__restrict__ out_off,
                                                       const int32_t *__restrict__ out_adj,
-                                                      int32_t *parent, unsigned long long *q,
-                                                      int64_t cap, unsigned *ctl) {
+                                                      int32_t *parent, uint32_t *visited,
+                                                      unsigned long long *q, int64_t cap,
+                                                      unsigned *ctl) {
     unsigned *head = ctl + kAsyncHead, *tail = ctl + kAsyncTail;
     unsigned *processed = ctl + kAsyncProcessed, *error = ctl + kAsyncError;
     const int lane = lane_id();
@@ -2508,7 +2523,8 @@ __global__ __launch_bounds__(kBlock) void k_bfs_async(const int32_t *__restrict_
                 const bool light = ready && end - beg <= kAsyncLaneDeg;
                 for (int k = 0; __ballot(light && beg + k < end); k++) {
                     const bool act = light && beg + k < end;
-                    async_claim(v, act ? out_adj[beg + k] : 0, parent, out_off, q, ctl, act);
+                    async_claim(v, act ? out_adj[beg + k] : 0, parent, visited, out_off, q, ctl,
+                                act);
                 }
                 unsigned long long heavy = __ballot(ready && !light);
                 while (heavy) {
@@ -2518,7 +2534,8 @@ __global__ __launch_bounds__(kBlock) void k_bfs_async(const int32_t *__restrict_
                     const int hb = __shfl(beg, l, 64), he = __shfl(end, l, 64);
                     for (int e0 = hb; e0 < he; e0 += 64) {
                         const bool act = e0 + lane < he;
-                        async_claim(hv, act ? out_adj[e0 + lane] : 0, parent, out_off, q, ctl, act);
+                        async_claim(hv, act ? out_adj[e0 + lane] : 0, parent, visited, out_off, q,
+                                    ctl, act);
                     }
                 }
                 // the items this wave queued are visible before it counts its own
@@ -2557,14 +2574,24 @@ __global__ __launch_bounds__(kBlock) void k_bfs_async(const int32_t *__restrict_
 
 // the source claimed by itself, its items queued
 __global__ void k_async_seed(int32_t src, const int32_t *out_off, int32_t *parent,
-                             unsigned long long *q, unsigned *ctl) {
+                             uint32_t *visited, unsigned long long *q, unsigned *ctl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         parent[src] = src;
+        visited[src >> 5] |= 1u << (src & 31);
         const int cnt = (out_off[src + 1] - out_off[src] + kAsyncItemEdges - 1) / kAsyncItemEdges;
         for (int j = 0; j < cnt; j++) q[j] = async_item(src, j);
         ctl[kAsyncTail] = (unsigned)cnt;
-        ctl[kAsyncReached] = 1u;
     }
+}
+
+// the vertices reached: the set bits of the visited bitmap
+__global__ void k_async_reached(const uint32_t *visited, int64_t nw, unsigned *out) {
+    unsigned c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw;
+         i += (int64_t)gridDim.x * blockDim.x)
+        c += __popc(visited[i]);
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane_id() == 0 && c) atomicAdd(out, c);
 }
 
 extern "C" int pg_bfs_async(pg_graph *g, int64_t src, int32_t *parent, int64_t *visited,
@@ -2579,7 +2606,10 @@ extern "C" int pg_bfs_async(pg_graph *g, int64_t src, int32_t *parent, int64_t *
     int32_t *par = nullptr;
     unsigned long long *q = nullptr;
     unsigned *ctl = nullptr;
+    uint32_t *vis = nullptr;
+    const int64_t nw = (g->n + 31) / 32;
     PG_TRY(workspace_t(g, "async_parent", (size_t)g->n, &par));
+    PG_TRY(workspace_t(g, "async_visited", (size_t)nw, &vis));
     PG_TRY(workspace_t(g, "async_queue", (size_t)cap, &q));
     PG_TRY(workspace_t(g, "async_ctl", (size_t)kAsyncCtlWords, &ctl));
     KernelTimer kt;
@@ -2587,12 +2617,14 @@ extern "C" int pg_bfs_async(pg_graph *g, int64_t src, int32_t *parent, int64_t *
     Timer tm;
     PG_TRY(tm.start());
     PG_TRY(fill(par, -1, g->n));
+    PG_TRY(fill(vis, 0u, nw));
     PG_TRY(fill(q, kAsyncEmpty, cap));
     PG_TRY(fill(ctl, 0u, kAsyncCtlWords));
-    PG_LAUNCH_K(k_async_seed, 1, s_int, g->out_off, par, q, ctl);
+    PG_LAUNCH_K(k_async_seed, 1, s_int, g->out_off, par, vis, q, ctl);
     PG_TRY(kt.begin());
-    PG_LAUNCH_K(k_bfs_async, 2048, g->out_off, g->out_adj, par, q, cap, ctl);
+    PG_LAUNCH_K(k_bfs_async, 2048, g->out_off, g->out_adj, par, vis, q, cap, ctl);
     PG_TRY(kt.end());
+    PG_LAUNCH_K(k_async_reached, grid_for(nw, 1024), vis, nw, ctl + kAsyncReached);
     double ms = 0;
     PG_TRY(tm.stop(&ms));
     unsigned h[kAsyncCtlWords];
