@@ -17,8 +17,13 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <fstream>
 #include <string>
+#include <thread>
 
 #include "pg_engine.h"
 #include "pg_internal.h"
@@ -1420,57 +1425,142 @@ static void delete_graph(pg_graph *g) {
     delete g;
 }
 
-int create_from_host(int64_t n, int64_t m, const int64_t *off, const int32_t *adj, const void *w,
-                     int w_dtype, uint32_t flags, pg_graph **out) {
+// ---- a graph from a CSR in device memory ------------------------------------------
+// Checks (offsets monotone from 0 to m, neighbours in range; the first offending
+// index is reported), then forms the (source, neighbour) keys of build_graph: an
+// edge's source is the number of vertices v >= 1 whose list starts at or before
+// it (an inclusive scan of start marks), so hubs cost no more than other
+// vertices.  PG_GRAPH_SKIP_SELF_LOOPS drops v -> v edges (IO-numa.h:225-268) with a
+// compaction scan, their weights with them.
+__global__ void k_csr_check(const int64_t *off, int64_t n, const int32_t *adj, int64_t m,
+                            unsigned long long *err) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        if (off[v] > off[v + 1]) atomicMin(&err[0], (unsigned long long)v);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride)
+        if (adj[e] < 0 || adj[e] >= n) atomicMin(&err[1], (unsigned long long)e);
+}
+
+__global__ void k_mark_starts(const int64_t *off, int64_t n, int64_t m, uint32_t *mark) {
+    for (int64_t v = 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        if (off[v] < m) atomicAdd(&mark[off[v]], 1u);
+}
+
+// keep[e] = 1 unless e is a self-loop that is dropped; keys when nothing is dropped
+__global__ void k_edge_keys(const uint32_t *src, const int32_t *adj, int64_t m, bool skip_loops,
+                            uint32_t *keep, uint64_t *keys) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = src[e];
+        const uint32_t d = (uint32_t)adj[e];
+        if (skip_loops) keep[e] = s != d ? 1u : 0u;
+        else keys[e] = ((uint64_t)s << 32) | d;
+    }
+}
+
+__global__ void k_edge_keys_compact(const uint32_t *src, const int32_t *adj, const uint32_t *keep,
+                                    const uint32_t *pos, const int32_t *w, int64_t m, uint64_t *keys,
+                                    int32_t *wout) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        if (!keep[e]) continue;
+        keys[pos[e]] = ((uint64_t)src[e] << 32) | (uint32_t)adj[e];
+        if (w) wout[pos[e]] = w[e];
+    }
+}
+
+int create_from_device(int64_t n, int64_t m, const int64_t *d_off, const int32_t *d_adj,
+                       const void *d_w, int w_dtype, uint32_t flags, pg_graph **out) {
+    *out = nullptr;
+    PG_REQUIRE(n > 0 && n < 0x7FFFFFFF, PG_ERR_ARG, "n out of range: %lld", (long long)n);
     PG_REQUIRE(m >= 0 && m < 0x7FFFFFFF, PG_ERR_ARG, "m out of range: %lld", (long long)m);
-    PG_REQUIRE(off && (adj || m == 0), PG_ERR_ARG, "null CSR arrays");
     PG_REQUIRE(w_dtype == PG_W_NONE || w_dtype == PG_W_I32 || w_dtype == PG_W_F32, PG_ERR_ARG,
                "bad weight dtype %d", w_dtype);
-    PG_REQUIRE(off[0] == 0 && off[n] == m, PG_ERR_ARG, "offsets must span [0, m]");
-    // host-side validation and keys (self-loop removal as IO-numa.h:225-268)
-    std::vector<uint64_t> keys;
-    std::vector<char> wbytes;
-    keys.reserve((size_t)m);
-    const size_t wsz = 4;
-    if (w_dtype != PG_W_NONE) wbytes.reserve((size_t)m * wsz);
-    for (int64_t v = 0; v < n; v++) {
-        PG_REQUIRE(off[v] <= off[v + 1], PG_ERR_ARG, "offsets not monotone at %lld", (long long)v);
-        for (int64_t e = off[v]; e < off[v + 1]; e++) {
-            const int32_t d = adj[e];
-            PG_REQUIRE(d >= 0 && d < n, PG_ERR_ARG, "edge %lld: neighbour %d out of range",
-                       (long long)e, d);
-            if ((flags & PG_GRAPH_SKIP_SELF_LOOPS) && d == v) continue;
-            keys.push_back(((uint64_t)v << 32) | (uint32_t)d);
-            if (w_dtype != PG_W_NONE) {
-                const char *src = reinterpret_cast<const char *>(w) + e * wsz;
-                wbytes.insert(wbytes.end(), src, src + wsz);
+    PG_REQUIRE(w_dtype == PG_W_NONE || d_w || m == 0, PG_ERR_ARG, "weights requested but w is NULL");
+    int64_t ends[2];
+    PG_CHECK_HIP(hipMemcpyAsync(&ends[0], d_off, 8, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipMemcpyAsync(&ends[1], d_off + n, 8, hipMemcpyDeviceToHost, g_stream));
+    DevBuf derr;
+    PG_TRY(derr.alloc(16));
+    PG_LAUNCH(k_fill<unsigned long long>, 1, derr.as<unsigned long long>(), ~0ull, 2);
+    PG_LAUNCH(k_csr_check, grid_for(std::max(n, m)), d_off, n, d_adj, m, derr.as<unsigned long long>());
+    unsigned long long err[2];
+    PG_CHECK_HIP(hipMemcpyAsync(err, derr.p, 16, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    PG_REQUIRE(ends[0] == 0 && ends[1] == m, PG_ERR_ARG, "offsets must span [0, m]");
+    PG_REQUIRE(err[0] == ~0ull, PG_ERR_ARG, "offsets not monotone at %lld", (long long)err[0]);
+    if (err[1] != ~0ull) {
+        int32_t d = 0;
+        PG_CHECK_HIP(hipMemcpy(&d, d_adj + err[1], 4, hipMemcpyDeviceToHost));
+        set_error("edge %lld: neighbour %d out of range", (long long)err[1], d);
+        return PG_ERR_ARG;
+    }
+    const bool skip = (flags & PG_GRAPH_SKIP_SELF_LOOPS) != 0;
+    DevBuf mark, src, keys, keep, pos, wc;
+    PG_TRY(mark.alloc(4 * (size_t)m + 16));
+    PG_TRY(src.alloc(4 * (size_t)m + 16));
+    PG_TRY(keys.alloc(8 * (size_t)m + 16));
+    int64_t mm = m;
+    const void *wk = d_w;
+    if (m > 0) {
+        PG_LAUNCH(k_fill<uint32_t>, grid_for(m), mark.as<uint32_t>(), 0u, m);
+        PG_LAUNCH(k_mark_starts, grid_for(n), d_off, n, m, mark.as<uint32_t>());
+        PG_TRY(inclusive_sum(mark.as<uint32_t>(), src.as<uint32_t>(), m));
+        if (!skip) {
+            PG_LAUNCH(k_edge_keys, grid_for(m), src.as<uint32_t>(), d_adj, m, false, nullptr,
+                      keys.as<uint64_t>());
+        } else {
+            PG_TRY(keep.alloc(4 * (size_t)m + 16));
+            PG_TRY(pos.alloc(4 * (size_t)m + 16));
+            PG_LAUNCH(k_edge_keys, grid_for(m), src.as<uint32_t>(), d_adj, m, true, keep.as<uint32_t>(),
+                      nullptr);
+            PG_TRY(exclusive_sum(keep.as<uint32_t>(), pos.as<uint32_t>(), m));
+            uint32_t last[2];
+            PG_CHECK_HIP(hipMemcpyAsync(&last[0], pos.as<uint32_t>() + m - 1, 4, hipMemcpyDeviceToHost,
+                                        g_stream));
+            PG_CHECK_HIP(hipMemcpyAsync(&last[1], keep.as<uint32_t>() + m - 1, 4,
+                                        hipMemcpyDeviceToHost, g_stream));
+            PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+            mm = (int64_t)last[0] + last[1];
+            if (d_w) {
+                PG_TRY(wc.alloc(4 * (size_t)mm + 16));
+                wk = wc.p;
             }
+            PG_LAUNCH(k_edge_keys_compact, grid_for(m), src.as<uint32_t>(), d_adj, keep.as<uint32_t>(),
+                      pos.as<uint32_t>(), static_cast<const int32_t *>(d_w), m, keys.as<uint64_t>(),
+                      wc.as<int32_t>());
         }
     }
-    PG_REQUIRE(w_dtype == PG_W_NONE || w, PG_ERR_ARG, "weights requested but w is NULL");
     PG_TRY(new_graph(n, out));
     pg_graph *g = *out;
     g->sym = (flags & PG_GRAPH_SYMMETRIC) != 0;
-    const int64_t mm = (int64_t)keys.size();
-    DevBuf dk, dw;
-    int r = dk.alloc(sizeof(uint64_t) * (mm + 1));
-    if (r == PG_OK && mm) {
-        hipError_t e = hipMemcpy(dk.p, keys.data(), sizeof(uint64_t) * mm, hipMemcpyHostToDevice);
-        if (e != hipSuccess) { set_error("upload failed: %s", hipGetErrorString(e)); r = PG_ERR_HIP; }
-    }
-    if (r == PG_OK && w_dtype != PG_W_NONE) {
-        r = dw.alloc(wbytes.size() + 16);
-        if (r == PG_OK && mm) {
-            hipError_t e = hipMemcpy(dw.p, wbytes.data(), wbytes.size(), hipMemcpyHostToDevice);
-            if (e != hipSuccess) { set_error("upload failed: %s", hipGetErrorString(e)); r = PG_ERR_HIP; }
-        }
-    }
-    if (r == PG_OK) r = build_graph(g, dk.as<uint64_t>(), mm, dw.p, w_dtype);
+    const int r = build_graph(g, keys.as<uint64_t>(), mm, w_dtype == PG_W_NONE ? nullptr : wk, w_dtype);
     if (r != PG_OK) {
         delete_graph(g);
         *out = nullptr;
     }
     return r;
+}
+
+// a host CSR: uploaded, then built as a device CSR
+int create_from_host(int64_t n, int64_t m, const int64_t *off, const int32_t *adj, const void *w,
+                     int w_dtype, uint32_t flags, pg_graph **out) {
+    *out = nullptr;
+    PG_REQUIRE(m >= 0 && m < 0x7FFFFFFF, PG_ERR_ARG, "m out of range: %lld", (long long)m);
+    PG_REQUIRE(off && (adj || m == 0), PG_ERR_ARG, "null CSR arrays");
+    PG_REQUIRE(w_dtype == PG_W_NONE || w || m == 0, PG_ERR_ARG, "weights requested but w is NULL");
+    DevBuf doff, dadj, dw;
+    PG_TRY(doff.alloc(8 * ((size_t)n + 1)));
+    PG_TRY(dadj.alloc(4 * (size_t)m + 16));
+    PG_CHECK_HIP(hipMemcpyAsync(doff.p, off, 8 * ((size_t)n + 1), hipMemcpyHostToDevice, g_stream));
+    if (m) PG_CHECK_HIP(hipMemcpyAsync(dadj.p, adj, 4 * (size_t)m, hipMemcpyHostToDevice, g_stream));
+    if (w_dtype != PG_W_NONE) {
+        PG_TRY(dw.alloc(4 * (size_t)m + 16));
+        if (m) PG_CHECK_HIP(hipMemcpyAsync(dw.p, w, 4 * (size_t)m, hipMemcpyHostToDevice, g_stream));
+    }
+    return create_from_device(n, m, doff.as<int64_t>(), dadj.as<int32_t>(),
+                              w_dtype == PG_W_NONE ? nullptr : dw.p, w_dtype, flags, out);
 }
 
 // device rMat: the first m distinct non-loop pairs of the counter sequence
@@ -1654,58 +1744,226 @@ extern "C" int pg_graph_rmat(int scale, int64_t m, double a, double b, double c,
     return create_rmat(scale, m, a, b, c, seed, flags, wseed, out);
 }
 
+// ---- PBBS text on the GPU (readGraphFromFile IO-numa.h:112-190, IO.h:186-272) ------
+// The file is read into pinned host memory by parallel preads and copied to the
+// device once.  There, each thread takes 16 bytes and marks the bytes that start a
+// word (not a separator, after a separator: IO-numa.h:90-109's words); per-block
+// counts and an exclusive scan number the words, and each word is parsed by the
+// thread that holds its first byte (an optional sign, then digits, as strtoll) into
+// its slot: words 3 .. 3+n-1 the offsets, then m neighbour ids, then m weights.
+inline bool txt_sep_host(char c) {
+    return c == ' ' || c == '\n' || c == '\r' || c == '\t' || c == 0;
+}
+
+constexpr int kTxtThreadBytes = 16;
+constexpr int kTxtBlockBytes = kBlock * kTxtThreadBytes;
+
+__device__ __forceinline__ bool txt_sep(unsigned c) {
+    return c == ' ' || c == '\n' || c == '\r' || c == '\t' || c == 0;
+}
+
+// bit k: byte p0 + k starts a word
+__device__ __forceinline__ unsigned txt_starts(const uint8_t *__restrict__ txt, int64_t size,
+                                               int64_t p0) {
+    if (p0 >= size) return 0u;
+    uint8_t b[kTxtThreadBytes];
+    if (p0 + kTxtThreadBytes <= size) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(txt + p0);
+        __builtin_memcpy(b, &v, 16);
+    } else {
+        for (int k = 0; k < kTxtThreadBytes; k++) b[k] = p0 + k < size ? txt[p0 + k] : ' ';
+    }
+    unsigned prev = p0 == 0 ? ' ' : txt[p0 - 1];
+    unsigned mask = 0;
+#pragma unroll
+    for (int k = 0; k < kTxtThreadBytes; k++) {
+        if (!txt_sep(b[k]) && txt_sep(prev)) mask |= 1u << k;
+        prev = b[k];
+    }
+    return mask;
+}
+
+__global__ __launch_bounds__(kBlock) void k_txt_count(const uint8_t *txt, int64_t size,
+                                                     uint32_t *bcount) {
+    const int64_t p0 = (int64_t)blockIdx.x * kTxtBlockBytes + (int64_t)threadIdx.x * kTxtThreadBytes;
+    unsigned c = __popc(txt_starts(txt, size, p0));
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+    __shared__ unsigned s[kBlock / 64];
+    if (lane_id() == 0) s[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tot = 0;
+        for (int i = 0; i < kBlock / 64; i++) tot += s[i];
+        bcount[blockIdx.x] = tot;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_txt_parse(const uint8_t *txt, int64_t size,
+                                                     const uint64_t *bbase, int64_t n, int64_t m,
+                                                     bool weighted, int64_t *off, int32_t *adj,
+                                                     int32_t *w) {
+    const int64_t p0 = (int64_t)blockIdx.x * kTxtBlockBytes + (int64_t)threadIdx.x * kTxtThreadBytes;
+    unsigned mask = txt_starts(txt, size, p0);
+    // the block's words before this thread's: a wave scan, then the waves' totals
+    const int lane = lane_id();
+    unsigned x = __popc(mask);
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    __shared__ unsigned s[kBlock / 64];
+    if (lane == 63) s[threadIdx.x >> 6] = x;
+    __syncthreads();
+    unsigned before = x - __popc(mask);
+    for (int i = 0; i < (int)(threadIdx.x >> 6); i++) before += s[i];
+    uint64_t j = bbase[blockIdx.x] + before;
+    const uint64_t nn = (uint64_t)n, mm = (uint64_t)m;
+    while (mask) {
+        const int k = __ffs(mask) - 1;
+        mask &= mask - 1;
+        int64_t p = p0 + k;
+        bool neg = false;
+        if (txt[p] == '-' || txt[p] == '+') {
+            neg = txt[p] == '-';
+            p++;
+        }
+        long long val = 0;
+        while (p < size) {
+            const unsigned c = txt[p];
+            if (c < '0' || c > '9') break;
+            val = val * 10 + (long long)(c - '0');
+            p++;
+        }
+        if (neg) val = -val;
+        if (j >= 3) {
+            const uint64_t i = j - 3;
+            if (i < nn) off[i] = val;
+            else if (i < nn + mm) adj[i - nn] = (int32_t)val;
+            else if (weighted && i < nn + 2 * mm) w[i - nn - mm] = (int32_t)val;
+        }
+        j++;
+    }
+}
+
+// the whole file into pinned host memory, read in parallel
+static int read_file_pinned(const char *path, uint8_t **buf, int64_t *size) {
+    *buf = nullptr;
+    const int fd = open(path, O_RDONLY);
+    PG_REQUIRE(fd >= 0, PG_ERR_ARG, "Unable to open file: %s", path);  // IO-numa.h:75-78
+    struct stat sb;
+    if (fstat(fd, &sb) != 0) {
+        close(fd);
+        set_error("Unable to open file: %s", path);
+        return PG_ERR_ARG;
+    }
+    const int64_t sz = (int64_t)sb.st_size;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, (size_t)sz + 16, hipHostMallocDefault) != hipSuccess) {
+        close(fd);
+        set_error("hipHostMalloc(%lld) failed", (long long)sz + 16);
+        return PG_ERR_OOM;
+    }
+    uint8_t *b = static_cast<uint8_t *>(p);
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(16, sz >> 26));
+    std::vector<int> ok(nt, 1);
+    std::vector<std::thread> th;
+    const int64_t chunk = (sz + nt - 1) / nt;
+    for (int i = 0; i < nt; i++) {
+        th.emplace_back([=, &ok]() {
+            int64_t pos = (int64_t)i * chunk;
+            const int64_t endp = std::min(sz, pos + chunk);
+            while (pos < endp) {
+                const ssize_t r = pread(fd, b + pos, (size_t)std::min<int64_t>(endp - pos, 1 << 30), pos);
+                if (r <= 0) {
+                    ok[i] = 0;
+                    return;
+                }
+                pos += r;
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+    close(fd);
+    for (int i = 0; i < nt; i++)
+        if (!ok[i]) {
+            (void)hipHostFree(p);
+            set_error("read failed: %s", path);
+            return PG_ERR_ARG;
+        }
+    std::memset(b + sz, 0, 16);
+    *buf = b;
+    *size = sz;
+    return PG_OK;
+}
+
 extern "C" int pg_graph_load_adj(const char *path, uint32_t flags, pg_graph **out) {
     PG_REQUIRE(initialised(), PG_ERR_STATE, "pg_init has not been called");
     PG_REQUIRE(path && out, PG_ERR_ARG, "null argument");
     *out = nullptr;
-    std::ifstream f(path, std::ios::in | std::ios::binary | std::ios::ate);
-    PG_REQUIRE(f.is_open(), PG_ERR_ARG, "Unable to open file: %s", path);  // IO-numa.h:75-78
-    const std::streamsize size = f.tellg();
-    f.seekg(0);
-    std::string text((size_t)size, '\0');
-    f.read(&text[0], size);
-    // whitespace-separated words (IO-numa.h:90-109)
-    const char *p = text.data(), *endp = p + text.size();
-    auto next_word = [&](const char **b, const char **e) -> bool {
-        while (p < endp && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == 0)) p++;
+    uint8_t *hb = nullptr;
+    int64_t size = 0;
+    PG_TRY(read_file_pinned(path, &hb, &size));
+    // the header words (IO-numa.h:112-122): the format's name, n, m
+    const char *p = reinterpret_cast<const char *>(hb), *endp = p + size;
+    auto next_word = [&](std::string *wd) -> bool {
+        while (p < endp && txt_sep_host(*p)) p++;
         if (p >= endp) return false;
-        *b = p;
-        while (p < endp && !(*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == 0)) p++;
-        *e = p;
+        const char *b = p;
+        while (p < endp && !txt_sep_host(*p)) p++;
+        wd->assign(b, p);
         return true;
     };
-    auto next_ll = [&](long long *x) -> bool {
-        const char *b, *e;
-        if (!next_word(&b, &e)) return false;
-        *x = strtoll(std::string(b, e).c_str(), nullptr, 10);
-        return true;
-    };
-    const char *b, *e;
-    PG_REQUIRE(next_word(&b, &e), PG_ERR_ARG, "Bad input file: %s", path);
-    const std::string head(b, e);
-    const bool weighted = head == "WeightedAdjacencyGraph";
-    PG_REQUIRE(weighted || head == "AdjacencyGraph", PG_ERR_ARG, "Bad input file: %s", path);
-    long long n = 0, m = 0;
-    PG_REQUIRE(next_ll(&n) && next_ll(&m) && n > 0 && m >= 0, PG_ERR_ARG, "Bad input file: %s", path);
-    std::vector<int64_t> off((size_t)n + 1);
-    std::vector<int32_t> adj((size_t)m), w(weighted ? (size_t)m : 0);
-    long long x;
-    for (long long i = 0; i < n; i++) {
-        PG_REQUIRE(next_ll(&x), PG_ERR_ARG, "Bad input file (offsets): %s", path);
-        off[i] = x;
+    std::string head, sn, sm;
+    const bool got = next_word(&head) && next_word(&sn) && next_word(&sm);
+    const bool weighted = got && head == "WeightedAdjacencyGraph";
+    const long long n = got ? strtoll(sn.c_str(), nullptr, 10) : 0;
+    const long long m = got ? strtoll(sm.c_str(), nullptr, 10) : 0;
+    if (!got || !(weighted || head == "AdjacencyGraph") || n <= 0 || m < 0 || n >= 0x7FFFFFFF ||
+        m >= 0x7FFFFFFF) {
+        (void)hipHostFree(hb);
+        set_error("Bad input file: %s", path);
+        return PG_ERR_ARG;
     }
-    off[n] = m;
-    for (long long i = 0; i < m; i++) {
-        PG_REQUIRE(next_ll(&x), PG_ERR_ARG, "Bad input file (edges): %s", path);
-        adj[i] = (int32_t)x;
+    DevBuf dtxt, dcnt, dbase, doff, dadj, dw;
+    int r = dtxt.alloc((size_t)size + 16);
+    if (r == PG_OK) {
+        const hipError_t e = hipMemcpyAsync(dtxt.p, hb, (size_t)size + 16, hipMemcpyHostToDevice, g_stream);
+        if (e == hipSuccess) (void)hipStreamSynchronize(g_stream);
+        if (e != hipSuccess) {
+            set_error("upload failed: %s", hipGetErrorString(e));
+            r = PG_ERR_HIP;
+        }
     }
-    for (long long i = 0; weighted && i < m; i++) {
-        PG_REQUIRE(next_ll(&x), PG_ERR_ARG, "Bad input file (weights): %s", path);
-        w[i] = (int32_t)x;
-    }
-    PG_REQUIRE(!next_word(&b, &e), PG_ERR_ARG, "Bad input file (length): %s", path);  // IO-numa.h:122
-    return create_from_host(n, m, off.data(), adj.data(), weighted ? w.data() : nullptr,
-                            weighted ? PG_W_I32 : PG_W_NONE, flags, out);
+    (void)hipHostFree(hb);
+    PG_TRY(r);
+    const int64_t nb = (size + kTxtBlockBytes - 1) / kTxtBlockBytes;
+    PG_REQUIRE(nb < 0x7FFFFFFF, PG_ERR_UNSUPPORTED, "file too large: %s", path);
+    PG_TRY(dcnt.alloc(4 * (size_t)nb + 16));
+    PG_TRY(dbase.alloc(8 * (size_t)nb + 16));
+    PG_LAUNCH(k_txt_count, nb, dtxt.as<uint8_t>(), size, dcnt.as<uint32_t>());
+    PG_TRY((exclusive_sum<uint32_t, uint64_t>(dcnt.as<uint32_t>(), dbase.as<uint64_t>(), nb)));
+    uint64_t lastb = 0;
+    uint32_t lastc = 0;
+    PG_CHECK_HIP(hipMemcpyAsync(&lastb, dbase.as<uint64_t>() + nb - 1, 8, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipMemcpyAsync(&lastc, dcnt.as<uint32_t>() + nb - 1, 4, hipMemcpyDeviceToHost, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    const uint64_t words = lastb + lastc;
+    const uint64_t need_off = 3 + (uint64_t)n, need_adj = need_off + (uint64_t)m;
+    const uint64_t need = weighted ? need_adj + (uint64_t)m : need_adj;
+    PG_REQUIRE(words >= need_off, PG_ERR_ARG, "Bad input file (offsets): %s", path);
+    PG_REQUIRE(words >= need_adj, PG_ERR_ARG, "Bad input file (edges): %s", path);
+    PG_REQUIRE(words >= need, PG_ERR_ARG, "Bad input file (weights): %s", path);
+    PG_REQUIRE(words == need, PG_ERR_ARG, "Bad input file (length): %s", path);  // IO-numa.h:122
+    PG_TRY(doff.alloc(8 * ((size_t)n + 1)));
+    PG_TRY(dadj.alloc(4 * (size_t)m + 16));
+    if (weighted) PG_TRY(dw.alloc(4 * (size_t)m + 16));
+    PG_LAUNCH(k_txt_parse, nb, dtxt.as<uint8_t>(), size, dbase.as<uint64_t>(), (int64_t)n, (int64_t)m,
+              weighted, doff.as<int64_t>(), dadj.as<int32_t>(), dw.as<int32_t>());
+    const int64_t mm = m;
+    PG_CHECK_HIP(hipMemcpyAsync(doff.as<int64_t>() + n, &mm, 8, hipMemcpyHostToDevice, g_stream));
+    PG_CHECK_HIP(hipStreamSynchronize(g_stream));
+    return create_from_device(n, m, doff.as<int64_t>(), dadj.as<int32_t>(), weighted ? dw.p : nullptr,
+                              weighted ? PG_W_I32 : PG_W_NONE, flags, out);
 }
 
 // The binary graph of readGraphFromBinary / readWghGraphFromBinary

@@ -132,6 +132,31 @@ def test_bp_scale24_10_iterations(pa, orc):
     assert np.array_equal(cg, cr)
 
 
+
+def test_load_adj_scale24_timed(pa, orc, tmp_path):
+    """PBBS AdjacencyGraph text of the scale-24 rMat (2^28 edges, about 2.6 GB)
+    written by the oracle, loaded by pg_graph_load_adj (read into pinned memory
+    in parallel, parsed on the GPU) and compared with the generator's graph; the
+    load is timed."""
+    import time
+    say("case: PBBS text load, scale 24")
+    g = pa.Graph.rmat(24, 1 << 28, seed=1)
+    off, adj, _ = g.download()
+    g.close()
+    hg = orc.Graph(len(off) - 1, off, adj)
+    path = str(tmp_path / "s24.adj")
+    t0 = time.perf_counter()
+    orc.write_adj(path, hg)
+    size = os.path.getsize(path)
+    say("  oracle wrote %d bytes in %.1f s" % (size, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    d = pa.Graph.load_adj(path)
+    dt = time.perf_counter() - t0
+    say("  pg_graph_load_adj: %.3f s (%.2f GB/s of text), n %d m %d" % (dt, size / dt / 1e9, d.n, d.m))
+    o2, a2, _ = d.download()
+    d.close()
+    assert np.array_equal(o2, off) and np.array_equal(a2, adj)
+
 # ---- traversal, scale 22 ---------------------------------------------------------------------
 def _check_bfs(off, adj, src, level, parent):
     n = off.size - 1

@@ -182,3 +182,65 @@ def test_load_dump_rejects_corruption(pa, orc, tmp_path):
         with pytest.raises(RuntimeError, match="Bad dump file"):
             pa.Graph.load_dump(p)
         say("dump corruption rejected:", what)
+
+
+# ---- PBBS text, parsed on the GPU ---------------------------------------------------
+def test_load_adj_text_forms(pa, orc, tmp_path):
+    """pg_graph_load_adj (IO-numa.h:112-190, IO.h:186-272) against the oracle's
+    reader: an unweighted and a weighted rMat as the oracle writes them; the same
+    words with tabs, CRLF line ends, runs of blanks, signs and no final newline;
+    self-loops kept or dropped (PG_GRAPH_SKIP_SELF_LOOPS, IO-numa.h:225-268)."""
+    g = orc.rmat(13, 1 << 17, seed=31)
+    w = orc.weights_i32(g, 5)
+    p = str(tmp_path / "g.adj")
+    orc.write_adj(p, g)
+    d = pa.Graph.load_adj(p)
+    o, a, _ = d.download()
+    assert np.array_equal(o, g.off) and np.array_equal(a, g.adj)
+    d.close()
+    pw = str(tmp_path / "g.wadj")
+    orc.write_adj(pw, g, w)
+    d = pa.Graph.load_adj(pw)
+    o, a, ww = d.download()
+    h, hw = orc.read_adj(pw)
+    assert np.array_equal(o, g.off) and np.array_equal(a, g.adj)
+    assert ww is not None and np.array_equal(ww, w) and np.array_equal(ww, hw)
+    d.close()
+    # the same small weighted graph with unusual separators, and a self-loop
+    off, adj, wt = [0, 2, 3, 5], [1, 2, 2, 0, 2], [7, +3, 9, 1, 4]
+    text = ("WeightedAdjacencyGraph\r\n3\t\t5\r\n" + " ".join(map(str, off[:3])) + "\n\n" +
+            "\t".join(map(str, adj)) + "   \r\n" + " +7 3\n9\t1  4")  # no final newline
+    q = str(tmp_path / "odd.wadj")
+    open(q, "w").write(text)
+    for skip in (False, True):
+        d = pa.Graph.load_adj(q, skip_self_loops=skip)
+        o, a, ww = d.download()
+        if skip:  # 2 -> 2 is dropped with its weight
+            assert list(o) == [0, 2, 3, 4] and list(a) == [1, 2, 2, 0] and list(ww) == [7, 3, 9, 1]
+        else:
+            assert list(o) == off and list(a) == adj and list(ww) == [7, 3, 9, 1, 4]
+        h, hw = orc.read_adj(q, skip_self_loops=skip)
+        assert np.array_equal(o, h.off) and np.array_equal(a, h.adj) and np.array_equal(ww, hw)
+        d.close()
+
+
+def test_load_adj_text_errors(pa, tmp_path):
+    """The reader's refusals (IO-numa.h:75-78, 116-122) as error codes: a missing
+    file, a wrong header, too few offsets, edges or weights, a word too many, and a
+    neighbour id out of range."""
+    cases = [
+        ("Graph\n1\n0\n0\n", "Bad input file"),
+        ("AdjacencyGraph\n3\n2\n0\n1\n", "Bad input file \\(offsets\\)"),
+        ("AdjacencyGraph\n3\n2\n0\n1\n1\n2\n", "Bad input file \\(edges\\)"),
+        ("WeightedAdjacencyGraph\n3\n2\n0\n1\n1\n2\n0\n5\n", "Bad input file \\(weights\\)"),
+        ("AdjacencyGraph\n3\n2\n0\n1\n1\n2\n0\n0\n", "Bad input file \\(length\\)"),
+        ("AdjacencyGraph\n3\n2\n0\n1\n1\n2\n7\n", "out of range"),
+    ]
+    for i, (text, msg) in enumerate(cases):
+        p = str(tmp_path / ("e%d.adj" % i))
+        open(p, "w").write(text)
+        say("case: %r" % msg)
+        with pytest.raises(RuntimeError, match=msg):
+            pa.Graph.load_adj(p)
+    with pytest.raises(RuntimeError, match="Unable to open"):
+        pa.Graph.load_adj(str(tmp_path / "missing.adj"))
