@@ -23,7 +23,12 @@ namespace pg {
         PG_CHECK_HIP(hipGetLastError());                                              \
     } while (0)
 
+// the residual statistic, and the gate of the device-side convergence test: a
+// launch enqueued after the test found convergence returns at once
 struct ResidualStat {
+    const int *gate;
+    static constexpr bool kGated = true;
+    __device__ bool gated() const { return *gate != 0; }
     using Stat = double;
     __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
     __device__ static void stat_commit(Stat *p, Stat v) {
@@ -117,40 +122,58 @@ __global__ void k_scale_f32(const float *a, int64_t n, float s, float *b) {
         b[i] = s * a[i];
 }
 
-static int read_residual(const double *slots, double *out) {
-    double h[kStatSlots];
-    PG_CHECK_HIP(hipMemcpyAsync(h, slots, sizeof(h), hipMemcpyDeviceToHost, stream()));
-    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+// After an iteration: its residual (the 64 slots summed in order, as a host loop
+// would), kept in hist[t]; the first t whose residual is <= tol sets the gate
+// (state[0]) and records t + 1 iterations used (state[1]).
+__global__ void k_res_check(const double *slots, double tol, int t, int *state, double *hist) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || state[0]) return;
     double s = 0;
-    for (int i = 0; i < kStatSlots; i++) s += h[i];
-    *out = s;
-    return PG_OK;
+    for (int i = 0; i < kStatSlots; i++) s += slots[i];
+    hist[t] = s;
+    if (s <= tol) {
+        state[1] = t + 1;
+        state[0] = 1;
+    }
 }
 
-// fixed-point loop shared by the forward and the adjoint
+// Fixed-point loop shared by the forward and the adjoint.  Iterations are enqueued
+// in batches of kFixBatch, each followed by the device-side test; the host reads the
+// state once per batch (not the residual after every iteration).  Launches past
+// convergence are gated off, so the result is the iterate of the first iteration
+// whose residual is <= tol, as with a test on the host after every iteration.
+constexpr int kFixBatch = 16;
+
 template <class MakeOp>
 static int fixed_point(pg_graph *g, const PullSchedule &sched, float *a, float *b, int max_iters,
                        double tol, KernelTimer &kt, MakeOp make, float **result, int *iters_used,
                        double *residual) {
-    double *slots = nullptr;
+    double *slots = nullptr, *hist = nullptr;
+    int *state = nullptr;
     PG_TRY(workspace_t(g, "wpr_res", (size_t)kStatSlots, &slots));
+    PG_TRY(workspace_t(g, "wpr_res_hist", (size_t)max_iters, &hist));
+    PG_TRY(workspace_t(g, "wpr_state", 4, &state));
+    PG_TRY(fill(state, 0, 4));
     float *cur = a, *nxt = b;
-    double res = 0;
-    int t = 0;
-    for (t = 0; t < max_iters;) {
-        PG_TRY(fill(slots, 0.0, kStatSlots));
-        auto op = make(cur, nxt);
-        PG_TRY(kt.begin());
-        PG_TRY(launch_pull(op, sched, slots));
-        PG_TRY(kt.end());
-        std::swap(cur, nxt);
-        t++;
-        PG_TRY(read_residual(slots, &res));
-        if (res <= tol) break;
+    int h[4] = {0, 0, 0, 0};
+    for (int t0 = 0; t0 < max_iters && !h[0]; t0 += kFixBatch) {
+        for (int t = t0; t < std::min(max_iters, t0 + kFixBatch); t++) {
+            PG_TRY(fill(slots, 0.0, kStatSlots));
+            auto op = make(state, cur, nxt);
+            PG_TRY(kt.begin());
+            PG_TRY(launch_pull(op, sched, slots));
+            PG_TRY(kt.end());
+            hipLaunchKernelGGL(k_res_check, dim3(1), dim3(64), 0, stream(), slots, tol, t, state, hist);
+            PG_CHECK_HIP(hipGetLastError());
+            std::swap(cur, nxt);
+        }
+        PG_CHECK_HIP(hipMemcpyAsync(h, state, sizeof(h), hipMemcpyDeviceToHost, stream()));
+        PG_CHECK_HIP(hipStreamSynchronize(stream()));
     }
-    *result = cur;
-    *iters_used = t;
-    *residual = res;
+    const int used = h[0] ? h[1] : max_iters;
+    // iteration t writes b when t is even, a when it is odd
+    *result = (used % 2 == 1) ? b : a;
+    *iters_used = used;
+    PG_CHECK_HIP(hipMemcpy(residual, hist + used - 1, sizeof(double), hipMemcpyDeviceToHost));
     return PG_OK;
 }
 
@@ -185,7 +208,9 @@ extern "C" int pg_wpr_forward(pg_graph *g, const float *wn, const float *tele, d
     int used = 0;
     double res = 0;
     PG_TRY(fixed_point(g, g->in_sched, xa, xb, max_iters, tol, kt,
-                       [&](float *cur, float *nxt) { return WprOp{{}, cur, nxt, w_in, tv, d, omd}; },
+                       [&](const int *gate, float *cur, float *nxt) {
+                           return WprOp{{gate}, cur, nxt, w_in, tv, d, omd};
+                       },
                        &res_ptr, &used, &res));
     PG_LAUNCH_A(k_copy_f32, grid_for(n), res_ptr, n, xs);  // kept for the adjoint
     double ms = 0;
@@ -233,7 +258,9 @@ extern "C" int pg_wpr_adjoint(pg_graph *g, const float *gvec, int max_iters, dou
     int used = 0;
     double res = 0;
     PG_TRY(fixed_point(g, g->out_sched, la, lb, max_iters, tol, kt,
-                       [&](float *cur, float *nxt) { return WprAdjOp{{}, cur, nxt, w_out, gv, d}; },
+                       [&](const int *gate, float *cur, float *nxt) {
+                           return WprAdjOp{{gate}, cur, nxt, w_out, gv, d};
+                       },
                        &lam, &used, &res));
     WprGradOp gop{lam, xs, gw, d};
     PG_TRY(launch_pull(gop, g->out_sched));

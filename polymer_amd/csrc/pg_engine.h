@@ -169,6 +169,15 @@ struct has_edge_token : std::false_type {};
 template <class Op>
 struct has_edge_token<Op, std::enable_if_t<Op::kEdgeToken>> : std::true_type {};
 
+// An Op may gate whole launches: with `static constexpr bool kGated = true` and
+// `bool gated()` (the same for every thread, read from device memory), a k_pull
+// launch whose gate is set returns at once (iterations enqueued past a
+// convergence found on the device).
+template <class Op, class = void>
+struct has_gate : std::false_type {};
+template <class Op>
+struct has_gate<Op, std::enable_if_t<Op::kGated>> : std::true_type {};
+
 template <class Op>
 __device__ __forceinline__ int edge_src(const Op &op, const int32_t *adj, int e) {
     if constexpr (has_edge_token<Op>::value) return op.edge_token(e);
@@ -385,6 +394,9 @@ __device__ __forceinline__ void commit_block_stat(typename Op::Stat st, typename
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_pull(Op op, PullArgs pa,
                                                  typename Op::Stat *stat_out) {
+    if constexpr (has_gate<Op>::value) {
+        if (op.gated()) return;  // uniform: every thread reads the same word
+    }
     commit_block_stat<Op>(pull_dispatch(op, pa, (int)blockIdx.x), stat_out);
 }
 
