@@ -593,6 +593,15 @@ def main():
                 "traffic_source": traffic_src,
                 "kernel": "pull iteration: " + kernels, "avg_launch_us": round(avg_kernel_s * 1e6, 2),
                 "bytes_per_launch": int(bytes_per_launch)}
+    # the 20 bytes per vertex of the model count the vertices without in-edges, which
+    # the pull visits in the first two iterations only (their rank cannot change
+    # after that): the same frac over the bytes of the vertices it touches
+    _, adj_in, _ = g.download()
+    zero_in = int(np.count_nonzero(np.bincount(adj_in, minlength=g.n) == 0))
+    del adj_in
+    touched = bytes_per_launch - 20.0 * zero_in * max(args.iters - 2, 0) / args.iters
+    roofline["zero_in_degree_vertices"] = zero_in
+    roofline["frac_touched"] = round(touched / avg_kernel_s / 1e9 / HBM_PEAK_GBS, 4)
     log("pull iteration avg %.1f us, %.1f GB/s algorithmic" % (avg_kernel_s * 1e6, achieved))
 
     # the same solve with the ranks copied to host memory in input order (the
