@@ -12,7 +12,11 @@ scale-22 configurations, in the build container where those sources exist:
     weights U[1,255] (the reference's weighted reader is integer-only); BP has no
     inputs of its own (its potentials are zero-filled, numa-BP.C:557-590).
 
-    python oracle/time_reference.py [--runs 3] [--out results.json]
+    python oracle/time_reference.py [--runs 3] [--out results.json] [--scale24]
+
+--scale24 adds numa-PageRank on the scale-24 rMat (n = 2^24, m = 2^28, 20
+iterations; the headline configuration), run once (its graph load alone takes
+minutes).
 
 numa-BFS and numa-Components can abort while their workers exit (a barrier that
 outlives the worker owning it), sometimes before the timing line is printed: such a
@@ -65,6 +69,7 @@ def main():
     ap.add_argument("--runs", type=int, default=3)
     ap.add_argument("--out", default=None)
     ap.add_argument("--only", default="", help="comma-separated program names (default: all)")
+    ap.add_argument("--scale24", action="store_true", help="also numa-PageRank at scale 24, once")
     args = ap.parse_args()
     only = set(x for x in args.only.split(",") if x)
 
@@ -129,6 +134,16 @@ def main():
             med, vals = run_timed([os.path.join(BIN, "numa-BellmanFord"), paths["wsym"], str(src), "-",
                                    "-s"], "BellmanFord", args.runs)
             add("numa-BellmanFord", "S22 symmetric, integer weights", med, vals, dict(source=src))
+        if args.scale24:
+            for f in paths.values():
+                os.remove(f)
+            del g, w, gs, ws
+            g = O.rmat(24, 1 << 28, seed=1)
+            p24 = os.path.join(td, "s24.adj")
+            O.write_adj(p24, g)
+            print("S24: n %d m %d" % (g.n, g.m), flush=True)
+            med, vals = run_timed([os.path.join(BIN, "numa-PageRank"), p24, "20", "-1"], "PageRank", 1)
+            add("numa-PageRank", "S24", med, vals, dict(iterations=20, edges_per_iteration=g.m))
     print(json.dumps(res, indent=1))
     if args.out:
         with open(args.out, "w") as f:
