@@ -408,6 +408,37 @@ def profile_extra(args):
         raise SystemExit("unknown --profile-extra %r" % name)
 
 
+def probe_gather(args):
+    """Random 8-byte gathers per second against the size of the gathered array:
+    torch.index_select of 2^27 uniform int32 indices from an fp64 array of 2^k
+    elements (each gather also streams its 4-byte index in and 8 bytes out).  The
+    ceiling of a pull whose functor gathers per edge from arrays larger than the
+    last-level caches (the drop-in drivers' p_curr[s] and V[s])."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    count = 1 << 27
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    for k in range(15, 28):
+        a = torch.zeros(1 << k, dtype=torch.float64, device=dev)
+        idx = torch.randint(0, 1 << k, (count,), generator=gen, device=dev, dtype=torch.int32)
+        out = torch.empty(count, dtype=torch.float64, device=dev)
+        torch.index_select(a, 0, idx, out=out)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        reps = 5
+        for _ in range(reps):
+            torch.index_select(a, 0, idx, out=out)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        print(json.dumps({"array_mb": round((8 << k) / 1e6, 2), "gathers": count,
+                          "ms": round(ms, 3), "ggather_per_s": round(count / ms / 1e6, 1)}),
+              flush=True)
+        del a, idx, out
+
+
 def measured_traffic(schedule):
     """HBM bytes per PageRank iteration from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary, when it was taken with the same pull schedule."""
@@ -519,6 +550,8 @@ def main():
                     help="the same for BP on the scale-24 rMat of the extras")
     ap.add_argument("--profile-extra", default=None,
                     help="the same for one extra workload: spmv, delta, bfs22, ppr, async")
+    ap.add_argument("--probe-gather", action="store_true",
+                    help="random 8-byte gather rate against the gathered array's size")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -546,6 +579,9 @@ def main():
         return
     if args.profile_extra:
         profile_extra(args)
+        return
+    if args.probe_gather:
+        probe_gather(args)
         return
 
     import polymer_amd as pa

@@ -214,6 +214,16 @@ inline void ensure_init() {
 
 inline void sync() { check_hip(hipStreamSynchronize(lib_stream()), "hipStreamSynchronize"); }
 
+// The end of a call whose results the program may read directly: with managed
+// arrays the host loops read them, so the call waits for its kernels; with
+// device-resident arrays the program reaches them only through toHost (which
+// waits) or further calls on the same stream, so the call returns at once.
+inline void sync_visible() {
+#ifndef POLYMER_DEVICE_RESIDENT
+    sync();
+#endif
+}
+
 inline void launched(const char *what) { check_hip(hipGetLastError(), what); }
 
 inline unsigned blocks_for(int64_t n, int threads = 256, int64_t cap = 65536) {
@@ -251,17 +261,19 @@ T *workspace_t(const char *name, size_t count) {
 // Memory the application's host loops and the kernels both address: managed
 // memory (one address on the host and the GPU), zero-filled like the reference's
 // anonymous mappings.  With POLYMER_DEVICE_RESIDENT: device memory.
-inline void *shared_alloc(size_t bytes) {
+inline void *shared_alloc(size_t bytes, bool zero = true) {
     ensure_init();
     void *p = nullptr;
     const size_t b = bytes ? bytes : 1;
 #ifdef POLYMER_DEVICE_RESIDENT
     check_hip(hipMalloc(&p, b), "hipMalloc");
-    check_hip(hipMemsetAsync(p, 0, b, lib_stream()), "hipMemsetAsync");
-    sync();
+    if (zero) {
+        check_hip(hipMemsetAsync(p, 0, b, lib_stream()), "hipMemsetAsync");
+        sync();
+    }
 #else
     check_hip(hipMallocManaged(&p, b, hipMemAttachGlobal), "hipMallocManaged");
-    std::memset(p, 0, b);
+    if (zero) std::memset(p, 0, b);
 #endif
     return p;
 }
@@ -285,6 +297,7 @@ const T *device_readable(const T *p, size_t count, const char *stage) {
     T *d = workspace_t<T>(stage, count);
     check_hip(hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, lib_stream()),
               "hipMemcpyAsync");
+    sync();  // the program may reuse its list once the call returns
     return d;
 }
 
@@ -584,6 +597,7 @@ struct Bins {
     int64_t start[4] = {0};
     int64_t count[4] = {0};
     int64_t nonzero = 0;  // destinations with in-edges
+    intT *first_in = nullptr;  // first in-neighbour of every vertex (-1: none)
 };
 inline std::map<const void *, Bins> &bins_cache() {
     static std::map<const void *, Bins> cache;
@@ -600,6 +614,7 @@ struct GraphOwner {
         auto it = bins_cache().find(key);
         if (it != bins_cache().end()) {
             if (it->second.ids) (void)hipFree(it->second.ids);
+            if (it->second.first_in) (void)hipFree(it->second.first_in);
             bins_cache().erase(it);
         }
         if (V) (void)hipFree(V);
@@ -640,6 +655,7 @@ template <class V>
 struct is_weighted<wghGraph<V>> : std::true_type {};
 
 inline const Bins &bins_of(const pg_view &v);
+inline void presize_workspaces(int64_t n);
 
 template <class G>
 void wrap_into(G &g, pg_graph *h, bool weighted) {
@@ -660,6 +676,7 @@ void wrap_into(G &g, pg_graph *h, bool weighted) {
     launched("k_fill_vertices");
     sync();
     (void)bins_of(g.view);  // the dense pull's in-degree bins, built with the graph
+    presize_workspaces((int64_t)g.n);
 }
 
 template <class G>
@@ -958,16 +975,31 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sm, ui
     return r;
 }
 
-// count of set flags per kChunk bytes
+// The set flags among b[i, i + 16) as the bits of a mask (bit j = flag i + j; i a
+// multiple of 16): one 16-byte load when b is 16-byte aligned and the block lies
+// inside [0, n), byte loads otherwise (a program's own offset array, the tail).
+__device__ __forceinline__ uint32_t flag_mask16(const bool *b, int64_t i, int64_t n) {
+    uint32_t m = 0;
+    if ((((uintptr_t)b) & 15u) == 0u && i + 16 <= n) {
+        const uint4 x = *reinterpret_cast<const uint4 *>(b + i);
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((w[k] >> (8 * j)) & 0xFFu) m |= 1u << (4 * k + j);
+    } else {
+        for (int j = 0; j < 16; j++)
+            if (i + j < n && b[i + j]) m |= 1u << j;
+    }
+    return m;
+}
+
+// count of set flags per kChunk bytes (16 per thread)
 __global__ __launch_bounds__(kT) void k_flag_counts(const bool *b, int64_t n, uint32_t *cnt) {
     __shared__ uint32_t sm[kT / 64 + 1];
-    const int64_t base = (int64_t)blockIdx.x * kChunk;
-    uint32_t c = 0;
-#pragma unroll 4
-    for (int k = 0; k < kChunk / kT; k++) {
-        const int64_t i = base + k * kT + threadIdx.x;
-        c += (i < n && b[i]) ? 1u : 0u;
-    }
+    const int64_t i = (int64_t)blockIdx.x * kChunk + 16 * (int64_t)threadIdx.x;
+    const uint32_t c = i < n ? (uint32_t)__popc(flag_mask16(b, i, n)) : 0u;
     uint32_t tot = 0;
     (void)block_excl_scan(c, sm, &tot);
     if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
@@ -988,19 +1020,19 @@ __global__ __launch_bounds__(kT) void k_scan_small(const uint32_t *in, uint32_t 
     if (threadIdx.x == 0) out[n] = carry;
 }
 
-// ordered compaction: the ids (+ offset) of the set flags, block by block
+// ordered compaction: the ids (+ offset) of the set flags, block by block; a
+// thread writes the ids of its 16 flags in order after its prefix
 __global__ __launch_bounds__(kT) void k_flag_write(const bool *b, int64_t n, const uint32_t *starts,
                                                    intT offset, intT *s) {
     __shared__ uint32_t sm[kT / 64 + 1];
-    const int64_t base = (int64_t)blockIdx.x * kChunk;
-    uint32_t out = starts[blockIdx.x];
-    for (int k = 0; k < kChunk / kT; k++) {
-        const int64_t i = base + k * kT + threadIdx.x;
-        const bool f = i < n && b[i];
-        uint32_t tot = 0;
-        const uint32_t x = block_excl_scan(f ? 1u : 0u, sm, &tot);
-        if (f) s[out + x] = (intT)i + offset;
-        out += tot;
+    const int64_t i = (int64_t)blockIdx.x * kChunk + 16 * (int64_t)threadIdx.x;
+    uint32_t mk = i < n ? flag_mask16(b, i, n) : 0u;
+    uint32_t tot = 0;
+    uint32_t out = starts[blockIdx.x] + block_excl_scan((uint32_t)__popc(mk), sm, &tot);
+    while (mk) {
+        const int j = __ffs(mk) - 1;
+        mk &= mk - 1u;
+        s[out++] = (intT)(i + j) + offset;
     }
 }
 
@@ -1060,8 +1092,8 @@ __global__ __launch_bounds__(kT) void k_deg_tiles(const intT *list, int64_t coun
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) scan[count] = run;
 }
 
-// scan[i] = sum of outdeg(list[0..i)), scan[count] = total; returns the total (host)
-inline uint32_t degree_scan(const intT *list, int64_t count, const int32_t *off, uint32_t *scan) {
+// scan[i] = sum of outdeg(list[0..i)), scan[count] = total, on the device
+inline void degree_scan(const intT *list, int64_t count, const int32_t *off, uint32_t *scan) {
     const int64_t nt = (count + kScanTile - 1) / kScanTile;
     uint32_t *sums = workspace_t<uint32_t>("deg_sums", (size_t)nt + 1);
     uint32_t *starts = workspace_t<uint32_t>("deg_starts", (size_t)nt + 1);
@@ -1072,9 +1104,23 @@ inline uint32_t degree_scan(const intT *list, int64_t count, const int32_t *off,
     hipLaunchKernelGGL(k_deg_tiles, dim3((unsigned)nt), dim3(kT), 0, lib_stream(), list, count, off,
                        (const uint32_t *)starts, scan);
     launched("degree scan");
-    uint32_t total = 0;
-    toHost(&total, scan + count, 1);
-    return total;
+}
+
+// The per-round workspaces of the edge maps sized for a graph of n vertices when
+// the graph is made (a list longer than n, possible with repeated successes, still
+// grows them on use), so a program's loop of rounds does not allocate.
+inline void presize_workspaces(int64_t n) {
+    const size_t nw = (size_t)(n + 31) / 32 + 1;
+    (void)workspace_t<uint32_t>("frontier_bits", nw);
+    (void)workspace_t<uint32_t>("pull_marks", nw);
+    (void)workspace_t<intT>("active_list", (size_t)n + 1);
+    (void)workspace_t<uint32_t>("push_scan", (size_t)n + 1);
+    const size_t nb = (size_t)(n + kChunk - 1) / kChunk + 1;
+    (void)workspace_t<uint32_t>("compact_cnt", nb);
+    (void)workspace_t<uint32_t>("compact_start", nb);
+    const size_t nt = (size_t)(n + kScanTile - 1) / kScanTile + 1;
+    (void)workspace_t<uint32_t>("deg_sums", nt);
+    (void)workspace_t<uint32_t>("deg_starts", nt);
 }
 
 template <bool W, class F>
@@ -1085,105 +1131,118 @@ __device__ __forceinline__ bool call_update_atomic(F &f, intT s, intT d, const i
         return f.updateAtomic(s, d);
 }
 
-// wave-aggregated append of val to list (capacity cap) when pred; the slot counter
-// is ctr[0]; appends beyond cap are counted in ctr[0] but not stored
-__device__ __forceinline__ void wave_append(intT *list, size_t cap, unsigned long long *ctr,
-                                            bool pred, intT val) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return;
-    const int l = lane();
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned long long base = 0;
-    if (l == leader) base = atomicAdd(&ctr[0], (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (pred) {
-        const unsigned long long pos = base + __popcll(mask & ((1ull << l) - 1ull));
-        if (pos < cap) list[pos] = val;
-    }
-}
-
-// Edge-balanced push over list[0, count) with degree scan `scan` (total edges):
-// block b takes the frontier's edges [b*epb, (b+1)*epb).  Mode 0 sets the next
-// flags (edgeMapDenseForward), 1 appends to the next list and adds the
-// destinations' out-degrees to ctr[1] (edgeMapSparseV3), 2 is mode 0 with the
-// edge's index in its source's list passed to updateAtomic (edgeMapDenseBP).
+// Edge-balanced push over list[0, count) with degree scan `scan` (scan[count] =
+// the total, read on the device, so the launch waits for no count): a fixed grid,
+// block b taking the frontier's edges [b*epb, (b+1)*epb), epb = total / grid,
+// in chunks of at most kPushEdgesMax.  Mode 0 sets the next flags
+// (edgeMapDenseForward), 1 appends to the next list and adds the destinations'
+// out-degrees to ctr[1..32] (edgeMapSparseV3), 2 is mode 0 with the edge's index in
+// its source's list passed to updateAtomic (edgeMapDenseBP).
 constexpr int kWin = 1024;
+constexpr int kPushEdgesMax = 2048;  // edges per chunk (the block's append stage)
+constexpr int kPushBlocks = 2048;    // grid of the push (8 workgroups per CU)
 
 template <bool W, int Mode, class F>
 __global__ __launch_bounds__(kT) void k_push_edges(F f, pg_view v, const intT *list,
-                                                   const uint32_t *scan, int64_t count,
-                                                   uint32_t total, uint32_t epb, bool *next,
+                                                   const uint32_t *scan, int64_t count, bool *next,
                                                    intT *out_list, size_t cap,
                                                    unsigned long long *ctr) {
     __shared__ uint32_t s_scan[kWin + 1];
     __shared__ int32_t s_beg[kWin];
     __shared__ int64_t s_j0, s_j1;
-    const uint32_t E0 = (uint32_t)blockIdx.x * epb;
-    const uint32_t E1 = min(total, E0 + epb);
-    if (threadIdx.x == 0) {
-        int64_t lo = 0, hi = count + 1;  // last j with scan[j] <= E0
-        while (lo + 1 < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (scan[mid] <= E0) lo = mid; else hi = mid;
-        }
-        s_j0 = lo;
-        lo = 0;
-        hi = count + 1;  // last j with scan[j] <= E1 - 1
-        while (lo + 1 < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (scan[mid] <= E1 - 1) lo = mid; else hi = mid;
-        }
-        s_j1 = lo;
-    }
-    __syncthreads();
-    const int64_t j0 = s_j0, nwin = s_j1 - s_j0 + 1;
-    const bool staged = nwin <= kWin;
-    if (staged) {
-        for (int64_t i = threadIdx.x; i < nwin; i += kT) {
-            s_scan[i] = scan[j0 + i];
-            s_beg[i] = v.out_off[list[j0 + i]];
-        }
-    }
-    __syncthreads();
+    // Mode 1: a chunk's appends are staged here and placed with one atomic on
+    // ctr[0] per chunk (a counter shared by every wave of the grid serialises)
+    __shared__ intT s_out[Mode == 1 ? kPushEdgesMax : 1];
+    __shared__ uint32_t s_cnt;
+    __shared__ unsigned long long s_base;
+    const uint32_t total = scan[count];
+    const uint32_t epb = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t B0 = min(total, (uint32_t)blockIdx.x * epb);
+    const uint32_t B1 = min(total, B0 + epb);
     const intE *w = W ? static_cast<const intE *>(v.out_w) : nullptr;
     unsigned long long outdeg = 0;
-    for (uint32_t k = E0 + threadIdx.x; k < E1; k += kT) {
-        int64_t j;
-        int32_t beg;
-        uint32_t sc;
+    for (uint32_t E0 = B0; E0 < B1; E0 += kPushEdgesMax) {  // block-uniform
+        const uint32_t E1 = min(B1, E0 + kPushEdgesMax);
+        if (threadIdx.x == 0) {
+            s_cnt = 0;
+            int64_t lo = 0, hi = count + 1;  // last j with scan[j] <= E0
+            while (lo + 1 < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (scan[mid] <= E0) lo = mid; else hi = mid;
+            }
+            s_j0 = lo;
+            lo = 0;
+            hi = count + 1;  // last j with scan[j] <= E1 - 1
+            while (lo + 1 < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (scan[mid] <= E1 - 1) lo = mid; else hi = mid;
+            }
+            s_j1 = lo;
+        }
+        __syncthreads();
+        const int64_t j0 = s_j0, nwin = s_j1 - s_j0 + 1;
+        const bool staged = nwin <= kWin;
         if (staged) {
-            int lo = 0, hi = (int)nwin - 1;  // last i with s_scan[i] <= k
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_scan[mid] <= k) lo = mid; else hi = mid - 1;
+            for (int64_t i = threadIdx.x; i < nwin; i += kT) {
+                s_scan[i] = scan[j0 + i];
+                s_beg[i] = v.out_off[list[j0 + i]];
             }
-            j = j0 + lo;
-            beg = s_beg[lo];
-            sc = s_scan[lo];
-        } else {
-            int64_t lo = j0, hi = s_j1;
-            while (lo < hi) {
-                const int64_t mid = (lo + hi + 1) >> 1;
-                if (scan[mid] <= k) lo = mid; else hi = mid - 1;
-            }
-            j = lo;
-            sc = scan[lo];
-            beg = v.out_off[list[lo]];
         }
-        const intT src = list[j];
-        const int64_t e = (int64_t)beg + (k - sc);
-        const intT ngh = v.out_adj[e];
-        bool act;
-        if constexpr (Mode == 2)
-            act = f.cond(ngh) && f.updateAtomic(src, ngh, (intT)(k - sc));
-        else
-            act = f.cond(ngh) && call_update_atomic<W>(f, src, ngh, w, e);
+        __syncthreads();
+        for (uint32_t k = E0 + threadIdx.x; k < E1; k += kT) {
+            int64_t j;
+            int32_t beg;
+            uint32_t sc;
+            if (staged) {
+                int lo = 0, hi = (int)nwin - 1;  // last i with s_scan[i] <= k
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_scan[mid] <= k) lo = mid; else hi = mid - 1;
+                }
+                j = j0 + lo;
+                beg = s_beg[lo];
+                sc = s_scan[lo];
+            } else {
+                int64_t lo = j0, hi = s_j1;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi + 1) >> 1;
+                    if (scan[mid] <= k) lo = mid; else hi = mid - 1;
+                }
+                j = lo;
+                sc = scan[lo];
+                beg = v.out_off[list[lo]];
+            }
+            const intT src = list[j];
+            const int64_t e = (int64_t)beg + (k - sc);
+            const intT ngh = v.out_adj[e];
+            bool act;
+            if constexpr (Mode == 2)
+                act = f.cond(ngh) && f.updateAtomic(src, ngh, (intT)(k - sc));
+            else
+                act = f.cond(ngh) && call_update_atomic<W>(f, src, ngh, w, e);
+            if constexpr (Mode == 1) {
+                const unsigned long long mask = __ballot(act);
+                if (mask) {
+                    const int l = lane(), leader = __ffsll((long long)mask) - 1;
+                    uint32_t at = 0;
+                    if (l == leader) at = atomicAdd(&s_cnt, (uint32_t)__popcll(mask));
+                    at = (uint32_t)__shfl((int)at, leader, 64);
+                    if (act) s_out[at + __popcll(mask & ((1ull << l) - 1ull))] = ngh;
+                }
+                if (act) outdeg += (unsigned long long)(v.out_off[ngh + 1] - v.out_off[ngh]);
+            } else {
+                if (act) next[ngh] = true;
+            }
+        }
         if constexpr (Mode == 1) {
-            wave_append(out_list, cap, ctr, act, ngh);
-            if (act) outdeg += (unsigned long long)(v.out_off[ngh + 1] - v.out_off[ngh]);
-        } else {
-            if (act) next[ngh] = true;
+            __syncthreads();
+            if (threadIdx.x == 0)
+                s_base = s_cnt ? atomicAdd(&ctr[0], (unsigned long long)s_cnt) : 0ull;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < s_cnt; i += kT)
+                if (s_base + i < cap) out_list[s_base + i] = s_out[i];
         }
+        __syncthreads();  // the chunk's LDS is reused by the next one
     }
     if constexpr (Mode == 1) {
         for (int o = 32; o >= 1; o >>= 1) outdeg += __shfl_xor(outdeg, o, 64);
@@ -1196,15 +1255,9 @@ inline void push_list(F &f, const pg_view &v, const intT *list, int64_t count, b
                       intT *out_list, size_t cap, unsigned long long *ctr) {
     if (count <= 0) return;
     uint32_t *scan = workspace_t<uint32_t>("push_scan", (size_t)count + 1);
-    const uint32_t total = degree_scan(list, count, v.out_off, scan);
-    if (total == 0) return;
-    // about 2048 edges per workgroup, fewer for small frontiers (at least 2048
-    // workgroups while each keeps a full wave of edges)
-    int64_t epb = std::min<int64_t>(2048, std::max<int64_t>(64, ((int64_t)total + 2047) / 2048));
-    const int64_t nb = ((int64_t)total + epb - 1) / epb;
-    hipLaunchKernelGGL((k_push_edges<W, Mode, F>), dim3((unsigned)nb), dim3(kT), 0, lib_stream(), f,
-                       v, list, (const uint32_t *)scan, count, total, (uint32_t)epb, next, out_list,
-                       cap, ctr);
+    degree_scan(list, count, v.out_off, scan);
+    hipLaunchKernelGGL((k_push_edges<W, Mode, F>), dim3(kPushBlocks), dim3(kT), 0, lib_stream(), f, v,
+                       list, (const uint32_t *)scan, count, next, out_list, cap, ctr);
     launched("edge push");
 }
 
@@ -1255,6 +1308,12 @@ __global__ __launch_bounds__(kT) void k_bin_fill(const int32_t *in_off, int64_t 
     }
 }
 
+__global__ void k_first_in(const int32_t *in_off, const intE *in_adj, int64_t n, intT *first_in) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        first_in[v] = in_off[v] < in_off[v + 1] ? in_adj[in_off[v]] : -1;
+}
+
 inline const Bins &bins_of(const pg_view &v) {
     auto it = bins_cache().find(v.in_off);
     if (it != bins_cache().end()) return it->second;
@@ -1279,6 +1338,9 @@ inline const Bins &bins_of(const pg_view &v) {
     toDevice(cnt, pos, 8);
     hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(v.n, kT, 4096)), dim3(kT), 0, lib_stream(),
                        v.in_off, (int64_t)v.n, cnt, b.ids);
+    b.first_in = dev_alloc<intT>((size_t)v.n + 1);
+    hipLaunchKernelGGL(k_first_in, dim3(blocks_for(v.n)), dim3(kT), 0, lib_stream(), v.in_off,
+                       v.in_adj, (int64_t)v.n, b.first_in);
     launched("k_bin_fill");
     sync();
     return bins_cache().emplace(v.in_off, b).first->second;
@@ -1311,48 +1373,97 @@ struct FrontierBits {
     __device__ bool operator[](intT s) const { return (w[s >> 5] >> (s & 31)) & 1u; }
 };
 
-// w[k] = the flags b[32k .. 32k+31]: a wave's ballot over 64 flags is two words
+// w[k] = the flags b[32k .. 32k+31], a thread per word (two 16-flag loads)
 __global__ void k_flag_bits(const bool *b, int64_t n, uint32_t *w) {
     const int64_t nw = (n + 31) / 32;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t v0 = wave * 64; v0 < n; v0 += nwaves * 64) {
-        const int64_t v = v0 + lane();
-        const unsigned long long bal = __ballot(v < n && b[v]);
-        const int64_t k = v0 / 32;
-        if (lane() == 0) w[k] = (uint32_t)bal;
-        if (lane() == 1 && k + 1 < nw) w[k + 1] = (uint32_t)(bal >> 32);
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nw;
+         k += (int64_t)gridDim.x * blockDim.x)
+        w[k] = flag_mask16(b, 32 * k, n) | (flag_mask16(b, 32 * k + 16, n) << 16);
+}
+
+template <bool W, class F>
+__device__ __forceinline__ bool call_update(F &f, intT s, intT d, const intE *w, int64_t e) {
+    if constexpr (W)
+        return f.update(s, d, w[e]);
+    else
+        return f.update(s, d);
+}
+
+// Dense pull (edgeMapDense, polymer.h:924-968) in id order.  A lane owns a
+// destination d with in-edges and cond(d): it tries d's first in-neighbour (from
+// first_in, read like cond's and the offsets' loads, coalesced), then, while cond(d)
+// holds, the rest of d's in-list in list order, four ids per step (a step's loads
+// issued together; the scan stops at the first step after cond(d) turns false),
+// when at most kLaneRest remain; a longer rest is marked in qbits (each wave writes its two
+// words whole: no atomics, nothing to clear) for k_pull_queue.  The owning lane calls
+// update (the reference's non-atomic form: no other lane touches d here) and stops
+// as soon as cond(d) is false, so a functor whose first success ends the scan (the
+// BFS claim) updates d from the first active in-neighbour in list order.
+constexpr int kLaneRest = 32;
+
+template <bool W, class F>
+__global__ __launch_bounds__(kT) void k_pull_first(F f, pg_view v, const intT *first_in,
+                                                   FrontierBits cur, bool *next, uint32_t *qbits) {
+    const int64_t n = v.n;
+    const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
+    const int64_t stride = (int64_t)gridDim.x * kT;
+    for (int64_t base = (int64_t)blockIdx.x * kT; base < n; base += stride) {  // block-uniform
+        const int64_t d64 = base + threadIdx.x;
+        const intT d = (intT)d64;
+        const intT s0 = d64 < n ? first_in[d64] : -1;
+        bool queued = false;
+        if (s0 >= 0 && f.cond(d)) {
+            const int beg = v.in_off[d], end = v.in_off[d + 1];
+            bool hit = cur[s0] && call_update<W>(f, s0, d, w, beg);
+            if (end - beg > 1 && f.cond(d)) {
+                if (end - beg - 1 > kLaneRest) {
+                    queued = true;
+                } else {
+                    bool go = true;
+                    for (int e0 = beg + 1; go && e0 < end; e0 += 4) {
+                        intT sv[4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) sv[k] = e0 + k < end ? v.in_adj[e0 + k] : -1;
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            if (go && sv[k] >= 0 && cur[sv[k]]) {
+                                if (call_update<W>(f, sv[k], d, w, e0 + k)) hit = true;
+                                if (!f.cond(d)) go = false;
+                            }
+                    }
+                }
+            }
+            if (hit) next[d] = true;
+        }
+        const unsigned long long qm = __ballot(queued);
+        const int64_t wd = (base + (threadIdx.x & ~63)) >> 5;  // this wave's first word
+        const int64_t nq = (n + 31) / 32;
+        if (lane() == 0 && wd < nq) qbits[wd] = (uint32_t)qm;
+        if (lane() == 1 && wd + 1 < nq) qbits[wd + 1] = (uint32_t)(qm >> 32);
     }
 }
 
-// G lanes per destination (G = 1, 16 or 64) over ids[0, count).  Within a group's
-// chunk of G in-neighbours the first active one (in list order) is applied
-// alone; if cond(d) still holds the chunk's other active ones are applied
-// together, and the scan stops as soon as cond(d) turns false.  A functor whose
-// first success ends the scan (the BFS claim) thus makes one update per
-// destination, on the in-neighbour the reference's sequential scan would take
-// first within the chunk; one that keeps cond true (a minimum, a sum) applies
-// every active in-neighbour.
-template <bool W, int G, class F>
-__global__ __launch_bounds__(kT) void k_pull_group(F f, pg_view v, FrontierBits cur, bool *next,
-                                                   const intT *ids, int64_t count) {
-    const int64_t slot = ((int64_t)blockIdx.x * kT + threadIdx.x) / G;
-    if (slot >= count) return;  // uniform within the group
-    const int l = lane() & (G - 1);
-    const intT d = ids[slot];
-    if (!group_any<G>(f.cond(d))) return;
+// The destinations k_pull_first marked, a wave each (a wave takes the marks of 64
+// destinations at a time), over their in-lists from the second entry, 64 entries
+// per step in list order: the first active entry of a step is applied alone, then,
+// if cond(d) still holds, the step's other active entries together; the scan stops
+// once cond(d) is false.  Lanes of a wave may update d at once here, so this part
+// calls updateAtomic.
+template <bool W, class F>
+__global__ __launch_bounds__(kT) void k_pull_queue(F f, pg_view v, FrontierBits cur, bool *next,
+                                                   const uint32_t *qbits) {
+    const int64_t ngroups = (v.n + 63) / 64, nq = (v.n + 31) / 32;
+    const int l = lane();
     const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
-    const int beg = v.in_off[d], end = v.in_off[d + 1];
-    bool hit = false;
-    if constexpr (G == 1) {
-        for (int e = beg; e < end; e++) {
-            const intT s = v.in_adj[e];
-            if (cur[s] && call_update_atomic<W>(f, s, d, w, e)) hit = true;
-            if (!f.cond(d)) break;
-        }
-        if (hit) next[d] = true;
-    } else {
-        for (int base = beg; base < end; base += G) {
+    const int64_t nwaves = ((int64_t)gridDim.x * kT) >> 6;
+    for (int64_t g = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6; g < ngroups; g += nwaves)
+    for (unsigned long long qm = (unsigned long long)qbits[2 * g] |
+                                 (2 * g + 1 < nq ? (unsigned long long)qbits[2 * g + 1] << 32 : 0ull);
+         qm; qm &= qm - 1ull) {
+        const intT d = (intT)(64 * g + __ffsll((long long)qm) - 1);
+        const int beg = v.in_off[d] + 1, end = v.in_off[d + 1];
+        bool hit = false;
+        for (int base = beg; base < end; base += 64) {
             const int e = base + l;
             intT s = 0;
             bool a = false;
@@ -1360,66 +1471,27 @@ __global__ __launch_bounds__(kT) void k_pull_group(F f, pg_view v, FrontierBits 
                 s = v.in_adj[e];
                 a = cur[s];
             }
-            const unsigned long long am = group_ballot<G>(a);
+            const unsigned long long am = __ballot(a);
             if (am) {
                 const int first = __ffsll((long long)am) - 1;
                 if (l == first && call_update_atomic<W>(f, s, d, w, e)) hit = true;
-                if (!group_any<G>(f.cond(d))) break;
+                if (!group_any<64>(f.cond(d))) break;
                 if (a && l != first && call_update_atomic<W>(f, s, d, w, e)) hit = true;
             }
-            if (!group_any<G>(f.cond(d))) break;
+            if (!group_any<64>(f.cond(d))) break;
         }
-        if (group_any<G>(hit) && l == 0) next[d] = true;
+        if (group_any<64>(hit) && l == 0) next[d] = true;
     }
-}
-
-// a 256-thread workgroup per destination over ids[0, count)
-template <bool W, class F>
-__global__ __launch_bounds__(kT) void k_pull_block(F f, pg_view v, FrontierBits cur, bool *next,
-                                                   const intT *ids) {
-    __shared__ int s_stop, s_hit;
-    const intT d = ids[blockIdx.x];
-    if (threadIdx.x == 0) {
-        s_stop = f.cond(d) ? 0 : 1;
-        s_hit = 0;
-    }
-    __syncthreads();
-    if (s_stop) return;  // uniform over the block
-    const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
-    const int beg = v.in_off[d], end = v.in_off[d + 1];
-    for (int base = beg; base < end; base += kT) {
-        const int e = base + (int)threadIdx.x;
-        if (e < end) {
-            const intT s = v.in_adj[e];
-            if (cur[s] && call_update_atomic<W>(f, s, d, w, e)) s_hit = 1;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0 && !f.cond(d)) s_stop = 1;
-        __syncthreads();
-        if (s_stop) break;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && s_hit) next[d] = true;
 }
 
 template <bool W, class F>
 inline void pull_dense(F &f, const pg_view &v, FrontierBits cur, bool *next) {
     const Bins &b = bins_of(v);
-    const int64_t nblk = b.count[3], nwave = b.count[2], n16 = b.count[1], n1 = b.count[0];
-    // heavy first: they set the tail of the launch sequence
-    if (nblk > 0)
-        hipLaunchKernelGGL((k_pull_block<W, F>), dim3((unsigned)nblk), dim3(kT), 0, lib_stream(), f, v,
-                           cur, next, (const intT *)(b.ids + b.start[3]));
-    if (nwave > 0)
-        hipLaunchKernelGGL((k_pull_group<W, 64, F>), dim3(blocks_for(nwave * 64, kT, 1ll << 31)),
-                           dim3(kT), 0, lib_stream(), f, v, cur, next,
-                           (const intT *)(b.ids + b.start[2]), nwave);
-    if (n16 > 0)
-        hipLaunchKernelGGL((k_pull_group<W, 16, F>), dim3(blocks_for(n16 * 16, kT, 1ll << 31)), dim3(kT),
-                           0, lib_stream(), f, v, cur, next, (const intT *)(b.ids + b.start[1]), n16);
-    if (n1 > 0)
-        hipLaunchKernelGGL((k_pull_group<W, 1, F>), dim3(blocks_for(n1, kT, 1ll << 31)), dim3(kT), 0,
-                           lib_stream(), f, v, cur, next, (const intT *)(b.ids + b.start[0]), n1);
+    uint32_t *qbits = workspace_t<uint32_t>("pull_marks", (size_t)(v.n + 31) / 32 + 1);
+    hipLaunchKernelGGL((k_pull_first<W, F>), dim3(blocks_for(v.n)), dim3(kT), 0, lib_stream(), f, v,
+                       (const intT *)b.first_in, cur, next, qbits);
+    hipLaunchKernelGGL((k_pull_queue<W, F>), dim3(blocks_for((v.n + 63) / 64 * 64, kT, 8192)), dim3(kT),
+                       0, lib_stream(), f, v, cur, next, (const uint32_t *)qbits);
     launched("dense pull");
 }
 
@@ -1549,11 +1621,27 @@ inline void reduce_dense(F &f, const pg_view &v, FrontierBits cur, bool *next) {
     launched("dense reduce");
 }
 
+// A wave's chunk of 1024 flags is loaded 16 per lane (lane l holds flags 16l ..
+// 16l + 15 as a mask) and visited lane-consecutive: in step r lane l takes vertex
+// 64r + l, its bit fetched from lane 4r + l/16, so the program's arrays indexed by
+// the vertex are read and written coalesced.
+__device__ __forceinline__ bool chunk_bit(uint32_t mk, int r) {
+    const int l = lane();
+    return (__shfl((int)mk, 4 * r + (l >> 4), 64) >> (l & 15)) & 1;
+}
+
 template <class F>
 __global__ void k_vertex_map_dense(F f, const bool *b, int64_t n, intT offset) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        if (b[i]) f((intT)i + offset);
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c * 1024 < n; c += nw) {
+        const int64_t base = c * 1024;  // wave-uniform
+        const int64_t mine = base + 16 * lane();
+        const uint32_t mk = mine < n ? flag_mask16(b, mine, n) : 0u;
+        if (!__ballot(mk != 0u)) continue;
+#pragma unroll 4
+        for (int r = 0; r < 16; r++)
+            if (chunk_bit(mk, r)) f((intT)(base + 64 * r + lane()) + offset);
+    }
 }
 
 template <class F>
@@ -1581,23 +1669,30 @@ __global__ __launch_bounds__(kT) void k_vertex_filter(F f, const bool *b, int64_
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < kT / 64; w++) c += sc[w];
-        if (c) atomicAdd(count, c);
+        if (c) atomicAdd(&count[blockIdx.x & 15], c);  // 16 slots, summed by the caller
     }
 }
 
 // |F| and the out-degree sum of a flag frontier: reduced per block, two atomics
-// per block (a grid of at most 2048 blocks)
+// per block (a grid of at most 2048 blocks) into one of 16 slot pairs
 __global__ __launch_bounds__(kT) void k_counter(const bool *b, int64_t n, intT offset,
                                                 const int32_t *out_off, unsigned long long *out) {
     __shared__ unsigned long long sc[kT / 64], se[kT / 64];
     unsigned long long c = 0, e = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        if (b[i]) {
-            const int64_t v = i + offset;
-            c++;
-            e += (unsigned long long)(out_off[v + 1] - out_off[v]);
-        }
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t ch = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; ch * 1024 < n; ch += nw) {
+        const int64_t base = ch * 1024;  // wave-uniform
+        const int64_t mine = base + 16 * lane();
+        const uint32_t mk = mine < n ? flag_mask16(b, mine, n) : 0u;
+        c += (unsigned long long)__popc(mk);
+        if (!__ballot(mk != 0u)) continue;
+#pragma unroll 4
+        for (int r = 0; r < 16; r++)
+            if (chunk_bit(mk, r)) {
+                const int64_t v = base + 64 * r + lane() + offset;
+                e += (unsigned long long)(out_off[v + 1] - out_off[v]);
+            }
+    }
     for (int o = 32; o > 0; o >>= 1) {
         c += __shfl_xor(c, o, 64);
         e += __shfl_xor(e, o, 64);
@@ -1612,8 +1707,11 @@ __global__ __launch_bounds__(kT) void k_counter(const bool *b, int64_t n, intT o
             c += sc[w];
             e += se[w];
         }
-        if (c) atomicAdd(&out[0], c);
-        if (e) atomicAdd(&out[1], e);
+        // 16 slot pairs (count, out-degree sum), summed by the caller: a pair shared
+        // by every block of the grid serialises its atomics
+        unsigned long long *slot = out + 2 * (blockIdx.x & 15);
+        if (c) atomicAdd(&slot[0], c);
+        if (e) atomicAdd(&slot[1], e);
     }
 }
 
@@ -1623,10 +1721,19 @@ __global__ void k_set_list(bool *b, const intT *s, int64_t m, intT offset) {
         b[s[i] - offset] = true;
 }
 
+// b[0, n) = v: 16-byte stores over the aligned body, byte stores for the rest
 __global__ void k_fill_bytes(bool *b, int64_t n, bool v) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = v;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nv = ((((uintptr_t)b) & 15u) == 0u) ? n / 16 : 0;
+    const uint32_t w = v ? 0x01010101u : 0u;
+    for (int64_t i = gt; i < nv; i += gs) reinterpret_cast<uint4 *>(b)[i] = make_uint4(w, w, w, w);
+    for (int64_t i = 16 * nv + gt; i < n; i += gs) b[i] = v;
+}
+
+// grid of the 16-flags-per-thread kernels over n flags
+inline unsigned blocks_for_flags(int64_t n, int64_t cap = 65536) {
+    return blocks_for((n + 15) / 16, kT, cap);
 }
 
 }  // namespace polymer_detail
@@ -1669,13 +1776,14 @@ struct LocalFrontier {
     bool isDense;
     size_t sCap = 0;        // capacity of s when the header allocated it
     bool sOwned = false;    // s was allocated here
-    unsigned long long *counters = nullptr;  // device: [0] count, [1..32] out-degree sums
+    unsigned long long *counters = nullptr;  // device: the count and out-degree sum slots of calls on it
 
     LocalFrontier(bool *_b, intT start, intT end)
         : n(end - start), m(0), outEdgesCount(0), startID(start), endID(end), b(_b), s(nullptr),
           isDense(true) {
         polymer_detail::ensure_init();
         counters = polymer_detail::dev_alloc<unsigned long long>(40);
+        reserveSparse((size_t)n + 1);  // made here, before a program's loop, not in a round
     }
     ~LocalFrontier() {
         if (s && sOwned) polymer_detail::shared_free(s);
@@ -1688,12 +1796,18 @@ struct LocalFrontier {
     PG_HD inline void setBit(intT index, bool val) { b[index - startID] = val; }
     PG_HD inline bool getBit(intT index) const { return b[index - startID]; }
 
-    // a list of at least cap entries owned by the frontier (kept across calls)
+    // a list of at least cap entries owned by the frontier, kept across calls: at
+    // least n + 1 entries (every frontier fits), doubled when a call needs more, so
+    // a loop of rounds allocates once (the reference mallocs and frees one per call)
     void reserveSparse(size_t cap) {
         if (s && sOwned && cap <= sCap) return;
-        if (s && sOwned) polymer_detail::shared_free(s);
-        s = static_cast<intT *>(polymer_detail::shared_alloc(sizeof(intT) * (cap ? cap : 1)));
-        sCap = cap;
+        const size_t grown = std::max(std::max(cap, (size_t)n + 1), s && sOwned ? 2 * sCap : 0);
+        if (s && sOwned) {
+            polymer_detail::sync();  // kernels queued before may still read the old list
+            polymer_detail::shared_free(s);
+        }
+        s = static_cast<intT *>(polymer_detail::shared_alloc(sizeof(intT) * grown, false));
+        sCap = grown;
         sOwned = true;
     }
 
@@ -1715,7 +1829,7 @@ struct LocalFrontier {
     void toDense() {
         if (!isDense) {
             using namespace polymer_detail;
-            hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for(n)), dim3(256), 0, lib_stream(), b,
+            hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for_flags(n)), dim3(256), 0, lib_stream(), b,
                                (int64_t)n, false);
             if (m > 0) {
                 const intT *sl = device_readable(s, (size_t)m, "todense_list");
@@ -1723,7 +1837,7 @@ struct LocalFrontier {
                                    (int64_t)m, startID);
             }
             launched("toDense");
-            sync();
+            sync_visible();
         }
         isDense = true;
     }
@@ -1869,7 +1983,7 @@ inline const bool *cur_flags(vertices *frontier) { return frontier->getArr(0); }
 inline FrontierBits cur_bits(vertices *frontier, int64_t n) {
     const int64_t nw = (n + 31) / 32;
     uint32_t *w = workspace_t<uint32_t>("frontier_bits", (size_t)nw + 1);
-    hipLaunchKernelGGL(k_flag_bits, dim3(blocks_for(nw * 32, kT)), dim3(kT), 0, lib_stream(),
+    hipLaunchKernelGGL(k_flag_bits, dim3(blocks_for(nw, kT)), dim3(kT), 0, lib_stream(),
                        cur_flags(frontier), n, w);
     launched("frontier bitmap");
     return FrontierBits{w};
@@ -1889,7 +2003,7 @@ void dense_push(G &GA, vertices *frontier, F &f, LocalFrontier *next) {
     intT *list = nullptr;
     const intT count = list_active(frontier, &list);
     push_list<W, Mode>(f, GA.view, list, count, next->b, nullptr, 0, next->counters);
-    sync();
+    sync_visible();
 }
 
 }  // namespace polymer_detail
@@ -1931,7 +2045,7 @@ bool *edgeMapDense(G GA, vertices *frontier, F f, LocalFrontier *next, bool para
     constexpr bool W = polymer_detail::is_weighted<G>::value;
     frontier->nextFrontiers[0] = next;
     polymer_detail::pull_dense<W>(f, GA.view, polymer_detail::cur_bits(frontier, GA.n), next->b);
-    polymer_detail::sync();
+    polymer_detail::sync_visible();
     return nullptr;
 }
 
@@ -1950,7 +2064,7 @@ bool *edgeMapDenseReduce(G GA, vertices *frontier, F f, LocalFrontier *next, boo
     constexpr bool W = is_weighted<G>::value;
     frontier->nextFrontiers[0] = next;
     reduce_dense<W>(f, GA.view, cur_bits(frontier, GA.n), next->b);
-    sync();
+    sync_visible();
     return nullptr;
 }
 
@@ -2083,16 +2197,20 @@ void vertexCounter(G &GA, LocalFrontier *frontier, int nodeNum, int subNum, int 
     using namespace polymer_detail;
     (void)totalSub;
     if (!frontier->isDense || nodeNum != 0 || subNum != 0) return;
-    check_hip(hipMemsetAsync(frontier->counters, 0, 2 * sizeof(unsigned long long), lib_stream()),
+    check_hip(hipMemsetAsync(frontier->counters, 0, 32 * sizeof(unsigned long long), lib_stream()),
               "hipMemsetAsync");
-    hipLaunchKernelGGL(k_counter, dim3(blocks_for(frontier->n, 256, 2048)), dim3(256), 0, lib_stream(),
+    hipLaunchKernelGGL(k_counter, dim3(blocks_for_flags(frontier->n, 2048)), dim3(256), 0, lib_stream(),
                        (const bool *)frontier->b, (int64_t)frontier->n, frontier->startID,
                        GA.view.out_off, frontier->counters);
     launched("vertexCounter");
-    unsigned long long c[2];
-    toHost(c, frontier->counters, 2);
-    frontier->m = (intT)c[0];
-    frontier->outEdgesCount = (intT)c[1];
+    unsigned long long c[32], cnt = 0, e = 0;
+    toHost(c, frontier->counters, 32);
+    for (int i = 0; i < 16; i++) {
+        cnt += c[2 * i];
+        e += c[2 * i + 1];
+    }
+    frontier->m = (intT)cnt;
+    frontier->outEdgesCount = (intT)e;
 }
 
 // vertexMap (polymer.h:2340-2381): f(i) for every active vertex
@@ -2103,7 +2221,7 @@ void vertexMap(vertices *V, F add, int nodeNum, int subNum = 0, int totalSub = 1
     if (subNum != 0) return;
     LocalFrontier *fr = V->getFrontier(nodeNum);
     if (V->isDense) {
-        hipLaunchKernelGGL(k_vertex_map_dense<F>, dim3(blocks_for(fr->n)), dim3(256), 0, lib_stream(),
+        hipLaunchKernelGGL(k_vertex_map_dense<F>, dim3(blocks_for_flags(fr->n)), dim3(256), 0, lib_stream(),
                            add, (const bool *)fr->b, (int64_t)fr->n, V->getOffset(nodeNum));
     } else if (fr->m > 0) {
         const intT *sl = device_readable(fr->s, (size_t)fr->m, "vmap_list");
@@ -2111,7 +2229,7 @@ void vertexMap(vertices *V, F add, int nodeNum, int subNum = 0, int totalSub = 1
                            add, sl, (int64_t)fr->m);
     }
     launched("vertexMap");
-    sync();
+    sync_visible();
 }
 
 // clearLocalFrontier (polymer.h:2383-2397)
@@ -2119,20 +2237,20 @@ inline void clearLocalFrontier(LocalFrontier *next, int nodeNum, int subNum, int
     using namespace polymer_detail;
     (void)nodeNum; (void)totalSub;
     if (subNum != 0) return;
-    hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for(next->n)), dim3(256), 0, lib_stream(), next->b,
+    hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for_flags(next->n)), dim3(256), 0, lib_stream(), next->b,
                        (int64_t)next->n, false);
     launched("clearLocalFrontier");
-    sync();
+    sync_visible();
 }
 
 // Every vertex of the frontier's range active (the apps' "all true" start,
 // e.g. numa-PageRank.C:427-429); m and outEdgesCount come from vertexCounter.
 inline void fillLocalFrontier(LocalFrontier *f) {
     using namespace polymer_detail;
-    hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for(f->n)), dim3(256), 0, lib_stream(), f->b,
+    hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for_flags(f->n)), dim3(256), 0, lib_stream(), f->b,
                        (int64_t)f->n, true);
     launched("fillLocalFrontier");
-    sync();
+    sync_visible();
     f->isDense = true;
     f->m = f->n;
 }
@@ -2144,13 +2262,13 @@ void vertexFilter(vertices *V, F filter, int nodeNum, bool *result) {
     using namespace polymer_detail;
     V->toDense();
     LocalFrontier *fr = V->getFrontier(nodeNum);
-    unsigned long long *cnt = workspace_t<unsigned long long>("filter_cnt", 1);
-    check_hip(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), lib_stream()), "hipMemsetAsync");
+    unsigned long long *cnt = workspace_t<unsigned long long>("filter_cnt", 16);
+    check_hip(hipMemsetAsync(cnt, 0, 16 * sizeof(unsigned long long), lib_stream()), "hipMemsetAsync");
     hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n, 256, 2048)), dim3(256), 0, lib_stream(), filter,
                        (const bool *)fr->b, (int64_t)0, (int64_t)fr->n, V->getOffset(nodeNum), result,
                        cnt);
     launched("vertexFilter");
-    sync();
+    sync_visible();
 }
 
 // vertexFilter (polymer.h:2411-2441): the same into a frontier; the count of
@@ -2162,15 +2280,16 @@ void vertexFilter(vertices *V, F filter, int nodeNum, int subNum, int totalSub, 
     if (subNum != 0) return;
     V->toDense();
     LocalFrontier *fr = V->getFrontier(nodeNum);
-    check_hip(hipMemsetAsync(result->counters, 0, sizeof(unsigned long long), lib_stream()),
+    check_hip(hipMemsetAsync(result->counters, 0, 16 * sizeof(unsigned long long), lib_stream()),
               "hipMemsetAsync");
     hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n, 256, 2048)), dim3(256), 0, lib_stream(), filter,
                        (const bool *)fr->b, (int64_t)0, (int64_t)fr->n, V->getOffset(nodeNum), result->b,
                        result->counters);
     launched("vertexFilter");
-    unsigned long long c = 0;
-    toHost(&c, result->counters, 1);
-    result->m += (intT)c;
+    unsigned long long c[16], tot = 0;
+    toHost(c, result->counters, 16);
+    for (int i = 0; i < 16; i++) tot += c[i];
+    result->m += (intT)tot;
     result->isDense = true;
 }
 
