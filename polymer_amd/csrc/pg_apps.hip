@@ -174,6 +174,12 @@ struct PrDeltaOp {
     __device__ static void stat_commit(Stat *p, Stat v) {
         atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
     }
+    // tiled pull: the fp64 contributions gathered in source order within a tile and
+    // added in LDS as doubles (the cold ones staged as doubles by the pre-pass)
+    static constexpr bool kTileWeighted = false;
+    __device__ double tile_raw(int s) const { return c[s]; }
+    __device__ double tile_fix(double x) const { return x; }
+    __device__ double tile_sum(double q) const { return q; }
 };
 
 // a push round: the listed sources' contributions added into nghSum (the
@@ -1547,6 +1553,20 @@ extern "C" int pg_pagerank_delta(pg_graph *g, int iters, double damping, double 
     PG_TRY(workspace_t(g, "prd_stats", ((size_t)iters + 1) * kStatSlots, &stats));
     Frontier A, B;
     PG_TRY(frontiers(g, &A, &B));
+    // pull rounds on the tiles when the graph's schedule has them: the tile set with
+    // a cold value array of doubles
+    const bool tiled = g->use_tiles;
+    TileSet tset{};
+    if (tiled) {
+        PG_TRY(ensure_tiles(g));
+        tset = g->tiles;
+        if (tset.m_cold > 0) {
+            double *cold64 = nullptr;
+            PG_TRY(workspace_t(g, "prd_cold64", (size_t)tset.m_cold + 16, &cold64));
+            tset.args.cold_val64 = cold64;
+            tset.cold.val64 = cold64;
+        }
+    }
     KernelTimer kt;
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
     Timer tm;
@@ -1575,7 +1595,10 @@ extern "C" int pg_pagerank_delta(pg_graph *g, int iters, double damping, double 
             op.cn = cn;
             op.p = p;
             op.first = t == 0 ? 1 : 0;
-            PG_TRY(launch_pull(op, g->in_sched, st_t));
+            if (tiled)
+                PG_TRY(launch_pull_tiled(op, g->in_sched, tset, st_t));
+            else
+                PG_TRY(launch_pull(op, g->in_sched, st_t));
         } else {
             // the sources with a nonzero contribution, as an ordered list
             PG_TRY(clear_frontier(g, A, false));

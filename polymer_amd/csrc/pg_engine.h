@@ -421,18 +421,30 @@ inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *s
 // workgroup streams its tile's (source, destination) pairs, sorted by source, and
 // adds the gathered values into per-destination accumulators in LDS.  An Op used
 // here also provides
-//     float tile_raw(int s);                      // the value gathered from source s
-//     unsigned long long tile_fix(float x);       // an edge's term as fixed point
-//     Acc tile_sum(unsigned long long q);         // an accumulated fixed-point sum
+//     V tile_raw(int s);                          // the value gathered from source s
+//     T tile_fix(V x);                            // an edge's term
+//     Acc tile_sum(T q);                          // an accumulated sum
 //     static constexpr bool kTileWeighted;        // term = tile_raw(s) * w(edge)
-// (weights from TileArgs::wt, the graph's edge weights in tile order).
-// Integer addition is associative, so the result does not depend on the order in
-// which the LDS atomics land.  The per-vertex protocol (active / finish / skip,
-// statistics) is the one k_pull follows.
-__device__ __forceinline__ unsigned ld_stream_u16(const uint16_t *p) {
-    return __builtin_nontemporal_load(p);
+// (weights from TileArgs::wt, the graph's edge weights in tile order) with V = float
+// and T = unsigned long long (fixed point: integer addition is associative, so the
+// result does not depend on the order in which the LDS atomics land), or V = T =
+// double (fp64 LDS adds, rounded in arrival order like any parallel fp64 sum; the
+// cold values then travel as doubles, TileArgs::cold_val64).  The per-vertex
+// protocol (active / finish / skip, statistics) is the one k_pull follows.
+template <class Op>
+using tile_val_t = decltype(static_cast<const Op *>(nullptr)->tile_raw(0));
+template <class Op>
+using tile_acc_t = decltype(static_cast<const Op *>(nullptr)->tile_fix(tile_val_t<Op>{}));
+
+__device__ __forceinline__ void lds_add(unsigned long long *p, unsigned long long v) { atomicAdd(p, v); }
+__device__ __forceinline__ void lds_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+template <class T>
+__device__ __forceinline__ const T *cold_values(const TileArgs &ta) {
+    if constexpr (sizeof(T) == 8) return static_cast<const T *>(ta.cold_val64);
+    else return ta.cold_val;
 }
-__device__ __forceinline__ unsigned long long ld_stream_u64(const unsigned long long *p) {
+__device__ __forceinline__ unsigned ld_stream_u16(const uint16_t *p) {
     return __builtin_nontemporal_load(p);
 }
 
@@ -460,7 +472,7 @@ __device__ __forceinline__ float tile_weight(const Op &, const float *wt, int64_
     else return 1.f;
 }
 template <class Op>
-__device__ __forceinline__ unsigned long long tile_term(const Op &op, float x, float wv) {
+__device__ __forceinline__ tile_acc_t<Op> tile_term(const Op &op, tile_val_t<Op> x, float wv) {
     if constexpr (Op::kTileWeighted) return op.tile_fix(x * wv);
     else return op.tile_fix(x);
 }
@@ -469,13 +481,16 @@ template <class Op>
 __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                                                              typename Op::Stat *stat_out) {
     using Stat = typename Op::Stat;
-    __shared__ unsigned long long acc[kTileMaxV];
+    using Val = tile_val_t<Op>;
+    using T = tile_acc_t<Op>;
+    static_assert(sizeof(T) == 8, "8-byte accumulators (the partial slots)");
+    __shared__ T acc[kTileMaxV];
     __shared__ Stat ssm[kTileThreads / 64];
     const TilePart part = ta.parts[blockIdx.x];
     const int t = part.tile;
     const int vb = ta.vbeg[t];
     const int nv = ta.vbeg[t + 1] - vb;  // <= kTileMaxV by construction
-    for (int i = threadIdx.x; i < nv; i += kTileThreads) acc[i] = 0ull;
+    for (int i = threadIdx.x; i < nv; i += kTileThreads) acc[i] = T(0);
     __syncthreads();
     constexpr int U = 4;  // measured on MI355X: 4 edges in flight per lane beat 2, 3, 8, 16
     constexpr unsigned kDstMask = (1u << kPackDstBits) - 1u;
@@ -520,7 +535,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                             s[k] = ta.esc[(int64_t)(-b[k] - 1) * kPackGroup + (i % kPackGroup)];
                     }
                 }
-                float x[U];
+                Val x[U];
 #pragma unroll
                 for (int k = 0; k < U; k++) x[k] = op.tile_raw(s[k]);
                 const int en = e + U * kTileThreads;
@@ -534,11 +549,11 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                     bn[k] = ta.gbase[i / kPackGroup];
                     wvn[k] = tile_weight(op, ta.wt, i);
                 }
-                unsigned long long q[U];
+                T q[U];
 #pragma unroll
                 for (int k = 0; k < U; k++) q[k] = tile_term(op, x[k], wv[k]);
 #pragma unroll
-                for (int k = 0; k < U; k++) atomicAdd(&acc[w[k] & kDstMask], q[k]);
+                for (int k = 0; k < U; k++) lds_add(&acc[w[k] & kDstMask], q[k]);
                 e = en;
                 if (!(e + (U - 1) * kTileThreads < cnt)) break;
 #pragma unroll
@@ -555,16 +570,17 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
             const int b = ta.gbase[i / kPackGroup];
             const int s = b >= 0 ? b + (int)(w >> kPackDstBits)
                                  : ta.esc[(int64_t)(-b - 1) * kPackGroup + (i % kPackGroup)];
-            atomicAdd(&acc[w & kDstMask], tile_term(op, op.tile_raw(s), tile_weight(op, ta.wt, i)));
+            lds_add(&acc[w & kDstMask], tile_term(op, op.tile_raw(s), tile_weight(op, ta.wt, i)));
         }
     }
     {  // cold sources: values streamed from the pre-pass
-        const float *__restrict__ val = ta.cold_val + part.cold_v;
+        const Val *__restrict__ val = cold_values<Val>(ta) + part.cold_v;
         const uint32_t *__restrict__ pk = ta.packed + part.cold_e;
         const int cnt = (int)(part.e1 - part.cold_e);
         int e = threadIdx.x;
         for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
-            float x[U], wv[U];
+            Val x[U];
+            float wv[U];
             unsigned d[U];
 #pragma unroll
             for (int k = 0; k < U; k++) {
@@ -573,17 +589,17 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                 wv[k] = tile_weight(op, ta.wt, part.cold_e + e + k * kTileThreads);
             }
 #pragma unroll
-            for (int k = 0; k < U; k++) atomicAdd(&acc[d[k]], tile_term(op, x[k], wv[k]));
+            for (int k = 0; k < U; k++) lds_add(&acc[d[k]], tile_term(op, x[k], wv[k]));
         }
         for (; e < cnt; e += kTileThreads)
-            atomicAdd(&acc[__builtin_nontemporal_load(pk + e) & kDstMask],
-                      tile_term(op, __builtin_nontemporal_load(val + e),
-                                tile_weight(op, ta.wt, part.cold_e + e)));
+            lds_add(&acc[__builtin_nontemporal_load(pk + e) & kDstMask],
+                    tile_term(op, __builtin_nontemporal_load(val + e),
+                              tile_weight(op, ta.wt, part.cold_e + e)));
     }
     __syncthreads();
     if (part.slot >= 0) {
         // one of several parts: store the partial sums; k_tile_finish adds them up
-        unsigned long long *p = ta.partial + part.slot;
+        T *p = reinterpret_cast<T *>(ta.partial) + part.slot;
         for (int i = threadIdx.x; i < nv; i += kTileThreads) p[i] = acc[i];
         return;  // uniform over the block
     }
@@ -614,31 +630,48 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
 // Cold pre-pass: one workgroup per slice of a source chunk's list stages the
 // chunk's kColdChunk source values in LDS, then writes each listed edge's value
 // to its place in the cold value array (runs of consecutive places per tile).
+// An 8-byte value type stages the chunk in two halves of kColdChunk / 2 sources (64
+// KiB of LDS each), every half scanning the part's list for its own sources.
 template <class Op>
 __global__ __launch_bounds__(kTileThreads) void k_cold_gather(Op op, ColdArgs ca) {
-    __shared__ float sv[kColdChunk];
+    using Val = tile_val_t<Op>;
+    constexpr int kHalves = sizeof(Val) == 8 ? 2 : 1;
+    constexpr int kStage = kColdChunk / kHalves;
+    __shared__ Val sv[kStage];
+    Val *__restrict__ out = sizeof(Val) == 8 ? static_cast<Val *>(ca.val64)
+                                             : reinterpret_cast<Val *>(ca.val);
     const ColdPart cp = ca.parts[blockIdx.x];
     const int64_t base = (int64_t)ca.cold_begin + (int64_t)cp.chunk * kColdChunk;
-    const int len = (int)min((int64_t)kColdChunk, ca.n - base);
-    for (int j = threadIdx.x; j < len; j += kTileThreads) sv[j] = op.tile_raw((int)(base + j));
-    __syncthreads();
     const uint32_t *__restrict__ pos = ca.pos + cp.a;
     const uint16_t *__restrict__ srcl = ca.srcl + cp.a;
     const int cnt = (int)(cp.b - cp.a);
-    constexpr int U = 8;
-    int e = threadIdx.x;
-    for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
-        uint32_t p[U];
-        unsigned l[U];
+    for (int h = 0; h < kHalves; h++) {
+        const int64_t hb = base + (int64_t)h * kStage;
+        const int len = (int)max((int64_t)0, min((int64_t)kStage, ca.n - hb));
+        if (h > 0) __syncthreads();  // the previous half's readers are done
+        for (int j = threadIdx.x; j < len; j += kTileThreads) sv[j] = op.tile_raw((int)(hb + j));
+        __syncthreads();
+        const unsigned lo = (unsigned)(h * kStage);
+        constexpr int U = 8;
+        int e = threadIdx.x;
+        for (; e + (U - 1) * kTileThreads < cnt; e += U * kTileThreads) {
+            uint32_t p[U];
+            unsigned l[U];
 #pragma unroll
-        for (int k = 0; k < U; k++) {
-            p[k] = __builtin_nontemporal_load(pos + e + k * kTileThreads);
-            l[k] = ld_stream_u16(srcl + e + k * kTileThreads);
+            for (int k = 0; k < U; k++) {
+                p[k] = __builtin_nontemporal_load(pos + e + k * kTileThreads);
+                l[k] = ld_stream_u16(srcl + e + k * kTileThreads);
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++)
+                if (kHalves == 1 || l[k] - lo < (unsigned)kStage) out[p[k]] = sv[l[k] - lo];
         }
-#pragma unroll
-        for (int k = 0; k < U; k++) ca.val[p[k]] = sv[l[k]];
+        for (; e < cnt; e += kTileThreads) {
+            const unsigned l = ld_stream_u16(srcl + e);
+            if (kHalves == 1 || l - lo < (unsigned)kStage)
+                out[__builtin_nontemporal_load(pos + e)] = sv[l - lo];
+        }
     }
-    for (; e < cnt; e += kTileThreads) ca.val[__builtin_nontemporal_load(pos + e)] = sv[ld_stream_u16(srcl + e)];
 }
 
 // Vertices of the tiles with several parts: one thread per vertex sums the
@@ -658,17 +691,19 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
         if (!op.active(v)) {
             op.skip(v);
         } else {
-            const unsigned long long *p = ta.partial + f.slot0 + i;
-            unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull, q3 = 0ull;
+            using T = tile_acc_t<Op>;
+            const T *p = reinterpret_cast<const T *>(ta.partial) + f.slot0 + i;
+            T q0 = T(0), q1 = T(0), q2 = T(0), q3 = T(0);
             int k = 0;
             for (; k + 3 < f.nparts; k += 4) {  // four slots in flight
-                q0 += ld_stream_u64(p + (int64_t)k * nv);
-                q1 += ld_stream_u64(p + (int64_t)(k + 1) * nv);
-                q2 += ld_stream_u64(p + (int64_t)(k + 2) * nv);
-                q3 += ld_stream_u64(p + (int64_t)(k + 3) * nv);
+                q0 += __builtin_nontemporal_load(p + (int64_t)k * nv);
+                q1 += __builtin_nontemporal_load(p + (int64_t)(k + 1) * nv);
+                q2 += __builtin_nontemporal_load(p + (int64_t)(k + 2) * nv);
+                q3 += __builtin_nontemporal_load(p + (int64_t)(k + 3) * nv);
             }
-            for (; k < f.nparts; k++) q0 += ld_stream_u64(p + (int64_t)k * nv);
-            const unsigned long long q = (q0 + q1) + (q2 + q3);  // integers: any order
+            for (; k < f.nparts; k++) q0 += __builtin_nontemporal_load(p + (int64_t)k * nv);
+            // integers: any order; doubles: this fixed order for a given part count
+            const T q = (q0 + q1) + (q2 + q3);
             st = op.finish(v, op.tile_sum(q), op.ctx(v));
         }
     }

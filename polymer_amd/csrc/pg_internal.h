@@ -135,6 +135,7 @@ struct ColdArgs {
     int32_t nparts;
     int32_t cold_begin;
     int64_t n;
+    void *val64;            // the same for an Op with 8-byte values (or null)
 };
 
 struct TileFin {     // kBlock vertices of a tile with several parts
@@ -164,6 +165,7 @@ struct TileArgs {
     int32_t nparts;
     int32_t nfin;
     const float *wt;             // edge weights in tile order (weighted graphs), or null
+    const void *cold_val64;      // cold values of an Op with 8-byte values (or null)
 };
 
 struct TileSet {
