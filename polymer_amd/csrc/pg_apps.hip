@@ -141,9 +141,9 @@ __device__ __forceinline__ unsigned long long prd_filter(int v, double sum, int 
         delta = sum * damp;
         pn = pv + delta;
     }
-    p[v] = pn;
+    if (pn != pv) p[v] = pn;  // an unchanged rank is not stored again
     const bool act = fabs(delta) > eps2 * pn;
-    const int od = out_off[v + 1] - out_off[v];
+    const int od = act ? out_off[v + 1] - out_off[v] : 0;
     cn[v] = (act && od > 0) ? delta / (double)od : 0.0;
     return act ? ((1ull << 32) | (unsigned long long)od) : 0ull;
 }
@@ -192,18 +192,29 @@ struct PrDeltaPushOp {
 };
 
 // the filter after a push round (a later round): every vertex, nghSum read and
-// cleared for the next round
+// cleared for the next round (only where the push added something: a -0.0 left in
+// place adds like +0.0), and the bitmap of the next contributions' nonzero entries
+// written for the next push round (a wave writes its two words)
 __global__ __launch_bounds__(kBlock) void k_prd_filter(int64_t n, double *ngh, double *p,
                                                        double *cn, const int32_t *out_off,
                                                        double damp, double eps2,
-                                                       unsigned long long *stats) {
+                                                       unsigned long long *stats, uint32_t *bits) {
     __shared__ unsigned long long ss[kBlock / 64];
     unsigned long long st = 0;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-         v += (int64_t)gridDim.x * blockDim.x) {
-        const double sum = ngh[v];
-        ngh[v] = 0.0;
-        st += prd_filter((int)v, sum, 0, p, cn, out_off, damp, 0.0, 0.0, eps2);
+    const int lane = lane_id();
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = base + lane;
+        bool nz = false;
+        if (v < n) {
+            const double sum = ngh[v];
+            if (sum != 0.0) ngh[v] = 0.0;
+            st += prd_filter((int)v, sum, 0, p, cn, out_off, damp, 0.0, 0.0, eps2);
+            nz = cn[v] != 0.0;
+        }
+        const unsigned long long m = __ballot(nz);
+        if (lane == 0) bits[base >> 5] = (uint32_t)m;
+        if (lane == 32 && base + 32 < n) bits[(base >> 5) + 1] = (uint32_t)(m >> 32);
     }
     for (int o = 32; o >= 1; o >>= 1) st += __shfl_xor(st, o, 64);
     if (lane_id() == 0) ss[threadIdx.x >> 6] = st;
@@ -1584,12 +1595,14 @@ extern "C" int pg_pagerank_delta(pg_graph *g, int iters, double damping, double 
     // round 1 runs on every vertex; round t+1 on the frontier round t left
     int64_t cnt = n, edges = g->m, tot_active = 0, tot_edges = 0;
     int dense_rounds = 0;
+    bool bits_ready = false;  // A.bits holds the nonzero entries of c
     for (int t = 0; t < iters; t++) {
         tot_active += cnt;
         tot_edges += edges;
         unsigned long long *st_t = &stats[(size_t)t * kStatSlots];
         PG_TRY(kt.begin());
-        if (t == 0 || edges >= g->m / kPrdPushDiv) {
+        const bool pull = t == 0 || edges >= g->m / kPrdPushDiv;
+        if (pull) {
             dense_rounds++;
             op.c = c;
             op.cn = cn;
@@ -1600,17 +1613,19 @@ extern "C" int pg_pagerank_delta(pg_graph *g, int iters, double damping, double 
             else
                 PG_TRY(launch_pull(op, g->in_sched, st_t));
         } else {
-            // the sources with a nonzero contribution, as an ordered list
+            // the sources with a nonzero contribution, as an ordered list (their
+            // bitmap written by the previous push round's filter, or formed here)
             PG_TRY(clear_frontier(g, A, false));
-            PG_LAUNCH_K(k_prd_bits, grid_for(n, 4096), c, n, A.bits);
+            if (!bits_ready) PG_LAUNCH_K(k_prd_bits, grid_for(n, 4096), c, n, A.bits);
             PG_TRY(bits_to_list(g, A.bits, A.list, &A.ctr[0]));
             int64_t lc = 0, le = 0;
             PG_TRY(frontier_stats(g, A, &lc, &le));
             PG_TRY(launch_push_global(PrDeltaPushOp{c, ngh}, g, g->out_off, g->out_adj, A.list, lc,
                                       le));
             PG_LAUNCH_K(k_prd_filter, grid_for(n, 4096), n, ngh, p, cn, g->out_off, damping,
-                        epsilon2, st_t);
+                        epsilon2, st_t, A.bits);
         }
+        bits_ready = !pull;
         PG_TRY(kt.end());
         std::swap(c, cn);
         if (t + 1 < iters) {  // the next frontier's size and out-degree sum
