@@ -110,6 +110,42 @@ def test_dropin_bfs(orc, thr):
         check_parents(h, src, parents, levels)
 
 
+def odd_graph(orc, n=6001, scale=13, m=1 << 17, seed=61):
+    """An rMat graph cut to its first n vertices: n is no multiple of 16, 32 or 64, so
+    the header's frontier kernels run their partial-word tails."""
+    g = orc.rmat(scale, m, seed=seed)
+    src = np.repeat(np.arange(g.n), np.diff(g.off))
+    keep = (src < n) & (g.adj < n)
+    s, d = src[keep], g.adj[keep]
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, s + 1, 1)
+    return orc.Graph(n, np.cumsum(off), d)
+
+
+def test_dropin_odd_sizes_from_file(orc):
+    """The drivers on PBBS files of a 6001-vertex graph: PageRank against the fp64
+    oracle, BFS levels (every round dense, and direction-switching) and CC labels
+    bit-exact on its symmetrised copy."""
+    g = odd_graph(orc)
+    gs = g.symmetrized()
+    assert g.n % 16 and gs.m > 0
+    with tempfile.TemporaryDirectory() as d:
+        fa, fs = os.path.join(d, "g.adj"), os.path.join(d, "gs.adj")
+        orc.write_adj(fa, g)
+        orc.write_adj(fs, gs)
+        r = run("pagerank", ["-file", fa, "-rounds", 20], np.float64, g.n)
+        ref, _ = orc.pagerank(g, 20, 0.85)
+        say("pagerank n %d: rel L1 %.3e" % (g.n, rel_l1(r, ref)))
+        assert rel_l1(r, ref) < 1e-9
+        for src, thr in ((0, 0), (5999, -1), (1234, -1)):
+            out = run("bfs", ["-file", fs, "-s", "-r", src, "-thr", thr], np.int32, 2 * g.n)
+            parents, levels = out[:g.n], out[g.n:]
+            assert np.array_equal(levels, orc.bfs_levels(gs, src)), (src, thr)
+            check_parents(gs, src, parents, levels)
+        lab = run("components", ["-file", fs, "-s"], np.int32, g.n)
+        assert np.array_equal(lab, orc.cc_unionfind(gs))
+
+
 def test_dropin_components(orc):
     g = orc.rmat(14, 1 << 18, seed=41)
     gs = g.symmetrized()
