@@ -1775,45 +1775,18 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
 }
 
 // ---- SpMV adjoint -----------------------------------------------------------------------
-// Backward step t of L = g . y_T (pull over the CSR: the transposed graph):
-//     mu_t[s] = sum_{e=(s->d)} w[e] mu_{t+1}[d]     (stored scaled by 2^-shift)
-//     grad[e] (+)= y_t[s] mu_{t+1}[d] 2^(E_t + F_{t+1})  (fp64; the first step stores)
-// with y_t = ys (stored as pg_spmv keeps it, true value ys 2^E_t) and mu_{t+1} stored
-// scaled by 2^-F_{t+1}; *fac = 2^(E_t + F_{t+1}).
-struct SpmvAdjOp {
-    using Acc = float;
-    using Ctx = double;  // y_t[s] * 2^(E_t + F_{t+1})
-    using Stat = unsigned;  // bits of max |mu_t| as stored
-    static constexpr bool kEarlyExit = false;
-    static constexpr bool kStats = true;
-    const float *mu;   // mu_{t+1}, stored
-    const float *w;    // CSR order
-    const float *ys;   // y_t, stored
-    float *mun;        // mu_t, stored
-    double *grad;      // CSR order
-    const double *fac;
-    const int32_t *shift;
-    int first;
-    __device__ bool active(int) const { return true; }
-    __device__ double ctx(int s) const { return (double)ys[s] * *fac; }
-    __device__ float identity() const { return 0.f; }
-    __device__ float combine(float a, float b) const { return a + b; }
-    __device__ float edge(int d, int e, const double &yf) const {
-        const float md = mu[d];
-        const double term = yf * (double)md;
-        grad[e] = first ? term : grad[e] + term;
-        return w[e] * md;
-    }
-    __device__ bool found(float, const double &) const { return false; }
-    __device__ Stat finish(int s, float sum, const double &) const {
-        const float r = ldexpf(sum, -*shift);
-        mun[s] = r;
-        return __float_as_uint(fabsf(r));
-    }
-    __device__ void skip(int) const {}
-    __device__ static Stat stat_reduce(Stat a, Stat b) { return a > b ? a : b; }
-    __device__ static void stat_commit(Stat *p, Stat v) { atomicMax(p, v); }
-};
+// Backward step t of L = g . y_T is the pull over the CSR (the transposed graph):
+//     mu_t[s] = sum_{e=(s->d)} w[e] mu_{t+1}[d]
+// run on the tiles of the CSR (every vertex a destination of its out-list, the
+// out-neighbours sorted within a tile: the in-degree order puts the hot ones first),
+// with SpmvOp's fixed point and power-of-two exponents: mu_t is stored scaled by
+// 2^-F_t.  Every mu_t is kept; the weight gradient
+//     grad[e] = sum_t y_t[s] mu_{t+1}[d] 2^(E_t + F_{t+1})   (fac[t] = 2^(E_t + F_{t+1}))
+// is then one pass over the edges, in fp64, with y_t and mu_{t+1} of up to
+// kGradRow steps transposed into one 128-byte row per vertex: a row gather per edge
+// (the source's row is shared by its out-edges) instead of a read-modify-write of
+// the gradient per edge and step.
+constexpr int kGradRow = 32;
 
 // Exponent bookkeeping of backward step t (one thread): shiftA[t] from the stored
 // max of mu_{t+1}; F[t] = F[t+1] + shiftA[t]; fac[t] = 2^(E_t + F[t+1]) where
@@ -1849,6 +1822,66 @@ __global__ void k_scatter_f64(const double *in, const int32_t *idx, int64_t m, d
         out[idx[i]] = in[i];
 }
 
+// src[e] = the source of CSR edge e (a wave per vertex)
+__global__ void k_out_src(const int32_t *off, int64_t n, int32_t *src) {
+    const int lane = lane_id();
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < n; v += nw)
+        for (int e = off[v] + lane; e < off[v + 1]; e += 64) src[e] = (int32_t)v;
+}
+
+// rows[v * kGradRow + j] = a[(t0 + j) n + v] for j < cnt, 0 for cnt <= j < kGradRow
+__global__ void k_rows_from_steps(const float *a, int64_t n, int t0, int cnt, float *rows) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        float4 *r = reinterpret_cast<float4 *>(rows + v * kGradRow);
+#pragma unroll
+        for (int q = 0; q < kGradRow / 4; q++) {
+            float x[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int j = 4 * q + k;
+                x[k] = j < cnt ? a[(int64_t)(t0 + j) * n + v] : 0.f;
+            }
+            r[q] = make_float4(x[0], x[1], x[2], x[3]);
+        }
+    }
+}
+
+// grad[e] (+)= sum_{j < cnt} fac[j] Y[s][j] M[d][j] in fp64 for the CSR edge e = (s -> d)
+// (first: store); a thread per edge, the rows' needed float4s loaded together
+__global__ __launch_bounds__(kBlock) void k_spmv_grad_rows(const int32_t *__restrict__ src,
+                                                           const int32_t *__restrict__ adj, int64_t m,
+                                                           const float *__restrict__ Y,
+                                                           const float *__restrict__ M,
+                                                           const double *__restrict__ fac, int cnt,
+                                                           int first, double *grad) {
+    __shared__ double sf[kGradRow];
+    if (threadIdx.x < kGradRow) sf[threadIdx.x] = (int)threadIdx.x < cnt ? fac[threadIdx.x] : 0.0;
+    __syncthreads();
+    const int nq = (cnt + 3) / 4;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const float4 *ys = reinterpret_cast<const float4 *>(Y + (int64_t)src[e] * kGradRow);
+        const float4 *md = reinterpret_cast<const float4 *>(M + (int64_t)__builtin_nontemporal_load(adj + e) * kGradRow);
+        float4 a[kGradRow / 4], b[kGradRow / 4];
+#pragma unroll
+        for (int q = 0; q < kGradRow / 4; q++) {
+            a[q] = q < nq ? ys[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            b[q] = q < nq ? md[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < kGradRow / 4; q++) {
+            acc += sf[4 * q] * ((double)a[q].x * (double)b[q].x);
+            acc += sf[4 * q + 1] * ((double)a[q].y * (double)b[q].y);
+            acc += sf[4 * q + 2] * ((double)a[q].z * (double)b[q].z);
+            acc += sf[4 * q + 3] * ((double)a[q].w * (double)b[q].w);
+        }
+        grad[e] = first ? acc : grad[e] + acc;
+    }
+}
+
 extern "C" int pg_spmv_adjoint(pg_graph *g, const float *gvec, double *grad_x0, double *grad_w,
                                uint32_t flags, pg_stats *st) {
     PG_TRY(check_graph(g));
@@ -1857,8 +1890,7 @@ extern "C" int pg_spmv_adjoint(pg_graph *g, const float *gvec, double *grad_x0, 
                "pg_spmv has not been run with PG_SPMV_SAVE on this graph");
     const int T = g->spmv_saved_iters;
     const int64_t n = g->n, m = g->m;
-    PG_TRY(ensure_out_schedule(g));
-    // fp32 weights in internal CSR order
+    // fp32 weights in internal CSR order (the largest out-weight sum)
     const float *w = nullptr;
     if (g->w_dtype == PG_W_F32) {
         w = g->out_w_f32;
@@ -1870,21 +1902,39 @@ extern "C" int pg_spmv_adjoint(pg_graph *g, const float *gvec, double *grad_x0, 
         if (!have) PG_LAUNCH_K(k_i32_to_f32, grid_for(m), g->out_w_i32, m, wf);
         w = wf;
     }
-    float *saved = nullptr, *ma = nullptr, *mb = nullptr;
-    int32_t *shiftF = nullptr, *shiftA = nullptr;
+    // once per graph: the tiles of the CSR, the edges' sources, the largest out-weight sum
+    PG_TRY(ensure_tiles_out(g));
+    const TileSet &to = g->tiles_out;
+    PG_REQUIRE(to.m_tiled == 0 || to.d_wt, PG_ERR_STATE, "CSR tiles carry no weights");
+    int32_t *osrc = nullptr;
+    unsigned *wmax = nullptr;
+    const bool have_src = g->ws.find("spmv_adj_src") != g->ws.end();
+    PG_TRY(workspace_t(g, "spmv_adj_src", (size_t)m + 1, &osrc));
+    PG_TRY(workspace_t(g, "spmv_adj_wmax", 2, &wmax));
+    if (!have_src) {
+        PG_LAUNCH_K(k_out_src, grid_for(n * 64), g->out_off, n, osrc);
+        PG_TRY(fill(wmax, 0u, 2));
+        PG_LAUNCH_K(k_in_weight_sum_max, grid_for(n, 4096), g->out_off, w, n, wmax);
+    }
+    float *saved = nullptr, *mu = nullptr, *yrows = nullptr, *mrows = nullptr;
+    int32_t *shiftF = nullptr, *shiftA = nullptr, *tsA = nullptr;
     unsigned *maxA = nullptr;
     long long *F = nullptr;
     double *fac = nullptr, *grad = nullptr;
+    const int tb = std::min(T, kGradRow);  // steps per gradient pass
     PG_TRY(workspace_t(g, "spmv_saved", (size_t)n * (size_t)(T + 1), &saved));
     PG_TRY(workspace_t(g, "spmv_shift", (size_t)T + 2, &shiftF));
-    PG_TRY(workspace_t(g, "spmv_adj_a", (size_t)n, &ma));
-    PG_TRY(workspace_t(g, "spmv_adj_b", (size_t)n, &mb));
+    PG_TRY(workspace_t(g, "spmv_adj_mu", (size_t)n * (size_t)(T + 1), &mu));
+    PG_TRY(workspace_t(g, "spmv_adj_yrows", (size_t)n * kGradRow, &yrows));
+    PG_TRY(workspace_t(g, "spmv_adj_mrows", (size_t)n * kGradRow, &mrows));
     PG_TRY(workspace_t(g, "spmv_adj_shift", (size_t)T + 2, &shiftA));
+    PG_TRY(workspace_t(g, "spmv_adj_tscale", (size_t)T + 2, &tsA));
     PG_TRY(workspace_t(g, "spmv_adj_max", (size_t)T + 2, &maxA));
     PG_TRY(workspace_t(g, "spmv_adj_F", (size_t)T + 2, &F));
     PG_TRY(workspace_t(g, "spmv_adj_fac", (size_t)T + 1, &fac));
     PG_TRY(workspace_t(g, "spmv_adj_grad", (size_t)m, &grad));
-    PG_TRY(from_input_order_f32(g, gvec, ma, 1));
+    float *muT = mu + (size_t)T * n;
+    PG_TRY(from_input_order_f32(g, gvec, muT, 1));
     KernelTimer kt;
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
     Timer tm;
@@ -1892,23 +1942,34 @@ extern "C" int pg_spmv_adjoint(pg_graph *g, const float *gvec, double *grad_x0, 
     PG_TRY(fill(maxA, 0u, T + 2));
     // max |g| (slot T+1) sets mu_T's scale; slot T then holds the max of mu_T as
     // stored, from which the first backward step takes its shift
-    PG_LAUNCH_K(k_absmax_bits, grid_for(n, 2048), ma, n, maxA + T + 1);
+    PG_LAUNCH_K(k_absmax_bits, grid_for(n, 2048), muT, n, maxA + T + 1);
     hipLaunchKernelGGL(k_spmv_adj_init, dim3(1), dim3(64), 0, stream(), maxA + T + 1, T, shiftA,
                        F);
     PG_CHECK_HIP(hipGetLastError());
-    PG_LAUNCH_K(k_ldexp_inplace, grid_for(n), ma, n, shiftA + T);
-    PG_LAUNCH_K(k_absmax_bits, grid_for(n, 2048), ma, n, maxA + T);
-    float *cur = ma, *nxt = mb;  // cur = mu_{t+1}
+    PG_LAUNCH_K(k_ldexp_inplace, grid_for(n), muT, n, shiftA + T);
+    PG_LAUNCH_K(k_absmax_bits, grid_for(n, 2048), muT, n, maxA + T);
     for (int t = T - 1; t >= 0; t--) {
         hipLaunchKernelGGL(k_spmv_adj_step, dim3(1), dim3(64), 0, stream(), maxA + t + 1, shiftF,
                            t, shiftA, F, fac);
         PG_CHECK_HIP(hipGetLastError());
-        SpmvAdjOp op{cur, w, saved + (size_t)t * n, nxt, grad, fac + t, shiftA + t,
-                     t == T - 1 ? 1 : 0};
+        hipLaunchKernelGGL(k_spmv_tscale, dim3(1), dim3(64), 0, stream(), maxA + t + 1, wmax,
+                           tsA + t);
+        PG_CHECK_HIP(hipGetLastError());
+        // mu_t (stored scaled by 2^-shiftA[t]) from mu_{t+1}
+        const SpmvOp op{mu + (size_t)(t + 1) * n, w, mu + (size_t)t * n, shiftA + t, tsA + t};
         PG_TRY(kt.begin());
-        PG_TRY(launch_pull(op, g->out_sched, maxA + t));
+        PG_TRY(launch_tiles(op, to, maxA + t));
         PG_TRY(kt.end());
-        std::swap(cur, nxt);
+    }
+    // the weight gradient, kGradRow steps per pass
+    for (int t0 = 0; t0 < T; t0 += tb) {
+        const int cnt = std::min(tb, T - t0);
+        PG_TRY(kt.begin());
+        PG_LAUNCH_K(k_rows_from_steps, grid_for(n), saved, n, t0, cnt, yrows);      // y_t
+        PG_LAUNCH_K(k_rows_from_steps, grid_for(n), mu + n, n, t0, cnt, mrows);     // mu_{t+1}
+        PG_LAUNCH_K(k_spmv_grad_rows, grid_for(m), osrc, g->out_adj, m, yrows, mrows, fac + t0,
+                    cnt, t0 == 0 ? 1 : 0, grad);
+        PG_TRY(kt.end());
     }
     double ms = 0;
     PG_TRY(tm.stop(&ms));
@@ -1923,7 +1984,7 @@ extern "C" int pg_spmv_adjoint(pg_graph *g, const float *gvec, double *grad_x0, 
     if (grad_x0) {  // mu_0 = stored * 2^F[0]
         double *tmp = nullptr;
         PG_TRY(workspace_t(g, "spmv_out", (size_t)n, &tmp));
-        PG_LAUNCH_K(k_to_f64_scaled, grid_for(n), cur, g->perm, n, F, tmp);
+        PG_LAUNCH_K(k_to_f64_scaled, grid_for(n), mu, g->perm, n, F, tmp);
         PG_CHECK_HIP(hipMemcpyAsync(grad_x0, tmp, 8 * n, hipMemcpyDeviceToHost, stream()));
     }
     if (grad_w) {

@@ -949,8 +949,7 @@ int build_schedule(pg_graph *g, bool in_direction) {
 }
 
 // ---- source-sorted tiles ----------------------------------------------------------------
-void free_tiles(pg_graph *g) {
-    TileSet &t = g->tiles;
+static void free_tile_set(TileSet &t) {
     if (t.d_vbeg) (void)hipFree(t.d_vbeg);
     if (t.d_parts) (void)hipFree(t.d_parts);
     if (t.d_fin) (void)hipFree(t.d_fin);
@@ -970,16 +969,23 @@ void free_tiles(pg_graph *g) {
     t = TileSet{};
 }
 
-// key = tile << 32 | source, value = the edge's index relative to e0, for the CSC
-// edges of the light positions [p0, p1); dloc[index] = destination offset within
-// the tile
+void free_tiles(pg_graph *g) {
+    free_tile_set(g->tiles);
+    free_tile_set(g->tiles_out);
+}
+
+// key = tile << 32 | source, value = the edge's index relative to e0, for the
+// edges of the positions [p0, p1) (a position's list: adj[off[p], off[p+1])); dloc[index]
+// = destination offset within the tile.  A wave per position (a hub's list is not
+// one thread's loop).
 __global__ void k_tile_keys(const int32_t *off, const int32_t *adj, int64_t p0, int64_t p1,
                             int64_t e0, uint64_t *keys, uint32_t *vals, uint16_t *dloc) {
-    for (int64_t p = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < p1;
-         p += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = lane_id();
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t p = p0 + (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6); p < p1; p += nw) {
         const int t = (int)((p - p0) / kTileMaxV);
         const uint16_t local = (uint16_t)((p - p0) % kTileMaxV);
-        for (int64_t e = off[p]; e < off[p + 1]; e++) {
+        for (int64_t e = off[p] + lane; e < off[p + 1]; e += 64) {
             keys[e - e0] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)adj[e];
             vals[e - e0] = (uint32_t)(e - e0);
             dloc[e - e0] = local;
@@ -1094,8 +1100,7 @@ __global__ void k_tile_pack(const int32_t *src, const uint16_t *dst, int64_t mt,
 
 // Packs the tile edges (source offset + destination in one word, escape rows for
 // wide groups) and frees the separate source / destination arrays.
-static int pack_tiles(pg_graph *g, int64_t mt, int32_t cold) {
-    TileSet &t = g->tiles;
+static int pack_tiles(TileSet &t, int64_t mt, int32_t cold) {
     const int64_t ng = (mt + kPackGroup - 1) / kPackGroup;
     DevBuf dmin, dwide;
     PG_TRY(dmin.alloc(sizeof(int32_t) * (ng + 1)));
@@ -1126,12 +1131,10 @@ static int pack_tiles(pg_graph *g, int64_t mt, int32_t cold) {
 // Builds the cold pre-pass of the tiles: for each tile the first edge whose source
 // is cold (tile lists are sorted by source), the cold value array in tile order,
 // and the pre-pass lists grouped by source chunk.  Fills the cold fields of parts.
-static int build_cold(pg_graph *g, const std::vector<int32_t> &vbeg,
+static int build_cold(TileSet &t, int64_t n, int32_t cold, const std::vector<int32_t> &vbeg,
                       const std::vector<int32_t> &off_h, std::vector<TilePart> &parts) {
-    TileSet &t = g->tiles;
     const int nt = (int)vbeg.size() - 1;
     const int64_t p0 = vbeg.front(), mt = (int64_t)off_h.back() - off_h[0];
-    const int32_t cold = (int32_t)g->tile_cold;
     std::vector<int64_t> tb((size_t)nt + 1);
     for (int k = 0; k <= nt; k++) tb[k] = (int64_t)off_h[vbeg[k] - p0] - off_h[0];
     DevBuf dtb, dcs, dcb;
@@ -1164,7 +1167,7 @@ static int build_cold(pg_graph *g, const std::vector<int32_t> &vbeg,
     PG_TRY(ch.alloc(sizeof(uint32_t) * mc));
     PG_LAUNCH(k_cold_keys, grid_for(mt), t.d_src, mt, dtb.as<int64_t>(), nt, dcs.as<int64_t>(),
               dcb.as<int64_t>(), cold, k0.as<uint64_t>(), v0.as<uint32_t>());
-    const int64_t nchunks = (g->n - cold + kColdChunk - 1) / kColdChunk;
+    const int64_t nchunks = (n - cold + kColdChunk - 1) / kColdChunk;
     PG_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(), mc,
                       32 + bits_for(nchunks + 1)));
     PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_cold_pos), sizeof(uint32_t) * (mc + 16)));
@@ -1190,26 +1193,23 @@ static int build_cold(pg_graph *g, const std::vector<int32_t> &vbeg,
         PG_CHECK_HIP(hipMemcpy(t.d_cold_parts, cparts.data(), sizeof(ColdPart) * cparts.size(),
                                hipMemcpyHostToDevice));
     t.cold = ColdArgs{t.d_cold_parts, t.d_cold_pos, t.d_cold_srcl, t.d_cold_val,
-                      (int32_t)cparts.size(), cold, g->n};
+                      (int32_t)cparts.size(), cold, n};
     return PG_OK;
 }
 
-int ensure_tiles(pg_graph *g) {
-    PullSchedule &s = g->in_sched;
-    PG_REQUIRE(s.built, PG_ERR_STATE, "tiles: the pull schedule is not built");
-    TileSet &t = g->tiles;
-    if (t.built && t.light_begin == s.light_begin && t.light_end == s.light_end &&
-        t.tile_edges == g->tile_edges && t.cold_begin == g->tile_cold)
-        return PG_OK;
+// The tiles of positions [p0, p1) of a pull over the lists adj[off[p], off[p+1])
+// (weights wf or wi in list order, or none): kTileMaxV consecutive positions a tile,
+// parts of about tile_edges edges, sources from g->tile_cold on cold.
+static int build_tile_set(pg_graph *g, TileSet &t, const int32_t *off, const int32_t *adj,
+                          const float *wf, const int32_t *wi, int64_t p0, int64_t p1) {
     drop_captures(g);
     PG_CHECK_HIP(hipStreamSynchronize(g_stream));
-    free_tiles(g);
-    const int64_t p0 = s.light_begin, p1 = s.light_end;
+    free_tile_set(t);
     PG_REQUIRE(p1 - p0 < ((int64_t)1 << 31), PG_ERR_UNSUPPORTED, "tiles: light range too large");
     // tiles: kTileMaxV consecutive light positions; parts: about tile_edges edges
     std::vector<int32_t> off_h((size_t)(p1 - p0 + 1));
     if (p1 > p0)
-        PG_CHECK_HIP(hipMemcpy(off_h.data(), g->in_off + p0, sizeof(int32_t) * off_h.size(),
+        PG_CHECK_HIP(hipMemcpy(off_h.data(), off + p0, sizeof(int32_t) * off_h.size(),
                                hipMemcpyDeviceToHost));
     std::vector<int32_t> vbeg;
     std::vector<TilePart> parts;
@@ -1247,7 +1247,7 @@ int ensure_tiles(pg_graph *g) {
     PG_CHECK_HIP(hipMemcpy(t.d_vbeg, vbeg.data(), sizeof(int32_t) * vbeg.size(), hipMemcpyHostToDevice));
     if (!fin.empty())
         PG_CHECK_HIP(hipMemcpy(t.d_fin, fin.data(), sizeof(TileFin) * fin.size(), hipMemcpyHostToDevice));
-    const bool weighted = g->w_dtype == PG_W_F32 || g->w_dtype == PG_W_I32;
+    const bool weighted = wf || wi;
     if (weighted && mt > 0)
         PG_CHECK_HIP(hipMalloc(reinterpret_cast<void **>(&t.d_wt), sizeof(float) * (mt + 16)));
     if (mt > 0) {
@@ -1258,15 +1258,12 @@ int ensure_tiles(pg_graph *g) {
         PG_TRY(v0.alloc(sizeof(uint32_t) * mt));
         PG_TRY(v1.alloc(sizeof(uint32_t) * mt));
         PG_TRY(dl.alloc(sizeof(uint16_t) * mt));
-        PG_LAUNCH(k_tile_keys, grid_for(p1 - p0), g->in_off, g->in_adj, p0, p1, (int64_t)off_h[0],
+        PG_LAUNCH(k_tile_keys, grid_for((p1 - p0) * 64), off, adj, p0, p1, (int64_t)off_h[0],
                   k0.as<uint64_t>(), v0.as<uint32_t>(), dl.as<uint16_t>());
         PG_TRY(sort_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint32_t>(), v1.as<uint32_t>(),
                           mt, 32 + bits_for(nt)));
         PG_LAUNCH(k_tile_split, grid_for(mt), k1.as<uint64_t>(), v1.as<uint32_t>(), mt,
-                  dl.as<uint16_t>(), (int64_t)off_h[0],
-                  g->w_dtype == PG_W_F32 ? g->in_w_f32 : (const float *)nullptr,
-                  g->w_dtype == PG_W_I32 ? g->in_w_i32 : (const int32_t *)nullptr, t.d_src,
-                  t.d_dst, t.d_wt);
+                  dl.as<uint16_t>(), (int64_t)off_h[0], wf, wi, t.d_src, t.d_dst, t.d_wt);
         PG_CHECK_HIP(hipStreamSynchronize(g_stream));
     }
     // cold split: every part's first cold edge; the pre-pass lists by source chunk
@@ -1277,8 +1274,8 @@ int ensure_tiles(pg_graph *g) {
     t.cold = ColdArgs{};
     t.cold_begin = g->tile_cold;
     const bool cold_split = mt > 0 && g->tile_cold >= 0 && g->tile_cold < g->n;
-    if (cold_split) PG_TRY(build_cold(g, vbeg, off_h, parts));
-    if (mt > 0) PG_TRY(pack_tiles(g, mt, cold_split ? (int32_t)g->tile_cold : (int32_t)g->n));
+    if (cold_split) PG_TRY(build_cold(t, g->n, (int32_t)g->tile_cold, vbeg, off_h, parts));
+    if (mt > 0) PG_TRY(pack_tiles(t, mt, cold_split ? (int32_t)g->tile_cold : (int32_t)g->n));
     // the next iteration's cold pre-pass may overlap this one's finish pass when no
     // vertex the finish pass writes is a cold source
     t.cold_overlap = cold_split && t.cold.nparts > 0;
@@ -1301,6 +1298,28 @@ int ensure_tiles(pg_graph *g) {
     t.light_end = p1;
     t.built = true;
     return PG_OK;
+}
+
+int ensure_tiles(pg_graph *g) {
+    PullSchedule &s = g->in_sched;
+    PG_REQUIRE(s.built, PG_ERR_STATE, "tiles: the pull schedule is not built");
+    TileSet &t = g->tiles;
+    if (t.built && t.light_begin == s.light_begin && t.light_end == s.light_end &&
+        t.tile_edges == g->tile_edges && t.cold_begin == g->tile_cold)
+        return PG_OK;
+    return build_tile_set(g, t, g->in_off, g->in_adj,
+                          g->w_dtype == PG_W_F32 ? g->in_w_f32 : (const float *)nullptr,
+                          g->w_dtype == PG_W_I32 ? g->in_w_i32 : (const int32_t *)nullptr,
+                          s.light_begin, s.light_end);
+}
+
+int ensure_tiles_out(pg_graph *g) {
+    TileSet &t = g->tiles_out;
+    if (t.built && t.tile_edges == g->tile_edges && t.cold_begin == g->tile_cold) return PG_OK;
+    return build_tile_set(g, t, g->out_off, g->out_adj,
+                          g->w_dtype == PG_W_F32 ? g->out_w_f32 : (const float *)nullptr,
+                          g->w_dtype == PG_W_I32 ? g->out_w_i32 : (const int32_t *)nullptr, 0,
+                          g->n);
 }
 
 int ensure_out_schedule(pg_graph *g) {

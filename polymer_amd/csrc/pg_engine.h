@@ -778,6 +778,27 @@ inline int launch_pull_tiled(const Op &op, const PullSchedule &s, const TileSet 
     return PG_OK;
 }
 
+// A pull whose every destination lies in the tiles (a tile set over all positions,
+// no hot schedule): the cold pre-pass, the tile parts, the finishing pass.
+template <class Op>
+inline int launch_tiles(const Op &op, const TileSet &t, typename Op::Stat *stat_out = nullptr) {
+    if (t.cold.nparts > 0)
+        hipLaunchKernelGGL(k_cold_gather<Op>, dim3(t.cold.nparts), dim3(kTileThreads), 0, stream(),
+                           op, t.cold);
+    if (t.args.nparts > 0)
+        hipLaunchKernelGGL(k_pull_tiles<Op>, dim3(t.args.nparts), dim3(kTileThreads), 0, stream(),
+                           op, t.args, stat_out);
+    if (t.args.nfin > 0)
+        hipLaunchKernelGGL(k_tile_finish<Op>, dim3(t.args.nfin), dim3(kBlock), 0, stream(), op,
+                           t.args, stat_out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("tile pull launch failed: %s", hipGetErrorString(e));
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
 // ---- push engine ------------------------------------------------------------------
 // kDense: sources are the vertex ids [0, count) filtered by op.src_active;
 // otherwise the sources are list[0, count).

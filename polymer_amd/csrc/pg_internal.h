@@ -245,6 +245,7 @@ struct pg_graph {
     pg::PullSchedule in_sched;   // pull over CSC (in-edges)
     pg::PullSchedule out_sched;  // pull over CSR (out-edges)
     pg::TileSet tiles;           // source-sorted tiles of the CSC light range
+    pg::TileSet tiles_out;       // the same over the CSR, every vertex (the SpMV adjoint)
     bool use_tiles = true;       // PageRank-family and SpMV solves use the tiles
     int64_t tile_max_degree = (int64_t)1 << 30;  // heavier vertices stay in the hot pull
     int64_t tile_edges = 524288; // target edges per tile part
@@ -304,6 +305,9 @@ int build_schedule(pg_graph *g, bool in_direction);
 int ensure_out_schedule(pg_graph *g);
 // builds (once per block threshold) the source-sorted tiles of the CSC light range
 int ensure_tiles(pg_graph *g);
+// builds (once per tile setting) the tiles of the pull over the CSR: every vertex a
+// destination of its out-lists, sources = the out-neighbours
+int ensure_tiles_out(pg_graph *g);
 void free_tiles(pg_graph *g);
 
 // device-wide helpers implemented in pg_build.hip (hipcub)

@@ -212,11 +212,13 @@ def test_wpr_adjoint_dense_and_fd(pa, orc):
 
 
 def test_spmv_adjoint_vs_oracle(pa, orc):
-    """pg_spmv_adjoint (numa-SPMV.C's iteration differentiated: a pull over the CSR
-    per step, the weight gradient accumulated per edge) against the oracle's fp64
-    adjoint, which tests/test_oracle.py pins by the dense-matrix gradient and
-    central differences: fp32 weights on the tiles and on the degree-binned pull,
-    integer weights, and a second loss on the same kept iterates."""
+    """pg_spmv_adjoint (numa-SPMV.C's iteration differentiated: a tiled pull over the
+    CSR per step, the weight gradient in passes of up to 32 steps over the edges)
+    against the oracle's fp64 adjoint, which tests/test_oracle.py pins by the
+    dense-matrix gradient and central differences: fp32 and integer weights, the
+    forward on the tiles and on the degree-binned pull, tiles with many parts and
+    cold sources, 40 steps (two gradient passes), and a second loss on the same
+    kept iterates."""
     say("case: spmv adjoint")
     g = orc.rmat(13, 1 << 17, seed=41)
     rng = np.random.default_rng(3)
@@ -228,9 +230,9 @@ def test_spmv_adjoint_vs_oracle(pa, orc):
             w = orc.weights_i32(g, 6)
             d = pa.Graph.from_csr(g.n, g.off, g.adj, w=w)
         x0 = rng.random(g.n).astype(np.float32)
-        for tiles in (1, 0):
-            d.set_tiles(tiles)
-            for T in (1, 6):
+        for tiles, te, cold in ((1, 0, 0), (0, 0, 0), (1, 4096, 1000)):
+            d.set_tiles(tiles, te, cold)
+            for T in ((1, 6, 40) if tiles and not te else (1, 6)):
                 y_plain, _ = d.spmv(x0, T)
                 y, _ = d.spmv(x0, T, save=True)
                 assert np.array_equal(y, y_plain)  # keeping the iterates changes nothing
@@ -240,8 +242,8 @@ def test_spmv_adjoint_vs_oracle(pa, orc):
                     rx, rw = orc.spmv_adjoint(g, w.astype(np.float64), x0.astype(np.float64),
                                               gv.astype(np.float64), T)
                     ex, ew = rel_l1(gx, rx), rel_l1(gw, rw)
-                    say("%s weights, tiles %d, T %d: dL/dx0 relL1 %.3e, dL/dw relL1 %.3e" %
-                        (wkind, tiles, T, ex, ew))
+                    say("%s weights, tiles %d %d %d, T %d: dL/dx0 relL1 %.3e, dL/dw relL1 %.3e" %
+                        (wkind, tiles, te, cold, T, ex, ew))
                     assert ex < TOL and ew < TOL
                     assert st.rounds == T
         d.close()
