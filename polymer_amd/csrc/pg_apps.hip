@@ -1006,6 +1006,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
     int32_t *level, int32_t *parent) {
     __shared__ int32_t s_v[kBlock], s_beg[kBlock];
     __shared__ int64_t s_pre[kBlock + 1];
+    __shared__ int64_t s_wsum[kBlock / 64];
     const int64_t rr = R.r();
     unsigned long long *cur = R.region(rr);
     if (cur[kBfsCtlMode] != kBfsModeTD) return;  // block-uniform
@@ -1018,17 +1019,24 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
     const int64_t gn = (int64_t)gridDim.x * kBlock;
     for (int64_t b0 = 0; b0 < nh; b0 += kBlock) {
         const int nbat = (int)min<int64_t>(kBlock, nh - b0);
+        // the batch's degree prefix by a block scan (a wave scan, then the wave sums)
+        long long x = 0;
         if ((int)threadIdx.x < nbat) {
             const int v = qhuge[b0 + threadIdx.x];
             s_v[threadIdx.x] = v;
             s_beg[threadIdx.x] = out_off[v];
-            s_pre[threadIdx.x + 1] = out_off[v + 1] - out_off[v];
+            x = out_off[v + 1] - out_off[v];
         }
+        for (int o = 1; o < 64; o <<= 1) {
+            const long long y = __shfl_up(x, o, 64);
+            if (lane_id() >= o) x += y;
+        }
+        if (lane_id() == 63) s_wsum[threadIdx.x >> 6] = x;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            s_pre[0] = 0;
-            for (int j = 1; j <= nbat; j++) s_pre[j] += s_pre[j - 1];
-        }
+        long long before = 0;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); w++) before += s_wsum[w];
+        s_pre[threadIdx.x + 1] = before + x;
+        if (threadIdx.x == 0) s_pre[0] = 0;
         __syncthreads();
         const int64_t tot = s_pre[nbat];
         for (int64_t k = gt; k < tot; k += gn) {
