@@ -227,8 +227,10 @@ def value_extra(pa, g, scale=24, m=1 << 28):
     out["spmv_scale24_algorithmic_gbs"] = round(gbs, 1)
     out["spmv_scale24_hbm_frac"] = round(gbs / HBM_PEAK_GBS, 4)
     # the adjoint of the same 20 iterations (loss g . y_20, g ~ N(0,1)): the forward
-    # keeps its iterates, the backward pulls over the CSR and accumulates the fp64
-    # weight gradient (28 bytes per edge and 12 per vertex per step)
+    # keeps its iterates; the backward is 20 tiled pulls over the CSR (12m + 12n bytes
+    # each, as the forward's) and one gradient pass (per edge its index, source,
+    # fp64 gradient and the destination's 20 backward values, 16 + 4*20 bytes; the
+    # iterates of both directions transposed into 128-byte rows, 2 * (4*20 + 128) n)
     gv = np.random.default_rng(4).standard_normal(gw.n).astype(np.float32)
     gw.spmv(x0, 20, out=False, save=True)
     _, stf = gw.spmv(x0, 20, out=False, save=True)
@@ -237,8 +239,8 @@ def value_extra(pa, g, scale=24, m=1 << 28):
     out["spmv_scale24_forward_saved_ms"] = round(stf.ms, 3)
     out["spmv_scale24_adjoint_ms"] = round(sta.ms, 3)
     out["spmv_adjoint_over_forward"] = round(sta.ms / st.ms, 3)
-    out["spmv_adjoint_algorithmic_gbs"] = round(
-        (28 * gw.m + 12 * gw.n) * 20 / (sta.ms / 1e3) / 1e9, 1)
+    adj_bytes = 20 * (12 * gw.m + 12 * gw.n) + gw.m * (16 + 4 * 20) + 2 * (4 * 20 + 128) * gw.n
+    out["spmv_adjoint_algorithmic_gbs"] = round(adj_bytes / (sta.ms / 1e3) / 1e9, 1)
     gw.close()
     g.bp_potentials(9)
     vpot = np.random.default_rng(8).uniform(0.1, 1.0, 2 * g.n).astype(np.float32)
@@ -360,6 +362,8 @@ def profile_extra(args):
               plain runs on the tiles (batched_extra)
       async   the asynchronous traversal from the first two scale-24 BFS sources
               (async_extra)
+      spmvadj the SpMV adjoint of 20 iterations on the scale-24 rMat, after the
+              forward that keeps its iterates (value_extra)
     (The drop-in drivers are profiled directly: rocprofv3 ... --
     polymer_amd/apps/bin/pagerank -rmat 24 268435456 1 -rounds 20.)"""
     import polymer_amd as pa
@@ -398,6 +402,14 @@ def profile_extra(args):
         for s in src[:2]:
             _, st = g.ppr(int(s), 20, out=False)
             log("ppr plain source %d: %.3f ms" % (s, st.ms))
+    elif name == "spmvadj":
+        gw = pa.Graph.rmat(24, 1 << 28, seed=1, weights="f32", wseed=3)
+        x0 = np.random.default_rng(3).random(gw.n).astype(np.float32)
+        gv = np.random.default_rng(4).standard_normal(gw.n).astype(np.float32)
+        for r in range(2):
+            _, stf = gw.spmv(x0, 20, out=False, save=True)
+            _, _, sta = gw.spmv_adjoint(gv, out=False)
+            log("spmv forward (kept iterates) run %d: %.3f ms, adjoint %.3f ms" % (r, stf.ms, sta.ms))
     elif name == "async":
         gs = pa.Graph.rmat(24, 1 << 28, seed=1, symmetrize=True)
         srcs = [s for s, _ in bfs24_runs(gs, 2)]
@@ -549,7 +561,7 @@ def main():
     ap.add_argument("--profile-bp", action="store_true",
                     help="the same for BP on the scale-24 rMat of the extras")
     ap.add_argument("--profile-extra", default=None,
-                    help="the same for one extra workload: spmv, delta, bfs22, ppr, async")
+                    help="the same for one extra workload: spmv, delta, bfs22, ppr, async, spmvadj")
     ap.add_argument("--probe-gather", action="store_true",
                     help="random 8-byte gather rate against the gathered array's size")
     ap.add_argument("--gpus", type=int, default=1)

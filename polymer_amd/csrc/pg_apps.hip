@@ -1790,10 +1790,10 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
 // with SpmvOp's fixed point and power-of-two exponents: mu_t is stored scaled by
 // 2^-F_t.  Every mu_t is kept; the weight gradient
 //     grad[e] = sum_t y_t[s] mu_{t+1}[d] 2^(E_t + F_{t+1})   (fac[t] = 2^(E_t + F_{t+1}))
-// is then one pass over the edges, in fp64, with y_t and mu_{t+1} of up to
-// kGradRow steps transposed into one 128-byte row per vertex: a row gather per edge
-// (the source's row is shared by its out-edges) instead of a read-modify-write of
-// the gradient per edge and step.
+// is then one pass over the edges, in fp64, with y_t and mu_{t+1} of up to kGradRow
+// steps transposed into one 128-byte row per vertex: a row gather per edge (the
+// source's row is shared by its out-edges) instead of a read-modify-write of the
+// gradient per edge and step.
 constexpr int kGradRow = 32;
 
 // Exponent bookkeeping of backward step t (one thread): shiftA[t] from the stored
@@ -1838,26 +1838,39 @@ __global__ void k_out_src(const int32_t *off, int64_t n, int32_t *src) {
         for (int e = off[v] + lane; e < off[v + 1]; e += 64) src[e] = (int32_t)v;
 }
 
-// rows[v * kGradRow + j] = a[(t0 + j) n + v] for j < cnt, 0 for cnt <= j < kGradRow
-__global__ void k_rows_from_steps(const float *a, int64_t n, int t0, int cnt, float *rows) {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-         v += (int64_t)gridDim.x * blockDim.x) {
-        float4 *r = reinterpret_cast<float4 *>(rows + v * kGradRow);
-#pragma unroll
-        for (int q = 0; q < kGradRow / 4; q++) {
-            float x[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int j = 4 * q + k;
-                x[k] = j < cnt ? a[(int64_t)(t0 + j) * n + v] : 0.f;
-            }
-            r[q] = make_float4(x[0], x[1], x[2], x[3]);
+// rows[v * kGradRow + j] = a[(t0 + j) n + v] for j < cnt, 0 for cnt <= j < kGradRow: a
+// wave reads its 64 vertices' values step by step (coalesced), stages the 64 rows in
+// LDS and writes them as 8 KB of consecutive 16-byte pieces (a lane writing its own
+// row's pieces touched 64 lines per store and wrote every line three times over)
+__global__ __launch_bounds__(kBlock) void k_rows_from_steps(const float *a, int64_t n, int t0,
+                                                            int cnt, float *rows) {
+    __shared__ float st[kBlock / 64][64 * (kGradRow + 1)];  // +1: rows start on distinct banks
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    float *sw = st[w];
+    // block-uniform steps of kBlock vertices (a wave takes 64 of them), so every wave
+    // of the block reaches each barrier
+    for (int64_t bb = (int64_t)blockIdx.x * kBlock; bb < n; bb += (int64_t)gridDim.x * kBlock) {
+        const int64_t base = bb + 64 * w;
+        const int64_t v = base + lane;
+        for (int j = 0; j < kGradRow; j++)
+            sw[lane * (kGradRow + 1) + j] = (j < cnt && v < n) ? a[(int64_t)(t0 + j) * n + v] : 0.f;
+        __syncthreads();
+        const int64_t nrows = max((int64_t)0, min((int64_t)64, n - base));
+        float4 *out = reinterpret_cast<float4 *>(rows + base * kGradRow);
+        for (int k = lane; k < nrows * (kGradRow / 4); k += 64) {
+            const int r = k / (kGradRow / 4), q = k % (kGradRow / 4);
+            const float *x = &sw[r * (kGradRow + 1) + 4 * q];
+            out[k] = make_float4(x[0], x[1], x[2], x[3]);
         }
+        __syncthreads();
     }
 }
 
 // grad[e] (+)= sum_{j < cnt} fac[j] Y[s][j] M[d][j] in fp64 for the CSR edge e = (s -> d)
-// (first: store); a thread per edge, the rows' needed float4s loaded together
+// (first: store); a thread per edge, the rows' needed float4s loaded together (the
+// source's row is shared by its out-edges, which sit side by side; reading the
+// source's values from the kept iterates step by step instead took a line per step
+// and source: 2.3x slower)
 __global__ __launch_bounds__(kBlock) void k_spmv_grad_rows(const int32_t *__restrict__ src,
                                                            const int32_t *__restrict__ adj, int64_t m,
                                                            const float *__restrict__ Y,
@@ -1870,7 +1883,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_grad_rows(const int32_t *__rest
     const int nq = (cnt + 3) / 4;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
          e += (int64_t)gridDim.x * blockDim.x) {
-        const float4 *ys = reinterpret_cast<const float4 *>(Y + (int64_t)src[e] * kGradRow);
+        const float4 *ys = reinterpret_cast<const float4 *>(Y + (int64_t)__builtin_nontemporal_load(src + e) * kGradRow);
         const float4 *md = reinterpret_cast<const float4 *>(M + (int64_t)__builtin_nontemporal_load(adj + e) * kGradRow);
         float4 a[kGradRow / 4], b[kGradRow / 4];
 #pragma unroll
@@ -1973,8 +1986,8 @@ extern "C" int pg_spmv_adjoint(pg_graph *g, const float *gvec, double *grad_x0, 
     for (int t0 = 0; t0 < T; t0 += tb) {
         const int cnt = std::min(tb, T - t0);
         PG_TRY(kt.begin());
-        PG_LAUNCH_K(k_rows_from_steps, grid_for(n), saved, n, t0, cnt, yrows);      // y_t
-        PG_LAUNCH_K(k_rows_from_steps, grid_for(n), mu + n, n, t0, cnt, mrows);     // mu_{t+1}
+        PG_LAUNCH_K(k_rows_from_steps, grid_for(n, 4096), saved, n, t0, cnt, yrows);   // y_t
+        PG_LAUNCH_K(k_rows_from_steps, grid_for(n, 4096), mu + n, n, t0, cnt, mrows);  // mu_{t+1}
         PG_LAUNCH_K(k_spmv_grad_rows, grid_for(m), osrc, g->out_adj, m, yrows, mrows, fac + t0,
                     cnt, t0 == 0 ? 1 : 0, grad);
         PG_TRY(kt.end());
