@@ -2742,7 +2742,10 @@ extern "C" int pg_bfs_async(pg_graph *g, int64_t src, int32_t *parent, int64_t *
     PG_TRY(fill(ctl, 0u, kAsyncCtlWords));
     PG_LAUNCH_K(k_async_seed, 1, s_int, g->out_off, par, vis, q, ctl);
     PG_TRY(kt.begin());
-    PG_LAUNCH_K(k_bfs_async, 2048, g->out_off, g->out_adj, par, vis, q, cap, ctl);
+    // one workgroup per CU: the waiting waves' polls contend, so fewer waves finish
+    // sooner (S24, per source: 2048 workgroups 18.9-22.9 ms, 1024 17.5-19.4, 512
+    // 17.1-17.7, 256 16.5-16.9, 128 21.1-21.3)
+    PG_LAUNCH_K(k_bfs_async, 256, g->out_off, g->out_adj, par, vis, q, cap, ctl);
     PG_TRY(kt.end());
     PG_LAUNCH_K(k_async_reached, grid_for(nw, 1024), vis, nw, ctl + kAsyncReached);
     double ms = 0;
