@@ -1413,9 +1413,17 @@ __global__ __launch_bounds__(kT) void k_pull_first(F f, pg_view v, const intT *f
         const intT s0 = d64 < n ? first_in[d64] : -1;
         bool queued = false;
         if (s0 >= 0 && f.cond(d)) {
-            const int beg = v.in_off[d], end = v.in_off[d + 1];
+            // the offsets are loaded only when the rest of the list is needed (a hit
+            // on the first in-neighbour usually ends the scan), or first when the
+            // update takes the first edge's weight
+            int beg = W ? v.in_off[d] : 0, end = 0;
             bool hit = cur[s0] && call_update<W>(f, s0, d, w, beg);
-            if (end - beg > 1 && f.cond(d)) {
+            const bool more = f.cond(d);
+            if (more) {
+                beg = v.in_off[d];
+                end = v.in_off[d + 1];
+            }
+            if (more && end - beg > 1) {
                 if (end - beg - 1 > kLaneRest) {
                     queued = true;
                 } else {
