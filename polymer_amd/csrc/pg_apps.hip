@@ -853,25 +853,86 @@ struct BfsRound {
     __device__ uint32_t *C(int64_t q) const { return pick((q + 1) % 3); }
 };
 
-// A top-down claim: V (n/8 bytes, L2-resident) screens out the vertices reached
-// before; an OR on V's word that returns it picks the one claimer of a destination,
-// which stores level and parent, sets the destination's bit in the next frontier
-// (a returnless OR) and counts it with its out-degree for the round's statistics.
+// Top-down claims of edges (v, d), made by the whole wave together (every lane
+// calls it) for J slices of 64 edges (d < 0: no edge), each stage's loads and
+// atomics issued for every slice before the first wait.  V (n/8 bytes,
+// L2-resident) screens out the vertices reached before; an OR on V's word that
+// returns it picks the one claimer of a destination, which stores level and
+// parent, sets the destination's bit in the next frontier and counts it with its
+// out-degree for the round's statistics.
 // Which frontier vertex becomes the parent is unspecified, as in the reference,
 // whose claim is a CAS of the parent from -1.  Parents need no initialisation: the
 // parent of an unreached vertex is set to -1 on output.
-__device__ __forceinline__ void bfs_claim(int v, int d, uint32_t *V, int32_t *level,
-                                          int32_t *parent, uint32_t *Fn, int32_t round,
-                                          const int32_t *__restrict__ out_off,
-                                          unsigned long long &cnt, unsigned long long &deg) {
-    const uint32_t b = 1u << (d & 31);
-    if (V[d >> 5] & b) return;
-    if (atomicOr(&V[d >> 5], b) & b) return;
-    level[d] = round;
-    parent[d] = v;
-    atomicOr(&Fn[d >> 5], b);
-    cnt++;
-    deg += (unsigned long long)(out_off[d + 1] - out_off[d]);
+// The ORs are made per run of consecutive lanes whose destinations share a V word:
+// neighbour lists are sorted and the ids are in degree order, so a hub's edges
+// reach dense id ranges, and one OR per lane would queue up to 32 atomics on one
+// word.  The run's first lane ORs the run's bits into V and the bits it won into
+// the next frontier; a lane wins if its bit was clear before and no earlier lane
+// of its run (a slice's runs) holds the same destination; two slices holding one
+// destination are told apart by their ORs.
+template <int J>
+__device__ __forceinline__ void bfs_claim_wave(const int (&v)[J], const int (&d)[J], uint32_t *V,
+                                               int32_t *level, int32_t *parent, uint32_t *Fn,
+                                               int32_t round, const int32_t *__restrict__ out_off,
+                                               unsigned long long &cnt,
+                                               unsigned long long &deg) {
+    const int lane = lane_id();
+    uint32_t b[J], vw[J];
+    int w[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        b[j] = d[j] >= 0 ? 1u << (d[j] & 31) : 0u;
+        w[j] = d[j] >= 0 ? d[j] >> 5 : 0;
+        vw[j] = d[j] >= 0 ? V[w[j]] : ~0u;
+    }
+    bool want[J], any = false;
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        want[j] = d[j] >= 0 && !(vw[j] & b[j]);
+        any = any || want[j];
+    }
+    if (!__ballot(any)) return;  // wave-uniform
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+    bool head[J];
+    int start[J];
+    uint32_t suf[J], before[J], old[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        // runs: a lane without a claim is a run of its own
+        const int key = want[j] ? w[j] : -1 - lane;
+        const int key_prev = __shfl_up(key, 1, 64);  // with every lane active
+        head[j] = lane == 0 || key_prev != key;
+        const unsigned long long H = __ballot(head[j]);
+        start[j] = 63 - __clzll((long long)(H & upto));
+        const unsigned long long after = H & ~upto;
+        const int end = after ? __ffsll((long long)after) - 2 : 63;
+        uint32_t s = want[j] ? b[j] : 0u, inc = s;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t ys = __shfl_down(s, o, 64);
+            const uint32_t yp = __shfl_up(inc, o, 64);
+            if (lane + o <= end) s |= ys;         // OR of lanes lane..end of the run
+            if (lane - o >= start[j]) inc |= yp;  // OR of lanes start..lane
+        }
+        suf[j] = s;
+        const uint32_t bp = __shfl_up(inc, 1, 64);
+        before[j] = lane == start[j] ? 0u : bp;
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) old[j] = head[j] && want[j] ? atomicOr(&V[w[j]], suf[j]) : 0u;
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        if (head[j] && want[j] && (suf[j] & ~old[j])) atomicOr(&Fn[w[j]], suf[j] & ~old[j]);
+        old[j] = __shfl(old[j], start[j], 64);
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        if (want[j] && !(old[j] & b[j]) && !(before[j] & b[j])) {
+            level[d[j]] = round;
+            parent[d[j]] = v[j];
+            cnt++;
+            deg += (unsigned long long)(out_off[d[j] + 1] - out_off[d[j]]);
+        }
+    }
 }
 
 // Top-down round over the frontier bitmap: no list and no scan, so the grid does
@@ -884,6 +945,9 @@ __device__ __forceinline__ void bfs_claim(int v, int d, uint32_t *V, int32_t *le
 // frontier is not one lane's (or one wave's) serial loop.
 constexpr int kBfsLaneDeg = 8;
 constexpr int kBfsWaveDeg = 1024;
+// slices of 64 edges a wave claims at once: the small out-lists pushed by their
+// lanes (bfs_push_bits), a queued big vertex's, the huge vertices' batches
+constexpr int kBfsLaneJ = 1, kBfsBigJ = 2, kBfsHugeJ = 2;
 
 __device__ __forceinline__ void bfs_push_bits(
     const uint32_t *__restrict__ F, int64_t nw, const int32_t *__restrict__ out_off,
@@ -896,18 +960,25 @@ __device__ __forceinline__ void bfs_push_bits(
          base < nw; base += nwaves * 64) {
         uint32_t x = base + lane < nw ? F[base + lane] : 0u;
         while (__ballot(x != 0u)) {
-            int v = -1, dg = 0;
+            int v = -1, dg = 0, beg = 0;
             if (x) {
                 v = (int)((base + lane) * 32 + __ffs(x) - 1);
                 x &= x - 1;
-                const int beg = out_off[v];
+                beg = out_off[v];
                 dg = out_off[v + 1] - beg;
-                if (dg <= kBfsLaneDeg) {
-                    for (int e = beg; e < beg + dg; e++)
-                        bfs_claim(v, out_adj[e], V, level, parent, Fn, round, out_off, cnt, deg);
-                    v = -1;
-                }
             }
+            // a small out-list is pushed by its own lane, the wave stepping together
+            const int own = v >= 0 && dg <= kBfsLaneDeg ? dg : 0;
+            for (int k = 0; __ballot(k < own); k += kBfsLaneJ) {
+                int vs[kBfsLaneJ], ds[kBfsLaneJ];
+#pragma unroll
+                for (int j = 0; j < kBfsLaneJ; j++) {
+                    vs[j] = v;
+                    ds[j] = k + j < own ? out_adj[beg + k + j] : -1;
+                }
+                bfs_claim_wave<kBfsLaneJ>(vs, ds, V, level, parent, Fn, round, out_off, cnt, deg);
+            }
+            if (v >= 0 && dg <= kBfsLaneDeg) v = -1;
             agg_append(qbig, &qctr[0], v >= 0 && dg <= kBfsWaveDeg, v);
             agg_append(qhuge, &qctr[1], v >= 0 && dg > kBfsWaveDeg, v);
         }
@@ -1039,14 +1110,21 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
         if (threadIdx.x == 0) s_pre[0] = 0;
         __syncthreads();
         const int64_t tot = s_pre[nbat];
-        for (int64_t k = gt; k < tot; k += gn) {
-            int lo = 0, hi = nbat - 1;  // last j with s_pre[j] <= k
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= k) lo = mid; else hi = mid - 1;
+        // a wave takes kBfsHugeJ consecutive 64-edge slices per step
+        for (int64_t k0 = (gt >> 6) * (64 * kBfsHugeJ); k0 < tot; k0 += gn * kBfsHugeJ) {
+            int vs[kBfsHugeJ], ds[kBfsHugeJ];
+#pragma unroll
+            for (int j = 0; j < kBfsHugeJ; j++) {
+                const int64_t k = k0 + 64 * j + lane_id();
+                int lo = 0, hi = nbat - 1;  // last i with s_pre[i] <= k
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= k) lo = mid; else hi = mid - 1;
+                }
+                vs[j] = s_v[lo];
+                ds[j] = k < tot ? out_adj[s_beg[lo] + (int)(k - s_pre[lo])] : -1;
             }
-            const int e = s_beg[lo] + (int)(k - s_pre[lo]);
-            bfs_claim(s_v[lo], out_adj[e], V, level, parent, Fn, round, out_off, cnt, deg);
+            bfs_claim_wave<kBfsHugeJ>(vs, ds, V, level, parent, Fn, round, out_off, cnt, deg);
         }
         __syncthreads();
     }
@@ -1063,9 +1141,16 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push_queue(
                 d[k] = e < end ? out_adj[e] : -1;
             }
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (d[k] >= 0)
-                    bfs_claim(v, d[k], V, level, parent, Fn, round, out_off, cnt, deg);
+            for (int k = 0; k < 4; k += kBfsBigJ) {
+                if (base + 64 * k >= end) break;  // wave-uniform
+                int vs[kBfsBigJ], ds[kBfsBigJ];
+#pragma unroll
+                for (int j = 0; j < kBfsBigJ; j++) {
+                    vs[j] = v;
+                    ds[j] = d[k + j];
+                }
+                bfs_claim_wave<kBfsBigJ>(vs, ds, V, level, parent, Fn, round, out_off, cnt, deg);
+            }
         }
     }
     bfs_flush_stats(cnt, deg, cur);
