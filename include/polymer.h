@@ -975,6 +975,92 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sm, ui
     return r;
 }
 
+// ---- totals a launch reports to the host ---------------------------------------------
+// The blocks of a counting launch add partial sums into kTotSlots spread slot pairs
+// of a device buffer that holds zeros between launches; one single-block kernel
+// after it adds the slots up into pinned host memory and clears them.  A call that
+// reports counts thus needs no clearing copy before its kernel and no copy back
+// after it (a copy into pageable memory measured 22 us against 12 us for a launch
+// and a synchronisation), and no block waits for the others (a last-block ticket
+// serialised 2048 blocks on one word).
+constexpr int kTotSlots = 32;  // slot pairs
+struct Totals {
+    unsigned long long *slots;  // device, 2 * kTotSlots words
+    unsigned long long *host;   // the kernels' address of the pinned words
+    unsigned long long *cpu;    // the host's address: the two totals and a sequence number
+    unsigned long long seq;     // the sequence number of the last finishing launch
+};
+inline Totals &totals() {
+    static Totals t{nullptr, nullptr, nullptr, 0};
+    if (!t.slots) {
+        ensure_init();
+        const size_t bytes = 2 * kTotSlots * sizeof(unsigned long long);
+        check_hip(hipMalloc(&t.slots, bytes), "hipMalloc");
+        check_hip(hipMemsetAsync(t.slots, 0, bytes, lib_stream()), "hipMemsetAsync");
+        check_hip(hipHostMalloc(&t.cpu, 4 * sizeof(unsigned long long), hipHostMallocDefault),
+                  "hipHostMalloc");
+        check_hip(hipHostGetDevicePointer((void **)&t.host, t.cpu, 0), "hipHostGetDevicePointer");
+        t.cpu[0] = t.cpu[1] = t.cpu[2] = 0;
+        sync();
+    }
+    return t;
+}
+// The two totals into the pinned words, then (after a system-scope fence) the
+// sequence number that tells the host they are there.
+__device__ __forceinline__ void publish_totals(unsigned long long *host, unsigned long long a,
+                                               unsigned long long b, unsigned long long seq) {
+    host[0] = a;
+    host[1] = b;
+    __threadfence_system();
+    host[2] = seq;
+    __threadfence_system();
+}
+// the slot pairs' sums published; the slots cleared for the next launch
+__global__ void k_totals(unsigned long long *slots, unsigned long long *host,
+                         unsigned long long seq) {
+    __shared__ unsigned long long s[2 * kTotSlots];
+    const int i = threadIdx.x;  // 2 * kTotSlots threads
+    s[i] = slots[i];
+    slots[i] = 0ull;
+    __syncthreads();
+    if (i == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int k = 0; k < kTotSlots; k++) {
+            a += s[2 * k];
+            b += s[2 * k + 1];
+        }
+        publish_totals(host, a, b, seq);
+    }
+}
+__global__ void k_publish_count(const uint32_t *count, unsigned long long *host,
+                                unsigned long long seq) {
+    publish_totals(host, *count, 0ull, seq);
+}
+inline void finish_totals() {
+    Totals &t = totals();
+    hipLaunchKernelGGL(k_totals, dim3(1), dim3(2 * kTotSlots), 0, lib_stream(), t.slots, t.host,
+                       ++t.seq);
+    launched("totals");
+}
+// The totals of the last publishing launch (finish_totals or the compaction's
+// scan).  Everything queued before that launch has finished once its sequence
+// number is visible, so the host spins on that word (a wake-up from a stream
+// synchronisation measured 12 us for a launch against 6 us for the spin) and
+// falls back to the synchronisation, which reports a failed kernel, after 2 ms.
+inline void read_totals(unsigned long long *a, unsigned long long *b) {
+    Totals &t = totals();
+    volatile unsigned long long *h = t.cpu;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (h[2] != t.seq) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+            sync();
+            break;
+        }
+    }
+    *a = h[0];
+    if (b) *b = h[1];
+}
+
 // The set flags among b[i, i + 16) as the bits of a mask (bit j = flag i + j; i a
 // multiple of 16): one 16-byte load when b is 16-byte aligned and the block lies
 // inside [0, n), byte loads otherwise (a program's own offset array, the tail).
@@ -1017,7 +1103,9 @@ __global__ __launch_bounds__(kT) void k_scan_small(const uint32_t *in, uint32_t 
         if (i < n) out[i] = carry + x;
         carry += tot;
     }
-    if (threadIdx.x == 0) out[n] = carry;
+    if (threadIdx.x == 0) {
+        out[n] = carry;
+    }
 }
 
 // ordered compaction: the ids (+ offset) of the set flags, block by block; a
@@ -1043,12 +1131,16 @@ inline intT compact_flags(const bool *b, int64_t n, intT offset, intT *s) {
     uint32_t *cnt = workspace_t<uint32_t>("compact_cnt", (size_t)nb + 1);
     uint32_t *st = workspace_t<uint32_t>("compact_start", (size_t)nb + 1);
     hipLaunchKernelGGL(k_flag_counts, dim3((unsigned)nb), dim3(kT), 0, lib_stream(), b, n, cnt);
+    Totals &t = totals();
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kT), 0, lib_stream(), (const uint32_t *)cnt, st, nb);
     hipLaunchKernelGGL(k_flag_write, dim3((unsigned)nb), dim3(kT), 0, lib_stream(), b, n,
                        (const uint32_t *)st, offset, s);
+    // the count, published once the list is written
+    hipLaunchKernelGGL(k_publish_count, dim3(1), dim3(1), 0, lib_stream(), (const uint32_t *)(st + nb),
+                       t.host, ++t.seq);
     launched("compaction");
-    uint32_t total = 0;
-    toHost(&total, st + nb, 1);
+    unsigned long long total = 0;
+    read_totals(&total, nullptr);
     return (intT)total;
 }
 
@@ -1142,7 +1234,7 @@ constexpr int kWin = 1024;
 constexpr int kPushEdgesMax = 2048;  // edges per chunk (the block's append stage)
 constexpr int kPushBlocks = 2048;    // grid of the push (8 workgroups per CU)
 
-template <bool W, int Mode, class F>
+template <bool W, int Mode, bool Small, class F>
 __global__ __launch_bounds__(kT) void k_push_edges(F f, pg_view v, const intT *list,
                                                    const uint32_t *scan, int64_t count, bool *next,
                                                    intT *out_list, size_t cap,
@@ -1155,7 +1247,40 @@ __global__ __launch_bounds__(kT) void k_push_edges(F f, pg_view v, const intT *l
     __shared__ intT s_out[Mode == 1 ? kPushEdgesMax : 1];
     __shared__ uint32_t s_cnt;
     __shared__ unsigned long long s_base;
-    const uint32_t total = scan[count];
+    uint32_t total;
+    if constexpr (Small) {
+        // a list of at most kWin vertices: every block forms the degree prefix of
+        // the whole list in LDS (kWin / kT consecutive entries per thread), so no
+        // scan kernels run before the push
+        __shared__ uint32_t sm[kT / 64 + 1];
+        uint32_t d[kWin / kT], run = 0;
+#pragma unroll
+        for (int k = 0; k < kWin / kT; k++) {
+            const int64_t i = (int64_t)threadIdx.x * (kWin / kT) + k;
+            int32_t b = 0, e = 0;
+            if (i < count) {
+                const intT u = list[i];
+                b = v.out_off[u];
+                e = v.out_off[u + 1];
+                s_beg[i] = b;
+            }
+            d[k] = (uint32_t)(e - b);
+            run += d[k];
+        }
+        uint32_t tot = 0;
+        uint32_t x = block_excl_scan(run, sm, &tot);
+#pragma unroll
+        for (int k = 0; k < kWin / kT; k++) {
+            const int64_t i = (int64_t)threadIdx.x * (kWin / kT) + k;
+            if (i < count) s_scan[i] = x;
+            x += d[k];
+        }
+        if (threadIdx.x == 0) s_scan[count] = tot;
+        __syncthreads();
+        total = tot;
+    } else {
+        total = scan[count];
+    }
     const uint32_t epb = (total + gridDim.x - 1) / gridDim.x;
     const uint32_t B0 = min(total, (uint32_t)blockIdx.x * epb);
     const uint32_t B1 = min(total, B0 + epb);
@@ -1165,24 +1290,29 @@ __global__ __launch_bounds__(kT) void k_push_edges(F f, pg_view v, const intT *l
         const uint32_t E1 = min(B1, E0 + kPushEdgesMax);
         if (threadIdx.x == 0) {
             s_cnt = 0;
-            int64_t lo = 0, hi = count + 1;  // last j with scan[j] <= E0
-            while (lo + 1 < hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                if (scan[mid] <= E0) lo = mid; else hi = mid;
+            if constexpr (Small) {
+                s_j0 = 0;
+                s_j1 = count - 1;
+            } else {
+                int64_t lo = 0, hi = count + 1;  // last j with scan[j] <= E0
+                while (lo + 1 < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (scan[mid] <= E0) lo = mid; else hi = mid;
+                }
+                s_j0 = lo;
+                lo = 0;
+                hi = count + 1;  // last j with scan[j] <= E1 - 1
+                while (lo + 1 < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (scan[mid] <= E1 - 1) lo = mid; else hi = mid;
+                }
+                s_j1 = lo;
             }
-            s_j0 = lo;
-            lo = 0;
-            hi = count + 1;  // last j with scan[j] <= E1 - 1
-            while (lo + 1 < hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                if (scan[mid] <= E1 - 1) lo = mid; else hi = mid;
-            }
-            s_j1 = lo;
         }
         __syncthreads();
         const int64_t j0 = s_j0, nwin = s_j1 - s_j0 + 1;
-        const bool staged = nwin <= kWin;
-        if (staged) {
+        const bool staged = Small || nwin <= kWin;
+        if (!Small && staged) {
             for (int64_t i = threadIdx.x; i < nwin; i += kT) {
                 s_scan[i] = scan[j0 + i];
                 s_beg[i] = v.out_off[list[j0 + i]];
@@ -1245,20 +1375,35 @@ __global__ __launch_bounds__(kT) void k_push_edges(F f, pg_view v, const intT *l
         __syncthreads();  // the chunk's LDS is reused by the next one
     }
     if constexpr (Mode == 1) {
+        // the successes' out-degree sum into slot pair blockIdx mod kTotSlots (ctr[0],
+        // the append position, is the first word of pair 0)
         for (int o = 32; o >= 1; o >>= 1) outdeg += __shfl_xor(outdeg, o, 64);
-        if (lane() == 0 && outdeg) atomicAdd(&ctr[1 + (blockIdx.x & 31)], outdeg);
+        if (lane() == 0 && outdeg) atomicAdd(&ctr[2 * (blockIdx.x % kTotSlots) + 1], outdeg);
     }
 }
 
 template <bool W, int Mode, class F>
 inline void push_list(F &f, const pg_view &v, const intT *list, int64_t count, bool *next,
-                      intT *out_list, size_t cap, unsigned long long *ctr) {
+                      intT *out_list, size_t cap, long long edges_hint = -1) {
     if (count <= 0) return;
-    uint32_t *scan = workspace_t<uint32_t>("push_scan", (size_t)count + 1);
-    degree_scan(list, count, v.out_off, scan);
-    hipLaunchKernelGGL((k_push_edges<W, Mode, F>), dim3(kPushBlocks), dim3(kT), 0, lib_stream(), f, v,
-                       list, (const uint32_t *)scan, count, next, out_list, cap, ctr);
+    const Totals &t = totals();  // Mode 1: successes and their out-degree sum in the slots
+    if (count <= kWin) {
+        // the degree prefix in each block's LDS; the grid from the frontier's
+        // out-degree sum when the caller knows it (a stale hint costs time only)
+        unsigned grid = kPushBlocks;
+        if (edges_hint > 0)
+            grid = (unsigned)std::min<long long>(kPushBlocks, (edges_hint + 1023) / 1024);
+        hipLaunchKernelGGL((k_push_edges<W, Mode, true, F>), dim3(grid), dim3(kT), 0, lib_stream(), f,
+                           v, list, (const uint32_t *)nullptr, count, next, out_list, cap, t.slots);
+    } else {
+        uint32_t *scan = workspace_t<uint32_t>("push_scan", (size_t)count + 1);
+        degree_scan(list, count, v.out_off, scan);
+        hipLaunchKernelGGL((k_push_edges<W, Mode, false, F>), dim3(kPushBlocks), dim3(kT), 0,
+                           lib_stream(), f, v, list, (const uint32_t *)scan, count, next, out_list,
+                           cap, t.slots);
+    }
     launched("edge push");
+    if constexpr (Mode == 1) finish_totals();
 }
 
 // Dense pull over the in-degree bins of the graph (Bins).  A destination with
@@ -1677,7 +1822,7 @@ __global__ __launch_bounds__(kT) void k_vertex_filter(F f, const bool *b, int64_
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < kT / 64; w++) c += sc[w];
-        if (c) atomicAdd(&count[blockIdx.x & 15], c);  // 16 slots, summed by the caller
+        if (c && count) atomicAdd(&count[2 * (blockIdx.x % kTotSlots)], c);  // spread slots
     }
 }
 
@@ -1715,9 +1860,9 @@ __global__ __launch_bounds__(kT) void k_counter(const bool *b, int64_t n, intT o
             c += sc[w];
             e += se[w];
         }
-        // 16 slot pairs (count, out-degree sum), summed by the caller: a pair shared
-        // by every block of the grid serialises its atomics
-        unsigned long long *slot = out + 2 * (blockIdx.x & 15);
+        // kTotSlots slot pairs (count, out-degree sum): a pair shared by every block
+        // of the grid serialises its atomics
+        unsigned long long *slot = out + 2 * (blockIdx.x % kTotSlots);
         if (c) atomicAdd(&slot[0], c);
         if (e) atomicAdd(&slot[1], e);
     }
@@ -1784,18 +1929,16 @@ struct LocalFrontier {
     bool isDense;
     size_t sCap = 0;        // capacity of s when the header allocated it
     bool sOwned = false;    // s was allocated here
-    unsigned long long *counters = nullptr;  // device: the count and out-degree sum slots of calls on it
 
     LocalFrontier(bool *_b, intT start, intT end)
         : n(end - start), m(0), outEdgesCount(0), startID(start), endID(end), b(_b), s(nullptr),
           isDense(true) {
         polymer_detail::ensure_init();
-        counters = polymer_detail::dev_alloc<unsigned long long>(40);
+        (void)polymer_detail::totals();
         reserveSparse((size_t)n + 1);  // made here, before a program's loop, not in a round
     }
     ~LocalFrontier() {
         if (s && sOwned) polymer_detail::shared_free(s);
-        if (counters) (void)hipFree(counters);
     }
     LocalFrontier(const LocalFrontier &) = delete;
     LocalFrontier &operator=(const LocalFrontier &) = delete;
@@ -2010,7 +2153,8 @@ void dense_push(G &GA, vertices *frontier, F &f, LocalFrontier *next) {
     constexpr bool W = is_weighted<G>::value && Mode != 2;
     intT *list = nullptr;
     const intT count = list_active(frontier, &list);
-    push_list<W, Mode>(f, GA.view, list, count, next->b, nullptr, 0, next->counters);
+    push_list<W, Mode>(f, GA.view, list, count, next->b, nullptr, 0,
+                       frontier->getFrontier(0)->outEdgesCount);
     sync_visible();
 }
 
@@ -2089,25 +2233,21 @@ void edgeMapSparseV3(G GA, vertices *frontier, F f, LocalFrontier *next, bool pa
     LocalFrontier *cur = frontier->getFrontier(0);
     const long long bufferLen = frontier->getEdgeStat();
     next->reserveSparse((size_t)(bufferLen > 0 ? bufferLen : 1));
-    check_hip(hipMemsetAsync(next->counters, 0, 40 * sizeof(unsigned long long), lib_stream()),
-              "hipMemsetAsync");
     const int64_t count = cur->m;
+    unsigned long long c = 0, e = 0;
     if (count > 0) {
         const intT *list = device_readable(cur->s, (size_t)count, "sparse_in");
-        push_list<W, 1>(f, GA.view, list, count, nullptr, next->s, next->sCap, next->counters);
+        push_list<W, 1>(f, GA.view, list, count, nullptr, next->s, next->sCap, bufferLen);
+        read_totals(&c, &e);
     }
-    unsigned long long c[40];
-    toHost(c, next->counters, 40);
-    unsigned long long e = 0;
-    for (int i = 1; i < 40; i++) e += c[i];
-    if (c[0] > next->sCap) {
+    if (c > next->sCap) {
         std::fprintf(stderr,
                      "polymer: edgeMapSparseV3: %llu successes exceed the list of %zu entries sized "
                      "from getEdgeStat()\n",
-                     c[0], next->sCap);
+                     c, next->sCap);
         std::abort();
     }
-    next->m = (intT)c[0];
+    next->m = (intT)c;
     next->outEdgesCount = (intT)e;
     next->isDense = false;
 }
@@ -2152,12 +2292,10 @@ void edgeMapSparseAsyncPipe(G GA, vertices *frontier, F f, LocalFrontier *next,
         check_hip(hipMemcpyAsync(qa, device_readable(cur->s, (size_t)count, "async_in"),
                                  sizeof(intT) * (size_t)count, hipMemcpyDefault, lib_stream()),
                   "hipMemcpyAsync");
-    unsigned long long *ctr = workspace_t<unsigned long long>("async_ctr", 40);
     while (count > 0) {
-        check_hip(hipMemsetAsync(ctr, 0, 40 * sizeof(unsigned long long), lib_stream()), "hipMemsetAsync");
-        push_list<W, 1>(f, GA.view, qa, count, nullptr, qb, cap, ctr);
+        push_list<W, 1>(f, GA.view, qa, count, nullptr, qb, cap);
         unsigned long long c = 0;
-        toHost(&c, ctr, 1);
+        read_totals(&c, nullptr);
         if (c > cap) fail("edgeMapSparseAsyncPipe: more activations in one pass than edges");
         count = (int64_t)c;
         std::swap(qa, qb);
@@ -2205,18 +2343,13 @@ void vertexCounter(G &GA, LocalFrontier *frontier, int nodeNum, int subNum, int 
     using namespace polymer_detail;
     (void)totalSub;
     if (!frontier->isDense || nodeNum != 0 || subNum != 0) return;
-    check_hip(hipMemsetAsync(frontier->counters, 0, 32 * sizeof(unsigned long long), lib_stream()),
-              "hipMemsetAsync");
     hipLaunchKernelGGL(k_counter, dim3(blocks_for_flags(frontier->n, 2048)), dim3(256), 0, lib_stream(),
                        (const bool *)frontier->b, (int64_t)frontier->n, frontier->startID,
-                       GA.view.out_off, frontier->counters);
+                       GA.view.out_off, totals().slots);
     launched("vertexCounter");
-    unsigned long long c[32], cnt = 0, e = 0;
-    toHost(c, frontier->counters, 32);
-    for (int i = 0; i < 16; i++) {
-        cnt += c[2 * i];
-        e += c[2 * i + 1];
-    }
+    finish_totals();
+    unsigned long long cnt = 0, e = 0;
+    read_totals(&cnt, &e);
     frontier->m = (intT)cnt;
     frontier->outEdgesCount = (intT)e;
 }
@@ -2270,11 +2403,9 @@ void vertexFilter(vertices *V, F filter, int nodeNum, bool *result) {
     using namespace polymer_detail;
     V->toDense();
     LocalFrontier *fr = V->getFrontier(nodeNum);
-    unsigned long long *cnt = workspace_t<unsigned long long>("filter_cnt", 16);
-    check_hip(hipMemsetAsync(cnt, 0, 16 * sizeof(unsigned long long), lib_stream()), "hipMemsetAsync");
     hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n, 256, 2048)), dim3(256), 0, lib_stream(), filter,
                        (const bool *)fr->b, (int64_t)0, (int64_t)fr->n, V->getOffset(nodeNum), result,
-                       cnt);
+                       (unsigned long long *)nullptr);  // no count
     launched("vertexFilter");
     sync_visible();
 }
@@ -2288,15 +2419,13 @@ void vertexFilter(vertices *V, F filter, int nodeNum, int subNum, int totalSub, 
     if (subNum != 0) return;
     V->toDense();
     LocalFrontier *fr = V->getFrontier(nodeNum);
-    check_hip(hipMemsetAsync(result->counters, 0, 16 * sizeof(unsigned long long), lib_stream()),
-              "hipMemsetAsync");
     hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n, 256, 2048)), dim3(256), 0, lib_stream(), filter,
                        (const bool *)fr->b, (int64_t)0, (int64_t)fr->n, V->getOffset(nodeNum), result->b,
-                       result->counters);
+                       totals().slots);
     launched("vertexFilter");
-    unsigned long long c[16], tot = 0;
-    toHost(c, result->counters, 16);
-    for (int i = 0; i < 16; i++) tot += c[i];
+    finish_totals();
+    unsigned long long tot = 0;
+    read_totals(&tot, nullptr);
     result->m += (intT)tot;
     result->isDense = true;
 }
