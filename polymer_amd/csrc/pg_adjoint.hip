@@ -166,8 +166,7 @@ static int fixed_point(pg_graph *g, const PullSchedule &sched, float *a, float *
             PG_CHECK_HIP(hipGetLastError());
             std::swap(cur, nxt);
         }
-        PG_CHECK_HIP(hipMemcpyAsync(h, state, sizeof(h), hipMemcpyDeviceToHost, stream()));
-        PG_CHECK_HIP(hipStreamSynchronize(stream()));
+        PG_TRY(fetch_words(h, state, sizeof(h)));
     }
     const int used = h[0] ? h[1] : max_iters;
     // iteration t writes b when t is even, a when it is odd

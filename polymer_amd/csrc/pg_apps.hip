@@ -1327,8 +1327,7 @@ static int clear_frontier(pg_graph *g, Frontier &f, bool bits = true) {
 // direction switch
 static int read_counts(Frontier &f, int64_t *count, int64_t *edges) {
     unsigned long long h[kCtrWords];
-    PG_CHECK_HIP(hipMemcpyAsync(h, f.ctr, sizeof(h), hipMemcpyDeviceToHost, stream()));
-    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    PG_TRY(fetch_words(h, f.ctr, sizeof(h)));
     unsigned long long c = h[0], d = h[1];
     for (int k = 0; k < kCtrSlots; k++) {
         c += h[4 + 2 * k];
@@ -1465,8 +1464,7 @@ int bfs_run(pg_graph *g, int64_t src, int64_t threshold, uint32_t flags, pg_stat
         launched_rounds += L;
         const unsigned long long *last =
             slots + (size_t)(launched_rounds % kBfsRegions) * kBfsRegion;
-        PG_CHECK_HIP(hipMemcpyAsync(hs, last, sizeof(hs), hipMemcpyDeviceToHost, stream()));
-        PG_CHECK_HIP(hipStreamSynchronize(stream()));
+        PG_TRY(fetch_words(hs, last, sizeof(hs)));
         unsigned long long c = 0;
         for (int i = 0; i < 64; i++) c += hs[2 * i];
         if (hs[kBfsCtlDone] || c == 0) break;
@@ -1723,8 +1721,7 @@ extern "C" int pg_pagerank_delta(pg_graph *g, int iters, double damping, double 
         std::swap(c, cn);
         if (t + 1 < iters) {  // the next frontier's size and out-degree sum
             unsigned long long hs[kStatSlots];
-            PG_CHECK_HIP(hipMemcpyAsync(hs, st_t, sizeof(hs), hipMemcpyDeviceToHost, stream()));
-            PG_CHECK_HIP(hipStreamSynchronize(stream()));
+            PG_TRY(fetch_words(hs, st_t, sizeof(hs)));
             unsigned long long sum = 0;
             for (int k = 0; k < kStatSlots; k++) sum += hs[k];
             cnt = (int64_t)(sum >> 32);
@@ -2488,8 +2485,7 @@ static int bp_death_times(pg_graph *g) {
         PG_LAUNCH_K(k_flags_to_bits, grid_for(n), fa, n, bits);
         PG_TRY(launch_pull(BpReachOp{fa, fb, D, bits, t + 1}, g->in_sched, cnt));
         unsigned long long h[kStatSlots];
-        PG_CHECK_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, stream()));
-        PG_CHECK_HIP(hipStreamSynchronize(stream()));
+        PG_TRY(fetch_words(h, cnt, sizeof(h)));
         unsigned long long left = 0;
         for (int k = 0; k < kStatSlots; k++) left += h[k];
         if (left == 0) state = 1;            // fixed point: the rest stay (255)

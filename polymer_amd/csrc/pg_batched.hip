@@ -738,8 +738,7 @@ __global__ void k_mssssp_seed(const int32_t *src, int k, unsigned long long *F, 
 
 static int sum_slots(const unsigned long long *d, unsigned long long *out) {
     unsigned long long h[kStatSlots];
-    PG_CHECK_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, stream()));
-    PG_CHECK_HIP(hipStreamSynchronize(stream()));
+    PG_TRY(fetch_words(h, d, sizeof(h)));
     unsigned long long s = 0;
     for (int i = 0; i < kStatSlots; i++) s += h[i];
     *out = s;
@@ -953,8 +952,7 @@ extern "C" int pg_msbfs(pg_graph *g, const int64_t *src, int k, int32_t *level, 
             PG_TRY(bits_to_list(g, Fany, list, &lctr[0]));
             PG_TRY(list_degree_sum(g, list, &lctr[0], g->out_off, lctr + 5, 2));
             unsigned long long h[kMsCtrWords];
-            PG_CHECK_HIP(hipMemcpyAsync(h, lctr, sizeof(h), hipMemcpyDeviceToHost, stream()));
-            PG_CHECK_HIP(hipStreamSynchronize(stream()));
+            PG_TRY(fetch_words(h, lctr, sizeof(h)));
             int64_t lc = (int64_t)h[0], le = 0;
             for (int j = 0; j < kStatSlots; j++) le += (int64_t)h[5 + 2 * j];
             pushed += le;

@@ -314,6 +314,10 @@ void free_tiles(pg_graph *g);
 int exclusive_scan_u32(pg_graph *g, const uint32_t *in, uint32_t *out, int64_t n);
 
 // event timing on the library stream
+// bytes (a multiple of 4, at most 4 KiB) of device memory into host memory once
+// everything queued on the library stream before the call has finished
+int fetch_words(void *host, const void *dev, size_t bytes);
+
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
     int start();
