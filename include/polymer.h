@@ -226,6 +226,16 @@ inline void sync_visible() {
 
 inline void launched(const char *what) { check_hip(hipGetLastError(), what); }
 
+// Bumped by every call that may change a frontier's flags or a program's device
+// data (copies in, vertex maps and filters, the edge maps, clearing and filling):
+// counts a dense edgeMap took of its output stay valid for vertexCounter only while
+// nothing else has run.
+inline unsigned long long &frontier_gen() {
+    static unsigned long long g = 1;
+    return g;
+}
+inline void frontiers_changed() { ++frontier_gen(); }
+
 inline unsigned blocks_for(int64_t n, int threads = 256, int64_t cap = 65536) {
     int64_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -325,6 +335,7 @@ static bitmask *numa_all_nodes_ptr = nullptr;
 /* ---- copies between host and the device arrays of an app ----------------- */
 template <class T>
 void toDevice(T *dev, const T *host, size_t count) {
+    polymer_detail::frontiers_changed();
     polymer_detail::check_hip(hipMemcpyAsync(dev, host, count * sizeof(T), hipMemcpyHostToDevice,
                                              polymer_detail::lib_stream()),
                               "hipMemcpyAsync");
@@ -1546,10 +1557,40 @@ __device__ __forceinline__ bool call_update(F &f, intT s, intT d, const intE *w,
 // BFS claim) updates d from the first active in-neighbour in list order.
 constexpr int kLaneRest = 32;
 
-template <bool W, class F>
+// (Count: the destinations this pass marks in next and their out-degree sum go to
+// the totals slots, for an output the caller cleared before)
+template <bool Count>
+__device__ __forceinline__ void count_to_slots(unsigned long long c, unsigned long long e,
+                                               unsigned long long *slots) {
+    if constexpr (Count) {
+        __shared__ unsigned long long sc[kT / 64], se[kT / 64];
+        for (int o = 32; o > 0; o >>= 1) {
+            c += __shfl_xor(c, o, 64);
+            e += __shfl_xor(e, o, 64);
+        }
+        if (lane() == 0) {
+            sc[threadIdx.x >> 6] = c;
+            se[threadIdx.x >> 6] = e;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kT / 64; w++) {
+                c += sc[w];
+                e += se[w];
+            }
+            unsigned long long *slot = slots + 2 * (blockIdx.x % kTotSlots);
+            if (c) atomicAdd(&slot[0], c);
+            if (e) atomicAdd(&slot[1], e);
+        }
+    }
+}
+
+template <bool W, bool Count, class F>
 __global__ __launch_bounds__(kT) void k_pull_first(F f, pg_view v, const intT *first_in,
-                                                   FrontierBits cur, bool *next, uint32_t *qbits) {
+                                                   FrontierBits cur, bool *next, uint32_t *qbits,
+                                                   unsigned long long *slots) {
     const int64_t n = v.n;
+    unsigned long long cnt = 0, deg = 0;
     const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
     const int64_t stride = (int64_t)gridDim.x * kT;
     for (int64_t base = (int64_t)blockIdx.x * kT; base < n; base += stride) {  // block-uniform
@@ -1586,7 +1627,13 @@ __global__ __launch_bounds__(kT) void k_pull_first(F f, pg_view v, const intT *f
                     }
                 }
             }
-            if (hit) next[d] = true;
+            if (hit) {
+                next[d] = true;
+                if constexpr (Count) {
+                    cnt++;
+                    deg += (unsigned long long)(v.out_off[d + 1] - v.out_off[d]);
+                }
+            }
         }
         const unsigned long long qm = __ballot(queued);
         const int64_t wd = (base + (threadIdx.x & ~63)) >> 5;  // this wave's first word
@@ -1594,6 +1641,7 @@ __global__ __launch_bounds__(kT) void k_pull_first(F f, pg_view v, const intT *f
         if (lane() == 0 && wd < nq) qbits[wd] = (uint32_t)qm;
         if (lane() == 1 && wd + 1 < nq) qbits[wd + 1] = (uint32_t)(qm >> 32);
     }
+    count_to_slots<Count>(cnt, deg, slots);
 }
 
 // The destinations k_pull_first marked, a wave each (a wave takes the marks of 64
@@ -1602,10 +1650,12 @@ __global__ __launch_bounds__(kT) void k_pull_first(F f, pg_view v, const intT *f
 // if cond(d) still holds, the step's other active entries together; the scan stops
 // once cond(d) is false.  Lanes of a wave may update d at once here, so this part
 // calls updateAtomic.
-template <bool W, class F>
+template <bool W, bool Count, class F>
 __global__ __launch_bounds__(kT) void k_pull_queue(F f, pg_view v, FrontierBits cur, bool *next,
-                                                   const uint32_t *qbits) {
+                                                   const uint32_t *qbits,
+                                                   unsigned long long *slots) {
     const int64_t ngroups = (v.n + 63) / 64, nq = (v.n + 31) / 32;
+    unsigned long long cnt = 0, deg = 0;
     const int l = lane();
     const intE *w = W ? static_cast<const intE *>(v.in_w) : nullptr;
     const int64_t nwaves = ((int64_t)gridDim.x * kT) >> 6;
@@ -1633,19 +1683,41 @@ __global__ __launch_bounds__(kT) void k_pull_queue(F f, pg_view v, FrontierBits 
             }
             if (!group_any<64>(f.cond(d))) break;
         }
-        if (group_any<64>(hit) && l == 0) next[d] = true;
+        if (group_any<64>(hit) && l == 0) {
+            next[d] = true;
+            if constexpr (Count) {
+                cnt++;
+                deg += (unsigned long long)(v.out_off[d + 1] - v.out_off[d]);
+            }
+        }
     }
+    count_to_slots<Count>(cnt, deg, slots);
 }
 
+// (counted: next was cleared before, and the number of destinations the pull marks
+// and their out-degree sum are published with finish_totals)
 template <bool W, class F>
-inline void pull_dense(F &f, const pg_view &v, FrontierBits cur, bool *next) {
+inline void pull_dense(F &f, const pg_view &v, FrontierBits cur, bool *next, bool counted = false) {
     const Bins &b = bins_of(v);
     uint32_t *qbits = workspace_t<uint32_t>("pull_marks", (size_t)(v.n + 31) / 32 + 1);
-    hipLaunchKernelGGL((k_pull_first<W, F>), dim3(blocks_for(v.n)), dim3(kT), 0, lib_stream(), f, v,
-                       (const intT *)b.first_in, cur, next, qbits);
-    hipLaunchKernelGGL((k_pull_queue<W, F>), dim3(blocks_for((v.n + 63) / 64 * 64, kT, 8192)), dim3(kT),
-                       0, lib_stream(), f, v, cur, next, (const uint32_t *)qbits);
-    launched("dense pull");
+    const unsigned gq = blocks_for((v.n + 63) / 64 * 64, kT, 8192);
+    if (counted) {
+        unsigned long long *slots = totals().slots;
+        // at most 8192 blocks, so the slots take few atomics
+        hipLaunchKernelGGL((k_pull_first<W, true, F>), dim3(blocks_for(v.n, kT, 8192)), dim3(kT), 0,
+                           lib_stream(), f, v, (const intT *)b.first_in, cur, next, qbits, slots);
+        hipLaunchKernelGGL((k_pull_queue<W, true, F>), dim3(gq), dim3(kT), 0, lib_stream(), f, v, cur,
+                           next, (const uint32_t *)qbits, slots);
+        launched("dense pull");
+        finish_totals();
+    } else {
+        hipLaunchKernelGGL((k_pull_first<W, false, F>), dim3(blocks_for(v.n)), dim3(kT), 0, lib_stream(),
+                           f, v, (const intT *)b.first_in, cur, next, qbits,
+                           (unsigned long long *)nullptr);
+        hipLaunchKernelGGL((k_pull_queue<W, false, F>), dim3(gq), dim3(kT), 0, lib_stream(), f, v, cur,
+                           next, (const uint32_t *)qbits, (unsigned long long *)nullptr);
+        launched("dense pull");
+    }
 }
 
 // edgeMapDenseReduce (polymer.h:1097-1168) over the in-degree bins: the G lanes of
@@ -1929,6 +2001,9 @@ struct LocalFrontier {
     bool isDense;
     size_t sCap = 0;        // capacity of s when the header allocated it
     bool sOwned = false;    // s was allocated here
+    // a dense edgeMap's own count of this frontier (read by vertexCounter while the
+    // publication with this sequence number is the last and nothing changed since)
+    unsigned long long pendingSeq = 0, pendingGen = 0;
 
     LocalFrontier(bool *_b, intT start, intT end)
         : n(end - start), m(0), outEdgesCount(0), startID(start), endID(end), b(_b), s(nullptr),
@@ -1980,6 +2055,7 @@ struct LocalFrontier {
     void toDense() {
         if (!isDense) {
             using namespace polymer_detail;
+            frontiers_changed();
             hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for_flags(n)), dim3(256), 0, lib_stream(), b,
                                (int64_t)n, false);
             if (m > 0) {
@@ -2151,6 +2227,7 @@ inline intT list_active(vertices *frontier, intT **list) {
 template <class G, class F, int Mode>
 void dense_push(G &GA, vertices *frontier, F &f, LocalFrontier *next) {
     constexpr bool W = is_weighted<G>::value && Mode != 2;
+    frontiers_changed();
     intT *list = nullptr;
     const intT count = list_active(frontier, &list);
     push_list<W, Mode>(f, GA.view, list, count, next->b, nullptr, 0,
@@ -2195,6 +2272,7 @@ bool *edgeMapDense(G GA, vertices *frontier, F f, LocalFrontier *next, bool para
                    Subworker_Partitioner &subworker = dummyPartitioner) {
     (void)parallel; (void)subworker;
     constexpr bool W = polymer_detail::is_weighted<G>::value;
+    polymer_detail::frontiers_changed();
     frontier->nextFrontiers[0] = next;
     polymer_detail::pull_dense<W>(f, GA.view, polymer_detail::cur_bits(frontier, GA.n), next->b);
     polymer_detail::sync_visible();
@@ -2214,6 +2292,7 @@ bool *edgeMapDenseReduce(G GA, vertices *frontier, F f, LocalFrontier *next, boo
     using namespace polymer_detail;
     (void)parallel; (void)subworker;
     constexpr bool W = is_weighted<G>::value;
+    frontiers_changed();
     frontier->nextFrontiers[0] = next;
     reduce_dense<W>(f, GA.view, cur_bits(frontier, GA.n), next->b);
     sync_visible();
@@ -2230,6 +2309,7 @@ void edgeMapSparseV3(G GA, vertices *frontier, F f, LocalFrontier *next, bool pa
     using namespace polymer_detail;
     (void)part; (void)subworker;
     constexpr bool W = is_weighted<G>::value;
+    frontiers_changed();
     LocalFrontier *cur = frontier->getFrontier(0);
     const long long bufferLen = frontier->getEdgeStat();
     next->reserveSparse((size_t)(bufferLen > 0 ? bufferLen : 1));
@@ -2282,6 +2362,7 @@ void edgeMapSparseAsyncPipe(G GA, vertices *frontier, F f, LocalFrontier *next,
     using namespace polymer_detail;
     (void)subworker;
     constexpr bool W = is_weighted<G>::value;
+    frontiers_changed();
     LocalFrontier *cur = frontier->getFrontier(0);
     if (cur->isDense) cur->toSparse();
     const size_t cap = (size_t)std::max(GA.m, GA.n) + 1;
@@ -2324,10 +2405,23 @@ void edgeMap(G GA, vertices *V, F f, LocalFrontier *next, intT threshold = -1, c
         V->toDense();
         next->sparseCounter = 0;
         clearLocalFrontier(next, 0, 0, 1);
-        if (option == DENSE_FORWARD)
+        if (option == DENSE_FORWARD) {
             edgeMapDenseForward(GA, V, f, next, part, subworker.dense_start, subworker.dense_end);
-        else
+        } else {
+#ifdef POLYMER_DEVICE_RESIDENT
+            // next was just cleared: the pull counts what it marks, for vertexCounter
+            // (with managed arrays the program's host code may write next's flags
+            // before it counts them, so the count is taken there instead)
+            using namespace polymer_detail;
+            constexpr bool W = is_weighted<G>::value;
+            V->nextFrontiers[0] = next;
+            pull_dense<W>(f, GA.view, cur_bits(V, GA.n), next->b, true);
+            next->pendingSeq = totals().seq;
+            next->pendingGen = frontier_gen();
+#else
             edgeMapDense(GA, V, f, next, option, subworker);
+#endif
+        }
         next->isDense = true;
     } else {
         V->toSparse();
@@ -2343,6 +2437,15 @@ void vertexCounter(G &GA, LocalFrontier *frontier, int nodeNum, int subNum, int 
     using namespace polymer_detail;
     (void)totalSub;
     if (!frontier->isDense || nodeNum != 0 || subNum != 0) return;
+    const unsigned long long pending = frontier->pendingSeq;
+    frontier->pendingSeq = 0;
+    if (pending && pending == totals().seq && frontier->pendingGen == frontier_gen()) {
+        unsigned long long cnt = 0, e = 0;  // the dense edgeMap that made it counted it
+        read_totals(&cnt, &e);
+        frontier->m = (intT)cnt;
+        frontier->outEdgesCount = (intT)e;
+        return;
+    }
     hipLaunchKernelGGL(k_counter, dim3(blocks_for_flags(frontier->n, 2048)), dim3(256), 0, lib_stream(),
                        (const bool *)frontier->b, (int64_t)frontier->n, frontier->startID,
                        GA.view.out_off, totals().slots);
@@ -2360,6 +2463,7 @@ void vertexMap(vertices *V, F add, int nodeNum, int subNum = 0, int totalSub = 1
     using namespace polymer_detail;
     (void)totalSub;
     if (subNum != 0) return;
+    frontiers_changed();  // the functor may write anything
     LocalFrontier *fr = V->getFrontier(nodeNum);
     if (V->isDense) {
         hipLaunchKernelGGL(k_vertex_map_dense<F>, dim3(blocks_for_flags(fr->n)), dim3(256), 0, lib_stream(),
@@ -2378,6 +2482,7 @@ inline void clearLocalFrontier(LocalFrontier *next, int nodeNum, int subNum, int
     using namespace polymer_detail;
     (void)nodeNum; (void)totalSub;
     if (subNum != 0) return;
+    frontiers_changed();
     hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for_flags(next->n)), dim3(256), 0, lib_stream(), next->b,
                        (int64_t)next->n, false);
     launched("clearLocalFrontier");
@@ -2388,6 +2493,7 @@ inline void clearLocalFrontier(LocalFrontier *next, int nodeNum, int subNum, int
 // e.g. numa-PageRank.C:427-429); m and outEdgesCount come from vertexCounter.
 inline void fillLocalFrontier(LocalFrontier *f) {
     using namespace polymer_detail;
+    frontiers_changed();
     hipLaunchKernelGGL(k_fill_bytes, dim3(blocks_for_flags(f->n)), dim3(256), 0, lib_stream(), f->b,
                        (int64_t)f->n, true);
     launched("fillLocalFrontier");
@@ -2401,6 +2507,7 @@ inline void fillLocalFrontier(LocalFrontier *f) {
 template <class F>
 void vertexFilter(vertices *V, F filter, int nodeNum, bool *result) {
     using namespace polymer_detail;
+    frontiers_changed();
     V->toDense();
     LocalFrontier *fr = V->getFrontier(nodeNum);
     hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n, 256, 2048)), dim3(256), 0, lib_stream(), filter,
@@ -2417,6 +2524,7 @@ void vertexFilter(vertices *V, F filter, int nodeNum, int subNum, int totalSub, 
     using namespace polymer_detail;
     (void)totalSub;
     if (subNum != 0) return;
+    frontiers_changed();
     V->toDense();
     LocalFrontier *fr = V->getFrontier(nodeNum);
     hipLaunchKernelGGL(k_vertex_filter<F>, dim3(blocks_for(fr->n, 256, 2048)), dim3(256), 0, lib_stream(), filter,
