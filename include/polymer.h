@@ -1103,20 +1103,30 @@ __global__ __launch_bounds__(kT) void k_flag_counts(const bool *b, int64_t n, ui
 }
 
 // out[i] = in[0] + ... + in[i-1] for i < n, out[n] = total; one workgroup (n small)
+// exclusive scan of in[0, n) into out, out[n] = the total, by one block: each pass
+// takes 16 * kT entries, 16 consecutive ones per thread, all loaded before the
+// block's scan (a pass of kT entries per block scan was a chain of dependent
+// load-and-scan steps: 30 us for 4096 entries)
 __global__ __launch_bounds__(kT) void k_scan_small(const uint32_t *in, uint32_t *out, int64_t n) {
     __shared__ uint32_t sm[kT / 64 + 1];
     uint32_t carry = 0;
-    for (int64_t base = 0; base < n; base += kT) {
-        const int64_t i = base + threadIdx.x;
-        const uint32_t v = i < n ? in[i] : 0u;
+    for (int64_t base = 0; base < n; base += 16 * kT) {  // block-uniform
+        const int64_t i0 = base + 16 * (int64_t)threadIdx.x;
+        uint32_t v[16], run = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = i0 + k < n ? in[i0 + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < 16; k++) run += v[k];
         uint32_t tot = 0;
-        const uint32_t x = block_excl_scan(v, sm, &tot);
-        if (i < n) out[i] = carry + x;
+        uint32_t x = carry + block_excl_scan(run, sm, &tot);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (i0 + k < n) out[i0 + k] = x;
+            x += v[k];
+        }
         carry += tot;
     }
-    if (threadIdx.x == 0) {
-        out[n] = carry;
-    }
+    if (threadIdx.x == 0) out[n] = carry;
 }
 
 // ordered compaction: the ids (+ offset) of the set flags, block by block; a
@@ -1158,18 +1168,31 @@ inline intT compact_flags(const bool *b, int64_t n, intT offset, intT *s) {
 // scan over the out-degrees of list[0, count): deg scan in tiles of 4096
 constexpr int kScanTile = 16 * kT;
 
+// the out-degrees of list[i0, i0 + 16) (0 past count), every load issued before
+// the first is used
+__device__ __forceinline__ void degrees16(const intT *list, int64_t count, const int32_t *off,
+                                          int64_t i0, uint32_t (&d)[16]) {
+    intT v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = i0 + k < count ? list[i0 + k] : -1;
+    int32_t a[16], b[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        a[k] = v[k] >= 0 ? off[v[k]] : 0;
+        b[k] = v[k] >= 0 ? off[v[k] + 1] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) d[k] = (uint32_t)(b[k] - a[k]);
+}
+
+// tiles of kScanTile entries, 16 consecutive ones per thread
 __global__ __launch_bounds__(kT) void k_deg_tile_sums(const intT *list, int64_t count,
                                                       const int32_t *off, uint32_t *sums) {
     __shared__ uint32_t sm[kT / 64 + 1];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile;
-    uint32_t c = 0;
-    for (int k = 0; k < kScanTile / kT; k++) {
-        const int64_t i = base + k * kT + threadIdx.x;
-        if (i < count) {
-            const intT v = list[i];
-            c += (uint32_t)(off[v + 1] - off[v]);
-        }
-    }
+    uint32_t d[16], c = 0;
+    degrees16(list, count, off, (int64_t)blockIdx.x * kScanTile + 16 * (int64_t)threadIdx.x, d);
+#pragma unroll
+    for (int k = 0; k < 16; k++) c += d[k];
     uint32_t tot = 0;
     (void)block_excl_scan(c, sm, &tot);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -1178,21 +1201,20 @@ __global__ __launch_bounds__(kT) void k_deg_tile_sums(const intT *list, int64_t 
 __global__ __launch_bounds__(kT) void k_deg_tiles(const intT *list, int64_t count, const int32_t *off,
                                                   const uint32_t *tile_start, uint32_t *scan) {
     __shared__ uint32_t sm[kT / 64 + 1];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile;
-    uint32_t run = tile_start[blockIdx.x];
-    for (int k = 0; k < kScanTile / kT; k++) {
-        const int64_t i = base + k * kT + threadIdx.x;
-        uint32_t d = 0;
-        if (i < count) {
-            const intT v = list[i];
-            d = (uint32_t)(off[v + 1] - off[v]);
-        }
-        uint32_t tot = 0;
-        const uint32_t x = block_excl_scan(d, sm, &tot);
-        if (i < count) scan[i] = run + x;
-        run += tot;
+    const int64_t i0 = (int64_t)blockIdx.x * kScanTile + 16 * (int64_t)threadIdx.x;
+    uint32_t d[16], run = 0;
+    degrees16(list, count, off, i0, d);
+#pragma unroll
+    for (int k = 0; k < 16; k++) run += d[k];
+    uint32_t tot = 0;
+    const uint32_t start = tile_start[blockIdx.x];
+    uint32_t x = start + block_excl_scan(run, sm, &tot);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (i0 + k < count) scan[i0 + k] = x;
+        x += d[k];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) scan[count] = run;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) scan[count] = start + tot;
 }
 
 // scan[i] = sum of outdeg(list[0..i)), scan[count] = total, on the device
@@ -1244,6 +1266,26 @@ __device__ __forceinline__ bool call_update_atomic(F &f, intT s, intT d, const i
 constexpr int kWin = 1024;
 constexpr int kPushEdgesMax = 2048;  // edges per chunk (the block's append stage)
 constexpr int kPushBlocks = 2048;    // grid of the push (8 workgroups per CU)
+
+// The last j in [0, count] with scan[j] <= key (scan non-decreasing, scan[count]
+// > key), found by the whole block: each step samples the remaining range at kT
+// evenly spaced entries with one load per thread and keeps the interval between
+// the last sample <= key and the next (three steps for a million entries, where a
+// single thread's binary search is a chain of twenty dependent loads).  Every
+// thread of the block calls it and gets the result.
+__device__ __forceinline__ int64_t block_search(const uint32_t *scan, int64_t count, uint32_t key) {
+    int64_t lo = 0, hi = count + 1;  // scan[lo] <= key < scan[hi] (scan[count + 1] taken as infinite)
+    while (hi - lo > 1) {  // block-uniform
+        const int64_t step = (hi - lo + kT - 1) / kT;
+        const int64_t p = lo + (int64_t)threadIdx.x * step;
+        const int below = p < hi && scan[p] <= key;  // true for a prefix of the threads
+        const int c = __syncthreads_count(below);    // at least 1: scan[lo] <= key
+        const int64_t nlo = lo + (int64_t)(c - 1) * step;
+        hi = min(hi, nlo + step);
+        lo = nlo;
+    }
+    return lo;
+}
 
 template <bool W, int Mode, bool Small, class F>
 __global__ __launch_bounds__(kT) void k_push_edges(F f, pg_view v, const intT *list,
@@ -1299,25 +1341,20 @@ __global__ __launch_bounds__(kT) void k_push_edges(F f, pg_view v, const intT *l
     unsigned long long outdeg = 0;
     for (uint32_t E0 = B0; E0 < B1; E0 += kPushEdgesMax) {  // block-uniform
         const uint32_t E1 = min(B1, E0 + kPushEdgesMax);
-        if (threadIdx.x == 0) {
-            s_cnt = 0;
-            if constexpr (Small) {
+        if constexpr (Small) {
+            if (threadIdx.x == 0) {
+                s_cnt = 0;
                 s_j0 = 0;
                 s_j1 = count - 1;
-            } else {
-                int64_t lo = 0, hi = count + 1;  // last j with scan[j] <= E0
-                while (lo + 1 < hi) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if (scan[mid] <= E0) lo = mid; else hi = mid;
-                }
-                s_j0 = lo;
-                lo = 0;
-                hi = count + 1;  // last j with scan[j] <= E1 - 1
-                while (lo + 1 < hi) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if (scan[mid] <= E1 - 1) lo = mid; else hi = mid;
-                }
-                s_j1 = lo;
+            }
+        } else {
+            // the list entries holding edges E0 and E1 - 1
+            const int64_t j0 = block_search(scan, count, E0);
+            const int64_t j1 = block_search(scan, count, E1 - 1);
+            if (threadIdx.x == 0) {
+                s_cnt = 0;
+                s_j0 = j0;
+                s_j1 = j1;
             }
         }
         __syncthreads();
