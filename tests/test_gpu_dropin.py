@@ -9,7 +9,8 @@ the checks are the ones the C-ABI paths meet:
     oracle;
   * BFS levels, CC labels, Bellman-Ford distances: bit-exact against the
     textbook BFS, union-find and Dijkstra; BFS parents checked for validity;
-  * BP (fp64, atomic products): the fp64 oracle on the entries it keeps in range.
+  * BP (fp64, atomic products): the fp64 oracle on the entries it keeps in range;
+  * the header's count reporting: header_checks.cpp against host recounts.
 """
 import os
 import subprocess
@@ -195,3 +196,17 @@ def test_dropin_bp(orc):
             say("bp iters %d: in-range %d of %d, relL1 %.3e" % (iters, ok.sum(), ok.size, err))
             assert ok.sum() > 0.3 * ok.size
             assert err < 1e-12
+
+
+def test_dropin_header_count_reporting():
+    """The header's count reporting (polymer_amd/apps/header_checks.cpp): a dense
+    edgeMap's own count of its output against a host recount and after the flags
+    change, pushes on both sides of the small-list limit (successes, their
+    out-degree sum, the output list), vertexFilter's count."""
+    exe = os.path.join(BIN, "header_checks")
+    if not os.path.exists(exe):
+        pytest.fail("driver %s is not built (run __graft_entry__.build())" % exe)
+    say("run: header_checks")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    say(p.stdout.strip())
+    assert p.returncode == 0 and "header_checks: ok" in p.stdout, p.stdout + p.stderr
