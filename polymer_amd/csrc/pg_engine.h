@@ -693,18 +693,24 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
         } else {
             using T = tile_acc_t<Op>;
             const T *p = reinterpret_cast<const T *>(ta.partial) + f.slot0 + i;
-            T q0 = T(0), q1 = T(0), q2 = T(0), q3 = T(0);
+            // eight slots in flight (a hub tile has dozens of parts, and its finishing
+            // blocks are the tail of the launch)
+            T q[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) q[j] = T(0);
             int k = 0;
-            for (; k + 3 < f.nparts; k += 4) {  // four slots in flight
-                q0 += __builtin_nontemporal_load(p + (int64_t)k * nv);
-                q1 += __builtin_nontemporal_load(p + (int64_t)(k + 1) * nv);
-                q2 += __builtin_nontemporal_load(p + (int64_t)(k + 2) * nv);
-                q3 += __builtin_nontemporal_load(p + (int64_t)(k + 3) * nv);
+            for (; k + 7 < f.nparts; k += 8) {
+                T x[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) x[j] = __builtin_nontemporal_load(p + (int64_t)(k + j) * nv);
+#pragma unroll
+                for (int j = 0; j < 8; j++) q[j] += x[j];
             }
-            for (; k < f.nparts; k++) q0 += __builtin_nontemporal_load(p + (int64_t)k * nv);
+            for (; k < f.nparts; k++) q[0] += __builtin_nontemporal_load(p + (int64_t)k * nv);
             // integers: any order; doubles: this fixed order for a given part count
-            const T q = (q0 + q1) + (q2 + q3);
-            st = op.finish(v, op.tile_sum(q), op.ctx(v));
+            const T qa = (q[0] + q[1]) + (q[2] + q[3]), qb = (q[4] + q[5]) + (q[6] + q[7]);
+            const T q8 = qa + qb;
+            st = op.finish(v, op.tile_sum(q8), op.ctx(v));
         }
     }
     if constexpr (Op::kStats) {
