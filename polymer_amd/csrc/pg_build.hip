@@ -23,6 +23,7 @@
 #include <unistd.h>
 
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <thread>
 
@@ -161,6 +162,8 @@ __global__ void k_fetch(const uint32_t *src, int nw, uint32_t *dst, unsigned lon
 int fetch_words(void *host, const void *dev, size_t bytes) {
     PG_REQUIRE(bytes % 4 == 0 && bytes <= 4 * (size_t)kFetchWords, PG_ERR_ARG,
                "fetch_words: %zu bytes", bytes);
+    static std::mutex mu;  // one pinned buffer: calls from several host threads take turns
+    std::lock_guard<std::mutex> lock(mu);
     if (!g_fetch.cpu) {
         PG_CHECK_HIP(hipHostMalloc((void **)&g_fetch.cpu, 4 * kFetchWords + 8, hipHostMallocDefault));
         PG_CHECK_HIP(hipHostGetDevicePointer((void **)&g_fetch.dev, g_fetch.cpu, 0));
