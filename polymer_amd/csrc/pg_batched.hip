@@ -460,7 +460,9 @@ __global__ void k_ppr_init(int64_t n, const int32_t *out_off, int32_t src, float
 // with few out-edges pushes: the frontier vertices (listed from Fany) OR their masks
 // into their out-neighbours' next masks (returnless 64-bit ORs), then
 // k_msbfs_settle keeps the bits not yet visited.  Either way a new bit of v records
-// the round as v's level for that source.  Statistics of the next frontier:
+// the round as v's level for that source, in v's own row of 64 levels (an [k][n]
+// layout scattered every bit's store over a line of its own; the rows need no
+// clearing, a column being read only where v's visited bit is set).  Statistics of the next frontier:
 // (count << 32) | out-degree sum.
 struct MsBfsOp {
     using Acc = unsigned long long;
@@ -472,7 +474,7 @@ struct MsBfsOp {
     const uint32_t *Fany;
     unsigned long long *Fn;
     unsigned long long *vis;
-    int32_t *level;  // [k][n]
+    int32_t *level;  // [n][64]: column c of v valid where vis[v] has bit c
     const int32_t *out_off;
     int64_t n;
     unsigned long long full;
@@ -492,7 +494,7 @@ struct MsBfsOp {
         while (x) {
             const int b = __ffsll((long long)x) - 1;
             x &= x - 1;
-            level[(int64_t)b * n + v] = round;
+            level[(int64_t)v * kRow + b] = round;
         }
         return (1ull << 32) | (unsigned long long)(out_off[v + 1] - out_off[v]);
     }
@@ -501,6 +503,13 @@ struct MsBfsOp {
     __device__ static void stat_commit(Stat *p, Stat v) {
         atomicAdd(&p[blockIdx.x & (kStatSlots - 1)], v);
     }
+    // the tiled pull: every in-neighbour's frontier mask ORed in LDS (a mask of 0 is
+    // not applied), no early exit
+    static constexpr bool kTileWeighted = false;
+    static constexpr bool kTileOr = true;
+    __device__ unsigned long long tile_raw(int s) const { return F[s]; }
+    __device__ unsigned long long tile_fix(unsigned long long x) const { return x; }
+    __device__ unsigned long long tile_sum(unsigned long long q) const { return q; }
 };
 
 struct MsBfsPushOp {
@@ -534,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_settle(int64_t n, unsigned lon
         while (x) {
             const int b = __ffsll((long long)x) - 1;
             x &= x - 1;
-            level[(int64_t)b * n + v] = round;
+            level[(int64_t)v * kRow + b] = round;
         }
         st += (1ull << 32) | (unsigned long long)(out_off[v + 1] - out_off[v]);
     }
@@ -555,17 +564,20 @@ __global__ void k_msbfs_seed(const int32_t *src, int k, int64_t n, unsigned long
             const int s = src[c];
             F[s] |= 1ull << c;
             vis[s] |= 1ull << c;
-            level[(int64_t)c * n + s] = 0;
+            level[(int64_t)s * kRow + c] = 0;
         }
     }
 }
 
-template <class T>
-__global__ void k_level_to_input(const T *lev, const int32_t *perm, int64_t n, int k, T *out) {
+// levels [n][64] (internal ids; a column valid where the visited mask has its bit)
+// to [k][n] in input order, -1 for a vertex a source did not reach
+__global__ void k_msbfs_levels_out(const int32_t *lev, const unsigned long long *vis,
+                                   const int32_t *perm, int64_t n, int k, int32_t *out) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * k;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = i / n, u = i % n;
-        out[i] = lev[c * n + perm[u]];
+        const int32_t v = perm[u];
+        out[i] = (vis[v] >> c) & 1ull ? lev[(int64_t)v * kRow + c] : -1;
     }
 }
 
@@ -901,8 +913,9 @@ extern "C" int pg_ppr_batched(pg_graph *g, const int64_t *src, int k, int iters,
 }
 
 // rounds whose frontier has fewer out-edges than m / kMsPushDiv push (measured on
-// S22 with 64 sources: 4.32 ms at m/2 against 4.53 at m/4 to m/32)
-constexpr int64_t kMsPushDiv = 2;
+// S22 with 64 sources, pull rounds on the tiles: 3.23 ms at m/4 to m/16 against
+// 3.37 at m/2; with the untiled pull m/2 was best, 4.32 ms against 4.53)
+constexpr int64_t kMsPushDiv = 8;
 constexpr int kMsCtrWords = 4 + 2 * kStatSlots;
 
 extern "C" int pg_msbfs(pg_graph *g, const int64_t *src, int k, int32_t *level, uint32_t flags,
@@ -919,20 +932,33 @@ extern "C" int pg_msbfs(pg_graph *g, const int64_t *src, int k, int32_t *level, 
     PG_TRY(workspace_t(g, "msbfs_f1", (size_t)n, &F1));
     PG_TRY(workspace_t(g, "msbfs_vis", (size_t)n, &vis));
     PG_TRY(workspace_t(g, "msbfs_cnt", (size_t)kStatSlots, &cnt));
-    PG_TRY(workspace_t(g, "msbfs_level", (size_t)n * k, &lev));
+    PG_TRY(workspace_t(g, "msbfs_level", (size_t)n * kRow, &lev));
     PG_TRY(workspace_t(g, "msbfs_src", (size_t)k, &dsrc));
     PG_TRY(workspace_t(g, "msbfs_any", (size_t)(n + 31) / 32 + 1, &Fany));
     PG_TRY(workspace_t(g, "msbfs_list", (size_t)n + 1, &list));
     PG_TRY(workspace_t(g, "msbfs_lctr", (size_t)kMsCtrWords, &lctr));
     PG_CHECK_HIP(hipMemcpyAsync(dsrc, s.data(), 4 * k, hipMemcpyHostToDevice, stream()));
     const unsigned long long full = (k == 64) ? ~0ull : ((1ull << k) - 1ull);
+    // pull rounds on the source-sorted tiles when the graph has them (the cold
+    // sources' masks staged as 8-byte words)
+    const bool tiled = g->use_tiles;
+    TileSet tset{};
+    if (tiled) {
+        PG_TRY(ensure_tiles(g));
+        tset = g->tiles;
+        if (tset.m_cold > 0) {
+            unsigned long long *cold64 = nullptr;
+            PG_TRY(workspace_t(g, "msbfs_cold64", (size_t)tset.m_cold + 16, &cold64));
+            tset.args.cold_val64 = cold64;
+            tset.cold.val64 = cold64;
+        }
+    }
     KernelTimer kt;
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
     Timer tm;
     PG_TRY(tm.start());
     PG_TRY(fill(F0, 0ull, n));
-    PG_TRY(fill(vis, 0ull, n));
-    PG_TRY(fill(lev, -1, n * k));
+    PG_TRY(fill(vis, 0ull, n));  // (levels need no clearing: a column counts once its bit is set)
     PG_LAUNCH_B(k_msbfs_seed, 1, dsrc, k, n, F0, vis, lev);
     unsigned long long *F = F0, *Fn = F1;
     int rounds = 0, dense_rounds = 0;
@@ -945,7 +971,10 @@ extern "C" int pg_msbfs(pg_graph *g, const int64_t *src, int k, int32_t *level, 
         if (rounds > 1 && fedges >= g->m / kMsPushDiv) {
             dense_rounds++;
             MsBfsOp op{F, Fany, Fn, vis, lev, g->out_off, n, full, rounds};
-            PG_TRY(launch_pull(op, g->in_sched, cnt));
+            if (tiled)
+                PG_TRY(launch_pull_tiled(op, g->in_sched, tset, cnt));
+            else
+                PG_TRY(launch_pull(op, g->in_sched, cnt));
         } else {
             // the frontier as a list, its size and out-degree sum
             PG_TRY(fill(lctr, 0ull, kMsCtrWords));
@@ -985,7 +1014,7 @@ extern "C" int pg_msbfs(pg_graph *g, const int64_t *src, int k, int32_t *level, 
     if (!level) return PG_OK;
     int32_t *tmp = nullptr;
     PG_TRY(workspace_t(g, "rows_out_i32", (size_t)n * k, &tmp));
-    PG_LAUNCH_B(k_level_to_input<int32_t>, grid_for(n * k), lev, g->perm, n, k, tmp);
+    PG_LAUNCH_B(k_msbfs_levels_out, grid_for(n * k), lev, vis, g->perm, n, k, tmp);
     PG_CHECK_HIP(hipMemcpyAsync(level, tmp, 4 * n * k, hipMemcpyDeviceToHost, stream()));
     PG_CHECK_HIP(hipStreamSynchronize(stream()));
     return PG_OK;

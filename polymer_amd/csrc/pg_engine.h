@@ -429,8 +429,10 @@ inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *s
 // and T = unsigned long long (fixed point: integer addition is associative, so the
 // result does not depend on the order in which the LDS atomics land), or V = T =
 // double (fp64 LDS adds, rounded in arrival order like any parallel fp64 sum; the
-// cold values then travel as doubles, TileArgs::cold_val64).  The per-vertex
-// protocol (active / finish / skip, statistics) is the one k_pull follows.
+// cold values then travel as doubles, TileArgs::cold_val64), or V = T = unsigned
+// long long with Op::kTileOr (64-bit masks ORed; the cold values as 8-byte words).
+// The per-vertex protocol (active / finish / skip, statistics) is the one k_pull
+// follows.
 template <class Op>
 using tile_val_t = decltype(static_cast<const Op *>(nullptr)->tile_raw(0));
 template <class Op>
@@ -438,6 +440,31 @@ using tile_acc_t = decltype(static_cast<const Op *>(nullptr)->tile_fix(tile_val_
 
 __device__ __forceinline__ void lds_add(unsigned long long *p, unsigned long long v) { atomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// Op::kTileOr (optional, with 64-bit integer values): the accumulators OR the
+// values instead of adding them (a zero value is not applied at all), and the
+// partial slots of a tile are ORed together
+template <class Op, class = void>
+struct tile_or_trait : std::false_type {};
+template <class Op>
+struct tile_or_trait<Op, std::void_t<decltype(Op::kTileOr)>>
+    : std::integral_constant<bool, Op::kTileOr> {};
+template <class Op>
+constexpr bool kTileOrOf = tile_or_trait<Op>::value;
+
+template <class Op, class T>
+__device__ __forceinline__ void lds_acc(T *p, T v) {
+    if constexpr (kTileOrOf<Op>) {
+        if (v) atomicOr(p, v);
+    } else {
+        lds_add(p, v);
+    }
+}
+template <class Op, class T>
+__device__ __forceinline__ T tile_combine(T a, T b) {
+    if constexpr (kTileOrOf<Op>) return a | b;
+    else return a + b;
+}
 
 template <class T>
 __device__ __forceinline__ const T *cold_values(const TileArgs &ta) {
@@ -553,7 +580,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
 #pragma unroll
                 for (int k = 0; k < U; k++) q[k] = tile_term(op, x[k], wv[k]);
 #pragma unroll
-                for (int k = 0; k < U; k++) lds_add(&acc[w[k] & kDstMask], q[k]);
+                for (int k = 0; k < U; k++) lds_acc<Op>(&acc[w[k] & kDstMask], q[k]);
                 e = en;
                 if (!(e + (U - 1) * kTileThreads < cnt)) break;
 #pragma unroll
@@ -570,7 +597,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
             const int b = ta.gbase[i / kPackGroup];
             const int s = b >= 0 ? b + (int)(w >> kPackDstBits)
                                  : ta.esc[(int64_t)(-b - 1) * kPackGroup + (i % kPackGroup)];
-            lds_add(&acc[w & kDstMask], tile_term(op, op.tile_raw(s), tile_weight(op, ta.wt, i)));
+            lds_acc<Op>(&acc[w & kDstMask], tile_term(op, op.tile_raw(s), tile_weight(op, ta.wt, i)));
         }
     }
     {  // cold sources: values streamed from the pre-pass
@@ -589,10 +616,10 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
                 wv[k] = tile_weight(op, ta.wt, part.cold_e + e + k * kTileThreads);
             }
 #pragma unroll
-            for (int k = 0; k < U; k++) lds_add(&acc[d[k]], tile_term(op, x[k], wv[k]));
+            for (int k = 0; k < U; k++) lds_acc<Op>(&acc[d[k]], tile_term(op, x[k], wv[k]));
         }
         for (; e < cnt; e += kTileThreads)
-            lds_add(&acc[__builtin_nontemporal_load(pk + e) & kDstMask],
+            lds_acc<Op>(&acc[__builtin_nontemporal_load(pk + e) & kDstMask],
                     tile_term(op, __builtin_nontemporal_load(val + e),
                               tile_weight(op, ta.wt, part.cold_e + e)));
     }
@@ -704,12 +731,14 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
 #pragma unroll
                 for (int j = 0; j < 8; j++) x[j] = __builtin_nontemporal_load(p + (int64_t)(k + j) * nv);
 #pragma unroll
-                for (int j = 0; j < 8; j++) q[j] += x[j];
+                for (int j = 0; j < 8; j++) q[j] = tile_combine<Op>(q[j], x[j]);
             }
-            for (; k < f.nparts; k++) q[0] += __builtin_nontemporal_load(p + (int64_t)k * nv);
+            for (; k < f.nparts; k++)
+                q[0] = tile_combine<Op>(q[0], __builtin_nontemporal_load(p + (int64_t)k * nv));
             // integers: any order; doubles: this fixed order for a given part count
-            const T qa = (q[0] + q[1]) + (q[2] + q[3]), qb = (q[4] + q[5]) + (q[6] + q[7]);
-            const T q8 = qa + qb;
+            const T qa = tile_combine<Op>(tile_combine<Op>(q[0], q[1]), tile_combine<Op>(q[2], q[3]));
+            const T qb = tile_combine<Op>(tile_combine<Op>(q[4], q[5]), tile_combine<Op>(q[6], q[7]));
+            const T q8 = tile_combine<Op>(qa, qb);
             st = op.finish(v, op.tile_sum(q8), op.ctx(v));
         }
     }
