@@ -1225,6 +1225,18 @@ struct CcPullOp {
         return 0ull;
     }
     __device__ void skip(int) const {}
+    __device__ static Stat stat_reduce(Stat a, Stat b) { return a + b; }
+    // the tiled pull: the smallest label of an active in-neighbour, in 64-bit LDS
+    // minima (an inactive source's all-ones value is not applied)
+    static constexpr bool kTileWeighted = false;
+    static constexpr int kTileReduce = kReduceMin;
+    __device__ uint32_t tile_raw(int s) const {
+        return bit_test(F, s) ? (uint32_t)label[s] : 0xFFFFFFFFu;
+    }
+    __device__ unsigned long long tile_fix(uint32_t x) const {
+        return x == 0xFFFFFFFFu ? ~0ull : (unsigned long long)x;
+    }
+    __device__ int tile_sum(unsigned long long q) const { return q == ~0ull ? 0x7FFFFFFF : (int)q; }
 };
 
 struct CcPushOp {
@@ -1495,6 +1507,9 @@ int cc_run(pg_graph *g, int64_t threshold, uint32_t flags, pg_stats *st, int32_t
     PG_TRY(workspace_t(g, "cc_prev", (size_t)g->n, &prev));
     Frontier A, B;
     PG_TRY(frontiers(g, &A, &B));
+    // dense rounds on the source-sorted tiles when the graph has them
+    const bool tiled = g->use_tiles;
+    if (tiled) PG_TRY(ensure_tiles(g));
     Timer tm;
     KernelTimer kt;
     kt.on = (flags & PG_PROFILE_KERNELS) != 0;
@@ -1522,7 +1537,10 @@ int cc_run(pg_graph *g, int64_t threshold, uint32_t flags, pg_stats *st, int32_t
         if (dense) {
             dense_rounds++;
             CcPullOp op{F->bits, N->bits, label};
-            PG_TRY(launch_pull(op, g->in_sched));
+            if (tiled)
+                PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles));
+            else
+                PG_TRY(launch_pull(op, g->in_sched));
         } else {
             CcPushOp op{N->bits, N->list, &N->ctr[0], label, prev, !compact};
             PG_TRY(launch_push_global(op, g, g->out_off, g->out_adj, F->list, cnt, edges));

@@ -506,7 +506,7 @@ struct MsBfsOp {
     // the tiled pull: every in-neighbour's frontier mask ORed in LDS (a mask of 0 is
     // not applied), no early exit
     static constexpr bool kTileWeighted = false;
-    static constexpr bool kTileOr = true;
+    static constexpr int kTileReduce = kReduceOr;
     __device__ unsigned long long tile_raw(int s) const { return F[s]; }
     __device__ unsigned long long tile_fix(unsigned long long x) const { return x; }
     __device__ unsigned long long tile_sum(unsigned long long q) const { return q; }

@@ -430,7 +430,8 @@ inline int launch_pull(const Op &op, const PullSchedule &s, typename Op::Stat *s
 // result does not depend on the order in which the LDS atomics land), or V = T =
 // double (fp64 LDS adds, rounded in arrival order like any parallel fp64 sum; the
 // cold values then travel as doubles, TileArgs::cold_val64), or V = T = unsigned
-// long long with Op::kTileOr (64-bit masks ORed; the cold values as 8-byte words).
+// long long with Op::kTileReduce = kReduceOr (64-bit masks ORed; the cold values as
+// 8-byte words), or V = uint32_t and T = unsigned long long with kReduceMin.
 // The per-vertex protocol (active / finish / skip, statistics) is the one k_pull
 // follows.
 template <class Op>
@@ -441,35 +442,45 @@ using tile_acc_t = decltype(static_cast<const Op *>(nullptr)->tile_fix(tile_val_
 __device__ __forceinline__ void lds_add(unsigned long long *p, unsigned long long v) { atomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(double *p, double v) { unsafeAtomicAdd(p, v); }
 
-// Op::kTileOr (optional, with 64-bit integer values): the accumulators OR the
-// values instead of adding them (a zero value is not applied at all), and the
-// partial slots of a tile are ORed together
+// Op::kTileReduce (optional, 64-bit integer accumulators): kReduceOr ORs the
+// values (masks; a zero value is not applied), kReduceMin keeps the smallest (an
+// all-ones value is not applied); the partial slots of a tile are combined the same
+// way.  Without it the values are added.
+constexpr int kReduceAdd = 0, kReduceOr = 1, kReduceMin = 2;
 template <class Op, class = void>
-struct tile_or_trait : std::false_type {};
+struct tile_reduce_trait : std::integral_constant<int, kReduceAdd> {};
 template <class Op>
-struct tile_or_trait<Op, std::void_t<decltype(Op::kTileOr)>>
-    : std::integral_constant<bool, Op::kTileOr> {};
+struct tile_reduce_trait<Op, std::void_t<decltype(Op::kTileReduce)>>
+    : std::integral_constant<int, Op::kTileReduce> {};
 template <class Op>
-constexpr bool kTileOrOf = tile_or_trait<Op>::value;
+constexpr int kTileReduceOf = tile_reduce_trait<Op>::value;
 
 template <class Op, class T>
 __device__ __forceinline__ void lds_acc(T *p, T v) {
-    if constexpr (kTileOrOf<Op>) {
+    if constexpr (kTileReduceOf<Op> == kReduceOr) {
         if (v) atomicOr(p, v);
+    } else if constexpr (kTileReduceOf<Op> == kReduceMin) {
+        if (v != ~T(0)) atomicMin(p, v);
     } else {
         lds_add(p, v);
     }
 }
 template <class Op, class T>
 __device__ __forceinline__ T tile_combine(T a, T b) {
-    if constexpr (kTileOrOf<Op>) return a | b;
+    if constexpr (kTileReduceOf<Op> == kReduceOr) return a | b;
+    else if constexpr (kTileReduceOf<Op> == kReduceMin) return a < b ? a : b;
     else return a + b;
+}
+template <class Op, class T>
+__device__ __forceinline__ T tile_identity() {
+    if constexpr (kTileReduceOf<Op> == kReduceMin) return ~T(0);
+    else return T(0);
 }
 
 template <class T>
 __device__ __forceinline__ const T *cold_values(const TileArgs &ta) {
     if constexpr (sizeof(T) == 8) return static_cast<const T *>(ta.cold_val64);
-    else return ta.cold_val;
+    else return reinterpret_cast<const T *>(ta.cold_val);
 }
 __device__ __forceinline__ unsigned ld_stream_u16(const uint16_t *p) {
     return __builtin_nontemporal_load(p);
@@ -517,7 +528,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pull_tiles(Op op, TileArgs ta,
     const int t = part.tile;
     const int vb = ta.vbeg[t];
     const int nv = ta.vbeg[t + 1] - vb;  // <= kTileMaxV by construction
-    for (int i = threadIdx.x; i < nv; i += kTileThreads) acc[i] = T(0);
+    for (int i = threadIdx.x; i < nv; i += kTileThreads) acc[i] = tile_identity<Op, T>();
     __syncthreads();
     constexpr int U = 4;  // measured on MI355X: 4 edges in flight per lane beat 2, 3, 8, 16
     constexpr unsigned kDstMask = (1u << kPackDstBits) - 1u;
@@ -724,7 +735,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
             // blocks are the tail of the launch)
             T q[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) q[j] = T(0);
+            for (int j = 0; j < 8; j++) q[j] = tile_identity<Op, T>();
             int k = 0;
             for (; k + 7 < f.nparts; k += 8) {
                 T x[8];
