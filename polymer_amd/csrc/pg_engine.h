@@ -509,9 +509,12 @@ __device__ __forceinline__ float tile_weight(const Op &, const float *wt, int64_
     if constexpr (Op::kTileWeighted) return __builtin_nontemporal_load(wt + i);
     else return 1.f;
 }
+// (a weighted Op that reduces by min adds the weight instead: its tile_fix_w(x, w)
+// makes the term, as a min-plus relaxation does)
 template <class Op>
 __device__ __forceinline__ tile_acc_t<Op> tile_term(const Op &op, tile_val_t<Op> x, float wv) {
-    if constexpr (Op::kTileWeighted) return op.tile_fix(x * wv);
+    if constexpr (Op::kTileWeighted && kTileReduceOf<Op> == kReduceMin) return op.tile_fix_w(x, wv);
+    else if constexpr (Op::kTileWeighted) return op.tile_fix(x * wv);
     else return op.tile_fix(x);
 }
 
