@@ -313,7 +313,7 @@ int pagerank_enqueue(pg_graph *g, int iters, double damping, KernelTimer *kt, fl
         if (kt) PG_TRY(kt->begin());
         if (tiled)  // the zero-in-degree vertices are settled after two iterations
             PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, &B.stats[(size_t)t * kStatSlots],
-                                     t >= 2, pre ? &next : nullptr, ahead && t > 0));
+                                     t >= 2, pre ? &next : nullptr, ahead && t > 0, true));
         else
             PG_TRY(launch_pull(op, g->in_sched, &B.stats[(size_t)t * kStatSlots]));
         if (kt) PG_TRY(kt->end());

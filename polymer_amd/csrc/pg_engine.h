@@ -782,10 +782,17 @@ __global__ __launch_bounds__(kBlock) void k_tile_finish(Op op, TileArgs ta,
 // on the side stream as soon as this iteration's tile parts are done, beside the
 // finishing pass; the next call, with cold_ready, waits for it instead of running
 // its own.  (The cold value array is read only by the tile parts, which are done.)
+// finish_aside: with ahead, the roles of the streams swap: the finishing pass runs
+// on the side stream and the next cold pre-pass follows the tile parts on this
+// stream, which saves a cross-stream hand-off on the critical path (the cold
+// pre-pass is the longer of the two).  Only for a caller that enqueues nothing
+// reading this pull's results (values or stat_out) before its next call, whose
+// cold_ready wait then covers the finishing pass.
 template <class Op>
 inline int launch_pull_tiled(const Op &op, const PullSchedule &s, const TileSet &t,
                              typename Op::Stat *stat_out = nullptr, bool skip_tail = false,
-                             const Op *ahead = nullptr, bool cold_ready = false) {
+                             const Op *ahead = nullptr, bool cold_ready = false,
+                             bool finish_aside = false) {
     if (cold_ready) {
         hipError_t e = hipStreamWaitEvent(stream(), t.ev_cold, 0);
         if (e != hipSuccess) {
@@ -803,6 +810,24 @@ inline int launch_pull_tiled(const Op &op, const PullSchedule &s, const TileSet 
     if (t.args.nparts > 0)
         hipLaunchKernelGGL(k_pull_tiles<Op>, dim3(t.args.nparts), dim3(kTileThreads), 0, stream(),
                            op, t.args, stat_out);
+    if (ahead && t.cold_overlap && t.cold.nparts > 0 && finish_aside) {
+        hipError_t e = hipEventRecord(t.ev_tiles, stream());
+        if (e == hipSuccess) e = hipStreamWaitEvent(side_stream(), t.ev_tiles, 0);
+        if (e == hipSuccess && t.args.nfin > 0)
+            hipLaunchKernelGGL(k_tile_finish<Op>, dim3(t.args.nfin), dim3(kBlock), 0,
+                               side_stream(), op, t.args, stat_out);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(t.ev_cold, side_stream());
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(k_cold_gather<Op>, dim3(t.cold.nparts), dim3(kTileThreads), 0,
+                               stream(), *ahead, t.cold);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error("tiled pull: finishing pass aside failed: %s", hipGetErrorString(e));
+            return PG_ERR_HIP;
+        }
+        return PG_OK;
+    }
     if (ahead && t.cold_overlap && t.cold.nparts > 0) {
         hipError_t e = hipEventRecord(t.ev_tiles, stream());
         if (e == hipSuccess) e = hipStreamWaitEvent(side_stream(), t.ev_tiles, 0);
