@@ -821,7 +821,8 @@ extern "C" int pg_ppr(pg_graph *g, int64_t src, int iters, double damping, float
         if (tiled)
             PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles,
                                      (PprOp::Stat *)nullptr, t >= 2 && t != iters - 1,
-                                     ahead && t + 1 < iters ? &next : nullptr, ahead && t > 0));
+                                     ahead && t + 1 < iters ? &next : nullptr, ahead && t > 0,
+                                     true));
         else
             PG_TRY(launch_pull(op, g->in_sched));
         PG_TRY(kt.end());
