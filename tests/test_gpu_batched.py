@@ -69,6 +69,22 @@ def test_ppr_batched(pa, orc, k):
     assert rel_l1(R[1], plain) < TOL
 
 
+def test_ppr_batched_short(pa, orc):
+    """1-3 iterations (the first iteration runs as a push from the sources' rows
+    unless it is the last), with a repeated source and a source without out-edges."""
+    say("case: ppr batched short")
+    g = orc.rmat(14, 1 << 18, seed=31)
+    d = pa.Graph.rmat(14, 1 << 18, seed=31)
+    outdeg = np.diff(g.off)
+    sink = int(np.flatnonzero(outdeg == 0)[0])
+    src = np.array([0, 77, 77, sink, 9999, int(np.argmax(outdeg))], np.int64)
+    for iters in (1, 2, 3):
+        R, _ = d.ppr_batched(src, iters, 0.85)
+        for c in range(len(src)):
+            ref = orc.ppr(g, onehot(g.n, src[c]), iters, 0.85)
+            assert rel_l1(R[c], ref) < TOL, (iters, c, int(src[c]))
+
+
 # ---- multi-source BFS / SSSP -------------------------------------------------------------
 def test_msbfs(pa, orc):
     say("case: msbfs 64 sources")
