@@ -140,10 +140,11 @@ struct PprRowOp {
 // 2.28 ms for iterations 0-2 with tests, 1.89 ms without)
 constexpr int kPprSkipIters = 2;
 
-// in-degree classes of the batched PPR iteration (measured on the scale-22 graph:
-// 39.5 ms for 20 iterations at these, 46 ms at 4096 / 2048 / 256, 66 ms with a
-// heavy limit of 65536)
-constexpr int32_t kPprHeavy = 8192, kPprChunk = 4096, kPprLight = 512;
+// in-degree classes of the batched PPR iteration, measured on the scale-22 rMat with 64
+// sources, 20 iterations: light limit 16/32/48/64/128/512/2048 -> 36.38/36.15/36.05/
+// 36.09/36.17/36.70/37.07 ms; heavy limit 4096 or 16384 -> 41.7 and 42.5 ms; chunk 2048
+// or 8192 -> 36.8 and 37.0 ms; a heavy limit of 65536 took 66 ms in an earlier form
+constexpr int32_t kPprHeavy = 8192, kPprChunk = 4096, kPprLight = 64;
 
 // r = d·sum + (1-d)[v is the column's source] for the lane's four columns; the rank
 // row on the last iteration, the contribution row r/outdeg when v has out-edges
