@@ -596,7 +596,9 @@ __global__ void k_msbfs_levels_out(const int32_t *lev, const unsigned long long 
 // a round with few active vertices streams little beyond the in-lists; a vertex
 // with no candidate reads nothing of its own row.  The improvements of the round
 // are counted per column into spread counters (the stop test).
-constexpr int32_t kMsLight = 256, kMsChunk = 2048;
+// (64 sources on the symmetrised weighted scale-22 rMat: light limit 64/128/256/512 ->
+// 59.0/58.9/59.4/60.9 ms; chunk 1024/4096 at a limit of 256 -> 59.8/63.2 ms)
+constexpr int32_t kMsLight = 128, kMsChunk = 2048;
 
 __device__ __forceinline__ void ms_relax4(int4 &best, unsigned nib, int4 ds, int we) {
     if (nib & 1u) best.x = min(best.x, ds.x + we);
