@@ -1904,8 +1904,12 @@ extern "C" int pg_spmv(pg_graph *g, int iters, const float *x0, double *y, uint3
                                t == 0 ? x0max : maxbits + t, wmax, tscale + t);
             // the zero-in-degree tail holds zeros in both buffers from iteration 2
             // (a fresh slot of the kept iterates does not: it is written every time)
+            // the finishing pass runs on the side stream beside the next cold pre-pass;
+            // the exponent shift below reads its maxima, so it waits for it
+            const bool aside = ahead && t + 1 < iters;
             PG_TRY(launch_pull_tiled(op, g->in_sched, g->tiles, maxbits + t + 1, t >= 2 && !save,
-                                     ahead && t + 1 < iters ? &next : nullptr, ahead && t > 0));
+                                     aside ? &next : nullptr, ahead && t > 0, true));
+            if (aside) PG_CHECK_HIP(hipStreamWaitEvent(stream(), g->tiles.ev_cold, 0));
         } else {
             PG_TRY(launch_pull(op, g->in_sched, maxbits + t + 1));
         }
